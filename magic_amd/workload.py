@@ -1,0 +1,93 @@
+"""Synthetic spectral inputs and grid sizing.
+
+Coefficients are c_lm = A_l exp(i phi) with phi uniform in [0, 2 pi) drawn
+from a SplitMix64 stream keyed by (seed, field, level); m = 0 keeps the real
+part only; l = 0 is zero for poloidal/toroidal potentials.  The generator is
+portable integer arithmetic so that C/C++/Python writers produce identical
+inputs.
+
+Grid sizing follows the reference's truncation rules
+(src/truncation.f90:66-71, prime_decomposition :162-192): n_theta =
+30 l_max / nalias, n_phi = 2 n_theta rounded up to a 2^i 3^j 5^k multiple of
+minc, then n_theta = n_phi_tot / 2.
+"""
+import numpy as np
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+SEED_BASE = 0x4D41474943000000
+
+
+def splitmix64(seed, n):
+    """n outputs of SplitMix64 started at ``seed`` (uint64 array)."""
+    with np.errstate(over="ignore"):
+        s = np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        z = s + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed, n):
+    return (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def stream_seed(config, field, level):
+    return (SEED_BASE + config) ^ ((field + 1) * 0x100000001B3) ^ ((level + 1) * 0xC2B2AE3D27D4EB4F)
+
+
+def lm_index(l_max, m_max=None, minc=1):
+    m_max = l_max if m_max is None else min(m_max, l_max)
+    ls, ms = [], []
+    for m in range(0, m_max + 1, minc):
+        ls.extend(range(m, l_max + 1))
+        ms.extend([m] * (l_max - m + 1))
+    return np.array(ls, dtype=np.int64), np.array(ms, dtype=np.int64)
+
+
+def spectrum(l, kind):
+    l = np.asarray(l, dtype=np.float64)
+    if kind == "l+1^-2":
+        return (l + 1.0) ** -2
+    if kind == "l^-5/3":
+        return np.maximum(l, 1.0) ** (-5.0 / 3.0)
+    if kind.startswith("l+1^-"):
+        return (l + 1.0) ** (-float(kind[5:]))
+    raise ValueError(kind)
+
+
+def coefficients(l_max, config, field, level, kind="l+1^-2", potential=True, m_max=None,
+                 minc=1, amp=1.0):
+    ls, ms = lm_index(l_max, m_max, minc)
+    ph = 2.0 * np.pi * uniform01(stream_seed(config, field, level), ls.size)
+    c = amp * spectrum(ls, kind) * np.exp(1j * ph)
+    c[ms == 0] = c[ms == 0].real
+    if potential:
+        c[ls == 0] = 0.0
+    return c.astype(np.complex128)
+
+
+def prime_decomposition(n):
+    """Smallest m >= n that factors as 2^i 3^j 5^k (src/truncation.f90:162-192)."""
+    m = n
+    while True:
+        k = m
+        for f in (2, 3, 5):
+            while k % f == 0:
+                k //= f
+        if k == 1:
+            return m
+        m += 1
+
+
+def grid_size(l_max, nalias=20, minc=1):
+    """(n_theta, n_phi) the reference derives from l_max (src/truncation.f90:66-71),
+    with the consistency checks of checkTruncation (:126-160)."""
+    n_theta = (30 * l_max) // nalias
+    n_phi_tot = prime_decomposition(2 * n_theta)
+    n_theta, n_phi = n_phi_tot // 2, n_phi_tot // minc
+    if n_phi_tot % minc or n_phi % 4 or n_phi_tot % 16 or n_theta % 4:
+        raise ValueError("l_max=%d, minc=%d gives an invalid grid (%d x %d)"
+                         % (l_max, minc, n_theta, n_phi))
+    return n_theta, n_phi

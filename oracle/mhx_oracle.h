@@ -1,0 +1,111 @@
+/*
+ * ORACLE - test infrastructure only.
+ *
+ * Plain-C FP64 restatement of the reference's native spherical-harmonic
+ * transform path (MagIC src/plms.f90, src/horizontal.f90, src/fft.f90,
+ * src/shtransforms.f90, src/sht_native.f90, src/get_nl.f90).  Only tests,
+ * the smoke check and the CPU-baseline leg of the benchmark may load it, and
+ * only as the checker.  The product library (libmhx) never calls it.
+ *
+ * Conventions (all as in the reference):
+ *   - coefficient arrays: complex128 interleaved (re,im), standard m-major
+ *     lm order (m = 0, minc, 2 minc, ..., l = m..l_max), length lm_max;
+ *   - grid arrays: float64 f[phi][theta] (theta fastest, Fortran f(nt,np)),
+ *     theta rows north/south interleaved (row 2k = theta_k, 2k+1 = pi-theta_k,
+ *     0-based);
+ *   - Fourier arrays: complex128 F[m][theta], m = 0..n_phi/2.
+ */
+#ifndef MHX_ORACLE_H
+#define MHX_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_plan orc_plan;
+
+/* Gauss-Legendre colatitudes (ascending from the north pole) and weights
+   (sum 2) for n points, src/horizontal.f90:279-340. */
+void orc_gauleg(int n, double *theta_ord, double *gauss);
+
+/* Fully normalised P_lm(cos theta) and sin(theta) dP_lm/dtheta, m-major,
+   for m = m_min..m_max step minc, l = m..l_max (norm=2 of
+   src/plms.f90:14-189). */
+void orc_plm_theta(double theta, int l_max, int m_min, int m_max, int minc,
+                   double *plm, double *dtheta_plm);
+
+orc_plan *orc_plan_create(int l_max, int m_max, int minc, int n_theta,
+                          int n_phi);
+void orc_plan_destroy(orc_plan *p);
+int orc_plan_lm_max(const orc_plan *p);
+int orc_plan_n_m_max(const orc_plan *p);
+/* copies of the plan's tables: theta_ord[nh], gauss[nh] (northern half),
+   plm/dplm/wplm/wdplm[nh][lm_max] (lm fastest), osin2[n_theta] (scrambled) */
+void orc_plan_tables(const orc_plan *p, double *theta_ord, double *gauss,
+                     double *plm, double *dplm, double *wplm, double *wdplm,
+                     double *osin2);
+
+/* phi FFTs, src/fft.f90:164-252: forward F = rfft(f)/n_phi, inverse the
+   unnormalised Hermitian synthesis (imaginary parts of modes 0 and n_phi/2
+   ignored).  F is [n_phi/2+1][n_theta] complex. */
+void orc_fft_forward(const orc_plan *p, const double *grid, double *F);
+void orc_fft_inverse(const orc_plan *p, const double *F, double *grid);
+
+/* native transforms, src/shtransforms.f90 */
+void orc_sph_to_spat(const orc_plan *p, const double *Slm, double *sc, int lcut);
+void orc_sph_to_grad_spat(const orc_plan *p, const double *Slm, double *gt,
+                          double *gp, int lcut);
+void orc_qst_to_spat(const orc_plan *p, const double *Q, const double *S,
+                     const double *T, double *vr, double *vt, double *vp,
+                     int lcut);
+void orc_sphtor_to_spat(const orc_plan *p, const double *S, const double *T,
+                        double *vt, double *vp, int lcut);
+void orc_axi_to_spat(const orc_plan *p, const double *Sl, double *sc);
+void orc_toraxi_to_spat(const orc_plan *p, const double *Tl, double *vt,
+                        double *vp, int lcut);
+void orc_spat_to_sph(const orc_plan *p, const double *f, double *flm, int lcut);
+void orc_spat_to_sph_tor(const orc_plan *p, const double *vt, const double *vp,
+                         double *slm, double *tlm, int lcut);
+
+/* module sht API, src/sht_native.f90 */
+void orc_scal_to_spat(const orc_plan *p, const double *S, double *f, int lcut);
+void orc_scal_to_grad_spat(const orc_plan *p, const double *S, double *gt,
+                           double *gp, int lcut);
+void orc_pol_to_grad_spat(const orc_plan *p, const double *S, double *gt,
+                          double *gp, int lcut);
+void orc_torpol_to_spat(const orc_plan *p, const double *W, const double *dW,
+                        const double *Z, double *vr, double *vt, double *vp,
+                        int lcut);
+void orc_pol_to_curlr_spat(const orc_plan *p, const double *Q, double *cvr,
+                           int lcut);
+void orc_torpol_to_curl_spat(const orc_plan *p, double or2, const double *B,
+                             const double *ddB, const double *J,
+                             const double *dJ, double *cvr, double *cvt,
+                             double *cvp, int lcut);
+void orc_torpol_to_dphspat(const orc_plan *p, const double *dW, const double *Z,
+                           double *dvt, double *dvp, int lcut);
+void orc_torpol_to_spat_IC(const orc_plan *p, double r, double r_icb,
+                           const double *W, const double *dW, const double *Z,
+                           double *br, double *bt, double *bp);
+void orc_torpol_to_curl_spat_IC(const orc_plan *p, double r, double r_icb,
+                                const double *dB, const double *ddB,
+                                const double *J, const double *dJ, double *cbr,
+                                double *cbt, double *cbp);
+void orc_scal_to_SH(const orc_plan *p, const double *f, double *flm, int lcut);
+void orc_spat_to_qst(const orc_plan *p, const double *f, const double *g,
+                     const double *h, double *q, double *s, double *t, int lcut);
+void orc_spat_to_sphertor(const orc_plan *p, const double *f, const double *g,
+                          double *flm, double *glm, int lcut);
+
+/* grid-space products of one bulk level (src/get_nl.f90:240-400 with
+   l_adv_curl, l_mag_LF, l_heat_nl, l_mag_nl; Boussinesq):
+   in[13] = vr vt vp cvr cvt cvp br bt bp cbr cbt cbp s,
+   out[12] = Adv(r,t,p) LF(r,t,p) VxB(r,t,p) VS(r,t,p).  Each array is one
+   n_theta x n_phi grid. */
+void orc_get_nl(const orc_plan *p, double or2, double or4, double orho1,
+                double LFfac, const double *const *in, double *const *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,267 @@
+"""ORACLE - test infrastructure only.
+
+ctypes front end of ``liboracle.so`` (the FP64 C restatement of MagIC's
+native transform path, ``mhx_oracle.c``) and of the reference harness
+``_ref/ref_harness`` (the reference's own Fortran modules).  Only the tests,
+``__graft_entry__.smoke`` and the CPU-baseline leg of ``bench.py`` may import
+this module, and only to check or time; the product never does.
+
+Array conventions follow the reference: coefficient vectors are complex128 of
+length lm_max in m-major order; grids are float64 arrays of shape
+(n_phi, n_theta) in C order, i.e. theta fastest like Fortran ``f(nt, np)``.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE, "liboracle.so"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        D = ctypes.c_double
+        I = ctypes.c_int
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.orc_plan_create.restype = P
+        L.orc_plan_create.argtypes = [I, I, I, I, I]
+        L.orc_plan_destroy.argtypes = [P]
+        L.orc_plan_lm_max.argtypes = [P]
+        L.orc_plan_n_m_max.argtypes = [P]
+        L.orc_gauleg.argtypes = [I, dp, dp]
+        L.orc_plm_theta.argtypes = [D, I, I, I, I, dp, dp]
+        L.orc_plan_tables.argtypes = [P] + [dp] * 7
+        for name in ("orc_fft_forward", "orc_fft_inverse"):
+            getattr(L, name).argtypes = [P, dp, dp]
+        L.orc_get_nl.argtypes = [P, D, D, D, D, ctypes.POINTER(dp), ctypes.POINTER(dp)]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def gauleg(n):
+    th = np.zeros(n)
+    w = np.zeros(n)
+    lib().orc_gauleg(n, _ptr(th), _ptr(w))
+    return th, w
+
+
+def plm_theta(theta, l_max, m_max=None, minc=1):
+    m_max = l_max if m_max is None else m_max
+    lm_max = sum(l_max - m + 1 for m in range(0, m_max + 1, minc))
+    p = np.zeros(lm_max)
+    d = np.zeros(lm_max)
+    lib().orc_plm_theta(float(theta), l_max, 0, m_max, minc, _ptr(p), _ptr(d))
+    return p, d
+
+
+def lm_index(l_max, m_max=None, minc=1):
+    """(lm2l, lm2m) of the standard m-major order (src/blocking.f90:309-317)."""
+    m_max = l_max if m_max is None else min(m_max, l_max)
+    ls, ms = [], []
+    for m in range(0, m_max + 1, minc):
+        for l in range(m, l_max + 1):
+            ls.append(l)
+            ms.append(m)
+    return np.array(ls), np.array(ms)
+
+
+class Plan:
+    """FP64 oracle plan for one truncation and grid."""
+
+    def __init__(self, l_max, n_theta, n_phi, m_max=None, minc=1):
+        m_max = l_max if m_max is None else min(m_max, l_max)
+        m_max = (m_max // minc) * minc
+        self.l_max, self.m_max, self.minc = l_max, m_max, minc
+        self.n_theta, self.n_phi = n_theta, n_phi
+        self._L = lib()
+        self.h = self._L.orc_plan_create(l_max, m_max, minc, n_theta, n_phi)
+        self.lm_max = self._L.orc_plan_lm_max(self.h)
+        self.n_m_max = self._L.orc_plan_n_m_max(self.h)
+        self.lm2l, self.lm2m = lm_index(l_max, m_max, minc)
+
+    def __del__(self):
+        try:
+            self._L.orc_plan_destroy(self.h)
+        except Exception:
+            pass
+
+    # -- helpers
+    def grid(self):
+        return np.zeros((self.n_phi, self.n_theta))
+
+    def coef(self):
+        return np.zeros(self.lm_max, dtype=np.complex128)
+
+    def tables(self):
+        nh = self.n_theta // 2
+        th = np.zeros(nh)
+        gw = np.zeros(nh)
+        t = [np.zeros((nh, self.lm_max)) for _ in range(4)]
+        o2 = np.zeros(self.n_theta)
+        self._L.orc_plan_tables(self.h, _ptr(th), _ptr(gw), *[_ptr(x) for x in t], _ptr(o2))
+        return dict(theta=th, gauss=gw, plm=t[0], dplm=t[1], wplm=t[2], wdplm=t[3], osin2=o2)
+
+    def _c(self, a, n=None):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        if a.shape != ((self.lm_max,) if n is None else (n,)):
+            raise ValueError("coefficient array has wrong shape %r" % (a.shape,))
+        return a
+
+    def _g(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != (self.n_phi, self.n_theta):
+            raise ValueError("grid has wrong shape %r" % (a.shape,))
+        return a
+
+    def _call(self, name, *args):
+        conv = []
+        for a in args:
+            if isinstance(a, np.ndarray):
+                conv.append(_ptr(a))
+            elif isinstance(a, float):
+                conv.append(ctypes.c_double(a))
+            else:
+                conv.append(ctypes.c_int(int(a)))
+        getattr(self._L, name)(ctypes.c_void_p(self.h), *conv)
+
+    # -- FFT
+    def fft_forward(self, g):
+        F = np.zeros((self.n_phi // 2 + 1, self.n_theta), dtype=np.complex128)
+        self._call("orc_fft_forward", self._g(g), F)
+        return F
+
+    def fft_inverse(self, F):
+        F = np.ascontiguousarray(F, dtype=np.complex128)
+        g = self.grid()
+        self._call("orc_fft_inverse", F, g)
+        return g
+
+    # -- module sht API (src/sht_native.f90)
+    def scal_to_spat(self, S, lcut):
+        f = self.grid()
+        self._call("orc_scal_to_spat", self._c(S), f, lcut)
+        return f
+
+    def scal_to_grad_spat(self, S, lcut):
+        a, b = self.grid(), self.grid()
+        self._call("orc_scal_to_grad_spat", self._c(S), a, b, lcut)
+        return a, b
+
+    def pol_to_grad_spat(self, S, lcut):
+        a, b = self.grid(), self.grid()
+        self._call("orc_pol_to_grad_spat", self._c(S), a, b, lcut)
+        return a, b
+
+    def torpol_to_spat(self, W, dW, Z, lcut):
+        a, b, c = self.grid(), self.grid(), self.grid()
+        self._call("orc_torpol_to_spat", self._c(W), self._c(dW), self._c(Z), a, b, c, lcut)
+        return a, b, c
+
+    def pol_to_curlr_spat(self, Q, lcut):
+        a = self.grid()
+        self._call("orc_pol_to_curlr_spat", self._c(Q), a, lcut)
+        return a
+
+    def torpol_to_curl_spat(self, or2, B, ddB, J, dJ, lcut):
+        a, b, c = self.grid(), self.grid(), self.grid()
+        self._call("orc_torpol_to_curl_spat", float(or2), self._c(B), self._c(ddB),
+                   self._c(J), self._c(dJ), a, b, c, lcut)
+        return a, b, c
+
+    def torpol_to_dphspat(self, dW, Z, lcut):
+        a, b = self.grid(), self.grid()
+        self._call("orc_torpol_to_dphspat", self._c(dW), self._c(Z), a, b, lcut)
+        return a, b
+
+    def sphtor_to_spat(self, S, T, lcut):
+        a, b = self.grid(), self.grid()
+        self._call("orc_sphtor_to_spat", self._c(S), self._c(T), a, b, lcut)
+        return a, b
+
+    def qst_to_spat(self, Q, S, T, lcut):
+        a, b, c = self.grid(), self.grid(), self.grid()
+        self._call("orc_qst_to_spat", self._c(Q), self._c(S), self._c(T), a, b, c, lcut)
+        return a, b, c
+
+    def torpol_to_spat_IC(self, r, r_icb, W, dW, Z):
+        a, b, c = self.grid(), self.grid(), self.grid()
+        self._call("orc_torpol_to_spat_IC", float(r), float(r_icb), self._c(W), self._c(dW),
+                   self._c(Z), a, b, c)
+        return a, b, c
+
+    def torpol_to_curl_spat_IC(self, r, r_icb, dB, ddB, J, dJ):
+        a, b, c = self.grid(), self.grid(), self.grid()
+        self._call("orc_torpol_to_curl_spat_IC", float(r), float(r_icb), self._c(dB),
+                   self._c(ddB), self._c(J), self._c(dJ), a, b, c)
+        return a, b, c
+
+    def axi_to_spat(self, Sl):
+        f = np.zeros(self.n_theta)
+        self._call("orc_axi_to_spat", self._c(Sl, self.l_max + 1), f)
+        return f
+
+    def toraxi_to_spat(self, Tl, lcut):
+        a, b = np.zeros(self.n_theta), np.zeros(self.n_theta)
+        self._call("orc_toraxi_to_spat", self._c(Tl, self.l_max + 1), a, b, lcut)
+        return a, b
+
+    def scal_to_SH(self, f, lcut):
+        out = self.coef()
+        self._call("orc_scal_to_SH", self._g(f), out, lcut)
+        return out
+
+    def spat_to_sphertor(self, f, g, lcut):
+        a, b = self.coef(), self.coef()
+        self._call("orc_spat_to_sphertor", self._g(f), self._g(g), a, b, lcut)
+        return a, b
+
+    def spat_to_qst(self, f, g, h, lcut):
+        a, b, c = self.coef(), self.coef(), self.coef()
+        self._call("orc_spat_to_qst", self._g(f), self._g(g), self._g(h), a, b, c, lcut)
+        return a, b, c
+
+    def get_nl(self, or2, or4, orho1, LFfac, grids):
+        """grids: 13 arrays (vr vt vp cvr cvt cvp br bt bp cbr cbt cbp s).
+        Returns 12 arrays Adv(3) LF(3) VxB(3) VS(3)."""
+        ins = [self._g(g) for g in grids]
+        outs = [self.grid() for _ in range(12)]
+        arr_in = (ctypes.POINTER(ctypes.c_double) * 13)(*[_ptr(a) for a in ins])
+        arr_out = (ctypes.POINTER(ctypes.c_double) * 12)(*[_ptr(a) for a in outs])
+        self._L.orc_get_nl(ctypes.c_void_p(self.h), float(or2), float(or4), float(orho1),
+                           float(LFfac), arr_in, arr_out)
+        return outs
+
+
+def have_ref():
+    return os.path.exists(REF_HARNESS)
+
+
+def run_ref(mode, l_max, minc, nalias, infile, outfile, nlev=1, nrep=1, threads=None,
+            cwd=None):
+    """Runs the reference harness (large automatic arrays: unlimited stack)."""
+    env = dict(os.environ)
+    env.setdefault("OMP_STACKSIZE", "1G")
+    if threads is not None:
+        env["OMP_NUM_THREADS"] = str(threads)
+    cmd = "ulimit -s unlimited; %s %s %d %d %d %s %s %d %d" % (
+        REF_HARNESS, mode, l_max, minc, nalias, infile, outfile, nlev, nrep)
+    return subprocess.run(["bash", "-c", cmd], env=env, cwd=cwd, check=True,
+                          capture_output=True, text=True)

@@ -1,0 +1,174 @@
+"""Writes the golden vectors in tests/golden/*.npz from the reference itself.
+
+Runs oracle/_ref/ref_harness (the reference's module sht, shtransforms,
+plms, fft, horizontal and get_nl compiled from the reference sources by
+oracle/Makefile) on seeded synthetic inputs and stores inputs and outputs.
+Run in the build container, where the reference tree exists:
+
+    make -C oracle && python tests/golden/make_golden.py
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from magic_amd import workload  # noqa: E402
+from oracle import oracle  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+# order of results written by ref_harness 'api' mode, per level:
+# (name, kind, count)  kind: g = grid, a = n_theta vector, c = coefficient vector
+API_OUTPUTS = [
+    ("scal_to_spat", "g", 1), ("scal_to_grad_spat", "g", 2), ("pol_to_grad_spat", "g", 2),
+    ("torpol_to_spat", "g", 3), ("pol_to_curlr_spat", "g", 1),
+    ("torpol_to_curl_spat", "g", 3), ("torpol_to_dphspat", "g", 2),
+    ("sphtor_to_spat", "g", 2), ("torpol_to_spat_IC", "g", 3),
+    ("torpol_to_curl_spat_IC", "g", 3), ("axi_to_spat", "a", 1),
+    ("toraxi_to_spat", "a", 2), ("scal_to_SH", "c", 1), ("spat_to_sphertor", "c", 2),
+    ("spat_to_qst", "c", 3),
+]
+
+
+def read_header(f):
+    h = np.fromfile(f, dtype="<i4", count=6)
+    return dict(zip(("l_max", "m_max", "minc", "n_theta", "n_phi", "lm_max"), h.tolist()))
+
+
+def make_api_inputs(l_max, minc, n_theta, n_phi, lcuts, config, real_m0=True):
+    ls, ms = workload.lm_index(l_max, None, minc)
+    levels = []
+    for lev, lcut in enumerate(lcuts):
+        coefs = []
+        for fld in range(11):
+            c = workload.coefficients(l_max, config, fld, lev, "l+1^-2", potential=False,
+                                      minc=minc)
+            if not real_m0:
+                u = workload.uniform01(workload.stream_seed(config, 50 + fld, lev), ls.size)
+                c = c + 1j * (ms == 0) * (u - 0.5) / (ls + 1.0) ** 2
+            coefs.append(c)
+        u = workload.uniform01(workload.stream_seed(config, 99, lev), 3 * n_phi * n_theta)
+        grids = (2.0 * u - 1.0).reshape(3, n_phi, n_theta)
+        levels.append(dict(lcut=lcut, r=0.3, r_icb=0.35, or2=1.0 / 0.81, coefs=coefs,
+                           grids=grids))
+    return levels
+
+
+def write_api_input(path, levels):
+    with open(path, "wb") as f:
+        for L in levels:
+            np.array([L["lcut"]], dtype="<i4").tofile(f)
+            np.array([L["r"], L["r_icb"], L["or2"]], dtype="<f8").tofile(f)
+            for c in L["coefs"]:
+                np.ascontiguousarray(c, dtype="<c16").tofile(f)
+            np.ascontiguousarray(L["grids"], dtype="<f8").tofile(f)
+
+
+def read_api_output(path, nlev):
+    res = []
+    with open(path, "rb") as f:
+        hd = read_header(f)
+        nt, npz, lm = hd["n_theta"], hd["n_phi"], hd["lm_max"]
+        for _ in range(nlev):
+            out = {}
+            for name, kind, cnt in API_OUTPUTS:
+                arrs = []
+                for _k in range(cnt):
+                    if kind == "g":
+                        arrs.append(np.fromfile(f, "<f8", nt * npz).reshape(npz, nt))
+                    elif kind == "a":
+                        arrs.append(np.fromfile(f, "<f8", nt))
+                    else:
+                        arrs.append(np.fromfile(f, "<c16", lm))
+                out[name] = np.stack(arrs)
+            res.append(out)
+        assert f.read() == b"", "trailing data in harness output"
+    return hd, res
+
+
+def api_case(tag, l_max, minc, lcuts, config, real_m0=True):
+    n_theta, n_phi = workload.grid_size(l_max, 20, minc)
+    levels = make_api_inputs(l_max, minc, n_theta, n_phi, lcuts, config, real_m0)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        write_api_input(fin, levels)
+        oracle.run_ref("api", l_max, minc, 20, fin, fout, nlev=len(levels), threads=4)
+        hd, res = read_api_output(fout, len(levels))
+    assert hd["n_theta"] == n_theta and hd["n_phi"] == n_phi, hd
+    data = dict(l_max=l_max, minc=minc, n_theta=n_theta, n_phi=n_phi, m_max=hd["m_max"],
+                lcut=np.array(lcuts), r=0.3, r_icb=0.35, or2=1.0 / 0.81)
+    for lev, (L, R) in enumerate(zip(levels, res)):
+        data["in_coefs_%d" % lev] = np.stack(L["coefs"])
+        data["in_grids_%d" % lev] = L["grids"]
+        for k, v in R.items():
+            data["out_%s_%d" % (k, lev)] = v
+    path = os.path.join(OUT, "api_%s.npz" % tag)
+    np.savez_compressed(path, **data)
+    print("wrote", path, os.path.getsize(path))
+
+
+def tables_case(tag, l_max, minc):
+    n_theta, n_phi = workload.grid_size(l_max, 20, minc)
+    with tempfile.TemporaryDirectory() as td:
+        fout = os.path.join(td, "t.bin")
+        oracle.run_ref("tables", l_max, minc, 20, "/dev/null", fout, threads=1)
+        with open(fout, "rb") as f:
+            hd = read_header(f)
+            nt, lm = hd["n_theta"], hd["lm_max"]
+            theta_ord = np.fromfile(f, "<f8", nt)
+            gauss = np.fromfile(f, "<f8", nt)
+            osin2 = np.fromfile(f, "<f8", nt)
+            cost = np.fromfile(f, "<f8", nt)
+            sint = np.fromfile(f, "<f8", nt)
+            tabs = [np.fromfile(f, "<f8", lm * (nt // 2)).reshape(nt // 2, lm) for _ in range(4)]
+            lm2l = np.fromfile(f, "<f8", lm)
+            lm2m = np.fromfile(f, "<f8", lm)
+            assert f.read() == b""
+    path = os.path.join(OUT, "tables_%s.npz" % tag)
+    np.savez_compressed(path, l_max=l_max, minc=minc, n_theta=n_theta, n_phi=n_phi,
+                        theta_ord=theta_ord, gauss=gauss, osin2=osin2, cos_theta=cost,
+                        sin_theta=sint, plm=tabs[0], dplm=tabs[1], wplm=tabs[2],
+                        wdplm=tabs[3], lm2l=lm2l.astype(np.int32), lm2m=lm2m.astype(np.int32))
+    print("wrote", path, os.path.getsize(path))
+
+
+def nl_case(tag, l_max):
+    n_theta, n_phi = workload.grid_size(l_max, 20, 1)
+    u = workload.uniform01(workload.stream_seed(7, 0, 0), 13 * n_phi * n_theta)
+    grids = (2.0 * u - 1.0).reshape(13, n_phi, n_theta)
+    rad = np.array([1.0 / 0.81, 1.0 / 0.6561, 1.25, 0.8])  # or2, or4, orho1, LFfac
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            rad.astype("<f8").tofile(f)
+            grids.astype("<f8").tofile(f)
+        oracle.run_ref("nl", l_max, 1, 20, fin, fout, threads=2)
+        with open(fout, "rb") as f:
+            read_header(f)
+            out = np.fromfile(f, "<f8", 12 * n_phi * n_theta).reshape(12, n_phi, n_theta)
+            assert f.read() == b""
+    path = os.path.join(OUT, "nl_%s.npz" % tag)
+    np.savez_compressed(path, l_max=l_max, n_theta=n_theta, n_phi=n_phi, radial=rad,
+                        in_grids=grids, out=out)
+    print("wrote", path, os.path.getsize(path))
+
+
+def main():
+    if not oracle.have_ref():
+        sys.exit("oracle/_ref/ref_harness is not built (make -C oracle)")
+    tables_case("L16_m1", 16, 1)
+    tables_case("L64_m4", 64, 4)
+    api_case("L16_m1", 16, 1, [16, 11, 0], config=1, real_m0=True)
+    api_case("L16_m1_cplxm0", 16, 1, [16], config=11, real_m0=False)
+    api_case("L16_m4", 16, 4, [16, 9], config=12)
+    api_case("L64_m1", 64, 1, [64], config=1)
+    api_case("L64_m4", 64, 4, [64, 40], config=13)
+    nl_case("L16", 16)
+
+
+if __name__ == "__main__":
+    main()
