@@ -1,0 +1,180 @@
+/*
+ * mhx - MI355X-native spherical-harmonic transforms and nonlinear-term path
+ * for MagIC-style dynamo codes.  C ABI (plain pointers and sizes).
+ *
+ * The per-call entry points mirror the reference's transform API, module sht
+ * (MagIC src/sht_native.f90:16-20), one function per public routine, with
+ * the same argument meaning and order after the leading plan handle.  The
+ * batched entry points replace the reference's per-radius loop
+ * (src/rIter.f90:190-444) with one launch per stage for all radial levels.
+ *
+ * Data conventions (identical to the reference):
+ *   coefficient vector: complex128 as interleaved (re, im) doubles, length
+ *       lm_max, standard m-major order: m = 0, minc, ..., m_max and
+ *       l = m..l_max inside each m (src/blocking.f90:309-317);
+ *   grid: n_theta x n_phi doubles, theta fastest (Fortran f(n_theta,n_phi)),
+ *       theta rows north/south interleaved: 0-based row 2k is colatitude
+ *       theta_k, row 2k+1 is pi - theta_k (src/horizontal.f90:141-191),
+ *       phi_j = j 2 pi / (n_phi minc);
+ *   vector grids in Robert form (sin(theta) v_theta, sin(theta) v_phi);
+ *   forward transforms scale by 1/n_phi (src/fft.f90:464).
+ *
+ * Pointers may be host or device memory; the library detects which.  Host
+ * pointers are staged through the plan's device buffers (PCIe-bound; the
+ * batched entry points with device pointers are the fast path).  All calls
+ * are synchronous with respect to the host unless they take a stream.
+ *
+ * Errors: every function returns 0 on success or a negative MHX_E* code; the
+ * message of the last error on the calling thread is mhx_last_error().
+ * The reference aborts the run instead (src/useful.f90:271-302); the
+ * Fortran shim (magic_amd/fortran/sht_mhx.f90) turns a nonzero code into
+ * that abort.
+ */
+#ifndef MHX_H
+#define MHX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHX_OK 0
+#define MHX_E_INVALID (-1)   /* bad argument (sizes, lcut, null pointer) */
+#define MHX_E_HIP (-2)       /* HIP runtime error */
+#define MHX_E_NOMEM (-3)     /* device allocation failed */
+#define MHX_E_UNSUPPORTED (-4)
+
+/* arithmetic of the Legendre stage */
+#define MHX_PREC_FP64 0      /* plain form: FP64 MFMA */
+#define MHX_PREC_SPLIT16 1   /* scaled FP16 hi/lo split, FP32 accumulation */
+
+typedef struct mhx_plan mhx_plan;
+
+const char *mhx_last_error(void);
+
+/* Plan for one truncation and grid; replaces initialize_sht +
+ * initialize_transforms (src/sht_native.f90:24-32,
+ * src/shtransforms.f90:38-91).  The P_lm tables are generated on the GPU in
+ * FP64 (recurrence of src/plms.f90:14-189) from Gauss-Legendre nodes
+ * (src/horizontal.f90:279-340).  n_theta must be a multiple of 4 and n_phi a
+ * multiple of 4 whose half factors into 2, 3 and 5 (src/fft.f90:55-73).
+ * max_levels sizes the batched workspaces (may be 0; grown on demand). */
+int mhx_plan_create(int l_max, int m_max, int minc, int n_theta, int n_phi,
+                    int precision, int max_levels, mhx_plan **plan);
+/* finalize_sht (src/sht_native.f90:34-38) */
+int mhx_plan_destroy(mhx_plan *plan);
+int mhx_plan_info(const mhx_plan *plan, int *lm_max, int *n_m_max,
+                  int *n_theta, int *n_phi);
+/* copies of the plan's FP64 tables in the reference's layout:
+ * theta_ord[n_theta/2], gauss[n_theta/2], plm/dplm[n_theta/2][lm_max]
+ * (lm fastest); any pointer may be NULL */
+int mhx_plan_tables(const mhx_plan *plan, double *theta_ord, double *gauss,
+                    double *plm, double *dplm);
+
+/* ---- module sht, one entry per public routine (src/sht_native.f90) ---- */
+/* :40-52 */
+int mhx_scal_to_spat(mhx_plan *p, const double *Slm, double *fieldc, int lcut);
+/* :54-68 */
+int mhx_scal_to_grad_spat(mhx_plan *p, const double *Slm, double *gradtc,
+                          double *gradpc, int lcut);
+/* :70-97 */
+int mhx_pol_to_grad_spat(mhx_plan *p, const double *Slm, double *gradtc,
+                         double *gradpc, int lcut);
+/* :99-127 */
+int mhx_torpol_to_spat(mhx_plan *p, const double *Wlm, const double *dWlm,
+                       const double *Zlm, double *vrc, double *vtc, double *vpc,
+                       int lcut);
+/* :129-141 */
+int mhx_sphtor_to_spat(mhx_plan *p, const double *dWlm, const double *Zlm,
+                       double *vtc, double *vpc, int lcut);
+/* :143-186 */
+int mhx_torpol_to_curl_spat_IC(mhx_plan *p, double r, double r_ICB,
+                               const double *dBlm, const double *ddBlm,
+                               const double *Jlm, const double *dJlm,
+                               double *cbr, double *cbt, double *cbp);
+/* :188-229 */
+int mhx_torpol_to_spat_IC(mhx_plan *p, double r, double r_ICB,
+                          const double *Wlm, const double *dWlm,
+                          const double *Zlm, double *Br, double *Bt, double *Bp);
+/* :231-272 */
+int mhx_torpol_to_dphspat(mhx_plan *p, const double *dWlm, const double *Zlm,
+                          double *dvtdp, double *dvpdp, int lcut);
+/* :274-300 */
+int mhx_pol_to_curlr_spat(mhx_plan *p, const double *Qlm, double *cvrc,
+                          int lcut);
+/* :302-336 */
+int mhx_torpol_to_curl_spat(mhx_plan *p, double or2, const double *Blm,
+                            const double *ddBlm, const double *Jlm,
+                            const double *dJlm, double *cvrc, double *cvtc,
+                            double *cvpc, int lcut);
+/* :338-346; f is intent(inout) in the reference and is left unchanged here */
+int mhx_scal_to_SH(mhx_plan *p, const double *f, double *fLM, int lcut);
+/* :348-364 */
+int mhx_spat_to_qst(mhx_plan *p, const double *f, const double *g,
+                    const double *h, double *qLM, double *sLM, double *tLM,
+                    int lcut);
+/* :366-379 */
+int mhx_spat_to_sphertor(mhx_plan *p, const double *f, const double *g,
+                         double *fLM, double *gLM, int lcut);
+/* :381-391; fl_ax has l_max+1 complex entries (m = 0), f has n_theta */
+int mhx_axi_to_spat(mhx_plan *p, const double *fl_ax, double *f);
+/* :393-405 */
+int mhx_toraxi_to_spat(mhx_plan *p, const double *fl_ax, double *ft,
+                       double *fp, int lcut);
+
+/* ---- batched over radial levels (device pointers, one launch per stage) ----
+ * Level i of an input/output coefficient array starts at ptr + 2*i*ld_lm
+ * doubles; level i of a grid at ptr + i*ld_grid doubles.  lcut[nlev] is the
+ * per-level cut-off (l_R(nR) of src/radial.f90:290-307), host memory.
+ * These replace nlev calls of the per-level routine of the same name. */
+int mhx_torpol_to_spat_batch(mhx_plan *p, int nlev, const double *W,
+                             const double *dW, const double *Z, size_t ld_lm,
+                             double *vr, double *vt, double *vp, size_t ld_grid,
+                             const int *lcut, void *stream);
+int mhx_spat_to_qst_batch(mhx_plan *p, int nlev, const double *f,
+                          const double *g, const double *h, size_t ld_grid,
+                          double *q, double *s, double *t, size_t ld_lm,
+                          const int *lcut, void *stream);
+
+/* ---- the nonlinear-term path of one radial loop, all levels at once ----
+ * Replaces transform_to_grid_space / get_nl / transform_to_lm_space of
+ * src/rIter.f90:466-712 for bulk levels of a Boussinesq MHD run with
+ * curl-form advection (l_adv_curl) and the derivative fields:
+ *   inverse: scal_to_spat(s), scal_to_grad_spat(s), torpol_to_spat(w,dw,z),
+ *            torpol_to_curl_spat(or2,w,ddw,z,dz), torpol_to_spat(dw,ddw,dz),
+ *            torpol_to_spat(b,db,aj), torpol_to_curl_spat(or2,b,ddb,aj,dj);
+ *   grid:    Adv = -curl(u) x u terms + Lorentz force (src/get_nl.f90:242-273,
+ *            merged as src/rIter.f90:650-656), VS = u s (:320-325),
+ *            VxB (:376-388), and the scalar u_h . grad_h s
+ *            = or2/sin^2(theta) (vt ds/dtheta_c + vp ds/dphi_c);
+ *   forward: spat_to_qst(Adv), spat_to_qst(VS), spat_to_qst(VxB),
+ *            scal_to_SH(u_h . grad_h s).
+ * in[11]  = w dw ddw z dz s b db ddb aj dj  (coefficients, ld_lm)
+ * out[10] = AdvrLM AdvtLM AdvpLM VSrLM VStLM VSpLM VxBrLM VxBtLM VxBpLM
+ *           uGsLM (coefficients, ld_lm)
+ * radial: per-level or2, or4, orho1 (nlev each) and LFfac; lcut[nlev].
+ * All arrays device memory except lcut/radial (host). */
+int mhx_nl_step(mhx_plan *p, int nlev, const double *const *in, size_t ld_lm,
+                double *const *out, const double *or2, const double *or4,
+                const double *orho1, double LFfac, const int *lcut,
+                void *stream);
+/* the same step, also writing the 18 inverse grid components
+ * (vr vt vp cvr cvt cvp dvr dvt dvp br bt bp cbr cbt cbp s dsdt dsdp), each
+ * grid component at grids + ((lev*18 + comp) * n_phi * n_theta) */
+int mhx_nl_step_grids(mhx_plan *p, int nlev, const double *const *in,
+                      size_t ld_lm, double *const *out, const double *or2,
+                      const double *or4, const double *orho1, double LFfac,
+                      const int *lcut, double *grids, void *stream);
+
+/* average durations (ms) of the stages of the last mhx_nl_step on this plan,
+ * measured with HIP events on the launch stream: [0] synthesis Legendre,
+ * [1] inverse FFT, [2] grid products, [3] forward FFT, [4] analysis Legendre.
+ * Requires mhx_set_timing(p, 1) before the step. */
+int mhx_set_timing(mhx_plan *p, int on);
+int mhx_stage_times(const mhx_plan *p, double *ms5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

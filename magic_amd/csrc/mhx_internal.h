@@ -1,0 +1,149 @@
+// Internal declarations shared by the mhx HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mhx.h"
+
+namespace mhx {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+#define MHX_HIP_CHECK(expr)                                                   \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess)                                                 \
+            return ::mhx::fail(MHX_E_HIP, std::string(#expr) + ": " +         \
+                                               hipGetErrorString(e_));        \
+    } while (0)
+
+// ----------------------------------------------------- Legendre job model
+// A transform "job" runs one kind of Legendre contraction for a group of
+// radial levels.  B columns are complex coefficient vectors built on the fly
+// from up to two input arrays:
+//     col = i^ipow * m^mpow * (g(l, lev) * x0 + hcoef * x1)
+// with g = 1, l(l+1) or or2[lev] l(l+1).
+enum GKind { G_ONE = 0, G_LL = 1, G_OR2LL = 2 };
+
+struct ColOp {
+    int src0;     // index into the job's source list
+    int src1;     // -1 if unused
+    int gkind;
+    int ipow;     // power of i (0..3)
+    int mpow;     // power of m (0..2)
+    double hcoef; // coefficient of x1
+    double scale;
+};
+
+// synthesis kinds (see mhx_legendre.hip for the product tables)
+enum SynKind { SYN_SCAL = 0, SYN_GRAD = 1, SYN_SG = 2, SYN_QST = 3, SYN_SPHTOR = 4 };
+// analysis kinds
+enum AnaKind { ANA_SH = 0, ANA_SPHTOR = 1, ANA_QST = 2 };
+
+constexpr int MAX_SRC = 4;
+constexpr int MAX_COL = 5;
+constexpr int MAX_OUT = 3;
+constexpr int LEV_GROUP = 8;   // radial levels per MFMA column tile (re/im -> 16)
+
+struct SynJob {
+    int kind;
+    int nlev;
+    // sources: coefficient arrays (complex, interleaved), level stride ld (complex)
+    const double *src[MAX_SRC];
+    size_t src_ld;
+    ColOp col[MAX_COL];
+    // outputs: Fourier arrays F[lev][comp][m][theta] complex; comp index and
+    // per-level stride (complex elements)
+    double *fout;
+    size_t fout_ld;      // complex elements per level
+    int out_comp[MAX_OUT];
+    int row_scale[MAX_OUT]; // 1: multiply by 1/sin^2(theta) (dphspat)
+    const int *lcut;     // device, per level
+    const double *or2;   // device, per level (may be null)
+};
+
+struct AnaJob {
+    int kind;
+    int nlev;
+    // inputs: Fourier arrays F[lev][comp][m][theta]
+    const double *fin;
+    size_t fin_ld;       // complex elements per level
+    int in_comp[3];      // f (SH/QST), vt, vp
+    // outputs: coefficient arrays
+    double *out[MAX_OUT];
+    size_t out_ld;       // complex elements per level
+    const int *lcut;     // device
+};
+
+// ------------------------------------------------------------------ plan
+struct Plan {
+    int l_max, m_max, minc, n_theta, n_phi, nh, n_m, lm_max, precision;
+    int nt16;   // theta tiles of 16 (northern half)
+    int nq4;    // theta quads (northern half)
+    int M;      // complex FFT length n_phi/2
+    std::vector<int> lstart;        // n_m
+    std::vector<int> fac;           // FFT radices of M
+    std::vector<size_t> syn_off;    // [mc*2+p] offsets (doubles) into syn tables
+    std::vector<size_t> ana_off;    // [mc*2+p]
+    size_t syn_size = 0, ana_size = 0;
+    std::vector<double> theta_ord, gauss;  // nh each (host)
+
+    // device
+    int *d_lstart = nullptr;
+    double *d_syn_P = nullptr, *d_syn_dP = nullptr;   // FP64 MFMA-tiled
+    double *d_ana_wP = nullptr, *d_ana_wdP = nullptr;
+    size_t *d_syn_off = nullptr, *d_ana_off = nullptr;
+    double *d_osin2 = nullptr;      // n_theta (scrambled)
+    double *d_theta = nullptr, *d_gauss = nullptr; // nh
+    double *d_tw = nullptr;         // M complex: exp(-2 pi i k/M)
+    double *d_twr = nullptr;        // (M+1) complex: exp(-2 pi i k/n_phi)
+
+    // workspaces
+    int ws_levels = 0;
+    double *ws_F = nullptr;     size_t ws_F_bytes = 0;    // Fourier, inverse side
+    double *ws_G = nullptr;     size_t ws_G_bytes = 0;    // grids
+    double *ws_F2 = nullptr;    size_t ws_F2_bytes = 0;   // Fourier, forward side
+    double *ws_G2 = nullptr;    size_t ws_G2_bytes = 0;   // product grids
+    double *ws_C = nullptr;     size_t ws_C_bytes = 0;    // staged coefficients
+    int *d_lm2l = nullptr;      // lm_max
+    int *d_lcut = nullptr;      int d_lcut_n = 0;
+    double *d_rad = nullptr;    int d_rad_n = 0;          // or2, or4, orho1
+    std::vector<double> h_rad;  // host copy of the per-level radial factors
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_user = nullptr;  // orders the plan stream after a caller's stream
+
+    bool timing = false;
+    double stage_ms[5] = {0, 0, 0, 0, 0};
+    hipEvent_t ev[6] = {};
+};
+
+// ------------------------------------------------------------- launchers
+int build_tables(Plan *p);
+int launch_syn(const Plan *p, const SynJob &job, hipStream_t s);
+int launch_ana(const Plan *p, const AnaJob &job, hipStream_t s);
+// inverse FFT: F[lev][comp][m][theta] (n_m modes) -> grid[lev][comp][phi][theta]
+int launch_ifft(const Plan *p, const double *F, size_t F_ld_lev, int ncomp,
+                int nlev, double *grid, size_t grid_ld_lev, hipStream_t s);
+// forward FFT: grid -> F, scale 1/n_phi, times 1/sin^2 for rows flagged in
+// osin2_mask (bit per comp), modes above m_lim zeroed
+int launch_fft(const Plan *p, const double *grid, size_t grid_ld_lev, int ncomp,
+               int nlev, unsigned osin2_mask, double *F, size_t F_ld_lev,
+               hipStream_t s);
+int launch_nl_products(const Plan *p, int nlev, const double *gin, double *gout,
+                       const double *rad, double LFfac, hipStream_t s);
+int launch_axi(const Plan *p, const double *fl, double *f, int lcut, int tor,
+               double *f2, hipStream_t s);
+
+}  // namespace mhx
+
+struct mhx_plan {
+    mhx::Plan p;
+};

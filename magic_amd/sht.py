@@ -1,0 +1,314 @@
+"""Host-side mirror of the reference's transform API (module sht,
+src/sht_native.f90:16-20) over the HIP library libmhx.so.
+
+``Plan`` exposes one method per public routine of module sht with the same
+name, argument meaning and order; arrays may be numpy arrays (host) or torch
+tensors on the GPU (device).  Coefficients are complex128 vectors of length
+lm_max in m-major order, grids float64 arrays of shape (n_phi, n_theta) in C
+order (theta fastest, north/south interleaved), exactly as the reference.
+
+There is no CPU fallback: if libmhx.so cannot be loaded, or no GPU is
+present, every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmhx.so")
+
+PREC_FP64 = 0
+PREC_SPLIT16 = 1
+
+_lib = None
+
+# (name, argument kinds) for every symbol of include/mhx.h; kinds:
+#   P plan, p pointer, i int, d double, z size_t, pp pointer-to-pointer
+_SIGS = {
+    "mhx_plan_create": "iiiiiiip",
+    "mhx_plan_destroy": "P",
+    "mhx_plan_info": "Ppppp",
+    "mhx_plan_tables": "Ppppp",
+    "mhx_scal_to_spat": "Pppi",
+    "mhx_scal_to_grad_spat": "Ppppi",
+    "mhx_pol_to_grad_spat": "Ppppi",
+    "mhx_torpol_to_spat": "Pppppppi",
+    "mhx_sphtor_to_spat": "Pppppi",
+    "mhx_torpol_to_curl_spat_IC": "Pddppppppp",
+    "mhx_torpol_to_spat_IC": "Pddpppppp",
+    "mhx_torpol_to_dphspat": "Pppppi",
+    "mhx_pol_to_curlr_spat": "Pppi",
+    "mhx_torpol_to_curl_spat": "Pdpppppppi",
+    "mhx_scal_to_SH": "Pppi",
+    "mhx_spat_to_qst": "Pppppppi",
+    "mhx_spat_to_sphertor": "Pppppi",
+    "mhx_axi_to_spat": "Ppp",
+    "mhx_toraxi_to_spat": "Ppppi",
+    "mhx_torpol_to_spat_batch": "Pipppzpppzpp",
+    "mhx_spat_to_qst_batch": "Pipppzpppzpp",
+    "mhx_nl_step": "Pipzppppdpp",
+    "mhx_nl_step_grids": "Pipzppppdppp",
+    "mhx_set_timing": "Pi",
+    "mhx_stage_times": "Pp",
+}
+_CT = {"P": ctypes.c_void_p, "p": ctypes.c_void_p, "i": ctypes.c_int, "d": ctypes.c_double,
+       "z": ctypes.c_size_t}
+
+
+class MhxError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads libmhx.so; raises if it is missing (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MhxError("libmhx.so is not built (python -m magic_amd.build)")
+        # torch ships its own HIP runtime (SONAME libamdhip64.so.7); loading
+        # torch first makes libmhx bind to that same runtime instead of a
+        # second copy, so tensors and kernels share one device context
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = ctypes.CDLL(LIB_PATH)
+        for name, kinds in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = ctypes.c_int
+            f.argtypes = [_CT.get(k, ctypes.c_void_p) for k in kinds]
+        L.mhx_last_error.restype = ctypes.c_char_p
+        L.mhx_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return ["mhx_last_error"] + list(_SIGS)
+
+
+def _check(rc):
+    if rc != 0:
+        raise MhxError("mhx error %d: %s" % (rc, lib().mhx_last_error().decode()))
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if _is_torch(a):
+        if not a.is_contiguous():
+            raise ValueError("torch tensors must be contiguous")
+        return ctypes.c_void_p(a.data_ptr())
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("arrays must be C-contiguous")
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Plan:
+    """A transform plan: initialize_sht for one truncation and grid
+    (src/sht_native.f90:24-32).  n_theta / n_phi default to the reference's
+    sizing for nalias = 20 (src/truncation.f90:66-71)."""
+
+    def __init__(self, l_max, n_theta=None, n_phi=None, m_max=None, minc=1,
+                 precision=PREC_FP64, max_levels=1):
+        if n_theta is None or n_phi is None:
+            from .workload import grid_size
+            n_theta, n_phi = grid_size(l_max, 20, minc)
+        self._L = lib()
+        h = ctypes.c_void_p()
+        _check(self._L.mhx_plan_create(l_max, 0 if m_max is None else m_max, minc, n_theta,
+                                       n_phi, precision, max_levels, ctypes.byref(h)))
+        self.h = h
+        lm, nm = ctypes.c_int(), ctypes.c_int()
+        _check(self._L.mhx_plan_info(h, ctypes.byref(lm), ctypes.byref(nm), None, None))
+        self.l_max, self.minc = l_max, minc
+        self.n_theta, self.n_phi = n_theta, n_phi
+        self.lm_max, self.n_m_max = lm.value, nm.value
+        self.m_max = (self.n_m_max - 1) * minc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.mhx_plan_destroy(self.h)
+            self.h = None
+
+    finalize_sht = close
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- array helpers (host numpy)
+    def grid(self):
+        return np.zeros((self.n_phi, self.n_theta))
+
+    def coef(self):
+        return np.zeros(self.lm_max, dtype=np.complex128)
+
+    def _c(self, a, n=None):
+        if _is_torch(a):
+            return a
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        if a.shape != ((self.lm_max,) if n is None else (n,)):
+            raise ValueError("coefficient array has shape %r" % (a.shape,))
+        return a
+
+    def _g(self, a):
+        if _is_torch(a):
+            return a
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != (self.n_phi, self.n_theta):
+            raise ValueError("grid has shape %r" % (a.shape,))
+        return a
+
+    def tables(self):
+        nh = self.n_theta // 2
+        th, gw = np.zeros(nh), np.zeros(nh)
+        P = np.zeros((nh, self.lm_max))
+        D = np.zeros((nh, self.lm_max))
+        _check(self._L.mhx_plan_tables(self.h, _ptr(th), _ptr(gw), _ptr(P), _ptr(D)))
+        return dict(theta=th, gauss=gw, plm=P, dplm=D)
+
+    # -- module sht (src/sht_native.f90)
+    def scal_to_spat(self, Slm, lcut, fieldc=None):
+        f = self.grid() if fieldc is None else fieldc
+        _check(self._L.mhx_scal_to_spat(self.h, _ptr(self._c(Slm)), _ptr(f), lcut))
+        return f
+
+    def scal_to_grad_spat(self, Slm, lcut):
+        a, b = self.grid(), self.grid()
+        _check(self._L.mhx_scal_to_grad_spat(self.h, _ptr(self._c(Slm)), _ptr(a), _ptr(b), lcut))
+        return a, b
+
+    def pol_to_grad_spat(self, Slm, lcut):
+        a, b = self.grid(), self.grid()
+        _check(self._L.mhx_pol_to_grad_spat(self.h, _ptr(self._c(Slm)), _ptr(a), _ptr(b), lcut))
+        return a, b
+
+    def torpol_to_spat(self, Wlm, dWlm, Zlm, lcut):
+        a, b, c = self.grid(), self.grid(), self.grid()
+        _check(self._L.mhx_torpol_to_spat(self.h, _ptr(self._c(Wlm)), _ptr(self._c(dWlm)),
+                                          _ptr(self._c(Zlm)), _ptr(a), _ptr(b), _ptr(c), lcut))
+        return a, b, c
+
+    def sphtor_to_spat(self, dWlm, Zlm, lcut):
+        a, b = self.grid(), self.grid()
+        _check(self._L.mhx_sphtor_to_spat(self.h, _ptr(self._c(dWlm)), _ptr(self._c(Zlm)),
+                                          _ptr(a), _ptr(b), lcut))
+        return a, b
+
+    def torpol_to_curl_spat_IC(self, r, r_ICB, dBlm, ddBlm, Jlm, dJlm):
+        a, b, c = self.grid(), self.grid(), self.grid()
+        _check(self._L.mhx_torpol_to_curl_spat_IC(
+            self.h, float(r), float(r_ICB), _ptr(self._c(dBlm)), _ptr(self._c(ddBlm)),
+            _ptr(self._c(Jlm)), _ptr(self._c(dJlm)), _ptr(a), _ptr(b), _ptr(c)))
+        return a, b, c
+
+    def torpol_to_spat_IC(self, r, r_ICB, Wlm, dWlm, Zlm):
+        a, b, c = self.grid(), self.grid(), self.grid()
+        _check(self._L.mhx_torpol_to_spat_IC(
+            self.h, float(r), float(r_ICB), _ptr(self._c(Wlm)), _ptr(self._c(dWlm)),
+            _ptr(self._c(Zlm)), _ptr(a), _ptr(b), _ptr(c)))
+        return a, b, c
+
+    def torpol_to_dphspat(self, dWlm, Zlm, lcut):
+        a, b = self.grid(), self.grid()
+        _check(self._L.mhx_torpol_to_dphspat(self.h, _ptr(self._c(dWlm)), _ptr(self._c(Zlm)),
+                                             _ptr(a), _ptr(b), lcut))
+        return a, b
+
+    def pol_to_curlr_spat(self, Qlm, lcut):
+        a = self.grid()
+        _check(self._L.mhx_pol_to_curlr_spat(self.h, _ptr(self._c(Qlm)), _ptr(a), lcut))
+        return a
+
+    def torpol_to_curl_spat(self, or2, Blm, ddBlm, Jlm, dJlm, lcut):
+        a, b, c = self.grid(), self.grid(), self.grid()
+        _check(self._L.mhx_torpol_to_curl_spat(
+            self.h, float(or2), _ptr(self._c(Blm)), _ptr(self._c(ddBlm)), _ptr(self._c(Jlm)),
+            _ptr(self._c(dJlm)), _ptr(a), _ptr(b), _ptr(c), lcut))
+        return a, b, c
+
+    def scal_to_SH(self, f, lcut):
+        out = self.coef()
+        _check(self._L.mhx_scal_to_SH(self.h, _ptr(self._g(f)), _ptr(out), lcut))
+        return out
+
+    def spat_to_qst(self, f, g, h, lcut):
+        a, b, c = self.coef(), self.coef(), self.coef()
+        _check(self._L.mhx_spat_to_qst(self.h, _ptr(self._g(f)), _ptr(self._g(g)),
+                                       _ptr(self._g(h)), _ptr(a), _ptr(b), _ptr(c), lcut))
+        return a, b, c
+
+    def spat_to_sphertor(self, f, g, lcut):
+        a, b = self.coef(), self.coef()
+        _check(self._L.mhx_spat_to_sphertor(self.h, _ptr(self._g(f)), _ptr(self._g(g)),
+                                            _ptr(a), _ptr(b), lcut))
+        return a, b
+
+    def axi_to_spat(self, fl_ax):
+        f = np.zeros(self.n_theta)
+        _check(self._L.mhx_axi_to_spat(self.h, _ptr(self._c(fl_ax, self.l_max + 1)), _ptr(f)))
+        return f
+
+    def toraxi_to_spat(self, fl_ax, lcut):
+        a, b = np.zeros(self.n_theta), np.zeros(self.n_theta)
+        _check(self._L.mhx_toraxi_to_spat(self.h, _ptr(self._c(fl_ax, self.l_max + 1)),
+                                          _ptr(a), _ptr(b), lcut))
+        return a, b
+
+    # -- batched over radial levels (torch tensors on the GPU)
+    def torpol_to_spat_batch(self, W, dW, Z, lcut, vr, vt, vp):
+        nlev = W.shape[0]
+        lc = (ctypes.c_int * nlev)(*[int(x) for x in lcut])
+        _check(self._L.mhx_torpol_to_spat_batch(
+            self.h, nlev, _ptr(W), _ptr(dW), _ptr(Z), self.lm_max, _ptr(vr), _ptr(vt), _ptr(vp),
+            self.n_phi * self.n_theta, lc, None))
+
+    def spat_to_qst_batch(self, f, g, h, lcut, q, s, t):
+        nlev = f.shape[0]
+        lc = (ctypes.c_int * nlev)(*[int(x) for x in lcut])
+        _check(self._L.mhx_spat_to_qst_batch(
+            self.h, nlev, _ptr(f), _ptr(g), _ptr(h), self.n_phi * self.n_theta, _ptr(q), _ptr(s),
+            _ptr(t), self.lm_max, lc, None))
+
+    def set_timing(self, on=True):
+        _check(self._L.mhx_set_timing(self.h, 1 if on else 0))
+
+    def stage_times(self):
+        ms = (ctypes.c_double * 5)()
+        _check(self._L.mhx_stage_times(self.h, ms))
+        return list(ms)
+
+
+NL_INPUTS = ("w", "dw", "ddw", "z", "dz", "s", "b", "db", "ddb", "aj", "dj")
+NL_OUTPUTS = ("AdvrLM", "AdvtLM", "AdvpLM", "VSrLM", "VStLM", "VSpLM", "VxBrLM", "VxBtLM",
+              "VxBpLM", "uGsLM")
+NL_GRIDS = ("vr", "vt", "vp", "cvr", "cvt", "cvp", "dvr", "dvt", "dvp", "br", "bt", "bp",
+            "cbr", "cbt", "cbp", "s", "dsdt", "dsdp")
+
+
+def nl_step(plan, inputs, outputs, or2, or4, orho1, LFfac, lcut, grids=None):
+    """The nonlinear-term path of one radial loop for all levels at once
+    (src/rIter.f90:466-712, bulk levels).  ``inputs``: 11 device tensors
+    (nlev, lm_max) complex128 in NL_INPUTS order; ``outputs``: 10 such tensors
+    (NL_OUTPUTS order), written in place.  ``grids`` (optional): a float64
+    device tensor (nlev, 18, n_phi, n_theta) that receives the inverse grids."""
+    L = plan._L
+    nlev = inputs[0].shape[0]
+    ins = (ctypes.c_void_p * 11)(*[t.data_ptr() for t in inputs])
+    outs = (ctypes.c_void_p * 10)(*[t.data_ptr() for t in outputs])
+    d = lambda v: (ctypes.c_double * nlev)(*[float(x) for x in v])  # noqa: E731
+    lc = (ctypes.c_int * nlev)(*[int(x) for x in lcut])
+    if grids is None:
+        _check(L.mhx_nl_step(plan.h, nlev, ins, plan.lm_max, outs, d(or2), d(or4), d(orho1),
+                             float(LFfac), lc, None))
+    else:
+        _check(L.mhx_nl_step_grids(plan.h, nlev, ins, plan.lm_max, outs, d(or2), d(or4),
+                                   d(orho1), float(LFfac), lc, _ptr(grids), None))

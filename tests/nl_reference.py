@@ -1,0 +1,57 @@
+"""The nonlinear-term step composed from the FP64 oracle, per level, in the
+reference's call order (src/rIter.f90:483-707, bulk level, l_adv_curl, with
+the derivative fields): the check for mhx_nl_step.  Test infrastructure."""
+import numpy as np
+
+from magic_amd import workload
+
+IN_SPECTRA = "l^-5/3"
+
+
+def make_inputs(l_max, nlev, config=2, kind=IN_SPECTRA, minc=1):
+    """11 fields (w dw ddw z dz s b db ddb aj dj) x nlev, complex128."""
+    arrs = []
+    for f in range(11):
+        arrs.append(np.stack([workload.coefficients(l_max, config, f, lev, kind,
+                                                    potential=(f != 5), minc=minc)
+                              for lev in range(nlev)]))
+    return arrs
+
+
+def radial(nlev):
+    r = np.linspace(0.55, 1.45, nlev)
+    return 1.0 / r ** 2, 1.0 / r ** 4, 1.0 / (1.0 + 0.1 * r), 0.8
+
+
+def oracle_step(o, inputs, or2, or4, orho1, LFfac, lcut):
+    """Returns (grids[nlev,18,nphi,nt], outs[10][nlev,lm_max]) from the oracle."""
+    nlev = inputs[0].shape[0]
+    w, dw, ddw, z, dz, s, b, db, ddb, aj, dj = inputs
+    G = np.zeros((nlev, 18, o.n_phi, o.n_theta))
+    outs = [np.zeros((nlev, o.lm_max), dtype=np.complex128) for _ in range(10)]
+    for i in range(nlev):
+        lc = int(lcut[i])
+        g = []
+        g += o.torpol_to_spat(w[i], dw[i], z[i], lc)
+        g += o.torpol_to_curl_spat(or2[i], w[i], ddw[i], z[i], dz[i], lc)
+        g += o.torpol_to_spat(dw[i], ddw[i], dz[i], lc)
+        g += o.torpol_to_spat(b[i], db[i], aj[i], lc)
+        g += o.torpol_to_curl_spat(or2[i], b[i], ddb[i], aj[i], dj[i], lc)
+        g.append(o.scal_to_spat(s[i], lc))
+        g += o.scal_to_grad_spat(s[i], lc)
+        G[i] = np.stack(g)
+        vr, vt, vp, cvr, cvt, cvp = g[0:6]
+        br, bt, bp, cbr, cbt, cbp = g[9:15]
+        sc, dsdt, dsdp = g[15:18]
+        nl = o.get_nl(or2[i], or4[i], orho1[i], LFfac,
+                      [vr, vt, vp, cvr, cvt, cvp, br, bt, bp, cbr, cbt, cbp, sc])
+        adv = [nl[k] + nl[3 + k] for k in range(3)]  # src/rIter.f90:650-656
+        vxb = nl[6:9]
+        vs = nl[9:12]
+        osin2 = o.tables()["osin2"][None, :]
+        ugs = or2[i] * osin2 * (vt * dsdt + vp * dsdp)
+        res = list(o.spat_to_qst(*adv, lc)) + list(o.spat_to_qst(*vs, lc)) + \
+            list(o.spat_to_qst(*vxb, lc)) + [o.scal_to_SH(ugs, lc)]
+        for k in range(10):
+            outs[k][i] = res[k]
+    return G, outs
