@@ -173,6 +173,11 @@ int mhx_nl_step_grids(mhx_plan *p, int nlev, const double *const *in,
  * Requires mhx_set_timing(p, 1) before the step. */
 int mhx_set_timing(mhx_plan *p, int on);
 int mhx_stage_times(const mhx_plan *p, double *ms5);
+/* durations (ms) of the individual launches of the last timed mhx_nl_step,
+ * in launch order: synthesis (s + gradient), synthesis QST x5 (u, curl u,
+ * du/dr, B, curl B), inverse FFT, grid products, forward FFT, analysis QST
+ * x3 (Adv, VS, VxB), analysis SH (u.grad s).  Returns the number written. */
+int mhx_launch_times(const mhx_plan *p, double *ms, int n);
 
 #ifdef __cplusplus
 }

@@ -121,8 +121,11 @@ struct Plan {
     hipEvent_t ev_user = nullptr;  // orders the plan stream after a caller's stream
 
     bool timing = false;
+    static constexpr int NEV = 16;
+    int n_launch = 0;                 // launches timed in the last step
+    double launch_ms[NEV] = {};       // their durations
     double stage_ms[5] = {0, 0, 0, 0, 0};
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[NEV] = {};
 };
 
 // ------------------------------------------------------------- launchers

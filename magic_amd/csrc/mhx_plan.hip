@@ -424,7 +424,7 @@ int mhx_plan_create(int l_max, int m_max, int minc, int n_theta, int n_phi, int 
         mhx_plan_destroy(h);
         return rc;
     }
-    for (int i = 0; i < 6; ++i) (void)hipEventCreate(&p->ev[i]);
+    for (int i = 0; i < Plan::NEV; ++i) (void)hipEventCreate(&p->ev[i]);
     (void)hipEventCreateWithFlags(&p->ev_user, hipEventDisableTiming);
     *plan = h;
     return MHX_OK;
@@ -455,7 +455,7 @@ int mhx_plan_destroy(mhx_plan *h)
     free_dev(p->ws_C);
     free_dev(p->d_lcut);
     free_dev(p->d_rad);
-    for (int i = 0; i < 6; ++i)
+    for (int i = 0; i < Plan::NEV; ++i)
         if (p->ev[i]) (void)hipEventDestroy(p->ev[i]);
     if (p->ev_user) (void)hipEventDestroy(p->ev_user);
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -809,7 +809,12 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
     MHX_HIP_CHECK(hipMemcpyAsync(P->d_rad, rad.data(), rad.size() * sizeof(double),
                                  hipMemcpyHostToDevice, P->stream));
     hipStream_t s = P->stream;
-    if (P->timing) MHX_HIP_CHECK(hipEventRecord(P->ev[0], s));
+    int nev = 0;
+    auto mark = [&]() -> int {
+        if (P->timing && nev < Plan::NEV) MHX_HIP_CHECK(hipEventRecord(P->ev[nev++], s));
+        return MHX_OK;
+    };
+    if ((rc = mark())) return rc;
 
     // inputs: 0 w, 1 dw, 2 ddw, 3 z, 4 dz, 5 s, 6 b, 7 db, 8 ddb, 9 aj, 10 dj
     enum { W = 0, DW, DDW, Z, DZ, S, B, DB, DDB, AJ, DJ };
@@ -820,7 +825,7 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
         j.col[0] = op(0);
         j.col[1] = op(0, G_ONE, 1, 1);
         j.out_comp[0] = 15; j.out_comp[1] = 16; j.out_comp[2] = 17;
-        if ((rc = launch_syn(P, j, s))) return rc;
+        if ((rc = launch_syn(P, j, s)) || (rc = mark())) return rc;
     }
     struct QSpec { int q, sidx, t0, t1; bool curl; int comp0; };
     // torpol_to_spat(a, b, c): Q = l(l+1) a, S = b, T = c
@@ -841,21 +846,20 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
         ColOp T = q.curl ? op(2, G_OR2LL, 0, 0, 1.0, 3, -1.0) : op(2);
         qst_cols(j, op(0, G_LL), op(1), T);
         j.out_comp[0] = q.comp0; j.out_comp[1] = q.comp0 + 1; j.out_comp[2] = q.comp0 + 2;
-        if ((rc = launch_syn(P, j, s))) return rc;
+        if ((rc = launch_syn(P, j, s)) || (rc = mark())) return rc;
     }
-    if (P->timing) MHX_HIP_CHECK(hipEventRecord(P->ev[1], s));
     size_t Fld = (size_t)18 * P->n_m * P->n_theta;
     size_t gsz = (size_t)P->n_phi * P->n_theta;
     double *G = grids ? grids : P->ws_G;
-    if ((rc = launch_ifft(P, P->ws_F, Fld, 18, nlev, G, 18 * gsz, s))) return rc;
-    if (P->timing) MHX_HIP_CHECK(hipEventRecord(P->ev[2], s));
-    if ((rc = launch_nl_products(P, nlev, G, P->ws_G2, P->d_rad, LFfac, s))) return rc;
-    if (P->timing) MHX_HIP_CHECK(hipEventRecord(P->ev[3], s));
+    if ((rc = launch_ifft(P, P->ws_F, Fld, 18, nlev, G, 18 * gsz, s)) || (rc = mark())) return rc;
+    if ((rc = launch_nl_products(P, nlev, G, P->ws_G2, P->d_rad, LFfac, s)) || (rc = mark()))
+        return rc;
     // 1/sin^2 for the horizontal components of the three vector products
     unsigned mask = (1u << 1) | (1u << 2) | (1u << 4) | (1u << 5) | (1u << 7) | (1u << 8);
     size_t F2ld = (size_t)10 * P->n_m * P->n_theta;
-    if ((rc = launch_fft(P, P->ws_G2, 10 * gsz, 10, nlev, mask, P->ws_F2, F2ld, s))) return rc;
-    if (P->timing) MHX_HIP_CHECK(hipEventRecord(P->ev[4], s));
+    if ((rc = launch_fft(P, P->ws_G2, 10 * gsz, 10, nlev, mask, P->ws_F2, F2ld, s)) ||
+        (rc = mark()))
+        return rc;
     for (int v = 0; v < 3; ++v) {
         AnaJob j = ana_job(P, ANA_QST, nlev, ld_lm);
         j.in_comp[0] = 3 * v;
@@ -864,21 +868,28 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
         j.out[0] = out[3 * v];
         j.out[1] = out[3 * v + 1];
         j.out[2] = out[3 * v + 2];
-        if ((rc = launch_ana(P, j, s))) return rc;
+        if ((rc = launch_ana(P, j, s)) || (rc = mark())) return rc;
     }
     {
         AnaJob j = ana_job(P, ANA_SH, nlev, ld_lm);
         j.in_comp[0] = 9;
         j.out[0] = out[9];
-        if ((rc = launch_ana(P, j, s))) return rc;
+        if ((rc = launch_ana(P, j, s)) || (rc = mark())) return rc;
     }
     if (P->timing) {
-        MHX_HIP_CHECK(hipEventRecord(P->ev[5], s));
-        MHX_HIP_CHECK(hipEventSynchronize(P->ev[5]));
-        for (int i = 0; i < 5; ++i) {
+        // launches: 6 synthesis, inverse FFT, products, forward FFT, 4 analysis
+        MHX_HIP_CHECK(hipEventSynchronize(P->ev[nev - 1]));
+        P->n_launch = nev - 1;
+        for (int i = 0; i + 1 < nev; ++i) {
             float ms = 0.f;
             (void)hipEventElapsedTime(&ms, P->ev[i], P->ev[i + 1]);
-            P->stage_ms[i] = ms;
+            P->launch_ms[i] = ms;
+        }
+        const int first[6] = {0, 6, 7, 8, 9, 13};
+        for (int st = 0; st < 5; ++st) {
+            double t = 0.0;
+            for (int i = first[st]; i < first[st + 1] && i < P->n_launch; ++i) t += P->launch_ms[i];
+            P->stage_ms[st] = t;
         }
     }
     return MHX_OK;
@@ -918,6 +929,14 @@ int mhx_stage_times(const mhx_plan *h, double *ms5)
     if (!h || !ms5) return fail(MHX_E_INVALID, "null argument");
     for (int i = 0; i < 5; ++i) ms5[i] = h->p.stage_ms[i];
     return MHX_OK;
+}
+
+int mhx_launch_times(const mhx_plan *h, double *ms, int n)
+{
+    if (!h || !ms) return fail(MHX_E_INVALID, "null argument");
+    int k = h->p.n_launch < n ? h->p.n_launch : n;
+    for (int i = 0; i < k; ++i) ms[i] = h->p.launch_ms[i];
+    return k;
 }
 
 }  // extern "C"

@@ -51,6 +51,7 @@ _SIGS = {
     "mhx_nl_step_grids": "Pipzppppdppp",
     "mhx_set_timing": "Pi",
     "mhx_stage_times": "Pp",
+    "mhx_launch_times": "Ppi",
 }
 _CT = {"P": ctypes.c_void_p, "p": ctypes.c_void_p, "i": ctypes.c_int, "d": ctypes.c_double,
        "z": ctypes.c_size_t}
@@ -285,6 +286,13 @@ class Plan:
         ms = (ctypes.c_double * 5)()
         _check(self._L.mhx_stage_times(self.h, ms))
         return list(ms)
+
+    def launch_times(self):
+        ms = (ctypes.c_double * 16)()
+        n = self._L.mhx_launch_times(self.h, ms, 16)
+        if n < 0:
+            _check(n)
+        return list(ms)[:n]
 
 
 NL_INPUTS = ("w", "dw", "ddw", "z", "dz", "s", "b", "db", "ddb", "aj", "dj")

@@ -8,7 +8,7 @@
 ! code calls it.
 !
 ! Usage:  ref_harness MODE l_max minc nalias infile outfile [nlev nrep]
-!   MODE = tables | api | nl | bench
+!   MODE = tables | api | nl | nlstep | bench
 !
 ! File formats are little-endian streams: int32 header words followed by
 ! float64 arrays in Fortran (column-major) order.
@@ -85,6 +85,8 @@ program ref_harness
       call do_nl()
    case ('bench')
       call do_bench()
+   case ('nlstep')
+      call do_nlstep()
    case default
       write(output_unit,*) 'unknown mode ', trim(mode)
       stop 2
@@ -214,6 +216,78 @@ contains
       close(v)
    end subroutine do_nl
 
+   subroutine nl_level(gsa, C, lcut, O, uGs)
+      !-- one bulk level of the nonlinear-term path in the reference's call
+      !   order (src/rIter.f90:483-707, l_adv_curl, derivative fields on)
+      type(grid_space_arrays_t), intent(inout) :: gsa
+      complex(cp), intent(in) :: C(:,:)
+      integer, intent(in) :: lcut
+      complex(cp), intent(out) :: O(:,:)
+      real(cp), intent(inout) :: uGs(:,:)
+      integer :: i
+      call scal_to_spat(C(:,6), gsa%sc, lcut)
+      call scal_to_grad_spat(C(:,6), gsa%dsdtc, gsa%dsdpc, lcut)
+      call torpol_to_spat(C(:,1), C(:,2), C(:,4), gsa%vrc, gsa%vtc, gsa%vpc, lcut)
+      call torpol_to_curl_spat(or2(1), C(:,1), C(:,3), C(:,4), C(:,5), &
+      &                        gsa%cvrc, gsa%cvtc, gsa%cvpc, lcut)
+      call torpol_to_spat(C(:,2), C(:,3), C(:,5), gsa%dvrdrc, gsa%dvtdrc, &
+      &                   gsa%dvpdrc, lcut)
+      call torpol_to_spat(C(:,7), C(:,8), C(:,10), gsa%brc, gsa%btc, gsa%bpc, lcut)
+      call torpol_to_curl_spat(or2(1), C(:,7), C(:,9), C(:,10), C(:,11), &
+      &                        gsa%cbrc, gsa%cbtc, gsa%cbpc, lcut)
+      call gsa%get_nl(0.0_cp, 1, 0, .false.)
+      !$omp parallel do default(shared)
+      do i=1,n_phi_max
+         gsa%Advr(:,i)=gsa%Advr(:,i)+gsa%LFr(:,i)
+         gsa%Advt(:,i)=gsa%Advt(:,i)+gsa%LFt(:,i)
+         gsa%Advp(:,i)=gsa%Advp(:,i)+gsa%LFp(:,i)
+         uGs(:,i)=or2(1)*O_sin_theta_E2(:)*(gsa%vtc(:,i)*gsa%dsdtc(:,i)+ &
+         &        gsa%vpc(:,i)*gsa%dsdpc(:,i))
+      end do
+      !$omp end parallel do
+      call spat_to_qst(gsa%Advr, gsa%Advt, gsa%Advp, O(:,1), O(:,2), O(:,3), lcut)
+      call spat_to_qst(gsa%VSr, gsa%VSt, gsa%VSp, O(:,4), O(:,5), O(:,6), lcut)
+      call spat_to_qst(gsa%VxBr, gsa%VxBt, gsa%VxBp, O(:,7), O(:,8), O(:,9), lcut)
+      call scal_to_SH(uGs, O(:,10), lcut)
+   end subroutine nl_level
+
+   subroutine do_nlstep()
+      !-- reads nlev levels; per level: lcut, or2, or4, orho1, LFfac and 11
+      !   coefficient arrays (w dw ddw z dz s b db ddb aj dj); writes the 18
+      !   inverse grids and the 10 forward outputs of nl_level
+      type(grid_space_arrays_t) :: gsa
+      complex(cp), allocatable :: C(:,:), O(:,:)
+      real(cp), allocatable :: uGs(:,:)
+      integer :: u, v, lev, i
+      integer(int32) :: lcut32
+      real(cp) :: o2, o4, orh, lff
+
+      allocate( C(lm_max,11), O(lm_max,10), uGs(n_theta_max,n_phi_max) )
+      open(newunit=u, file=trim(infile), access='stream', form='unformatted', &
+      &    status='old')
+      open(newunit=v, file=trim(outfile), access='stream', form='unformatted', &
+      &    status='replace')
+      call write_header(v)
+      do lev=1,nlev
+         read(u) lcut32, o2, o4, orh, lff
+         call set_radial(o2, o4, orh)
+         LFfac=lff
+         if ( lev == 1 ) call gsa%initialize()
+         do i=1,11
+            read(u) C(:,i)
+         end do
+         call nl_level(gsa, C, int(lcut32), O, uGs)
+         write(v) gsa%vrc, gsa%vtc, gsa%vpc, gsa%cvrc, gsa%cvtc, gsa%cvpc
+         write(v) gsa%dvrdrc, gsa%dvtdrc, gsa%dvpdrc, gsa%brc, gsa%btc, gsa%bpc
+         write(v) gsa%cbrc, gsa%cbtc, gsa%cbpc, gsa%sc, gsa%dsdtc, gsa%dsdpc
+         do i=1,10
+            write(v) O(:,i)
+         end do
+      end do
+      close(u)
+      close(v)
+   end subroutine do_nlstep
+
    subroutine do_bench()
       !-- Times the per-level transform sequence of a Boussinesq MHD bulk
       !   level (src/rIter.f90:483-707 with l_adv_curl and the derivative
@@ -250,32 +324,7 @@ contains
       do rep=0,nrep
          call system_clock(t0, rate)
          do lev=1,nlev
-            call scal_to_spat(C(:,6,lev), gsa%sc, l_max)
-            call scal_to_grad_spat(C(:,6,lev), gsa%dsdtc, gsa%dsdpc, l_max)
-            call torpol_to_spat(C(:,1,lev), C(:,2,lev), C(:,4,lev), &
-            &                   gsa%vrc, gsa%vtc, gsa%vpc, l_max)
-            call torpol_to_curl_spat(or2(1), C(:,1,lev), C(:,3,lev), C(:,4,lev), &
-            &                        C(:,5,lev), gsa%cvrc, gsa%cvtc, gsa%cvpc, l_max)
-            call torpol_to_spat(C(:,2,lev), C(:,3,lev), C(:,5,lev), &
-            &                   gsa%dvrdrc, gsa%dvtdrc, gsa%dvpdrc, l_max)
-            call torpol_to_spat(C(:,7,lev), C(:,8,lev), C(:,10,lev), &
-            &                   gsa%brc, gsa%btc, gsa%bpc, l_max)
-            call torpol_to_curl_spat(or2(1), C(:,7,lev), C(:,9,lev), C(:,10,lev), &
-            &                        C(:,11,lev), gsa%cbrc, gsa%cbtc, gsa%cbpc, l_max)
-            call gsa%get_nl(0.0_cp, 1, 0, .false.)
-            !$omp parallel do default(shared)
-            do i=1,n_phi_max
-               gsa%Advr(:,i)=gsa%Advr(:,i)+gsa%LFr(:,i)
-               gsa%Advt(:,i)=gsa%Advt(:,i)+gsa%LFt(:,i)
-               gsa%Advp(:,i)=gsa%Advp(:,i)+gsa%LFp(:,i)
-               uGs(:,i)=or2(1)*O_sin_theta_E2(:)*(gsa%vtc(:,i)*gsa%dsdtc(:,i)+ &
-               &        gsa%vpc(:,i)*gsa%dsdpc(:,i))
-            end do
-            !$omp end parallel do
-            call spat_to_qst(gsa%Advr, gsa%Advt, gsa%Advp, O(:,1), O(:,2), O(:,3), l_max)
-            call spat_to_qst(gsa%VSr, gsa%VSt, gsa%VSp, O(:,4), O(:,5), O(:,6), l_max)
-            call spat_to_qst(gsa%VxBr, gsa%VxBt, gsa%VxBp, O(:,7), O(:,8), O(:,9), l_max)
-            call scal_to_SH(uGs, O(:,10), l_max)
+            call nl_level(gsa, C(:,:,lev), l_max, O, uGs)
             checksum=checksum+abs(O(2,1))
          end do
          call system_clock(t1)

@@ -157,6 +157,43 @@ def nl_case(tag, l_max):
     print("wrote", path, os.path.getsize(path))
 
 
+def nlstep_case(tag, l_max, nlev, write=True):
+    """The nonlinear-term step (ref_harness nlstep mode) on nl_reference's inputs."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from nl_reference import make_inputs, radial
+    n_theta, n_phi = workload.grid_size(l_max)
+    inputs = make_inputs(l_max, nlev)
+    or2, or4, orho1, lff = radial(nlev)
+    lcut = [l_max] * nlev
+    lcut[-1] = l_max - 3
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            for i in range(nlev):
+                np.array([lcut[i]], dtype="<i4").tofile(f)
+                np.array([or2[i], or4[i], orho1[i], lff], dtype="<f8").tofile(f)
+                for k in range(11):
+                    np.ascontiguousarray(inputs[k][i], dtype="<c16").tofile(f)
+        oracle.run_ref("nlstep", l_max, 1, 20, fin, fout, nlev=nlev, threads=4)
+        with open(fout, "rb") as f:
+            hd = read_header(f)
+            lm = hd["lm_max"]
+            G = np.zeros((nlev, 18, n_phi, n_theta))
+            O = np.zeros((10, nlev, lm), dtype=np.complex128)
+            for i in range(nlev):
+                G[i] = np.fromfile(f, "<f8", 18 * n_phi * n_theta).reshape(18, n_phi, n_theta)
+                for k in range(10):
+                    O[k, i] = np.fromfile(f, "<c16", lm)
+            assert f.read() == b""
+    data = dict(l_max=l_max, nlev=nlev, n_theta=n_theta, n_phi=n_phi, lcut=np.array(lcut),
+                grids=G, outs=O)
+    if write:
+        path = os.path.join(OUT, "nlstep_%s.npz" % tag)
+        np.savez_compressed(path, **data)
+        print("wrote", path, os.path.getsize(path))
+    return data
+
+
 def main():
     if not oracle.have_ref():
         sys.exit("oracle/_ref/ref_harness is not built (make -C oracle)")
@@ -168,6 +205,7 @@ def main():
     api_case("L64_m1", 64, 1, [64], config=1)
     api_case("L64_m4", 64, 4, [64, 40], config=13)
     nl_case("L16", 16)
+    nlstep_case("L16", 16, 3)
 
 
 if __name__ == "__main__":

@@ -26,6 +26,25 @@ def _run(p, inputs, rad, lcut, with_grids=True):
     return g, [t.cpu().numpy() for t in tout]
 
 
+# Grid-space products amplify FP64 rounding where two products nearly
+# cancel (AdvpLM on these inputs: the reference and the oracle themselves
+# differ by ~4e-12 at l_max=64); the transforms alone are held to 1e-12.
+OUT_TOL = 1e-11
+
+
+def test_nl_step_matches_reference_golden():
+    """the full step against the reference's own composition (ref_harness)"""
+    from conftest import load_golden
+    g = load_golden("nlstep_L16.npz")
+    L, nlev = int(g["l_max"]), int(g["nlev"])
+    p = sht.Plan(L, int(g["n_theta"]), int(g["n_phi"]), max_levels=nlev)
+    G, outs = _run(p, make_inputs(L, nlev), radial(nlev), list(g["lcut"]))
+    for c in range(18):
+        assert rel_l2(G[:, c], g["grids"][:, c]) < 1e-12, ("grid", sht.NL_GRIDS[c])
+    for k in range(10):
+        assert rel_l2(outs[k], g["outs"][k]) < OUT_TOL, ("out", sht.NL_OUTPUTS[k])
+
+
 @pytest.mark.parametrize("L,nlev", [(16, 3), (64, 9)])
 def test_nl_step_matches_oracle(L, nlev):
     from oracle import oracle
@@ -44,7 +63,7 @@ def test_nl_step_matches_oracle(L, nlev):
         assert err < 1e-12, ("grid", sht.NL_GRIDS[c], err)
     for k in range(10):
         err = rel_l2(outs[k], outr[k])
-        assert err < 1e-12, ("out", sht.NL_OUTPUTS[k], err)
+        assert err < OUT_TOL, ("out", sht.NL_OUTPUTS[k], err)
 
 
 def test_nl_step_full_size_properties():

@@ -55,6 +55,23 @@ def test_oracle_get_nl_matches_reference():
         assert rel_l2(out[k], g["out"][k]) < 1e-14, k
 
 
+def test_oracle_nl_step_matches_reference():
+    """The nonlinear-term step composed from the oracle in the reference's
+    call order against the reference's own composition (ref_harness nlstep).
+    AdvpLM is ill-conditioned on these inputs (the two curl products nearly
+    cancel): the reference and the oracle differ there by ~1e-13 at l_max=16
+    and ~4e-12 at l_max=64 although every transform agrees to ~1e-15."""
+    from nl_reference import make_inputs, oracle_step, radial
+    g = load_golden("nlstep_L16.npz")
+    L, nlev = int(g["l_max"]), int(g["nlev"])
+    o = oracle.Plan(L, int(g["n_theta"]), int(g["n_phi"]))
+    G, outs = oracle_step(o, make_inputs(L, nlev), *radial(nlev), list(g["lcut"]))
+    for c in range(18):
+        assert rel_l2(G[:, c], g["grids"][:, c]) < 1e-14, c
+    for k in range(10):
+        assert rel_l2(outs[k], g["outs"][k]) < (1e-12 if k == 2 else 1e-14), k
+
+
 # ---------------------------------------------------------- independent
 def test_gauleg_vs_numpy():
     for n in (24, 96, 384):
