@@ -33,9 +33,7 @@ LAUNCHES = [
     ("syn_qst_du", "k_syn<SYN_QST>", "flop", 20),
     ("syn_qst_b", "k_syn<SYN_QST>", "flop", 20),
     ("syn_qst_curl_b", "k_syn<SYN_QST>", "flop", 20),
-    ("ifft", "k_ifft", "fft", 18),
-    ("products", "k_nl_products", "grid", 25),
-    ("fft", "k_fft", "fft", 10),
+    ("grid_stage", "k_nl_fused", "fused", 0),
     ("ana_qst_adv", "k_ana<ANA_QST>", "flop", 20),
     ("ana_qst_vs", "k_ana<ANA_QST>", "flop", 20),
     ("ana_qst_vxb", "k_ana<ANA_QST>", "flop", 20),
@@ -50,9 +48,9 @@ def log(*a):
 def algorithmic(kind, w, nh, lm_max, n_theta, n_phi, n_m, nlev):
     if kind == "flop":
         return w * nh * lm_max * nlev
-    if kind == "fft":   # bytes: real grid + complex half-spectrum kept modes, per comp
-        return w * (n_theta * n_phi * 8 + n_theta * n_m * 16) * nlev
-    return w * n_theta * n_phi * 8 * nlev
+    # fused grid stage, bytes: 18 Fourier comps in, 10 out (n_theta x n_m
+    # complex128 each) and the 3 du/dr grids written (n_theta x n_phi float64)
+    return ((18 + 10) * n_theta * n_m * 16 + 3 * n_theta * n_phi * 8) * nlev
 
 
 def cpu_baseline(l_max, nlev_sample):
@@ -196,7 +194,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "full nonlinear step (5 QST + scalar + gradient inverse, "
-                               "grid products, 3 QST + scalar forward), FP64 plain form",
+                               "fused FFTs + grid products, 3 QST + scalar forward), "
+                               "FP64 plain form",
                    "l_max": L, "n_theta": nt, "n_phi": nph, "n_r": nlev,
                    "precision": "fp64"},
         "legendre_tflops": total_flop / (sum(t for (_, _, k, _), t in zip(LAUNCHES, lt)

@@ -167,16 +167,17 @@ int mhx_nl_step_grids(mhx_plan *p, int nlev, const double *const *in,
                       const double *or4, const double *orho1, double LFfac,
                       const int *lcut, double *grids, void *stream);
 
-/* average durations (ms) of the stages of the last mhx_nl_step on this plan,
+/* durations (ms) of the stages of the last mhx_nl_step on this plan,
  * measured with HIP events on the launch stream: [0] synthesis Legendre,
- * [1] inverse FFT, [2] grid products, [3] forward FFT, [4] analysis Legendre.
+ * [1] fused grid stage (inverse FFTs, grid products, forward FFTs), [2] and
+ * [3] zero (stages fused into [1]), [4] analysis Legendre.
  * Requires mhx_set_timing(p, 1) before the step. */
 int mhx_set_timing(mhx_plan *p, int on);
 int mhx_stage_times(const mhx_plan *p, double *ms5);
 /* durations (ms) of the individual launches of the last timed mhx_nl_step,
  * in launch order: synthesis (s + gradient), synthesis QST x5 (u, curl u,
- * du/dr, B, curl B), inverse FFT, grid products, forward FFT, analysis QST
- * x3 (Adv, VS, VxB), analysis SH (u.grad s).  Returns the number written. */
+ * du/dr, B, curl B), fused grid stage, analysis QST x3 (Adv, VS, VxB),
+ * analysis SH (u.grad s).  Returns the number written. */
 int mhx_launch_times(const mhx_plan *p, double *ms, int n);
 
 #ifdef __cplusplus

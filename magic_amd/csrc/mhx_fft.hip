@@ -1,293 +1,465 @@
-// phi FFTs, grid-space products and the axisymmetric transforms.
+// phi FFTs, the fused grid stage of the nonlinear step, and the
+// axisymmetric transforms.
 //
-// Real FFTs of length N = n_phi over blocks of theta rows, computed as a
-// complex FFT of length M = N/2 in LDS (Stockham autosort, radices 4/2/3/5,
-// the radix set of src/fft.f90:67-73) plus the half-length pre/post step.
-// Normalisation as the reference: forward F_k = (1/N) sum_j x_j e^{-2 pi i jk/N}
-// (src/fft.f90:262-268,464), inverse unnormalised Hermitian synthesis using
-// only the real parts of modes 0 and N/2 (src/fft.f90:386-398).
+// Real FFTs of length N = n_phi are computed as complex FFTs of length
+// M = N/2 in LDS (Stockham autosort, ping-pong between two buffers, radices
+// 4/2/3/5 = the radix set of src/fft.f90:67-73) plus the half-length
+// pre/post step.  Normalisation as the reference: forward
+// F_k = (1/N) sum_j x_j e^{-2 pi i jk/N} (src/fft.f90:262-268,464), inverse
+// the unnormalised Hermitian synthesis using only the real parts of modes 0
+// and N/2 (src/fft.f90:386-398).
 //
-// Grids are theta-fastest (Fortran f(n_theta, n_phi)); a block of TB
-// consecutive theta rows is read/written as TB-element runs per phi.
+// Fourier rows are contiguous: F[lev][comp][theta][m], m < n_m (modes up to
+// N/2 beyond n_m are zero, as src/shtransforms.f90:224-232).
 #include "mhx_internal.h"
 
 namespace mhx {
 
 namespace {
 
-constexpr int NTH = 256;
-constexpr int MAXB = 8;  // butterflies per thread per pass (rows*M/R <= 2048)
+template <typename T> struct CT;
+template <> struct CT<double> { typedef double2 t; };
+template <> struct CT<float> { typedef float2 t; };
+template <typename T> using cpx = typename CT<T>::t;
 
-struct FFTArgs {
-    int M, N, nfac, fac[16];
-    const double2 *tw;   // exp(-2 pi i k / M), k < M
-    const double2 *twr;  // exp(-2 pi i k / N), k <= M
-};
-
-__device__ inline double2 cmul(double2 a, double2 b)
+template <typename C> __device__ inline C mk(decltype(C::x) a, decltype(C::x) b)
 {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    C r;
+    r.x = a;
+    r.y = b;
+    return r;
 }
-__device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
-__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-// multiply by sign*i
-__device__ inline double2 cmuli(double2 a, int sign)
+template <typename C> __device__ inline C cmul(C a, C b)
 {
-    return sign > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+    return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename C> __device__ inline C cconj(C a) { return mk<C>(a.x, -a.y); }
+template <typename C> __device__ inline C cadd(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
+template <typename C> __device__ inline C csub(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y); }
+template <typename C> __device__ inline C cmuli(C a, int sign)  // * (sign i)
+{
+    return sign > 0 ? mk<C>(-a.y, a.x) : mk<C>(a.y, -a.x);
 }
 
 // in-register DFT of size R with kernel exp(sign 2 pi i / R)
-template <int R> __device__ inline void dft(double2 *x, int sign);
+template <int R, typename C> struct Dft;
+template <typename C> struct Dft<2, C> {
+    __device__ static void run(C *x, int)
+    {
+        C a = x[0], b = x[1];
+        x[0] = cadd(a, b);
+        x[1] = csub(a, b);
+    }
+};
+template <typename C> struct Dft<4, C> {
+    __device__ static void run(C *x, int sign)
+    {
+        C a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
+        C b0 = cadd(x[1], x[3]), b1 = cmuli(csub(x[1], x[3]), sign);
+        x[0] = cadd(a0, b0);
+        x[2] = csub(a0, b0);
+        x[1] = cadd(a1, b1);
+        x[3] = csub(a1, b1);
+    }
+};
+template <typename C> struct Dft<3, C> {
+    __device__ static void run(C *x, int sign)
+    {
+        typedef decltype(C::x) T;
+        const T c = T(-0.5), s = T(0.86602540378443864676) * T(sign);
+        C t1 = cadd(x[1], x[2]), t2 = csub(x[1], x[2]);
+        C y0 = cadd(x[0], t1);
+        C m1 = mk<C>(x[0].x + c * t1.x, x[0].y + c * t1.y);
+        C m2 = mk<C>(-s * t2.y, s * t2.x);
+        x[0] = y0;
+        x[1] = cadd(m1, m2);
+        x[2] = csub(m1, m2);
+    }
+};
+template <typename C> struct Dft<5, C> {
+    __device__ static void run(C *x, int sign)
+    {
+        typedef decltype(C::x) T;
+        const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+        const T s1 = T(0.95105651629515357212) * T(sign), s2 = T(0.58778525229247312917) * T(sign);
+        C a1 = cadd(x[1], x[4]), b1 = csub(x[1], x[4]);
+        C a2 = cadd(x[2], x[3]), b2 = csub(x[2], x[3]);
+        C y0 = cadd(x[0], cadd(a1, a2));
+        C r1 = mk<C>(x[0].x + c1 * a1.x + c2 * a2.x, x[0].y + c1 * a1.y + c2 * a2.y);
+        C r2 = mk<C>(x[0].x + c2 * a1.x + c1 * a2.x, x[0].y + c2 * a1.y + c1 * a2.y);
+        C i1 = mk<C>(-(s1 * b1.y + s2 * b2.y), s1 * b1.x + s2 * b2.x);
+        C i2 = mk<C>(-(s2 * b1.y - s1 * b2.y), s2 * b1.x - s1 * b2.x);
+        x[0] = y0;
+        x[1] = cadd(r1, i1);
+        x[4] = csub(r1, i1);
+        x[2] = cadd(r2, i2);
+        x[3] = csub(r2, i2);
+    }
+};
 
-template <> __device__ inline void dft<2>(double2 *x, int)
-{
-    double2 a = x[0], b = x[1];
-    x[0] = cadd(a, b);
-    x[1] = csub(a, b);
-}
-template <> __device__ inline void dft<4>(double2 *x, int sign)
-{
-    double2 a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
-    double2 b0 = cadd(x[1], x[3]), b1 = cmuli(csub(x[1], x[3]), sign);
-    x[0] = cadd(a0, b0);
-    x[2] = csub(a0, b0);
-    x[1] = cadd(a1, b1);
-    x[3] = csub(a1, b1);
-}
-template <> __device__ inline void dft<3>(double2 *x, int sign)
-{
-    const double c = -0.5, s = 0.86602540378443864676 * sign;
-    double2 t1 = cadd(x[1], x[2]), t2 = csub(x[1], x[2]);
-    double2 y0 = cadd(x[0], t1);
-    double2 m1 = make_double2(x[0].x + c * t1.x, x[0].y + c * t1.y);
-    double2 m2 = make_double2(-s * t2.y, s * t2.x);  // i s t2
-    x[0] = y0;
-    x[1] = cadd(m1, m2);
-    x[2] = csub(m1, m2);
-}
-template <> __device__ inline void dft<5>(double2 *x, int sign)
-{
-    const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
-    const double s1 = 0.95105651629515357212 * sign, s2 = 0.58778525229247312917 * sign;
-    double2 a1 = cadd(x[1], x[4]), b1 = csub(x[1], x[4]);
-    double2 a2 = cadd(x[2], x[3]), b2 = csub(x[2], x[3]);
-    double2 y0 = cadd(x[0], cadd(a1, a2));
-    double2 r1 = make_double2(x[0].x + c1 * a1.x + c2 * a2.x, x[0].y + c1 * a1.y + c2 * a2.y);
-    double2 r2 = make_double2(x[0].x + c2 * a1.x + c1 * a2.x, x[0].y + c2 * a1.y + c1 * a2.y);
-    // i (s1 b1 + s2 b2), i (s2 b1 - s1 b2)
-    double2 i1 = make_double2(-(s1 * b1.y + s2 * b2.y), s1 * b1.x + s2 * b2.x);
-    double2 i2 = make_double2(-(s2 * b1.y - s1 * b2.y), s2 * b1.x - s1 * b2.x);
-    x[0] = y0;
-    x[1] = cadd(r1, i1);
-    x[4] = csub(r1, i1);
-    x[2] = cadd(r2, i2);
-    x[3] = csub(r2, i2);
-}
+struct FFTArgs {
+    int M, N, nfac, fac[16];
+    const void *tw;   // exp(-2 pi i k / M), k < M (precision of the kernel)
+    const void *twr;  // exp(-2 pi i k / N), k <= M
+};
 
-// one Stockham pass over nrows rows of M complex values at buf[row*ldr + k]
-template <int R>
-__device__ void pass(double2 *buf, int ldr, int nrows, int M, int L, const double2 *tw, int sign)
+// one Stockham pass from `in` to `out` over G rows of M complex values
+template <int R, typename C>
+__device__ void pass_pp(const C *in, C *out, int G, int M, int L, const C *tw, int sign)
 {
     const int T = M / R;
-    const int total = nrows * T;
     const int tstride = M / (L * R);
-    double2 y[MAXB][R];
+    const int nthr = blockDim.x;
+    for (int idx = threadIdx.x; idx < G * T; idx += nthr) {
+        int g = idx / T, j = idx - g * T;
+        int k = j % L;
+        const C *ip = in + g * M;
+        C x[R];
 #pragma unroll
-    for (int b = 0; b < MAXB; ++b) {
-        int idx = threadIdx.x + b * NTH;
-        if (idx < total) {
-            int row = idx / T, j = idx - row * T;
-            int k = j % L;
-            double2 x[R];
-#pragma unroll
-            for (int s = 0; s < R; ++s) {
-                double2 v = buf[row * ldr + j + s * T];
-                if (s > 0) {
-                    double2 w = tw[s * k * tstride];
-                    if (sign > 0) w = cconj(w);
-                    v = cmul(v, w);
-                }
-                x[s] = v;
+        for (int s = 0; s < R; ++s) {
+            C v = ip[j + s * T];
+            if (s > 0) {
+                C w = tw[s * k * tstride];
+                if (sign > 0) w = cconj(w);
+                v = cmul(v, w);
             }
-            dft<R>(x, sign);
-#pragma unroll
-            for (int s = 0; s < R; ++s) y[b][s] = x[s];
+            x[s] = v;
         }
-    }
-    __syncthreads();
+        Dft<R, C>::run(x, sign);
+        C *op = out + g * M + (j - k) * R + k;
 #pragma unroll
-    for (int b = 0; b < MAXB; ++b) {
-        int idx = threadIdx.x + b * NTH;
-        if (idx < total) {
-            int row = idx / T, j = idx - row * T;
-            int k = j % L;
-            int base = (j - k) * R + k;
-#pragma unroll
-            for (int s = 0; s < R; ++s) buf[row * ldr + base + s * L] = y[b][s];
-        }
+        for (int s = 0; s < R; ++s) op[s * L] = x[s];
     }
-    __syncthreads();
 }
 
-__device__ void fft_rows(double2 *buf, int ldr, int nrows, const FFTArgs &f, int sign)
+// complex FFT of G rows held in buffer a (b is scratch); returns the buffer
+// that holds the result
+template <typename C>
+__device__ C *fft_pp(C *a, C *b, int G, const FFTArgs &f, const C *tw, int sign)
 {
     int L = 1;
     for (int s = 0; s < f.nfac; ++s) {
         int R = f.fac[s];
         switch (R) {
-        case 4: pass<4>(buf, ldr, nrows, f.M, L, f.tw, sign); break;
-        case 2: pass<2>(buf, ldr, nrows, f.M, L, f.tw, sign); break;
-        case 3: pass<3>(buf, ldr, nrows, f.M, L, f.tw, sign); break;
-        default: pass<5>(buf, ldr, nrows, f.M, L, f.tw, sign); break;
+        case 4: pass_pp<4>(a, b, G, f.M, L, tw, sign); break;
+        case 2: pass_pp<2>(a, b, G, f.M, L, tw, sign); break;
+        case 3: pass_pp<3>(a, b, G, f.M, L, tw, sign); break;
+        default: pass_pp<5>(a, b, G, f.M, L, tw, sign); break;
         }
+        __syncthreads();
+        C *t = a;
+        a = b;
+        b = t;
         L *= R;
+    }
+    return a;
+}
+
+// Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/N} (X_k - conj X_{M-k}), real
+// parts only for X_0 and X_M; X is read from a global Fourier row (zero for
+// k >= n_m), Z written to LDS row z
+template <typename C>
+__device__ void irfft_pre(const C *__restrict__ X, int n_m, C *z, int M, const C *twr, int tid0,
+                          int nthr)
+{
+    typedef decltype(C::x) T;
+    for (int k = tid0; k <= M / 2; k += nthr) {
+        int k2 = M - k;
+        C a = (k < n_m) ? X[k] : mk<C>(T(0), T(0));
+        C c = (k2 < n_m) ? X[k2] : mk<C>(T(0), T(0));
+        if (k == 0) {
+            a.y = T(0);
+            c.y = T(0);
+        }
+        C za = cadd(cadd(a, cconj(c)), cmuli(cmul(cconj(twr[k]), csub(a, cconj(c))), 1));
+        z[k] = za;
+        if (k2 < M && k2 != k) {
+            C zb = cadd(cadd(c, cconj(a)), cmuli(cmul(cconj(twr[k2]), csub(c, cconj(a))), 1));
+            z[k2] = zb;
+        }
     }
 }
 
-// inverse: F[lev][comp][k][theta] (k < n_m) -> grid[lev][comp][phi][theta]
-__global__ __launch_bounds__(NTH) void k_ifft(FFTArgs f, const double *__restrict__ F, size_t F_ld,
-                                              int n_m, int n_theta, int ncomp, int TB, int ntb,
-                                              double *__restrict__ g, size_t g_ld)
+// X_k = (E_k + e^{-2 pi i k/N} O_k) * scale for k < n_m from the FFT of
+// z_n = x_{2n} + i x_{2n+1}
+template <typename C>
+__device__ inline C rfft_post(const C *z, int k, int M, const C *twr, decltype(C::x) scale)
 {
-    extern __shared__ double2 sbuf[];
-    const int M = f.M, N = f.N, ldr = M + 1;
+    typedef decltype(C::x) T;
+    C zk = z[k == M ? 0 : k];
+    C zc = cconj(z[k == 0 ? 0 : M - k]);
+    C E = mk<C>(T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y + zc.y));
+    C D = csub(zk, zc);
+    C O = mk<C>(T(0.5) * D.y, T(-0.5) * D.x);
+    C X = cadd(E, cmul(twr[k], O));
+    X.x *= scale;
+    X.y *= scale;
+    if (k == 0) X.y = T(0);
+    return X;
+}
+
+template <typename C> __device__ void load_twiddles(C *s_tw, C *s_twr, const FFTArgs &f)
+{
+    const C *tw = reinterpret_cast<const C *>(f.tw);
+    const C *twr = reinterpret_cast<const C *>(f.twr);
+    for (int k = threadIdx.x; k < f.M; k += blockDim.x) s_tw[k] = tw[k];
+    for (int k = threadIdx.x; k <= f.M; k += blockDim.x) s_twr[k] = twr[k];
+}
+
+// ---------------------------------------------------------------------------
+// API path: Fourier rows <-> theta-fastest grids (Fortran f(n_theta, n_phi))
+// WG: one (lev, comp) and TB consecutive theta rows.
+template <typename T>
+__global__ void k_ifft_rows(FFTArgs f, const T *__restrict__ F, size_t F_ld, int F_ncomp, int n_m,
+                            int n_theta, int ncomp, int TB, int ntb, T *__restrict__ g,
+                            size_t g_ld_lev, size_t g_ld_comp)
+{
+    typedef cpx<T> C;
+    extern __shared__ char smem_raw[];
+    C *s_tw = reinterpret_cast<C *>(smem_raw);
+    C *s_twr = s_tw + f.M;
+    C *A = s_twr + f.M + 1;
+    C *B = A + (size_t)TB * f.M;
+    const int M = f.M, N = f.N;
     int b = blockIdx.x;
     int tb = b % ntb;
     int comp = (b / ntb) % ncomp;
     int lev = b / (ntb * ncomp);
     int th0 = tb * TB;
     int nrows = min(TB, n_theta - th0);
-    const double2 *Fp = reinterpret_cast<const double2 *>(F) + (size_t)lev * F_ld + (size_t)comp * n_m * n_theta;
-    // load Hermitian half spectrum, zero beyond n_m
-    for (int idx = threadIdx.x; idx < (M + 1) * TB; idx += NTH) {
-        int r = idx % TB, k = idx / TB;
-        double2 v = make_double2(0.0, 0.0);
-        if (r < nrows && k < n_m) v = Fp[(size_t)k * n_theta + th0 + r];
-        sbuf[r * ldr + k] = v;
-    }
+    load_twiddles(s_tw, s_twr, f);
     __syncthreads();
-    // Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/N} (X_k - conj X_{M-k}),
-    // real parts only for X_0 and X_M
-    const int npair = M / 2 + 1;
-    double2 za[MAXB], zb[MAXB];
-#pragma unroll
-    for (int bb = 0; bb < MAXB; ++bb) {
-        int idx = threadIdx.x + bb * NTH;
-        if (idx < npair * TB) {
-            int r = idx % TB, k = idx / TB, k2 = M - k;
-            double2 a = sbuf[r * ldr + k], c = sbuf[r * ldr + k2];
-            if (k == 0) { a.y = 0.0; c.y = 0.0; }
-            double2 w1 = cconj(f.twr[k]), w2 = cconj(f.twr[k2]);
-            za[bb] = cadd(cadd(a, cconj(c)), cmuli(cmul(w1, csub(a, cconj(c))), 1));
-            zb[bb] = cadd(cadd(c, cconj(a)), cmuli(cmul(w2, csub(c, cconj(a))), 1));
+    // one thread team per row for the pre-step: threads split over rows
+    const C *Frow0 = reinterpret_cast<const C *>(F) + (size_t)lev * F_ld +
+                     ((size_t)comp * n_theta + th0) * n_m;
+    {
+        int per = blockDim.x / TB;  // threads per row
+        int r = threadIdx.x / per, t = threadIdx.x % per;
+        if (r < TB) {
+            if (r < nrows) irfft_pre(Frow0 + (size_t)r * n_m, n_m, A + r * M, M, s_twr, t, per);
+            else
+                for (int k = t; k < M; k += per) A[r * M + k] = mk<C>(T(0), T(0));
         }
     }
     __syncthreads();
-#pragma unroll
-    for (int bb = 0; bb < MAXB; ++bb) {
-        int idx = threadIdx.x + bb * NTH;
-        if (idx < npair * TB) {
-            int r = idx % TB, k = idx / TB, k2 = M - k;
-            sbuf[r * ldr + k] = za[bb];
-            if (k2 < M && k2 != k) sbuf[r * ldr + k2] = zb[bb];
-        }
-    }
-    __syncthreads();
-    fft_rows(sbuf, ldr, TB, f, +1);
-    double *gp = g + (size_t)lev * g_ld + (size_t)comp * N * n_theta;
-    for (int idx = threadIdx.x; idx < N * TB; idx += NTH) {
+    C *res = fft_pp(A, B, TB, f, s_tw, +1);
+    T *gp = g + (size_t)lev * g_ld_lev + (size_t)comp * g_ld_comp;
+    for (int idx = threadIdx.x; idx < N * TB; idx += blockDim.x) {
         int r = idx % TB, j = idx / TB;
         if (r < nrows) {
-            double2 z = sbuf[r * ldr + (j >> 1)];
+            C z = res[r * M + (j >> 1)];
             gp[(size_t)j * n_theta + th0 + r] = (j & 1) ? z.y : z.x;
         }
     }
 }
 
-// forward: grid -> F (k < n_m), scale 1/N, times 1/sin^2(theta) when the
-// comp's bit is set in osin2_mask, modes m > m_lim[lev] zeroed (m_lim
-// nullptr: none)
-__global__ __launch_bounds__(NTH) void k_fft(FFTArgs f, const double *__restrict__ g, size_t g_ld,
-                                             int n_m, int n_theta, int ncomp, int TB, int ntb,
-                                             unsigned osin2_mask, const double *__restrict__ osin2,
-                                             double *__restrict__ F, size_t F_ld)
+template <typename T>
+__global__ void k_fft_rows(FFTArgs f, const T *__restrict__ g, size_t g_ld_lev, size_t g_ld_comp,
+                           int n_m, int n_theta, int ncomp, int TB, int ntb, unsigned osin2_mask,
+                           const double *__restrict__ osin2, T *__restrict__ F, size_t F_ld)
 {
-    extern __shared__ double2 sbuf[];
-    const int M = f.M, N = f.N, ldr = M + 1;
+    typedef cpx<T> C;
+    extern __shared__ char smem_raw[];
+    C *s_tw = reinterpret_cast<C *>(smem_raw);
+    C *s_twr = s_tw + f.M;
+    C *A = s_twr + f.M + 1;
+    C *B = A + (size_t)TB * f.M;
+    const int M = f.M, N = f.N;
     int b = blockIdx.x;
     int tb = b % ntb;
     int comp = (b / ntb) % ncomp;
     int lev = b / (ntb * ncomp);
     int th0 = tb * TB;
     int nrows = min(TB, n_theta - th0);
-    const double *gp = g + (size_t)lev * g_ld + (size_t)comp * N * n_theta;
-    for (int idx = threadIdx.x; idx < N * TB; idx += NTH) {
+    load_twiddles(s_tw, s_twr, f);
+    const T *gp = g + (size_t)lev * g_ld_lev + (size_t)comp * g_ld_comp;
+    for (int idx = threadIdx.x; idx < N * TB; idx += blockDim.x) {
         int r = idx % TB, j = idx / TB;
-        double v = (r < nrows) ? gp[(size_t)j * n_theta + th0 + r] : 0.0;
-        double *z = reinterpret_cast<double *>(&sbuf[r * ldr + (j >> 1)]);
+        T v = (r < nrows) ? gp[(size_t)j * n_theta + th0 + r] : T(0);
+        T *z = reinterpret_cast<T *>(&A[r * M + (j >> 1)]);
         z[j & 1] = v;
     }
     __syncthreads();
-    fft_rows(sbuf, ldr, TB, f, -1);
+    C *res = fft_pp(A, B, TB, f, s_tw, -1);
     const bool os = (osin2_mask >> comp) & 1u;
-    const double invN = 1.0 / (double)N;
-    double2 *Fp = reinterpret_cast<double2 *>(F) + (size_t)lev * F_ld + (size_t)comp * n_m * n_theta;
-    for (int idx = threadIdx.x; idx < n_m * TB; idx += NTH) {
-        int r = idx % TB, k = idx / TB;
+    C *Fp = reinterpret_cast<C *>(F) + (size_t)lev * F_ld + ((size_t)comp * n_theta + th0) * n_m;
+    for (int idx = threadIdx.x; idx < n_m * TB; idx += blockDim.x) {
+        int k = idx % n_m, r = idx / n_m;
         if (r >= nrows) continue;
-        double2 zk = sbuf[r * ldr + (k == M ? 0 : k)];
-        double2 zc = cconj(sbuf[r * ldr + (k == 0 ? 0 : M - k)]);
-        double2 E = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
-        double2 D = csub(zk, zc);                       // 2 i O
-        double2 O = make_double2(0.5 * D.y, -0.5 * D.x); // D / (2i)
-        double2 X = cadd(E, cmul(f.twr[k], O));
-        double sc = invN;
-        if (os) sc *= osin2[th0 + r];
-        X.x *= sc;
-        X.y *= sc;
-        if (k == 0) X.y = 0.0;
-        Fp[(size_t)k * n_theta + th0 + r] = X;
+        T sc = T(1.0 / (double)N);
+        if (os) sc *= T(osin2[th0 + r]);
+        Fp[(size_t)r * n_m + k] = rfft_post(res + r * M, k, M, s_twr, sc);
     }
 }
 
-// Boussinesq bulk-level products, one point per thread.
-// gin comps: vr vt vp cvr cvt cvp dvr dvt dvp br bt bp cbr cbt cbp s dsdt dsdp
-// gout comps: Advr Advt Advp VSr VSt VSp VxBr VxBt VxBp uGs
-__global__ __launch_bounds__(NTH) void k_nl_products(int nlev, int n_theta, int n_phi,
-                                                     const double *__restrict__ gin,
-                                                     double *__restrict__ gout,
-                                                     const double *__restrict__ osin2,
-                                                     const double *__restrict__ rad, double LFfac)
+// ---------------------------------------------------------------------------
+// Fused grid stage of the nonlinear step: for one radial level and one theta
+// row, inverse FFTs of the 18 synthesised components, the grid products
+// (src/get_nl.f90:242-273 curl advection + Lorentz force merged as
+// src/rIter.f90:650-656, :320-325 V s, :376-388 V x B, and u_h . grad_h s),
+// and forward FFTs of the 10 products.  Grid values live in registers (P
+// points per thread); only Fourier rows are read and written.
+//
+// Fin comps:  vr vt vp | cvr cvt cvp | dvr dvt dvp | br bt bp | cbr cbt cbp |
+//             s dsdt dsdp
+// Fout comps: Advr Advt Advp | VSr VSt VSp | VxBr VxBt VxBp | uGs
+// du/dr grids (not consumed by the products) are written to dgrid
+// [lev][3][theta][phi]; with full_grids != nullptr all 18 inverse grids are
+// also written in the reference layout [lev][comp][phi][theta].
+constexpr int PMAX = 3;
+
+template <typename T, int G>
+__global__ void k_nl_fused(FFTArgs f, const T *__restrict__ Fin, T *__restrict__ Fout, int n_m,
+                           int n_theta, int nlev, const double *__restrict__ osin2,
+                           const double *__restrict__ rad, double LFfac_d,
+                           T *__restrict__ dgrid, T *__restrict__ full_grids)
 {
-    const size_t npts = (size_t)n_theta * n_phi;
-    size_t i = (size_t)blockIdx.x * NTH + threadIdx.x;
-    int lev = blockIdx.y;
-    if (i >= npts || lev >= nlev) return;
-    const double *a = gin + (size_t)lev * 18 * npts + i;
-    double *o = gout + (size_t)lev * 10 * npts + i;
-    const double or2 = rad[lev], or4 = rad[nlev + lev], orho1 = rad[2 * nlev + lev];
-    const double o2 = osin2[i % n_theta];
-    double vr = a[0], vt = a[npts], vp = a[2 * npts];
-    double cvr = a[3 * npts], cvt = a[4 * npts], cvp = a[5 * npts];
-    double br = a[9 * npts], bt = a[10 * npts], bp = a[11 * npts];
-    double cbr = a[12 * npts], cbt = a[13 * npts], cbp = a[14 * npts];
-    double s = a[15 * npts], dsdt = a[16 * npts], dsdp = a[17 * npts];
-    // src/get_nl.f90:262-273 (curl form) + :242-258 (Lorentz force),
-    // merged as src/rIter.f90:650-656
-    o[0] = -o2 * (cvt * vp - cvp * vt) + LFfac * o2 * (cbt * bp - cbp * bt);
-    o[npts] = -or4 * (cvp * vr - cvr * vp) + LFfac * or4 * (cbp * br - cbr * bp);
-    o[2 * npts] = -or4 * (cvr * vt - cvt * vr) + LFfac * or4 * (cbr * bt - cbt * br);
-    // :320-325
-    o[3 * npts] = vr * s;
-    o[4 * npts] = or2 * vt * s;
-    o[5 * npts] = or2 * vp * s;
-    // :376-388
-    o[6 * npts] = orho1 * o2 * (vt * bp - vp * bt);
-    o[7 * npts] = orho1 * or4 * (vp * br - vr * bp);
-    o[8 * npts] = orho1 * or4 * (vr * bt - vt * br);
-    // horizontal advection of s
-    o[9 * npts] = or2 * o2 * (vt * dsdt + vp * dsdp);
+    typedef cpx<T> C;
+    extern __shared__ char smem_raw[];
+    const int M = f.M, N = f.N;
+    C *s_tw = reinterpret_cast<C *>(smem_raw);
+    C *s_twr = s_tw + M;
+    C *A = s_twr + M + 1;
+    C *B = A + (size_t)G * M;
+    const int th = blockIdx.x, lev = blockIdx.y;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const T o2 = T(osin2[th]);
+    const T or2 = T(rad[lev]), or4 = T(rad[nlev + lev]), orho1 = T(rad[2 * nlev + lev]);
+    const T LFfac = T(LFfac_d);
+    load_twiddles(s_tw, s_twr, f);
+    __syncthreads();
+
+    const size_t rowsz = (size_t)n_m;                   // complex per row
+    const C *Fi = reinterpret_cast<const C *>(Fin) + (size_t)lev * 18 * n_theta * rowsz;
+    C *Fo = reinterpret_cast<C *>(Fout) + (size_t)lev * 10 * n_theta * rowsz;
+
+    T u[3][PMAX], adv[3][PMAX], bb[3][PMAX], vxb[3][PMAX], vs[3][PMAX], ugs[PMAX];
+    T val[3][PMAX];
+
+    // inverse FFT of comps c0..c0+2 (in groups of G rows) into val
+    auto inverse3 = [&](int c0) {
+#pragma unroll
+        for (int g0 = 0; g0 < 3; g0 += G) {
+            constexpr int gn = G;
+            int per = nthr / gn;
+            int r = tid / per, t = tid % per;
+            if (r < gn)
+                irfft_pre(Fi + ((size_t)(c0 + g0 + r) * n_theta + th) * rowsz, n_m, A + r * M, M,
+                          s_twr, t, per);
+            __syncthreads();
+            C *res = fft_pp(A, B, gn, f, s_tw, +1);
+#pragma unroll
+            for (int p = 0; p < PMAX; ++p) {
+                int j = tid + p * nthr;
+#pragma unroll
+                for (int r2 = 0; r2 < gn; ++r2) {
+                    T v = T(0);
+                    if (j < N) {
+                        C z = res[r2 * M + (j >> 1)];
+                        v = (j & 1) ? z.y : z.x;
+                        int comp = c0 + g0 + r2;
+                        if (full_grids)
+                            full_grids[(((size_t)lev * 18 + comp) * N + j) * n_theta + th] = v;
+                        if (comp >= 6 && comp < 9)
+                            dgrid[(((size_t)lev * 3 + comp - 6) * n_theta + th) * N + j] = v;
+                    }
+                    val[g0 + r2][p] = v;
+                }
+            }
+            __syncthreads();
+        }
+    };
+    // forward FFT of 3 (or 1) product rows given per point, rows scaled by sc[]
+    auto forward = [&](T (*src)[PMAX], int ncomp, int c0, const T *sc) {
+#pragma unroll
+        for (int g0 = 0; g0 < 3; g0 += G) {
+            if (g0 >= ncomp) break;
+            const int gn = min(G, ncomp - g0);
+#pragma unroll
+            for (int p = 0; p < PMAX; ++p) {
+                int j = tid + p * nthr;
+                if (j < N)
+#pragma unroll
+                    for (int r2 = 0; r2 < G; ++r2) {
+                        if (r2 >= gn) break;
+                        T *z = reinterpret_cast<T *>(&A[r2 * M + (j >> 1)]);
+                        z[j & 1] = src[g0 + r2][p] * sc[g0 + r2];
+                    }
+            }
+            __syncthreads();
+            C *res = fft_pp(A, B, gn, f, s_tw, -1);
+            const T invN = T(1.0 / (double)N);
+            for (int idx = tid; idx < gn * n_m; idx += nthr) {
+                int r2 = idx / n_m, k = idx - r2 * n_m;
+                Fo[((size_t)(c0 + g0 + r2) * n_theta + th) * rowsz + k] =
+                    rfft_post(res + r2 * M, k, M, s_twr, invN);
+            }
+            __syncthreads();
+        }
+    };
+
+    // u
+    inverse3(0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int p = 0; p < PMAX; ++p) u[c][p] = val[c][p];
+    // curl u: Adv = -curl(u) x u terms (src/get_nl.f90:262-273)
+    inverse3(3);
+#pragma unroll
+    for (int p = 0; p < PMAX; ++p) {
+        T cvr = val[0][p], cvt = val[1][p], cvp = val[2][p];
+        adv[0][p] = -o2 * (cvt * u[2][p] - cvp * u[1][p]);
+        adv[1][p] = -or4 * (cvp * u[0][p] - cvr * u[2][p]);
+        adv[2][p] = -or4 * (cvr * u[1][p] - cvt * u[0][p]);
+    }
+    // du/dr: written to dgrid only
+    inverse3(6);
+    // B: V x B (src/get_nl.f90:376-388)
+    inverse3(9);
+#pragma unroll
+    for (int p = 0; p < PMAX; ++p) {
+        bb[0][p] = val[0][p];
+        bb[1][p] = val[1][p];
+        bb[2][p] = val[2][p];
+        vxb[0][p] = orho1 * o2 * (u[1][p] * bb[2][p] - u[2][p] * bb[1][p]);
+        vxb[1][p] = orho1 * or4 * (u[2][p] * bb[0][p] - u[0][p] * bb[2][p]);
+        vxb[2][p] = orho1 * or4 * (u[0][p] * bb[1][p] - u[1][p] * bb[0][p]);
+    }
+    // curl B: Lorentz force added to Adv (src/get_nl.f90:242-258,
+    // src/rIter.f90:650-656)
+    inverse3(12);
+#pragma unroll
+    for (int p = 0; p < PMAX; ++p) {
+        T cbr = val[0][p], cbt = val[1][p], cbp = val[2][p];
+        adv[0][p] += LFfac * o2 * (cbt * bb[2][p] - cbp * bb[1][p]);
+        adv[1][p] += LFfac * or4 * (cbp * bb[0][p] - cbr * bb[2][p]);
+        adv[2][p] += LFfac * or4 * (cbr * bb[1][p] - cbt * bb[0][p]);
+    }
+    {
+        const T sc[3] = {T(1), o2, o2};  // 1/sin^2 on the horizontal comps
+        forward(adv, 3, 0, sc);
+        forward(vxb, 3, 6, sc);
+    }
+    // s, ds/dtheta, ds/dphi: V s (src/get_nl.f90:320-325), u_h . grad_h s
+    inverse3(15);
+#pragma unroll
+    for (int p = 0; p < PMAX; ++p) {
+        T s = val[0][p];
+        vs[0][p] = u[0][p] * s;
+        vs[1][p] = or2 * u[1][p] * s;
+        vs[2][p] = or2 * u[2][p] * s;
+        ugs[p] = or2 * o2 * (u[1][p] * val[1][p] + u[2][p] * val[2][p]);
+    }
+    {
+        const T sc[3] = {T(1), o2, o2};
+        forward(vs, 3, 3, sc);
+        T tmp[1][PMAX];
+#pragma unroll
+        for (int p = 0; p < PMAX; ++p) tmp[0][p] = ugs[p];
+        const T one[1] = {T(1)};
+        forward(tmp, 1, 9, one);
+    }
 }
 
 // axisymmetric transforms: src/shtransforms.f90:374-403 (tor = 0) and
@@ -332,62 +504,81 @@ FFTArgs make_args(const Plan *p)
     f.N = p->n_phi;
     f.nfac = (int)p->fac.size();
     for (int i = 0; i < f.nfac && i < 16; ++i) f.fac[i] = p->fac[i];
-    f.tw = reinterpret_cast<const double2 *>(p->d_tw);
-    f.twr = reinterpret_cast<const double2 *>(p->d_twr);
+    f.tw = p->d_tw;
+    f.twr = p->d_twr;
     return f;
 }
 
+// rows per block of the API kernels: two LDS buffers of TB rows + twiddles
 int rows_per_block(const Plan *p)
 {
-    // LDS: TB (M+1) complex; registers: butterflies per pass TB M / 2 and
-    // pre-processing pairs TB (M/2 + 1) must fit NTH * MAXB
     int TB = 16;
-    while (TB > 1 && ((size_t)TB * (p->M + 1) * sizeof(double2) > 64 * 1024 ||
-                      TB * p->M > 2 * NTH * MAXB || TB * (p->M / 2 + 1) > NTH * MAXB))
-        TB >>= 1;
+    while (TB > 1 && ((size_t)(2 * TB + 2) * (p->M + 1) * sizeof(double2) > 64 * 1024)) TB >>= 1;
     return TB;
+}
+
+size_t api_shm(const Plan *p, int TB)
+{
+    return ((size_t)2 * TB * p->M + 2 * p->M + 1) * sizeof(double2);
 }
 
 }  // namespace
 
-int launch_ifft(const Plan *p, const double *F, size_t F_ld_lev, int ncomp, int nlev, double *grid,
-                size_t grid_ld_lev, hipStream_t s)
+int launch_ifft(const Plan *p, const double *F, size_t F_ld_lev, int F_ncomp, int ncomp, int nlev,
+                double *grid, size_t grid_ld_lev, size_t grid_ld_comp, hipStream_t s)
 {
     if (nlev <= 0) return MHX_OK;
     FFTArgs f = make_args(p);
     int TB = rows_per_block(p);
     int ntb = (p->n_theta + TB - 1) / TB;
-    size_t shm = (size_t)TB * (p->M + 1) * sizeof(double2);
-    dim3 grid_dim((unsigned)(ntb * ncomp * nlev));
-    k_ifft<<<grid_dim, NTH, shm, s>>>(f, F, F_ld_lev, p->n_m, p->n_theta, ncomp, TB, ntb, grid,
-                                      grid_ld_lev);
+    size_t shm = api_shm(p, TB);
+    int nthr = 256;
+    if (nthr % TB) return fail(MHX_E_INVALID, "row block does not divide the block size");
+    k_ifft_rows<double><<<dim3((unsigned)(ntb * ncomp * nlev)), nthr, shm, s>>>(
+        f, F, F_ld_lev, F_ncomp, p->n_m, p->n_theta, ncomp, TB, ntb, grid, grid_ld_lev,
+        grid_ld_comp);
     MHX_HIP_CHECK(hipGetLastError());
     return MHX_OK;
 }
 
-int launch_fft(const Plan *p, const double *grid, size_t grid_ld_lev, int ncomp, int nlev,
-               unsigned osin2_mask, double *F, size_t F_ld_lev, hipStream_t s)
+int launch_fft(const Plan *p, const double *grid, size_t grid_ld_lev, size_t grid_ld_comp,
+               int ncomp, int nlev, unsigned osin2_mask, double *F, size_t F_ld_lev, hipStream_t s)
 {
     if (nlev <= 0) return MHX_OK;
     FFTArgs f = make_args(p);
     int TB = rows_per_block(p);
     int ntb = (p->n_theta + TB - 1) / TB;
-    size_t shm = (size_t)TB * (p->M + 1) * sizeof(double2);
-    dim3 grid_dim((unsigned)(ntb * ncomp * nlev));
-    k_fft<<<grid_dim, NTH, shm, s>>>(f, grid, grid_ld_lev, p->n_m, p->n_theta, ncomp, TB, ntb,
-                                     osin2_mask, p->d_osin2, F, F_ld_lev);
+    size_t shm = api_shm(p, TB);
+    k_fft_rows<double><<<dim3((unsigned)(ntb * ncomp * nlev)), 256, shm, s>>>(
+        f, grid, grid_ld_lev, grid_ld_comp, p->n_m, p->n_theta, ncomp, TB, ntb, osin2_mask,
+        p->d_osin2, F, F_ld_lev);
     MHX_HIP_CHECK(hipGetLastError());
     return MHX_OK;
 }
 
-int launch_nl_products(const Plan *p, int nlev, const double *gin, double *gout, const double *rad,
-                       double LFfac, hipStream_t s)
+int fused_threads(int n_phi)
+{
+    int nt = 64 * ((n_phi + 3 * 64 - 1) / (3 * 64));
+    return nt > 1024 ? 1024 : nt;
+}
+
+int launch_nl_fused(const Plan *p, int nlev, const double *Fin, double *Fout, const double *rad,
+                    double LFfac, double *dgrid, double *full_grids, hipStream_t s)
 {
     if (nlev <= 0) return MHX_OK;
-    size_t npts = (size_t)p->n_theta * p->n_phi;
-    dim3 grid_dim((unsigned)((npts + NTH - 1) / NTH), (unsigned)nlev);
-    k_nl_products<<<grid_dim, NTH, 0, s>>>(nlev, p->n_theta, p->n_phi, gin, gout, p->d_osin2, rad,
-                                           LFfac);
+    FFTArgs f = make_args(p);
+    int nthr = fused_threads(p->n_phi);
+    if (nthr * PMAX < p->n_phi) return fail(MHX_E_UNSUPPORTED, "n_phi too large for the fused stage");
+    int G = ((size_t)(6 * p->M + 2 * p->M + 1) * sizeof(double2) <= 64 * 1024) ? 3 : 1;
+    size_t shm = ((size_t)2 * G * p->M + 2 * p->M + 1) * sizeof(double2);
+    if (shm > 64 * 1024) return fail(MHX_E_UNSUPPORTED, "n_phi too large for the fused stage (LDS)");
+    dim3 grid((unsigned)p->n_theta, (unsigned)nlev);
+    if (G == 3)
+        k_nl_fused<double, 3><<<grid, nthr, shm, s>>>(f, Fin, Fout, p->n_m, p->n_theta, nlev,
+                                                       p->d_osin2, rad, LFfac, dgrid, full_grids);
+    else
+        k_nl_fused<double, 1><<<grid, nthr, shm, s>>>(f, Fin, Fout, p->n_m, p->n_theta, nlev,
+                                                       p->d_osin2, rad, LFfac, dgrid, full_grids);
     MHX_HIP_CHECK(hipGetLastError());
     return MHX_OK;
 }

@@ -132,16 +132,23 @@ struct Plan {
 int build_tables(Plan *p);
 int launch_syn(const Plan *p, const SynJob &job, hipStream_t s);
 int launch_ana(const Plan *p, const AnaJob &job, hipStream_t s);
-// inverse FFT: F[lev][comp][m][theta] (n_m modes) -> grid[lev][comp][phi][theta]
-int launch_ifft(const Plan *p, const double *F, size_t F_ld_lev, int ncomp,
-                int nlev, double *grid, size_t grid_ld_lev, hipStream_t s);
-// forward FFT: grid -> F, scale 1/n_phi, times 1/sin^2 for rows flagged in
-// osin2_mask (bit per comp), modes above m_lim zeroed
-int launch_fft(const Plan *p, const double *grid, size_t grid_ld_lev, int ncomp,
-               int nlev, unsigned osin2_mask, double *F, size_t F_ld_lev,
+// Fourier rows F[lev][comp][theta][m] (m < n_m, complex) <-> grids in the
+// reference layout (theta fastest).  F points at the first comp; F_ld_lev is
+// the level stride in complex elements; grid comp c of level l starts at
+// grid + l*grid_ld_lev + c*grid_ld_comp (doubles).
+int launch_ifft(const Plan *p, const double *F, size_t F_ld_lev, int F_ncomp, int ncomp,
+                int nlev, double *grid, size_t grid_ld_lev, size_t grid_ld_comp,
+                hipStream_t s);
+// forward: scale 1/n_phi, times 1/sin^2(theta) for comps flagged in osin2_mask
+int launch_fft(const Plan *p, const double *grid, size_t grid_ld_lev, size_t grid_ld_comp,
+               int ncomp, int nlev, unsigned osin2_mask, double *F, size_t F_ld_lev,
                hipStream_t s);
-int launch_nl_products(const Plan *p, int nlev, const double *gin, double *gout,
-                       const double *rad, double LFfac, hipStream_t s);
+// fused grid stage of the nonlinear step: Fin (18 comps per level) -> Fout
+// (10 comps per level); du/dr grids to dgrid [lev][3][theta][phi]; all 18
+// inverse grids in the reference layout to full_grids when not null
+int launch_nl_fused(const Plan *p, int nlev, const double *Fin, double *Fout,
+                    const double *rad, double LFfac, double *dgrid, double *full_grids,
+                    hipStream_t s);
 int launch_axi(const Plan *p, const double *fl, double *f, int lcut, int tor,
                double *f2, hipStream_t s);
 

@@ -97,6 +97,19 @@ __device__ inline d4 mfma64(double a, double b, d4 c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2).
+// Remap the linear block id so that each XCD works through a contiguous range
+// of logical blocks; with the order index m fastest, the blocks resident on
+// one XCD at a time cover neighbouring m, whose Fourier values share cache
+// lines of the row-contiguous F[lev][comp][theta][m] layout.
+__device__ inline int xcd_logical(int b, int nb)
+{
+    const int nx = 8;
+    int per = nb / nx, rem = nb % nx;
+    int x = b % nx, i = b / nx;
+    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + i;
+}
+
 // i^ipow * m^mpow * scale applied to (re, im)
 __device__ inline void apply_factor(double &re, double &im, int ipow, int mpow, double m,
                                     double scale)
@@ -118,19 +131,20 @@ __global__ __launch_bounds__(WG) void k_syn(SynArgs a)
 {
     using S = SynSpec<KIND>;
     constexpr int NOUT = S::NOUT, NCOL = S::NCOL, NPROD = S::NPROD;
-    // B chunk: [col][parity][32 kk][16 cols]; reused by the epilogue
-    constexpr int BS_DOUBLES = NCOL * 2 * CHUNK * 16;
-    constexpr int EP_DOUBLES = 4 * LEV_GROUP * 32 * 2;  // per wave: 8 lev x 32 theta x re/im
-    __shared__ double smem[BS_DOUBLES > EP_DOUBLES ? BS_DOUBLES : EP_DOUBLES];
+    // B chunk: [col][parity][32 kk][16 cols]
+    __shared__ double smem[NCOL * 2 * CHUNK * 16];
 
     const SynJob &job = a.job;
-    const int mc = blockIdx.x;
+    const int lb = xcd_logical(blockIdx.x, gridDim.x);
+    const int mc = lb % a.n_m;
+    const int ng = (job.nlev + LEV_GROUP - 1) / LEV_GROUP;
+    const int gy = (lb / a.n_m) % ng, gz = lb / (a.n_m * ng);
     const int m = mc * a.minc;
     const double dm = (double)m;
-    const int lev0 = blockIdx.y * LEV_GROUP;
+    const int lev0 = gy * LEV_GROUP;
     const int nlev_here = min(LEV_GROUP, job.nlev - lev0);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tt = blockIdx.z * 4 + wave;
+    const int tt = gz * 4 + wave;
     const bool active = tt < a.nt16;
     const int K0 = (a.l_max - m) / 2 + 1, K1 = (a.l_max - m + 1) / 2;
     const int Kq[2] = {(K0 + 3) / 4, (K1 + 3) / 4};
@@ -212,43 +226,27 @@ __global__ __launch_bounds__(WG) void k_syn(SynArgs a)
         }
     }
 
-    // epilogue: north = Sym + Anti, south = Sym - Anti; transpose through LDS
-    // into contiguous theta runs of F[lev][comp][m][theta]
-    __syncthreads();
-    double *ep = smem + wave * (LEV_GROUP * 32 * 2);
-    const int col = lane & 15, levl_c = col >> 1, ri = col & 1;
+    // epilogue: north = Sym + Anti, south = Sym - Anti, written to
+    // F[lev][comp][theta][m] (theta rows north/south interleaved)
+    if (active) {
+        const int col = lane & 15, levl = col >> 1, ri = col & 1;
+        if (levl < nlev_here) {
+            double *fl = job.fout + (size_t)(lev0 + levl) * job.fout_ld * 2;
 #pragma unroll
-    for (int o = 0; o < NOUT; ++o) {
-        if (active) {
+            for (int o = 0; o < NOUT; ++o) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                int row = (lane >> 4) + 4 * j;
-                int th = tt * 16 + row;
-                double sc = 1.0;
-                if (job.row_scale[o] && th < a.nh) sc = a.osin2[2 * th];
-                double n = (acc[o][0][j] + acc[o][1][j]) * sc;
-                double s = (acc[o][0][j] - acc[o][1][j]) * sc;
-                ep[(levl_c * 32 + 2 * row) * 2 + ri] = n;
-                ep[(levl_c * 32 + 2 * row + 1) * 2 + ri] = s;
-            }
-        }
-        __syncthreads();
-        if (active) {
-            // 8 levels x 32 theta complex = 256 complex per wave, 4 per lane
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                int e = r * 64 + lane;
-                int levl = e >> 5, thl = e & 31;
-                int theta = tt * 32 + thl;
-                if (levl < nlev_here && theta < a.n_theta) {
-                    size_t o2 = ((size_t)(lev0 + levl) * job.fout_ld +
-                                 ((size_t)job.out_comp[o] * a.n_m + mc) * a.n_theta + theta) * 2;
-                    double2 v = make_double2(ep[(levl * 32 + thl) * 2], ep[(levl * 32 + thl) * 2 + 1]);
-                    *reinterpret_cast<double2 *>(job.fout + o2) = v;
+                for (int j = 0; j < 4; ++j) {
+                    int th = tt * 16 + (lane >> 4) + 4 * j;
+                    if (th >= a.nh) continue;
+                    double sc = job.row_scale[o] ? a.osin2[2 * th] : 1.0;
+                    double n = (acc[o][0][j] + acc[o][1][j]) * sc;
+                    double sv = (acc[o][0][j] - acc[o][1][j]) * sc;
+                    size_t base = ((size_t)job.out_comp[o] * a.n_theta + 2 * th) * a.n_m + mc;
+                    fl[base * 2 + ri] = n;
+                    fl[(base + a.n_m) * 2 + ri] = sv;
                 }
             }
         }
-        __syncthreads();
     }
 }
 
@@ -263,13 +261,16 @@ __global__ __launch_bounds__(WG) void k_ana(AnaArgs a)
     __shared__ double smem[BS_DOUBLES > EP_DOUBLES ? BS_DOUBLES : EP_DOUBLES];
 
     const AnaJob &job = a.job;
-    const int mc = blockIdx.x;
+    const int lb = xcd_logical(blockIdx.x, gridDim.x);
+    const int mc = lb % a.n_m;
+    const int ng = (job.nlev + LEV_GROUP - 1) / LEV_GROUP;
+    const int gy = (lb / a.n_m) % ng, gz = lb / (a.n_m * ng);
     const int m = mc * a.minc;
     const double dm = (double)m;
-    const int lev0 = blockIdx.y * LEV_GROUP;
+    const int lev0 = gy * LEV_GROUP;
     const int nlev_here = min(LEV_GROUP, job.nlev - lev0);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int t = blockIdx.z * 4 + wave;  // pair tile: l = m + 32 t + 0..31
+    const int t = gz * 4 + wave;  // pair tile: l = m + 32 t + 0..31
     const int K0 = (a.l_max - m) / 2 + 1;
     const int ntile = (K0 + 15) / 16;
     const bool active = t < ntile;
@@ -317,11 +318,12 @@ __global__ __launch_bounds__(WG) void k_ana(AnaArgs a)
                 for (int k = 0; k < nsrc; ++k) {
                     int comp = job.in_comp[k];
                     if (comp < 0) continue;
-                    size_t o = ((size_t)lev * job.fin_ld + ((size_t)comp * a.n_m + mc) * a.n_theta + 2 * th) * 2;
+                    size_t o = ((size_t)lev * job.fin_ld + ((size_t)comp * a.n_theta + 2 * th) * a.n_m + mc) * 2;
+                    size_t o2 = o + (size_t)a.n_m * 2;
                     nr[k] = job.fin[o];
                     ni[k] = job.fin[o + 1];
-                    sr[k] = job.fin[o + 2];
-                    si[k] = job.fin[o + 3];
+                    sr[k] = job.fin[o2];
+                    si[k] = job.fin[o2 + 1];
                 }
             }
 #pragma unroll
@@ -419,7 +421,7 @@ int launch_syn(const Plan *p, const SynJob &job, hipStream_t s)
     a.nh = p->nh;
     a.n_theta = p->n_theta;
     a.nt16 = p->nt16;
-    dim3 grid(p->n_m, (job.nlev + LEV_GROUP - 1) / LEV_GROUP, (p->nt16 + 3) / 4);
+    dim3 grid((unsigned)(p->n_m * ((job.nlev + LEV_GROUP - 1) / LEV_GROUP) * ((p->nt16 + 3) / 4)));
     switch (job.kind) {
     case SYN_SCAL: k_syn<SYN_SCAL><<<grid, WG, 0, s>>>(a); break;
     case SYN_GRAD: k_syn<SYN_GRAD><<<grid, WG, 0, s>>>(a); break;
@@ -449,7 +451,7 @@ int launch_ana(const Plan *p, const AnaJob &job, hipStream_t s)
     a.n_theta = p->n_theta;
     a.nq4 = p->nq4;
     int ntile0 = ((p->l_max) / 2 + 1 + 15) / 16;  // pair tiles at m = 0
-    dim3 grid(p->n_m, (job.nlev + LEV_GROUP - 1) / LEV_GROUP, (ntile0 + 3) / 4);
+    dim3 grid((unsigned)(p->n_m * ((job.nlev + LEV_GROUP - 1) / LEV_GROUP) * ((ntile0 + 3) / 4)));
     switch (job.kind) {
     case ANA_SH: k_ana<ANA_SH><<<grid, WG, 0, s>>>(a); break;
     case ANA_SPHTOR: k_ana<ANA_SPHTOR><<<grid, WG, 0, s>>>(a); break;

@@ -178,7 +178,7 @@ int ifft_to(Plan *p, int comp0, int ncomp, int nlev, double *const *dst, size_t 
     size_t Fld = (size_t)18 * p->n_m * p->n_theta;
     for (int c = 0; c < ncomp; ++c) {
         const double *F = p->ws_F + (size_t)(comp0 + c) * p->n_m * p->n_theta * 2;
-        int rc = launch_ifft(p, F, Fld, 1, nlev, dst[c], ld_grid, p->stream);
+        int rc = launch_ifft(p, F, Fld, 1, 1, nlev, dst[c], ld_grid, 0, p->stream);
         if (rc) return rc;
     }
     return MHX_OK;
@@ -191,7 +191,7 @@ int fft_from(Plan *p, int ncomp, int nlev, const double *const *src, size_t ld_g
     size_t Fld = (size_t)10 * p->n_m * p->n_theta;
     for (int c = 0; c < ncomp; ++c) {
         double *F = p->ws_F2 + (size_t)c * p->n_m * p->n_theta * 2;
-        int rc = launch_fft(p, src[c], ld_grid, 1, nlev, osin2[c] ? 1u : 0u, F, Fld, p->stream);
+        int rc = launch_fft(p, src[c], ld_grid, 0, 1, nlev, osin2[c] ? 1u : 0u, F, Fld, p->stream);
         if (rc) return rc;
     }
     return MHX_OK;
@@ -848,16 +848,8 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
         j.out_comp[0] = q.comp0; j.out_comp[1] = q.comp0 + 1; j.out_comp[2] = q.comp0 + 2;
         if ((rc = launch_syn(P, j, s)) || (rc = mark())) return rc;
     }
-    size_t Fld = (size_t)18 * P->n_m * P->n_theta;
-    size_t gsz = (size_t)P->n_phi * P->n_theta;
-    double *G = grids ? grids : P->ws_G;
-    if ((rc = launch_ifft(P, P->ws_F, Fld, 18, nlev, G, 18 * gsz, s)) || (rc = mark())) return rc;
-    if ((rc = launch_nl_products(P, nlev, G, P->ws_G2, P->d_rad, LFfac, s)) || (rc = mark()))
-        return rc;
-    // 1/sin^2 for the horizontal components of the three vector products
-    unsigned mask = (1u << 1) | (1u << 2) | (1u << 4) | (1u << 5) | (1u << 7) | (1u << 8);
-    size_t F2ld = (size_t)10 * P->n_m * P->n_theta;
-    if ((rc = launch_fft(P, P->ws_G2, 10 * gsz, 10, nlev, mask, P->ws_F2, F2ld, s)) ||
+    // fused: inverse FFTs, grid products, forward FFTs; du/dr grids to ws_G
+    if ((rc = launch_nl_fused(P, nlev, P->ws_F, P->ws_F2, P->d_rad, LFfac, P->ws_G, grids, s)) ||
         (rc = mark()))
         return rc;
     for (int v = 0; v < 3; ++v) {
@@ -877,7 +869,7 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
         if ((rc = launch_ana(P, j, s)) || (rc = mark())) return rc;
     }
     if (P->timing) {
-        // launches: 6 synthesis, inverse FFT, products, forward FFT, 4 analysis
+        // launches: 6 synthesis, fused grid stage, 4 analysis
         MHX_HIP_CHECK(hipEventSynchronize(P->ev[nev - 1]));
         P->n_launch = nev - 1;
         for (int i = 0; i + 1 < nev; ++i) {
@@ -885,7 +877,7 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
             (void)hipEventElapsedTime(&ms, P->ev[i], P->ev[i + 1]);
             P->launch_ms[i] = ms;
         }
-        const int first[6] = {0, 6, 7, 8, 9, 13};
+        const int first[6] = {0, 6, 7, 7, 7, 11};
         for (int st = 0; st < 5; ++st) {
             double t = 0.0;
             for (int i = first[st]; i < first[st + 1] && i < P->n_launch; ++i) t += P->launch_ms[i];
