@@ -1,17 +1,20 @@
 """Benchmark of the nonlinear-term path on one MI355X.
 
 A step is one pass of the full nonlinear-term path over all radial levels of
-the workload (inverse transforms 5 x QST + scalar + gradient, grid products,
-forward transforms 3 x QST + scalar), with the coefficient inputs resident in
-HBM.  The metric is radial-level transform pairs per second: levels x steps
-/ time.  Prints one JSON line (last line of stdout).
+the workload (inverse transforms 5 x QST + scalar + gradient, fused FFTs and
+grid products, forward transforms 3 x QST + scalar), with the coefficient
+inputs resident in HBM.  The metric is radial-level transform pairs per
+second: levels x steps / time, for the split-precision form (scaled FP16 x3
+Legendre MFMA, FP32 grids); the FP64 plain form is timed beside it and the
+relative L2 error of every output field against it is reported.  Prints one
+JSON line (last line of stdout).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--nlev 97]
 """
 import argparse
 import json
 import os
-import subprocess
+import re
 import sys
 import tempfile
 import time
@@ -20,12 +23,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "SHT pairs/s at l_max=255 (split-prec vs FP64 GPU) at ≤1e-6 rel err vs FP64"
-PEAK_FP64_TFLOPS = 78.6
+PEAK = {"fp64": 78.6, "fp16": 2500.0}   # dense MFMA TFLOP/s
 PEAK_HBM_GBS = 8000.0
 
 # launch order of mhx_nl_step and the algorithmic work of each launch:
-# Legendre weights are flops per (N_h * lm_max) per level (FMA = 2), FFT and
-# product launches count bytes per level.
+# Legendre weights are flops per (N_h * lm_max) per level (FMA = 2), charged
+# the FP64-equivalent count in both forms; the fused grid stage counts bytes.
 LAUNCHES = [
     ("syn_scal_grad", "k_syn<SYN_SG>", "flop", 12),
     ("syn_qst_u", "k_syn<SYN_QST>", "flop", 20),
@@ -45,12 +48,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic(kind, w, nh, lm_max, n_theta, n_phi, n_m, nlev):
+def algorithmic(kind, w, nh, lm_max, n_theta, n_phi, n_m, nlev, word):
+    """flops, or bytes for the fused grid stage: 18 Fourier comps in, 10 out
+    (n_theta x n_m complex each) and the 3 du/dr grids (n_theta x n_phi)."""
     if kind == "flop":
         return w * nh * lm_max * nlev
-    # fused grid stage, bytes: 18 Fourier comps in, 10 out (n_theta x n_m
-    # complex128 each) and the 3 du/dr grids written (n_theta x n_phi float64)
-    return ((18 + 10) * n_theta * n_m * 16 + 3 * n_theta * n_phi * 8) * nlev
+    return ((18 + 10) * n_theta * n_m * 2 * word + 3 * n_theta * n_phi * word) * nlev
 
 
 def cpu_baseline(l_max, nlev_sample):
@@ -66,7 +69,6 @@ def cpu_baseline(l_max, nlev_sample):
                            threads=threads)
             wall = time.time() - t0
             line = open(out).readline()
-        import re
         kv = dict(re.findall(r"(\w+)=\s*([^\s]+)", line))
         mean_s = float(kv["mean_s"])
         return {"value": nlev_sample / mean_s, "unit": "radial-level transform pairs/s",
@@ -76,15 +78,79 @@ def cpu_baseline(l_max, nlev_sample):
     # without the reference build: the FP64 oracle, single thread, one level
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from magic_amd import workload
-    from nl_reference import make_inputs, oracle_step, radial
+    from nl_reference import oracle_step, radial
     nt, nph = workload.grid_size(l_max)
     o = oracle.Plan(l_max, nt, nph)
-    inputs = make_inputs(l_max, 1)
+    inputs = workload.nl_inputs(l_max, 1)
     t0 = time.time()
     oracle_step(o, inputs, *radial(1), [l_max])
     dt = time.time() - t0
     return {"value": 1.0 / dt, "unit": "radial-level transform pairs/s", "cores": 1,
             "kind": "port", "sample": "1 radial level of the l_max=%d nonlinear step" % l_max}
+
+
+def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut):
+    import numpy as np
+    import torch
+
+    from magic_amd import sht, workload
+    nt, nph = workload.grid_size(L)
+    p = sht.Plan(L, nt, nph, precision=precision, max_levels=nlev)
+    dev = tin[0].device
+    tout = [torch.zeros((nlev, p.lm_max), dtype=torch.complex128, device=dev) for _ in range(10)]
+    for _ in range(warmup):
+        sht.nl_step(p, tin, tout, rad[0], rad[1], rad[2], rad[3], lcut)
+    torch.cuda.synchronize()
+    p.set_timing(True)
+    lt = np.zeros(len(LAUNCHES))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sht.nl_step(p, tin, tout, rad[0], rad[1], rad[2], rad[3], lcut)
+        lt += np.array(p.launch_times())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    p.set_timing(False)
+    lt /= steps
+    word = 4 if precision == sht.PREC_SPLIT16 else 8
+    nh, lm_max, n_m = nt // 2, p.lm_max, p.n_m_max
+    per_kernel = {}
+    for (name, kern, kind, w), t in zip(LAUNCHES, lt):
+        a = algorithmic(kind, w, nh, lm_max, nt, nph, n_m, nlev, word)
+        d = per_kernel.setdefault(kern, {"ms": 0.0, "n": 0, "work": 0.0, "kind": kind})
+        d["ms"] += t
+        d["n"] += 1
+        d["work"] += a
+    kernels = {k: {"avg_ms": v["ms"] / v["n"], "launches": v["n"],
+                   ("TFLOP/s" if v["kind"] == "flop" else "GB/s"):
+                       (v["work"] / (v["ms"] * 1e-3) / (1e12 if v["kind"] == "flop" else 1e9))}
+               for k, v in per_kernel.items()}
+    flop_ms = sum(t for (_, _, k, _), t in zip(LAUNCHES, lt) if k == "flop")
+    total_flop = sum(algorithmic(k, w, nh, lm_max, nt, nph, n_m, nlev, word)
+                     for _, _, k, w in LAUNCHES if k == "flop")
+    outs = [t.cpu().numpy() for t in tout]
+    p.close()
+    return {"pairs": nlev * steps / dt, "ms_per_step": 1e3 * dt / steps,
+            "per_kernel": per_kernel, "kernels": kernels,
+            "legendre_tflops": total_flop / (flop_ms * 1e-3) / 1e12, "outs": outs}
+
+
+def roofline(form, peak_key):
+    pk = form["per_kernel"]
+    dom = max(pk, key=lambda k: pk[k]["ms"])
+    d = pk[dom]
+    avg_ms = d["ms"] / d["n"]
+    work = d["work"] / d["n"]
+    if d["kind"] == "flop":
+        achieved = work / (avg_ms * 1e-3) / 1e12
+        r = {"bound": "mfma", "achieved": achieved, "peak": PEAK[peak_key], "unit": "TFLOP/s",
+             "frac": achieved / PEAK[peak_key]}
+    else:
+        achieved = work / (avg_ms * 1e-3) / 1e9
+        r = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+             "frac": achieved / PEAK_HBM_GBS}
+    r.update({"kernel": dom, "avg_launch_ms": avg_ms, "traffic": None})
+    return r
 
 
 def main():
@@ -96,6 +162,7 @@ def main():
     ap.add_argument("--nlev", type=int, default=97)
     ap.add_argument("--cpu-levels", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form", choices=["both", "split16", "fp64"], default="both")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         log("bench.py measures one GPU only: the transform path is independent per radial "
@@ -113,65 +180,33 @@ def main():
     dev = torch.device("cuda:0")
     L, nlev = args.l_max, args.nlev
     nt, nph = workload.grid_size(L)
-    log("plan l_max=%d %dx%d nlev=%d" % (L, nt, nph, nlev))
-    p = sht.Plan(L, nt, nph, max_levels=nlev)
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from nl_reference import make_inputs, radial
-    inputs = make_inputs(L, nlev)
-    rad = radial(nlev)
+    log("l_max=%d %dx%d nlev=%d" % (L, nt, nph, nlev))
+    inputs = workload.nl_inputs(L, nlev)
+    r = np.linspace(0.55, 1.45, nlev)
+    rad = (1.0 / r ** 2, 1.0 / r ** 4, 1.0 / (1.0 + 0.1 * r), 0.8)
     lcut = [L] * nlev
     tin = [torch.from_numpy(x).to(dev) for x in inputs]
-    tout = [torch.zeros((nlev, p.lm_max), dtype=torch.complex128, device=dev) for _ in range(10)]
     torch.cuda.synchronize()
 
-    for _ in range(args.warmup):
-        sht.nl_step(p, tin, tout, rad[0], rad[1], rad[2], rad[3], lcut)
-    torch.cuda.synchronize()
-    log("warmup done")
-
-    p.set_timing(True)
-    lt = np.zeros(len(LAUNCHES))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sht.nl_step(p, tin, tout, rad[0], rad[1], rad[2], rad[3], lcut)
-        lt += np.array(p.launch_times())
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    p.set_timing(False)
-    lt /= args.steps
-    ms_per_step = 1e3 * dt / args.steps
-    pairs = nlev * args.steps / dt
-    log("timed %d steps: %.3f ms/step, %.1f pairs/s" % (args.steps, ms_per_step, pairs))
-
-    # per-kernel averages and the dominant kernel
-    nh, lm_max, n_m = nt // 2, p.lm_max, p.n_m_max
-    per_kernel = {}
-    for (name, kern, kind, w), t in zip(LAUNCHES, lt):
-        a = algorithmic(kind, w, nh, lm_max, nt, nph, n_m, nlev)
-        d = per_kernel.setdefault(kern, {"ms": 0.0, "n": 0, "work": 0.0, "kind": kind})
-        d["ms"] += t
-        d["n"] += 1
-        d["work"] += a
-    dom = max(per_kernel, key=lambda k: per_kernel[k]["ms"])
-    dk = per_kernel[dom]
-    avg_ms = dk["ms"] / dk["n"]
-    work = dk["work"] / dk["n"]
-    if dk["kind"] == "flop":
-        achieved = work / (avg_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS}
-    else:
-        achieved = work / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM_GBS}
-    roof.update({"kernel": dom, "avg_launch_ms": avg_ms, "traffic": None})
-    total_flop = sum(algorithmic(k, w, nh, lm_max, nt, nph, n_m, nlev)
-                     for _, _, k, w in LAUNCHES if k == "flop")
-    kernels = {k: {"avg_ms": v["ms"] / v["n"], "launches": v["n"],
-                   ("TFLOP/s" if v["kind"] == "flop" else "GB/s"):
-                       (v["work"] / (v["ms"] * 1e-3) / (1e12 if v["kind"] == "flop" else 1e9))}
-               for k, v in per_kernel.items()}
+    forms = {}
+    if args.form in ("both", "fp64"):
+        forms["fp64"] = run_form(sht.PREC_FP64, L, nlev, args.steps, args.warmup, tin, rad, lcut)
+        log("fp64: %.3f ms/step, %.1f pairs/s" % (forms["fp64"]["ms_per_step"],
+                                                  forms["fp64"]["pairs"]))
+    if args.form in ("both", "split16"):
+        forms["split16"] = run_form(sht.PREC_SPLIT16, L, nlev, args.steps, args.warmup, tin,
+                                    rad, lcut)
+        log("split16: %.3f ms/step, %.1f pairs/s" % (forms["split16"]["ms_per_step"],
+                                                     forms["split16"]["pairs"]))
+    main_key = "split16" if "split16" in forms else "fp64"
+    main = forms[main_key]
+    errs = None
+    if "fp64" in forms and "split16" in forms:
+        errs = {}
+        for k, name in enumerate(sht.NL_OUTPUTS):
+            a, b = forms["split16"]["outs"][k], forms["fp64"]["outs"][k]
+            errs[name] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        log("max rel L2 error split vs fp64: %.3e" % max(errs.values()))
 
     cpu = None
     if not args.no_cpu:
@@ -182,28 +217,35 @@ def main():
             log("cpu baseline failed: %r" % (e,))
     res = {
         "metric": METRIC,
-        "value": pairs,
+        "value": main["pairs"],
         "unit": "radial-level transform pairs/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": main["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f16x3-f32acc" if main_key == "split16" else "f64",
         "data": "synthetic",
         "config": {"workload": "full nonlinear step (5 QST + scalar + gradient inverse, "
-                               "fused FFTs + grid products, 3 QST + scalar forward), "
-                               "FP64 plain form",
+                               "fused FFTs + grid products, 3 QST + scalar forward)",
                    "l_max": L, "n_theta": nt, "n_phi": nph, "n_r": nlev,
-                   "precision": "fp64"},
-        "legendre_tflops": total_flop / (sum(t for (_, _, k, _), t in zip(LAUNCHES, lt)
-                                             if k == "flop") * 1e-3) / 1e12,
-        "kernels": kernels,
-        "roofline": roof,
+                   "precision": main_key},
+        "legendre_tflops_fp64_equiv": main["legendre_tflops"],
+        "kernels": main["kernels"],
+        "roofline": roofline(main, "fp16" if main_key == "split16" else "fp64"),
         "cpu_baseline": cpu,
     }
+    if "fp64" in forms and main_key != "fp64":
+        f = forms["fp64"]
+        res["fp64_form"] = {"value": f["pairs"], "ms_per_step": f["ms_per_step"],
+                            "legendre_tflops": f["legendre_tflops"], "kernels": f["kernels"],
+                            "roofline": roofline(f, "fp64")}
+        res["speedup_vs_fp64"] = main["pairs"] / f["pairs"]
+    if errs is not None:
+        res["rel_l2_err_vs_fp64"] = errs
+        res["max_rel_l2_err_vs_fp64"] = max(errs.values())
     print(json.dumps(res))
 
 

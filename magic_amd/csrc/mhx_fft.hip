@@ -9,8 +9,10 @@
 // the unnormalised Hermitian synthesis using only the real parts of modes 0
 // and N/2 (src/fft.f90:386-398).
 //
-// Fourier rows are contiguous: F[lev][comp][theta][m], m < n_m (modes up to
-// N/2 beyond n_m are zero, as src/shtransforms.f90:224-232).
+// Fourier layouts: the inverse side F[lev][comp][m][theta] (written in
+// contiguous theta runs by the synthesis), the forward side
+// F2[lev][comp][theta][m] (written in contiguous rows by the FFT kernels);
+// m < n_m, modes up to N/2 beyond n_m are zero (src/shtransforms.f90:224-232).
 #include "mhx_internal.h"
 
 namespace mhx {
@@ -156,17 +158,17 @@ __device__ C *fft_pp(C *a, C *b, int G, const FFTArgs &f, const C *tw, int sign)
 }
 
 // Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/N} (X_k - conj X_{M-k}), real
-// parts only for X_0 and X_M; X is read from a global Fourier row (zero for
-// k >= n_m), Z written to LDS row z
+// parts only for X_0 and X_M; X_k is read from global memory at X[k xs]
+// (zero for k >= n_m), Z written to LDS row z
 template <typename C>
-__device__ void irfft_pre(const C *__restrict__ X, int n_m, C *z, int M, const C *twr, int tid0,
-                          int nthr)
+__device__ void irfft_pre(const C *__restrict__ X, size_t xs, int n_m, C *z, int M, const C *twr,
+                          int tid0, int nthr)
 {
     typedef decltype(C::x) T;
     for (int k = tid0; k <= M / 2; k += nthr) {
         int k2 = M - k;
-        C a = (k < n_m) ? X[k] : mk<C>(T(0), T(0));
-        C c = (k2 < n_m) ? X[k2] : mk<C>(T(0), T(0));
+        C a = (k < n_m) ? X[(size_t)k * xs] : mk<C>(T(0), T(0));
+        C c = (k2 < n_m) ? X[(size_t)k2 * xs] : mk<C>(T(0), T(0));
         if (k == 0) {
             a.y = T(0);
             c.y = T(0);
@@ -229,14 +231,14 @@ __global__ void k_ifft_rows(FFTArgs f, const T *__restrict__ F, size_t F_ld, int
     int nrows = min(TB, n_theta - th0);
     load_twiddles(s_tw, s_twr, f);
     __syncthreads();
-    // one thread team per row for the pre-step: threads split over rows
-    const C *Frow0 = reinterpret_cast<const C *>(F) + (size_t)lev * F_ld +
-                     ((size_t)comp * n_theta + th0) * n_m;
+    // inverse-side Fourier layout F[lev][comp][m][theta]; one thread team
+    // per theta row for the pre-step
+    const C *F0 = reinterpret_cast<const C *>(F) + (size_t)lev * F_ld + (size_t)comp * n_m * n_theta + th0;
     {
         int per = blockDim.x / TB;  // threads per row
         int r = threadIdx.x / per, t = threadIdx.x % per;
         if (r < TB) {
-            if (r < nrows) irfft_pre(Frow0 + (size_t)r * n_m, n_m, A + r * M, M, s_twr, t, per);
+            if (r < nrows) irfft_pre(F0 + r, (size_t)n_theta, n_m, A + r * M, M, s_twr, t, per);
             else
                 for (int k = t; k < M; k += per) A[r * M + k] = mk<C>(T(0), T(0));
         }
@@ -300,9 +302,11 @@ __global__ void k_fft_rows(FFTArgs f, const T *__restrict__ g, size_t g_ld_lev, 
 // and forward FFTs of the 10 products.  Grid values live in registers (P
 // points per thread); only Fourier rows are read and written.
 //
-// Fin comps:  vr vt vp | cvr cvt cvp | dvr dvt dvp | br bt bp | cbr cbt cbp |
+// Fin comps (layout [lev][comp][m][theta]):
+//             vr vt vp | cvr cvt cvp | dvr dvt dvp | br bt bp | cbr cbt cbp |
 //             s dsdt dsdp
-// Fout comps: Advr Advt Advp | VSr VSt VSp | VxBr VxBt VxBp | uGs
+// Fout (layout [lev][comp][theta][m]):
+//             Advr Advt Advp | VSr VSt VSp | VxBr VxBt VxBp | uGs
 // du/dr grids (not consumed by the products) are written to dgrid
 // [lev][3][theta][phi]; with full_grids != nullptr all 18 inverse grids are
 // also written in the reference layout [lev][comp][phi][theta].
@@ -321,7 +325,15 @@ __global__ void k_nl_fused(FFTArgs f, const T *__restrict__ Fin, T *__restrict__
     C *s_twr = s_tw + M;
     C *A = s_twr + M + 1;
     C *B = A + (size_t)G * M;
-    const int th = blockIdx.x, lev = blockIdx.y;
+    // XCD-aware order: consecutive theta rows (which share the cache lines
+    // of F[lev][comp][m][theta]) run on the same XCD at the same time
+    int lb;
+    {
+        const int nx = 8, nb = gridDim.x, b = blockIdx.x;
+        int per = nb / nx, rem = nb % nx, x = b % nx, i = b / nx;
+        lb = (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + i;
+    }
+    const int th = lb % n_theta, lev = lb / n_theta;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const T o2 = T(osin2[th]);
     const T or2 = T(rad[lev]), or4 = T(rad[nlev + lev]), orho1 = T(rad[2 * nlev + lev]);
@@ -329,8 +341,8 @@ __global__ void k_nl_fused(FFTArgs f, const T *__restrict__ Fin, T *__restrict__
     load_twiddles(s_tw, s_twr, f);
     __syncthreads();
 
-    const size_t rowsz = (size_t)n_m;                   // complex per row
-    const C *Fi = reinterpret_cast<const C *>(Fin) + (size_t)lev * 18 * n_theta * rowsz;
+    const size_t rowsz = (size_t)n_m;                   // complex per forward row
+    const C *Fi = reinterpret_cast<const C *>(Fin) + (size_t)lev * 18 * n_theta * rowsz + th;
     C *Fo = reinterpret_cast<C *>(Fout) + (size_t)lev * 10 * n_theta * rowsz;
 
     T u[3][PMAX], adv[3][PMAX], bb[3][PMAX], vxb[3][PMAX], vs[3][PMAX], ugs[PMAX];
@@ -344,8 +356,8 @@ __global__ void k_nl_fused(FFTArgs f, const T *__restrict__ Fin, T *__restrict__
             int per = nthr / gn;
             int r = tid / per, t = tid % per;
             if (r < gn)
-                irfft_pre(Fi + ((size_t)(c0 + g0 + r) * n_theta + th) * rowsz, n_m, A + r * M, M,
-                          s_twr, t, per);
+                irfft_pre(Fi + (size_t)(c0 + g0 + r) * n_m * n_theta, (size_t)n_theta, n_m,
+                          A + r * M, M, s_twr, t, per);
             __syncthreads();
             C *res = fft_pp(A, B, gn, f, s_tw, +1);
 #pragma unroll
@@ -562,25 +574,41 @@ int fused_threads(int n_phi)
     return nt > 1024 ? 1024 : nt;
 }
 
-int launch_nl_fused(const Plan *p, int nlev, const double *Fin, double *Fout, const double *rad,
-                    double LFfac, double *dgrid, double *full_grids, hipStream_t s)
+template <typename T>
+int launch_fused_t(const Plan *p, int nlev, const void *Fin, void *Fout, const double *rad,
+                   double LFfac, void *dgrid, void *full_grids, hipStream_t s)
 {
-    if (nlev <= 0) return MHX_OK;
     FFTArgs f = make_args(p);
+    if (sizeof(T) == 4) {
+        f.tw = p->d_tw32;
+        f.twr = p->d_twr32;
+    }
     int nthr = fused_threads(p->n_phi);
     if (nthr * PMAX < p->n_phi) return fail(MHX_E_UNSUPPORTED, "n_phi too large for the fused stage");
-    int G = ((size_t)(6 * p->M + 2 * p->M + 1) * sizeof(double2) <= 64 * 1024) ? 3 : 1;
-    size_t shm = ((size_t)2 * G * p->M + 2 * p->M + 1) * sizeof(double2);
+    const size_t cs = 2 * sizeof(T);
+    int G = ((size_t)(6 * p->M + 2 * p->M + 1) * cs <= 64 * 1024) ? 3 : 1;
+    size_t shm = ((size_t)2 * G * p->M + 2 * p->M + 1) * cs;
     if (shm > 64 * 1024) return fail(MHX_E_UNSUPPORTED, "n_phi too large for the fused stage (LDS)");
-    dim3 grid((unsigned)p->n_theta, (unsigned)nlev);
+    dim3 grid((unsigned)(p->n_theta * nlev));
+    const T *fi = static_cast<const T *>(Fin);
+    T *fo = static_cast<T *>(Fout), *dg = static_cast<T *>(dgrid), *fg = static_cast<T *>(full_grids);
     if (G == 3)
-        k_nl_fused<double, 3><<<grid, nthr, shm, s>>>(f, Fin, Fout, p->n_m, p->n_theta, nlev,
-                                                       p->d_osin2, rad, LFfac, dgrid, full_grids);
+        k_nl_fused<T, 3><<<grid, nthr, shm, s>>>(f, fi, fo, p->n_m, p->n_theta, nlev, p->d_osin2,
+                                                  rad, LFfac, dg, fg);
     else
-        k_nl_fused<double, 1><<<grid, nthr, shm, s>>>(f, Fin, Fout, p->n_m, p->n_theta, nlev,
-                                                       p->d_osin2, rad, LFfac, dgrid, full_grids);
+        k_nl_fused<T, 1><<<grid, nthr, shm, s>>>(f, fi, fo, p->n_m, p->n_theta, nlev, p->d_osin2,
+                                                  rad, LFfac, dg, fg);
     MHX_HIP_CHECK(hipGetLastError());
     return MHX_OK;
+}
+
+int launch_nl_fused(const Plan *p, int nlev, const void *Fin, void *Fout, const double *rad,
+                    double LFfac, void *dgrid, void *full_grids, hipStream_t s)
+{
+    if (nlev <= 0) return MHX_OK;
+    if (p->precision == MHX_PREC_SPLIT16)
+        return launch_fused_t<float>(p, nlev, Fin, Fout, rad, LFfac, dgrid, full_grids, s);
+    return launch_fused_t<double>(p, nlev, Fin, Fout, rad, LFfac, dgrid, full_grids, s);
 }
 
 int launch_axi(const Plan *p, const double *fl, double *f, int lcut, int tor, double *f2,

@@ -105,13 +105,22 @@ struct Plan {
     double *d_theta = nullptr, *d_gauss = nullptr; // nh
     double *d_tw = nullptr;         // M complex: exp(-2 pi i k/M)
     double *d_twr = nullptr;        // (M+1) complex: exp(-2 pi i k/n_phi)
+    float *d_tw32 = nullptr, *d_twr32 = nullptr;  // the same in FP32
+
+    // split-precision tables (MHX_PREC_SPLIT16): FP16 hi/lo pairs in the
+    // 16x16x32 MFMA operand layout, with power-of-2 row factors
+    int nq32 = 0;                   // theta chunks of 32 (northern half)
+    size_t syn16_size = 0, ana16_size = 0;   // halves
+    _Float16 *d_syn16_P = nullptr, *d_syn16_dP = nullptr;
+    _Float16 *d_ana16_P = nullptr, *d_ana16_dP = nullptr;
+    size_t *d_syn16_off = nullptr, *d_ana16_off = nullptr, *d_ana16_soff = nullptr;
+    float *d_syn16_rs = nullptr, *d_ana16_rs = nullptr;
 
     // workspaces
     int ws_levels = 0;
     double *ws_F = nullptr;     size_t ws_F_bytes = 0;    // Fourier, inverse side
     double *ws_G = nullptr;     size_t ws_G_bytes = 0;    // grids
     double *ws_F2 = nullptr;    size_t ws_F2_bytes = 0;   // Fourier, forward side
-    double *ws_G2 = nullptr;    size_t ws_G2_bytes = 0;   // product grids
     double *ws_C = nullptr;     size_t ws_C_bytes = 0;    // staged coefficients
     int *d_lm2l = nullptr;      // lm_max
     int *d_lcut = nullptr;      int d_lcut_n = 0;
@@ -130,11 +139,15 @@ struct Plan {
 
 // ------------------------------------------------------------- launchers
 int build_tables(Plan *p);
+int build_split_tables(Plan *p);
+int launch_syn16(const Plan *p, const SynJob &job, hipStream_t s);
+int launch_ana16(const Plan *p, const AnaJob &job, hipStream_t s);
 int launch_syn(const Plan *p, const SynJob &job, hipStream_t s);
 int launch_ana(const Plan *p, const AnaJob &job, hipStream_t s);
-// Fourier rows F[lev][comp][theta][m] (m < n_m, complex) <-> grids in the
-// reference layout (theta fastest).  F points at the first comp; F_ld_lev is
-// the level stride in complex elements; grid comp c of level l starts at
+// Fourier data (complex, m < n_m): inverse side F[lev][comp][m][theta],
+// forward side F2[lev][comp][theta][m]; grids in the reference layout (theta
+// fastest).  F points at the first comp; F_ld_lev is the level stride in
+// complex elements; grid comp c of level l starts at
 // grid + l*grid_ld_lev + c*grid_ld_comp (doubles).
 int launch_ifft(const Plan *p, const double *F, size_t F_ld_lev, int F_ncomp, int ncomp,
                 int nlev, double *grid, size_t grid_ld_lev, size_t grid_ld_comp,
@@ -146,8 +159,9 @@ int launch_fft(const Plan *p, const double *grid, size_t grid_ld_lev, size_t gri
 // fused grid stage of the nonlinear step: Fin (18 comps per level) -> Fout
 // (10 comps per level); du/dr grids to dgrid [lev][3][theta][phi]; all 18
 // inverse grids in the reference layout to full_grids when not null
-int launch_nl_fused(const Plan *p, int nlev, const double *Fin, double *Fout,
-                    const double *rad, double LFfac, double *dgrid, double *full_grids,
+// (FP32 Fourier data and grids for a split-precision plan)
+int launch_nl_fused(const Plan *p, int nlev, const void *Fin, void *Fout,
+                    const double *rad, double LFfac, void *dgrid, void *full_grids,
                     hipStream_t s);
 int launch_axi(const Plan *p, const double *fl, double *f, int lcut, int tor,
                double *f2, hipStream_t s);

@@ -100,8 +100,8 @@ __device__ inline d4 mfma64(double a, double b, d4 c)
 // Blocks are dispatched round-robin over the 8 XCDs (each with its own L2).
 // Remap the linear block id so that each XCD works through a contiguous range
 // of logical blocks; with the order index m fastest, the blocks resident on
-// one XCD at a time cover neighbouring m, whose Fourier values share cache
-// lines of the row-contiguous F[lev][comp][theta][m] layout.
+// one XCD at a time cover neighbouring m, whose values share cache lines of
+// the row-contiguous forward-side layout F2[lev][comp][theta][m].
 __device__ inline int xcd_logical(int b, int nb)
 {
     const int nx = 8;
@@ -131,8 +131,10 @@ __global__ __launch_bounds__(WG) void k_syn(SynArgs a)
 {
     using S = SynSpec<KIND>;
     constexpr int NOUT = S::NOUT, NCOL = S::NCOL, NPROD = S::NPROD;
-    // B chunk: [col][parity][32 kk][16 cols]
-    __shared__ double smem[NCOL * 2 * CHUNK * 16];
+    // B chunk: [col][parity][32 kk][16 cols]; reused by the epilogue
+    constexpr int BS_DOUBLES = NCOL * 2 * CHUNK * 16;
+    constexpr int EP_DOUBLES = 4 * LEV_GROUP * 32 * 2;  // per wave: 8 lev x 32 theta x re/im
+    __shared__ double smem[BS_DOUBLES > EP_DOUBLES ? BS_DOUBLES : EP_DOUBLES];
 
     const SynJob &job = a.job;
     const int lb = xcd_logical(blockIdx.x, gridDim.x);
@@ -155,46 +157,62 @@ __global__ __launch_bounds__(WG) void k_syn(SynArgs a)
     for (int o = 0; o < NOUT; ++o)
         for (int s = 0; s < 2; ++s) acc[o][s] = d4{0.0, 0.0, 0.0, 0.0};
 
-    const int nchunk = (Kq[0] + KQC - 1) / KQC;
-    for (int c = 0; c < nchunk; ++c) {
-        __syncthreads();
-        // stage B: (parity, kk, level) triples
-        for (int idx = threadIdx.x; idx < 2 * CHUNK * LEV_GROUP; idx += WG) {
+    // B staging is software-pipelined: the input coefficients of chunk c+1
+    // are loaded into registers while the MFMAs of chunk c run, then built
+    // into columns in LDS.  The table fragments of a chunk are all issued
+    // before its MFMAs.
+    constexpr int NE = (2 * CHUNK * LEV_GROUP) / WG;  // staging entries per thread
+    double pxr[NE][MAX_SRC], pxi[NE][MAX_SRC];
+    double pll[NE];
+    bool pval[NE];
+    auto load_B = [&](int c) {
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            int idx = threadIdx.x + e * WG;
             int levl = idx % LEV_GROUP;
             int kkl = (idx / LEV_GROUP) % CHUNK;
             int par = idx / (LEV_GROUP * CHUNK);
-            int kk = c * CHUNK + kkl;
-            int l = m + par + 2 * kk;
+            int l = m + par + 2 * (c * CHUNK + kkl);
             int lev = lev0 + levl;
             bool valid = (levl < nlev_here) && (l <= a.l_max);
             if (valid) valid = l <= job.lcut[lev];
-            double xr[MAX_SRC], xi[MAX_SRC];
-            if (valid) {
-                size_t o = ((size_t)lev * job.src_ld + (size_t)(l0 + l - m)) * 2;
+            pval[e] = valid;
+            size_t o = ((size_t)lev * job.src_ld + (size_t)(l0 + l - m)) * 2;
 #pragma unroll
-                for (int sidx = 0; sidx < MAX_SRC; ++sidx) {
-                    if (job.src[sidx]) {
-                        xr[sidx] = job.src[sidx][o];
-                        xi[sidx] = job.src[sidx][o + 1];
-                    } else {
-                        xr[sidx] = 0.0;
-                        xi[sidx] = 0.0;
-                    }
-                }
+            for (int sidx = 0; sidx < MAX_SRC; ++sidx) {
+                double2 v = make_double2(0.0, 0.0);
+                if (valid && job.src[sidx]) v = *reinterpret_cast<const double2 *>(job.src[sidx] + o);
+                pxr[e][sidx] = v.x;
+                pxi[e][sidx] = v.y;
             }
-            double ll = (double)l * (double)(l + 1);
+        }
+    };
+    auto store_B = [&]() {
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            int idx = threadIdx.x + e * WG;
+            int levl = idx % LEV_GROUP;
+            int kkl = (idx / LEV_GROUP) % CHUNK;
+            int par = idx / (LEV_GROUP * CHUNK);
+            int lev = lev0 + levl;
 #pragma unroll
             for (int col = 0; col < NCOL; ++col) {
                 double re = 0.0, im = 0.0;
-                if (valid) {
+                if (pval[e]) {
                     const ColOp &op = job.col[col];
-                    double g = op.gkind == G_ONE ? 1.0 : (op.gkind == G_LL ? ll : job.or2[lev] * ll);
-                    re = g * xr[op.src0];
-                    im = g * xi[op.src0];
-                    if (op.src1 >= 0) {
-                        re += op.hcoef * xr[op.src1];
-                        im += op.hcoef * xi[op.src1];
+                    double g = 1.0;
+                    if (op.gkind != G_ONE) {
+                        double ll = pll[e];
+                        g = (op.gkind == G_LL) ? ll : job.or2[lev] * ll;
                     }
+                    double xr0 = 0.0, xi0 = 0.0, xr1 = 0.0, xi1 = 0.0;
+#pragma unroll
+                    for (int sidx = 0; sidx < MAX_SRC; ++sidx) {
+                        if (op.src0 == sidx) { xr0 = pxr[e][sidx]; xi0 = pxi[e][sidx]; }
+                        if (op.src1 == sidx) { xr1 = pxr[e][sidx]; xi1 = pxi[e][sidx]; }
+                    }
+                    re = g * xr0 + op.hcoef * xr1;
+                    im = g * xi0 + op.hcoef * xi1;
                     apply_factor(re, im, op.ipow, op.mpow, dm, op.scale);
                 }
                 double *b = smem + (((col * 2 + par) * CHUNK + kkl) * 16 + levl * 2);
@@ -202,51 +220,103 @@ __global__ __launch_bounds__(WG) void k_syn(SynArgs a)
                 b[1] = im;
             }
         }
-        __syncthreads();
+    };
+
+    const int nchunk = (Kq[0] + KQC - 1) / KQC;
+    auto set_ll = [&](int c) {
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            int idx = threadIdx.x + e * WG;
+            int kkl = (idx / LEV_GROUP) % CHUNK;
+            int par = idx / (LEV_GROUP * CHUNK);
+            double l = (double)(m + par + 2 * (c * CHUNK + kkl));
+            pll[e] = l * (l + 1.0);
+        }
+    };
+    load_B(0);
+    set_ll(0);
+    store_B();
+    __syncthreads();
+    const double *tP[2], *tdP[2];
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+        tP[par] = a.sP + a.off[2 * mc + par] + (size_t)tt * 64 + lane;
+        tdP[par] = a.sdP + a.off[2 * mc + par] + (size_t)tt * 64 + lane;
+    }
+    const size_t qstride = (size_t)a.nt16 * 64;
+    for (int c = 0; c < nchunk; ++c) {
+        double aP[2][KQC], adP[2][KQC];
+        int nq[2];
+#pragma unroll
+        for (int par = 0; par < 2; ++par) {
+            nq[par] = min(KQC, Kq[par] - c * KQC);
+#pragma unroll
+            for (int q = 0; q < KQC; ++q) {
+                bool ok = active && q < nq[par];
+                size_t kq = (size_t)(c * KQC + q);
+                aP[par][q] = ok ? tP[par][kq * qstride] : 0.0;
+                adP[par][q] = ok ? tdP[par][kq * qstride] : 0.0;
+            }
+        }
+        if (c + 1 < nchunk) load_B(c + 1);
         if (active) {
 #pragma unroll
-            for (int par = 0; par < 2; ++par) {
-                int nq = min(KQC, Kq[par] - c * KQC);
-                const double *tP = a.sP + a.off[2 * mc + par] + (size_t)tt * 64 + lane;
-                const double *tdP = a.sdP + a.off[2 * mc + par] + (size_t)tt * 64 + lane;
-                for (int q = 0; q < nq; ++q) {
-                    size_t kq = (size_t)(c * KQC + q);
-                    double aP = tP[kq * a.nt16 * 64];
-                    double adP = tdP[kq * a.nt16 * 64];
+            for (int par = 0; par < 2; ++par)
+#pragma unroll
+                for (int q = 0; q < KQC; ++q) {
+                    if (q >= nq[par]) break;
 #pragma unroll
                     for (int pr = 0; pr < NPROD; ++pr) {
                         const Prod P = S::prod[pr];
                         double b = smem[((P.col * 2 + par) * CHUNK + q * 4 + (lane >> 4)) * 16 + (lane & 15)];
                         // P-table: even parity -> Sym (0); dP-table: even -> Anti (1)
                         int tgt = (P.tab == 0) ? par : (1 - par);
-                        acc[P.acc][tgt] = mfma64(P.tab == 0 ? aP : adP, b, acc[P.acc][tgt]);
+                        acc[P.acc][tgt] = mfma64(P.tab == 0 ? aP[par][q] : adP[par][q], b, acc[P.acc][tgt]);
                     }
                 }
-            }
         }
+        __syncthreads();
+        if (c + 1 < nchunk) {
+            set_ll(c + 1);
+            store_B();
+        }
+        __syncthreads();
     }
 
-    // epilogue: north = Sym + Anti, south = Sym - Anti, written to
-    // F[lev][comp][theta][m] (theta rows north/south interleaved)
-    if (active) {
-        const int col = lane & 15, levl = col >> 1, ri = col & 1;
-        if (levl < nlev_here) {
-            double *fl = job.fout + (size_t)(lev0 + levl) * job.fout_ld * 2;
+    // epilogue: north = Sym + Anti, south = Sym - Anti; transposed through
+    // LDS into contiguous runs of 32 theta rows of F[lev][comp][m][theta]
+    __syncthreads();
+    double *ep = smem + wave * (LEV_GROUP * 32 * 2);
+    const int col = lane & 15, levl_c = col >> 1, ri = col & 1;
 #pragma unroll
-            for (int o = 0; o < NOUT; ++o) {
+    for (int o = 0; o < NOUT; ++o) {
+        if (active) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    int th = tt * 16 + (lane >> 4) + 4 * j;
-                    if (th >= a.nh) continue;
-                    double sc = job.row_scale[o] ? a.osin2[2 * th] : 1.0;
-                    double n = (acc[o][0][j] + acc[o][1][j]) * sc;
-                    double sv = (acc[o][0][j] - acc[o][1][j]) * sc;
-                    size_t base = ((size_t)job.out_comp[o] * a.n_theta + 2 * th) * a.n_m + mc;
-                    fl[base * 2 + ri] = n;
-                    fl[(base + a.n_m) * 2 + ri] = sv;
+            for (int j = 0; j < 4; ++j) {
+                int row = (lane >> 4) + 4 * j;
+                int th = tt * 16 + row;
+                double sc = 1.0;
+                if (job.row_scale[o] && th < a.nh) sc = a.osin2[2 * th];
+                ep[(levl_c * 32 + 2 * row) * 2 + ri] = (acc[o][0][j] + acc[o][1][j]) * sc;
+                ep[(levl_c * 32 + 2 * row + 1) * 2 + ri] = (acc[o][0][j] - acc[o][1][j]) * sc;
+            }
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int e = r * 64 + lane;
+                int levl = e >> 5, thl = e & 31;
+                int theta = tt * 32 + thl;
+                if (levl < nlev_here && theta < a.n_theta) {
+                    size_t o2 = ((size_t)(lev0 + levl) * job.fout_ld +
+                                 ((size_t)job.out_comp[o] * a.n_m + mc) * a.n_theta + theta) * 2;
+                    *reinterpret_cast<double2 *>(job.fout + o2) =
+                        make_double2(ep[(levl * 32 + thl) * 2], ep[(levl * 32 + thl) * 2 + 1]);
                 }
             }
         }
+        __syncthreads();
     }
 }
 
@@ -280,11 +350,11 @@ __global__ __launch_bounds__(WG) void k_ana(AnaArgs a)
     // SH: f | SPHTOR: vt, -i m vp, -vp, -i m vt | QST: f + SPHTOR's
     int csrc[NCOL], cip[NCOL], cmp[NCOL];
     double csc[NCOL];
-    if (KIND == ANA_SH) {
+    if constexpr (KIND == ANA_SH) {
         csrc[0] = 0; cip[0] = 0; cmp[0] = 0; csc[0] = 1.0;
     } else {
         const int b = (KIND == ANA_QST) ? 1 : 0;
-        if (KIND == ANA_QST) { csrc[0] = 0; cip[0] = 0; cmp[0] = 0; csc[0] = 1.0; }
+        if constexpr (KIND == ANA_QST) { csrc[0] = 0; cip[0] = 0; cmp[0] = 0; csc[0] = 1.0; }
         csrc[b + 0] = 1; cip[b + 0] = 0; cmp[b + 0] = 0; csc[b + 0] = 1.0;
         csrc[b + 1] = 2; cip[b + 1] = 1; cmp[b + 1] = 1; csc[b + 1] = -1.0;
         csrc[b + 2] = 2; cip[b + 2] = 0; cmp[b + 2] = 0; csc[b + 2] = -1.0;

@@ -95,9 +95,10 @@ int ensure_ws(Plan *p, int nlev)
     size_t gc = (size_t)p->n_phi * p->n_theta * 8;  // bytes per grid comp
     int rc;
     if ((rc = ensure_buf(p->ws_F, p->ws_F_bytes, fc * 18 * nlev))) return rc;
-    if ((rc = ensure_buf(p->ws_G, p->ws_G_bytes, gc * 18 * nlev))) return rc;
+    // grids: the 3 du/dr components of the nonlinear step, or up to 3 grids
+    // of a per-call transform
+    if ((rc = ensure_buf(p->ws_G, p->ws_G_bytes, gc * 3 * nlev))) return rc;
     if ((rc = ensure_buf(p->ws_F2, p->ws_F2_bytes, fc * 10 * nlev))) return rc;
-    if ((rc = ensure_buf(p->ws_G2, p->ws_G2_bytes, gc * 10 * nlev))) return rc;
     if ((rc = ensure_buf(p->ws_C, p->ws_C_bytes, (size_t)p->lm_max * 16 * 8))) return rc;
     if (p->d_lcut_n < nlev) {
         free_dev(p->d_lcut);
@@ -350,8 +351,6 @@ int mhx_plan_create(int l_max, int m_max, int minc, int n_theta, int n_phi, int 
     if (n_phi < 8 || n_phi % 4) return fail(MHX_E_INVALID, "n_phi must be a multiple of 4");
     if (precision != MHX_PREC_FP64 && precision != MHX_PREC_SPLIT16)
         return fail(MHX_E_INVALID, "unknown precision");
-    if (precision != MHX_PREC_FP64)
-        return fail(MHX_E_UNSUPPORTED, "split precision is not built into this library version");
     std::vector<int> fac;
     if (!factorize(n_phi / 2, fac)) return fail(MHX_E_INVALID, "n_phi/2 must factor into 2, 3, 5");
     int n_m = m_max / minc + 1;
@@ -407,11 +406,13 @@ int mhx_plan_create(int l_max, int m_max, int minc, int n_theta, int n_phi, int 
         MHX_HIP_CHECK(hipMemcpy(dst, vec.data(), bytes, hipMemcpyHostToDevice));
         return MHX_OK;
     };
+    std::vector<float> tw32(tw.begin(), tw.end()), twr32(twr.begin(), twr.end());
     int rc;
     int *d_lm2l = nullptr;
     if ((rc = up(p->d_lstart, p->lstart)) || (rc = up(p->d_theta, p->theta_ord)) ||
         (rc = up(p->d_gauss, p->gauss)) || (rc = up(p->d_osin2, osin2)) ||
-        (rc = up(p->d_tw, tw)) || (rc = up(p->d_twr, twr)) || (rc = up(d_lm2l, lm2l))) {
+        (rc = up(p->d_tw, tw)) || (rc = up(p->d_twr, twr)) || (rc = up(d_lm2l, lm2l)) ||
+        (rc = up(p->d_tw32, tw32)) || (rc = up(p->d_twr32, twr32))) {
         mhx_plan_destroy(h);
         return rc;
     }
@@ -420,7 +421,9 @@ int mhx_plan_create(int l_max, int m_max, int minc, int n_theta, int n_phi, int 
         mhx_plan_destroy(h);
         return fail(MHX_E_HIP, "hipStreamCreate failed");
     }
-    if ((rc = build_tables(p)) || (rc = ensure_ws(p, std::max(1, max_levels)))) {
+    if ((rc = build_tables(p)) ||
+        (precision == MHX_PREC_SPLIT16 && (rc = build_split_tables(p))) ||
+        (rc = ensure_ws(p, std::max(1, max_levels)))) {
         mhx_plan_destroy(h);
         return rc;
     }
@@ -448,10 +451,20 @@ int mhx_plan_destroy(mhx_plan *h)
     free_dev(p->d_tw);
     free_dev(p->d_twr);
     free_dev(p->d_lm2l);
+    free_dev(p->d_tw32);
+    free_dev(p->d_twr32);
+    free_dev(p->d_syn16_P);
+    free_dev(p->d_syn16_dP);
+    free_dev(p->d_ana16_P);
+    free_dev(p->d_ana16_dP);
+    free_dev(p->d_syn16_off);
+    free_dev(p->d_ana16_off);
+    free_dev(p->d_ana16_soff);
+    free_dev(p->d_syn16_rs);
+    free_dev(p->d_ana16_rs);
     free_dev(p->ws_F);
     free_dev(p->ws_G);
     free_dev(p->ws_F2);
-    free_dev(p->ws_G2);
     free_dev(p->ws_C);
     free_dev(p->d_lcut);
     free_dev(p->d_rad);
@@ -809,6 +822,9 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
     MHX_HIP_CHECK(hipMemcpyAsync(P->d_rad, rad.data(), rad.size() * sizeof(double),
                                  hipMemcpyHostToDevice, P->stream));
     hipStream_t s = P->stream;
+    // a split-precision plan runs the Legendre stage in scaled FP16 x3 and
+    // keeps Fourier data and grids in FP32; the per-call API stays FP64
+    const bool split = P->precision == MHX_PREC_SPLIT16;
     int nev = 0;
     auto mark = [&]() -> int {
         if (P->timing && nev < Plan::NEV) MHX_HIP_CHECK(hipEventRecord(P->ev[nev++], s));
@@ -825,7 +841,7 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
         j.col[0] = op(0);
         j.col[1] = op(0, G_ONE, 1, 1);
         j.out_comp[0] = 15; j.out_comp[1] = 16; j.out_comp[2] = 17;
-        if ((rc = launch_syn(P, j, s)) || (rc = mark())) return rc;
+        if ((rc = split ? launch_syn16(P, j, s) : launch_syn(P, j, s)) || (rc = mark())) return rc;
     }
     struct QSpec { int q, sidx, t0, t1; bool curl; int comp0; };
     // torpol_to_spat(a, b, c): Q = l(l+1) a, S = b, T = c
@@ -846,7 +862,7 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
         ColOp T = q.curl ? op(2, G_OR2LL, 0, 0, 1.0, 3, -1.0) : op(2);
         qst_cols(j, op(0, G_LL), op(1), T);
         j.out_comp[0] = q.comp0; j.out_comp[1] = q.comp0 + 1; j.out_comp[2] = q.comp0 + 2;
-        if ((rc = launch_syn(P, j, s)) || (rc = mark())) return rc;
+        if ((rc = split ? launch_syn16(P, j, s) : launch_syn(P, j, s)) || (rc = mark())) return rc;
     }
     // fused: inverse FFTs, grid products, forward FFTs; du/dr grids to ws_G
     if ((rc = launch_nl_fused(P, nlev, P->ws_F, P->ws_F2, P->d_rad, LFfac, P->ws_G, grids, s)) ||
@@ -860,13 +876,13 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
         j.out[0] = out[3 * v];
         j.out[1] = out[3 * v + 1];
         j.out[2] = out[3 * v + 2];
-        if ((rc = launch_ana(P, j, s)) || (rc = mark())) return rc;
+        if ((rc = split ? launch_ana16(P, j, s) : launch_ana(P, j, s)) || (rc = mark())) return rc;
     }
     {
         AnaJob j = ana_job(P, ANA_SH, nlev, ld_lm);
         j.in_comp[0] = 9;
         j.out[0] = out[9];
-        if ((rc = launch_ana(P, j, s)) || (rc = mark())) return rc;
+        if ((rc = split ? launch_ana16(P, j, s) : launch_ana(P, j, s)) || (rc = mark())) return rc;
     }
     if (P->timing) {
         // launches: 6 synthesis, fused grid stage, 4 analysis

@@ -68,6 +68,33 @@ def coefficients(l_max, config, field, level, kind="l+1^-2", potential=True, m_m
     return c.astype(np.complex128)
 
 
+# potential -> field l-factor of each input of the nonlinear step (w dw ddw z
+# dz s b db ddb aj dj): v_r ~ l(l+1) W, the horizontal components ~ l dW,
+# l Z, ..., the temperature itself
+NL_FIELD_ORDER = ("w", "dw", "ddw", "z", "dz", "s", "b", "db", "ddb", "aj", "dj")
+
+
+def nl_inputs(l_max, nlev, config=2, kind="l^-5/3", minc=1):
+    """Coefficient inputs of the nonlinear step whose synthesised fields
+    (velocity, vorticity, magnetic field, current density, temperature) have
+    amplitude spectrum `kind` with random phases: each potential is drawn with
+    that spectrum and divided by the l-factor its transform applies (l(l+1)
+    for the poloidal potentials w and b, l for the others, 1 for s)."""
+    ls, _ = lm_index(l_max, None, minc)
+    lsafe = np.maximum(ls, 1).astype(np.float64)
+    out = []
+    for f, name in enumerate(NL_FIELD_ORDER):
+        if name in ("w", "b"):
+            fac = 1.0 / (lsafe * (lsafe + 1.0))
+        elif name == "s":
+            fac = np.ones_like(lsafe)
+        else:
+            fac = 1.0 / lsafe
+        out.append(np.stack([coefficients(l_max, config, f, lev, kind, potential=(name != "s"),
+                                          minc=minc) * fac for lev in range(nlev)]))
+    return out
+
+
 def prime_decomposition(n):
     """Smallest m >= n that factors as 2^i 3^j 5^k (src/truncation.f90:162-192)."""
     m = n

@@ -9,13 +9,10 @@ IN_SPECTRA = "l^-5/3"
 
 
 def make_inputs(l_max, nlev, config=2, kind=IN_SPECTRA, minc=1):
-    """11 fields (w dw ddw z dz s b db ddb aj dj) x nlev, complex128."""
-    arrs = []
-    for f in range(11):
-        arrs.append(np.stack([workload.coefficients(l_max, config, f, lev, kind,
-                                                    potential=(f != 5), minc=minc)
-                              for lev in range(nlev)]))
-    return arrs
+    """11 fields (w dw ddw z dz s b db ddb aj dj) x nlev, complex128, built
+    by magic_amd.workload.nl_inputs (the synthesised fields have the given
+    amplitude spectrum)."""
+    return workload.nl_inputs(l_max, nlev, config, kind, minc)
 
 
 def radial(nlev):

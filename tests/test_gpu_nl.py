@@ -26,10 +26,8 @@ def _run(p, inputs, rad, lcut, with_grids=True):
     return g, [t.cpu().numpy() for t in tout]
 
 
-# Grid-space products amplify FP64 rounding where two products nearly
-# cancel (AdvpLM on these inputs: the reference and the oracle themselves
-# differ by ~4e-12 at l_max=64); the transforms alone are held to 1e-12.
-OUT_TOL = 1e-11
+# FP64 plain form: every grid and every output within 1e-12 relative L2
+OUT_TOL = 1e-12
 
 
 def test_nl_step_matches_reference_golden():
