@@ -19,25 +19,10 @@ sys.path.insert(0, ROOT)
 from magic_amd import workload  # noqa: E402
 from oracle import oracle  # noqa: E402
 
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+from api_cases import read_header, write_api_input, read_api_output  # noqa: E402
+
 OUT = os.path.dirname(os.path.abspath(__file__))
-
-# order of results written by ref_harness 'api' mode, per level:
-# (name, kind, count)  kind: g = grid, a = n_theta vector, c = coefficient vector
-API_OUTPUTS = [
-    ("scal_to_spat", "g", 1), ("scal_to_grad_spat", "g", 2), ("pol_to_grad_spat", "g", 2),
-    ("torpol_to_spat", "g", 3), ("pol_to_curlr_spat", "g", 1),
-    ("torpol_to_curl_spat", "g", 3), ("torpol_to_dphspat", "g", 2),
-    ("sphtor_to_spat", "g", 2), ("torpol_to_spat_IC", "g", 3),
-    ("torpol_to_curl_spat_IC", "g", 3), ("axi_to_spat", "a", 1),
-    ("toraxi_to_spat", "a", 2), ("scal_to_SH", "c", 1), ("spat_to_sphertor", "c", 2),
-    ("spat_to_qst", "c", 3),
-]
-
-
-def read_header(f):
-    h = np.fromfile(f, dtype="<i4", count=6)
-    return dict(zip(("l_max", "m_max", "minc", "n_theta", "n_phi", "lm_max"), h.tolist()))
-
 
 def make_api_inputs(l_max, minc, n_theta, n_phi, lcuts, config, real_m0=True):
     ls, ms = workload.lm_index(l_max, None, minc)
@@ -56,38 +41,6 @@ def make_api_inputs(l_max, minc, n_theta, n_phi, lcuts, config, real_m0=True):
         levels.append(dict(lcut=lcut, r=0.3, r_icb=0.35, or2=1.0 / 0.81, coefs=coefs,
                            grids=grids))
     return levels
-
-
-def write_api_input(path, levels):
-    with open(path, "wb") as f:
-        for L in levels:
-            np.array([L["lcut"]], dtype="<i4").tofile(f)
-            np.array([L["r"], L["r_icb"], L["or2"]], dtype="<f8").tofile(f)
-            for c in L["coefs"]:
-                np.ascontiguousarray(c, dtype="<c16").tofile(f)
-            np.ascontiguousarray(L["grids"], dtype="<f8").tofile(f)
-
-
-def read_api_output(path, nlev):
-    res = []
-    with open(path, "rb") as f:
-        hd = read_header(f)
-        nt, npz, lm = hd["n_theta"], hd["n_phi"], hd["lm_max"]
-        for _ in range(nlev):
-            out = {}
-            for name, kind, cnt in API_OUTPUTS:
-                arrs = []
-                for _k in range(cnt):
-                    if kind == "g":
-                        arrs.append(np.fromfile(f, "<f8", nt * npz).reshape(npz, nt))
-                    elif kind == "a":
-                        arrs.append(np.fromfile(f, "<f8", nt))
-                    else:
-                        arrs.append(np.fromfile(f, "<c16", lm))
-                out[name] = np.stack(arrs)
-            res.append(out)
-        assert f.read() == b"", "trailing data in harness output"
-    return hd, res
 
 
 def api_case(tag, l_max, minc, lcuts, config, real_m0=True):
