@@ -163,6 +163,9 @@ def main():
     ap.add_argument("--cpu-levels", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["both", "split16", "fp64"], default="both")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the bench, run one device-to-device copy of this many MiB "
+                         "(a kernel of known traffic, for scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         log("bench.py measures one GPU only: the transform path is independent per radial "
@@ -246,6 +249,15 @@ def main():
     if errs is not None:
         res["rel_l2_err_vs_fp64"] = errs
         res["max_rel_l2_err_vs_fp64"] = max(errs.values())
+    if args.calib_copy_mib > 0:
+        n = args.calib_copy_mib << 18  # float32 elements
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: %d MiB read + %d MiB written" % (args.calib_copy_mib,
+                                                                args.calib_copy_mib))
     print(json.dumps(res))
 
 

@@ -318,6 +318,8 @@ __global__ void k_nl_fused(FFTArgs f, const T *__restrict__ Fin, T *__restrict__
                            const double *__restrict__ rad, double LFfac_d,
                            T *__restrict__ dgrid, T *__restrict__ full_grids)
 {
+    // split form (T = float): theta-blocked forward side
+    constexpr bool BLK = sizeof(T) == 4;
     typedef cpx<T> C;
     extern __shared__ char smem_raw[];
     const int M = f.M, N = f.N;
@@ -342,8 +344,9 @@ __global__ void k_nl_fused(FFTArgs f, const T *__restrict__ Fin, T *__restrict__
     __syncthreads();
 
     const size_t rowsz = (size_t)n_m;                   // complex per forward row
+    const int ntb = (n_theta + 15) / 16;
     const C *Fi = reinterpret_cast<const C *>(Fin) + (size_t)lev * 18 * n_theta * rowsz + th;
-    C *Fo = reinterpret_cast<C *>(Fout) + (size_t)lev * 10 * n_theta * rowsz;
+    C *Fo = reinterpret_cast<C *>(Fout) + (size_t)lev * 10 * (BLK ? ntb * 16 : n_theta) * rowsz;
 
     T u[3][PMAX], adv[3][PMAX], bb[3][PMAX], vxb[3][PMAX], vs[3][PMAX], ugs[PMAX];
     T val[3][PMAX];
@@ -403,8 +406,11 @@ __global__ void k_nl_fused(FFTArgs f, const T *__restrict__ Fin, T *__restrict__
             const T invN = T(1.0 / (double)N);
             for (int idx = tid; idx < gn * n_m; idx += nthr) {
                 int r2 = idx / n_m, k = idx - r2 * n_m;
-                Fo[((size_t)(c0 + g0 + r2) * n_theta + th) * rowsz + k] =
-                    rfft_post(res + r2 * M, k, M, s_twr, invN);
+                C v = rfft_post(res + r2 * M, k, M, s_twr, invN);
+                if constexpr (BLK)
+                    Fo[(((size_t)(c0 + g0 + r2) * ntb + (th >> 4)) * rowsz + k) * 16 + (th & 15)] = v;
+                else
+                    Fo[((size_t)(c0 + g0 + r2) * n_theta + th) * rowsz + k] = v;
             }
             __syncthreads();
         }
@@ -509,6 +515,242 @@ __global__ void k_axi(int l_max, int nh, const double *__restrict__ plm_syn,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused grid stage, slot form.  Each complex FFT of length M = 12 Q is done
+// by one "slot" of Q lanes inside a single wave (Q divides 64), every lane
+// holding 12 points; the Stockham passes (radices 4, 2, 3 dividing 12) run
+// in place in the slot's LDS row with no block barrier: within a wave the
+// LDS operations execute in program order, and each pass loads all of its
+// inputs before it stores any output.  Six slots work on one theta row:
+// the 18 inverse FFTs in three rounds of six, the 10 forward FFTs in two.
+// Between rounds the block redistributes the row so that each thread holds
+// NPT grid points of every component for the products.
+
+// compiler-level ordering of LDS accesses between the lanes of one wave
+__device__ inline void wave_order()
+{
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+
+template <int R, int L, int M, typename C>
+__device__ inline void slot_pass(C *buf, int t, const C *tw, int sign)
+{
+    constexpr int Q = M / 12, NB = 12 / R;  // lanes per slot, butterflies per lane
+    constexpr int T = M / R, tstride = M / (L * R);
+    C x[NB][R];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = t + Q * b;
+        const int k = j % L;
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            C v = buf[j + s * T];
+            if (s > 0) {
+                C w = tw[s * k * tstride];
+                if (sign > 0) w = cconj(w);
+                v = cmul(v, w);
+            }
+            x[b][s] = v;
+        }
+    }
+    wave_order();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = t + Q * b;
+        const int k = j % L;
+        Dft<R, C>::run(x[b], sign);
+        C *op = buf + (j - k) * R + k;
+#pragma unroll
+        for (int s = 0; s < R; ++s) op[s * L] = x[b][s];
+    }
+    wave_order();
+}
+
+// M = 3 4^A 2^B: A radix-4 passes, B radix-2 passes, one radix-3 pass (the
+// order of the plan's factorisation)
+constexpr int pow4(int a) { return a == 0 ? 1 : 4 * pow4(a - 1); }
+template <int A, int B> struct SlotFFT {
+    static constexpr int M = 3 * pow4(A) * (B ? 2 : 1);
+    template <int I, typename C> __device__ static void radix4(C *buf, int t, const C *tw, int sign)
+    {
+        if constexpr (I < A) {
+            slot_pass<4, pow4(I), M>(buf, t, tw, sign);
+            radix4<I + 1>(buf, t, tw, sign);
+        }
+    }
+    template <typename C> __device__ static void run(C *buf, int t, const C *tw, int sign)
+    {
+        radix4<0>(buf, t, tw, sign);
+        if constexpr (B) slot_pass<2, pow4(A), M>(buf, t, tw, sign);
+        slot_pass<3, pow4(A) * (B ? 2 : 1), M>(buf, t, tw, sign);
+    }
+};
+
+constexpr int NPT = 4;  // grid points per thread (N <= 4 * threads)
+
+template <typename T, int A, int B>
+__global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict__ Fin,
+                                                  T *__restrict__ Fout, int n_m, int n_theta,
+                                                  int nlev, const double *__restrict__ osin2,
+                                                  const double *__restrict__ rad, double LFfac_d,
+                                                  T *__restrict__ dgrid, T *__restrict__ full_grids)
+{
+    typedef cpx<T> C;
+    constexpr bool BLK = sizeof(T) == 4;  // split form: theta-blocked F2
+    extern __shared__ char smem_raw[];
+    typedef SlotFFT<A, B> FFT;
+    constexpr int M = FFT::M, N = 2 * M, Q = M / 12;
+    C *s_tw = reinterpret_cast<C *>(smem_raw);
+    C *s_twr = s_tw + M;
+    C *rows = s_twr + M + 1;  // 6 slot rows of M
+    int lb;
+    {
+        const int nx = 8, nb = gridDim.x, b = blockIdx.x;
+        int per = nb / nx, rem = nb % nx, x = b % nx, i = b / nx;
+        lb = (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + i;
+    }
+    const int th = lb % n_theta, lev = lb / n_theta;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int slot = tid / Q, t = tid - slot * Q;
+    const T o2 = T(osin2[th]);
+    const T or2 = T(rad[lev]), or4 = T(rad[nlev + lev]), orho1 = T(rad[2 * nlev + lev]);
+    const T LFfac = T(LFfac_d);
+    load_twiddles(s_tw, s_twr, f);
+    __syncthreads();
+
+    const int ntb = (n_theta + 15) / 16;
+    const C *Fi = reinterpret_cast<const C *>(Fin) + (size_t)lev * 18 * n_theta * n_m + th;
+    C *Fo = reinterpret_cast<C *>(Fout) + (size_t)lev * 10 * (BLK ? ntb * 16 : n_theta) * n_m;
+
+    T u[3][NPT], adv[3][NPT], vxb[3][NPT];
+    T val[6][NPT];
+
+    // inverse FFTs of six comps into val: round 0 = u, curl u (comps 0-5),
+    // round 1 = B, curl B (9-14), round 2 = du/dr (6-8), s, grad s (15-17)
+    auto inverse6 = [&](int round) {
+        if (slot < 6) {
+            C *z = rows + slot * M;
+            const int comp = round == 0 ? slot : (round == 1 ? 9 + slot : (slot < 3 ? 6 + slot : 12 + slot));
+            const C *X = Fi + (size_t)comp * n_m * n_theta;
+            // Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/N} (X_k - conj X_{M-k})
+            for (int k = t; k <= M / 2; k += Q) {
+                const int k2 = M - k;
+                C a = (k < n_m) ? X[(size_t)k * n_theta] : mk<C>(T(0), T(0));
+                C c = (k2 < n_m) ? X[(size_t)k2 * n_theta] : mk<C>(T(0), T(0));
+                if (k == 0) {
+                    a.y = T(0);
+                    c.y = T(0);
+                }
+                z[k] = cadd(cadd(a, cconj(c)), cmuli(cmul(cconj(s_twr[k]), csub(a, cconj(c))), 1));
+                if (k2 < M && k2 != k)
+                    z[k2] = cadd(cadd(c, cconj(a)), cmuli(cmul(cconj(s_twr[k2]), csub(c, cconj(a))), 1));
+            }
+            wave_order();
+            FFT::run(z, t, s_tw, +1);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < NPT; ++p) {
+            const int j = tid + p * nthr;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) {
+                T v = T(0);
+                if (j < N) {
+                    C zz = rows[r * M + (j >> 1)];
+                    v = (j & 1) ? zz.y : zz.x;
+                    const int comp = round == 0 ? r : (round == 1 ? 9 + r : (r < 3 ? 6 + r : 12 + r));
+                    if (full_grids) full_grids[(((size_t)lev * 18 + comp) * N + j) * n_theta + th] = v;
+                    if (round == 2 && r < 3) dgrid[(((size_t)lev * 3 + r) * n_theta + th) * N + j] = v;
+                }
+                val[r][p] = v;
+            }
+        }
+        __syncthreads();
+    };
+
+    // forward FFTs of the product rows held in val: round 0 = Adv (F2 comps
+    // 0-2) and V x B (6-8), round 1 = V s (3-5) and u_h . grad_h s (9);
+    // 1/sin^2 on the horizontal comps of the vector products
+    auto forward6 = [&](int round) {
+        const int nc = round == 0 ? 6 : 4;
+#pragma unroll
+        for (int p = 0; p < NPT; ++p) {
+            const int j = tid + p * nthr;
+            if (j < N) {
+#pragma unroll
+                for (int r = 0; r < 6; ++r) {
+                    if (r >= nc) break;
+                    const bool horiz = (r % 3 != 0) && !(round == 1 && r == 3);
+                    const T v = horiz ? val[r][p] * o2 : val[r][p];
+                    reinterpret_cast<T *>(&rows[r * M + (j >> 1)])[j & 1] = v;
+                }
+            }
+        }
+        __syncthreads();
+        if (slot < nc) {
+            C *z = rows + slot * M;
+            FFT::run(z, t, s_tw, -1);
+            const int comp = round == 0 ? (slot < 3 ? slot : slot + 3) : (slot < 3 ? slot + 3 : 9);
+            const T invN = T(1.0 / (double)N);
+            for (int k = t; k < n_m; k += Q) {
+                C v = rfft_post(z, k, M, s_twr, invN);
+                if constexpr (BLK)
+                    Fo[(((size_t)comp * ntb + (th >> 4)) * n_m + k) * 16 + (th & 15)] = v;
+                else
+                    Fo[((size_t)comp * n_theta + th) * n_m + k] = v;
+            }
+        }
+        __syncthreads();
+    };
+
+    // u, curl u: Adv = -curl(u) x u terms (src/get_nl.f90:262-273)
+    inverse6(0);
+#pragma unroll
+    for (int p = 0; p < NPT; ++p) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[c][p] = val[c][p];
+        const T cvr = val[3][p], cvt = val[4][p], cvp = val[5][p];
+        adv[0][p] = -o2 * (cvt * u[2][p] - cvp * u[1][p]);
+        adv[1][p] = -or4 * (cvp * u[0][p] - cvr * u[2][p]);
+        adv[2][p] = -or4 * (cvr * u[1][p] - cvt * u[0][p]);
+    }
+    // B: V x B (src/get_nl.f90:376-388); curl B: Lorentz force added to Adv
+    // (src/get_nl.f90:242-258, src/rIter.f90:650-656)
+    inverse6(1);
+#pragma unroll
+    for (int p = 0; p < NPT; ++p) {
+        const T br = val[0][p], bt = val[1][p], bp = val[2][p];
+        const T cbr = val[3][p], cbt = val[4][p], cbp = val[5][p];
+        vxb[0][p] = orho1 * o2 * (u[1][p] * bp - u[2][p] * bt);
+        vxb[1][p] = orho1 * or4 * (u[2][p] * br - u[0][p] * bp);
+        vxb[2][p] = orho1 * or4 * (u[0][p] * bt - u[1][p] * br);
+        adv[0][p] += LFfac * o2 * (cbt * bp - cbp * bt);
+        adv[1][p] += LFfac * or4 * (cbp * br - cbr * bp);
+        adv[2][p] += LFfac * or4 * (cbr * bt - cbt * br);
+    }
+#pragma unroll
+    for (int p = 0; p < NPT; ++p)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            val[c][p] = adv[c][p];
+            val[3 + c][p] = vxb[c][p];
+        }
+    forward6(0);
+    // du/dr (to dgrid only); s, ds/dtheta, ds/dphi: V s (src/get_nl.f90:320-325)
+    // and u_h . grad_h s
+    inverse6(2);
+#pragma unroll
+    for (int p = 0; p < NPT; ++p) {
+        const T sv = val[3][p], dst = val[4][p], dsp = val[5][p];
+        val[0][p] = u[0][p] * sv;
+        val[1][p] = or2 * u[1][p] * sv;
+        val[2][p] = or2 * u[2][p] * sv;
+        val[3][p] = or2 * o2 * (u[1][p] * dst + u[2][p] * dsp);
+    }
+    forward6(1);
+}
+
 FFTArgs make_args(const Plan *p)
 {
     FFTArgs f;
@@ -574,6 +816,41 @@ int fused_threads(int n_phi)
     return nt > 1024 ? 1024 : nt;
 }
 
+// the slot form applies when M = 3 4^A 2^B with 1 <= A <= 4, B <= 1 and
+// M / 12 lanes per slot dividing 64: M = 12 .. 768 (n_phi = 24 .. 1536)
+bool slot_form(const Plan *p, int &A, int &B)
+{
+    int m = p->M;
+    if (m % 3) return false;
+    m /= 3;
+    A = 0;
+    while (m % 4 == 0) { m /= 4; ++A; }
+    B = (m == 2);
+    if (m != 1 && m != 2) return false;
+    return A >= 1 && A <= 4 && !(A == 4 && B);
+}
+
+template <typename T, int A, int B>
+int launch_slots(const Plan *p, const FFTArgs &f, int nlev, const void *Fin, void *Fout,
+                 const double *rad, double LFfac, void *dgrid, void *full_grids, hipStream_t s)
+{
+    constexpr int M = SlotFFT<A, B>::M, Q = M / 12;
+    int nthr = 6 * Q;
+    if (nthr < 64) nthr = 64;
+    nthr = (nthr + 63) / 64 * 64;
+    if (NPT * nthr < p->n_phi) return fail(MHX_E_UNSUPPORTED, "n_phi too large for the fused stage");
+    const size_t shm = ((size_t)2 * M + 1 + 6 * (size_t)M) * 2 * sizeof(T);
+    if (shm > 64 * 1024)
+        MHX_HIP_CHECK(hipFuncSetAttribute((const void *)k_nl_slots<T, A, B>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    dim3 grid((unsigned)(p->n_theta * nlev));
+    k_nl_slots<T, A, B><<<grid, nthr, shm, s>>>(f, static_cast<const T *>(Fin), static_cast<T *>(Fout),
+                                                p->n_m, p->n_theta, nlev, p->d_osin2, rad, LFfac,
+                                                static_cast<T *>(dgrid), static_cast<T *>(full_grids));
+    MHX_HIP_CHECK(hipGetLastError());
+    return MHX_OK;
+}
+
 template <typename T>
 int launch_fused_t(const Plan *p, int nlev, const void *Fin, void *Fout, const double *rad,
                    double LFfac, void *dgrid, void *full_grids, hipStream_t s)
@@ -582,6 +859,20 @@ int launch_fused_t(const Plan *p, int nlev, const void *Fin, void *Fout, const d
     if (sizeof(T) == 4) {
         f.tw = p->d_tw32;
         f.twr = p->d_twr32;
+    }
+    int A, B;
+    if (slot_form(p, A, B)) {
+        const int key = 2 * A + B;
+        switch (key) {
+        case 2: return launch_slots<T, 1, 0>(p, f, nlev, Fin, Fout, rad, LFfac, dgrid, full_grids, s);
+        case 3: return launch_slots<T, 1, 1>(p, f, nlev, Fin, Fout, rad, LFfac, dgrid, full_grids, s);
+        case 4: return launch_slots<T, 2, 0>(p, f, nlev, Fin, Fout, rad, LFfac, dgrid, full_grids, s);
+        case 5: return launch_slots<T, 2, 1>(p, f, nlev, Fin, Fout, rad, LFfac, dgrid, full_grids, s);
+        case 6: return launch_slots<T, 3, 0>(p, f, nlev, Fin, Fout, rad, LFfac, dgrid, full_grids, s);
+        case 7: return launch_slots<T, 3, 1>(p, f, nlev, Fin, Fout, rad, LFfac, dgrid, full_grids, s);
+        case 8: return launch_slots<T, 4, 0>(p, f, nlev, Fin, Fout, rad, LFfac, dgrid, full_grids, s);
+        default: break;
+        }
     }
     int nthr = fused_threads(p->n_phi);
     if (nthr * PMAX < p->n_phi) return fail(MHX_E_UNSUPPORTED, "n_phi too large for the fused stage");

@@ -83,6 +83,44 @@ struct AnaJob {
     const int *lcut;     // device
 };
 
+// Legendre kernels of the nonlinear step in the split form (mhx_nlsplit.hip)
+enum NSynKind { NS_SG = 0, NS_QST = 1, NS_QSTC = 2 };
+enum NAnaKind { NA_QST = 0, NA_SH = 1 };
+
+struct NSynArgs {
+    const double *src[4];   // coefficient arrays, complex, level stride src_ld
+    size_t src_ld;
+    float *fout;            // F[lev][comp][m][theta] FP32 complex
+    size_t fout_ld;         // complex elements per level
+    int comp0;              // first output comp
+    const int *lcut;        // device, per level
+    const double *or2;      // device, per level
+    int nlev;
+    // filled by the launcher
+    const _Float16 *tP, *tdP;
+    const size_t *off;
+    const float *rowscale;
+    const int *lstart;
+    int l_max, minc, n_m, nh, n_theta, nt16, tpb, ngz;
+};
+
+struct NAnaArgs {
+    const float *fin;       // F2[lev][comp][theta/16][m][theta%16] FP32 complex
+    size_t fin_ld;          // complex elements per level
+    int in_comp[3];         // f (QST, SH), vt, vp
+    double *out[3];         // coefficient arrays, complex, level stride out_ld
+    size_t out_ld;
+    const int *lcut;        // device
+    int nlev;
+    // filled by the launcher
+    const _Float16 *tP, *tdP;
+    const size_t *off;
+    const float *rowscale;
+    const size_t *soff;
+    const int *lstart;
+    int l_max, m_max, minc, n_m, nh, n_theta, ntb, nq32, ngz;
+};
+
 // ------------------------------------------------------------------ plan
 struct Plan {
     int l_max, m_max, minc, n_theta, n_phi, nh, n_m, lm_max, precision;
@@ -122,6 +160,7 @@ struct Plan {
     double *ws_G = nullptr;     size_t ws_G_bytes = 0;    // grids
     double *ws_F2 = nullptr;    size_t ws_F2_bytes = 0;   // Fourier, forward side
     double *ws_C = nullptr;     size_t ws_C_bytes = 0;    // staged coefficients
+    double *ws_H = nullptr;     size_t ws_H_bytes = 0;    // nl step host arrays, staged
     int *d_lm2l = nullptr;      // lm_max
     int *d_lcut = nullptr;      int d_lcut_n = 0;
     double *d_rad = nullptr;    int d_rad_n = 0;          // or2, or4, orho1
@@ -141,6 +180,8 @@ struct Plan {
 int build_tables(Plan *p);
 int build_split_tables(Plan *p);
 int launch_syn16(const Plan *p, const SynJob &job, hipStream_t s);
+int launch_nsyn16(const Plan *p, int kind, const NSynArgs &args, hipStream_t s);
+int launch_nana16(const Plan *p, int kind, const NAnaArgs &args, hipStream_t s);
 int launch_ana16(const Plan *p, const AnaJob &job, hipStream_t s);
 int launch_syn(const Plan *p, const SynJob &job, hipStream_t s);
 int launch_ana(const Plan *p, const AnaJob &job, hipStream_t s);
@@ -159,7 +200,8 @@ int launch_fft(const Plan *p, const double *grid, size_t grid_ld_lev, size_t gri
 // fused grid stage of the nonlinear step: Fin (18 comps per level) -> Fout
 // (10 comps per level); du/dr grids to dgrid [lev][3][theta][phi]; all 18
 // inverse grids in the reference layout to full_grids when not null
-// (FP32 Fourier data and grids for a split-precision plan)
+// (FP32 Fourier data and grids for a split-precision plan; there Fout is
+// theta-blocked, F2[lev][comp][theta/16][m][theta%16])
 int launch_nl_fused(const Plan *p, int nlev, const void *Fin, void *Fout,
                     const double *rad, double LFfac, void *dgrid, void *full_grids,
                     hipStream_t s);

@@ -91,7 +91,9 @@ int ensure_buf(double *&buf, size_t &have, size_t need)
 // per-level workspaces of the batched paths
 int ensure_ws(Plan *p, int nlev)
 {
-    size_t fc = (size_t)p->n_m * p->n_theta * 16;   // bytes per Fourier comp
+    // bytes per Fourier comp; theta padded to whole blocks of 16 rows (the
+    // split form's theta-blocked forward side)
+    size_t fc = (size_t)p->n_m * ((p->n_theta + 15) / 16 * 16) * 16;
     size_t gc = (size_t)p->n_phi * p->n_theta * 8;  // bytes per grid comp
     int rc;
     if ((rc = ensure_buf(p->ws_F, p->ws_F_bytes, fc * 18 * nlev))) return rc;
@@ -466,6 +468,7 @@ int mhx_plan_destroy(mhx_plan *h)
     free_dev(p->ws_G);
     free_dev(p->ws_F2);
     free_dev(p->ws_C);
+    free_dev(p->ws_H);
     free_dev(p->d_lcut);
     free_dev(p->d_rad);
     for (int i = 0; i < Plan::NEV; ++i)
@@ -812,6 +815,31 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
     if ((rc = order_after(P, stream)) || (rc = ensure_ws(P, nlev)) ||
         (rc = upload_lcut(P, lcut, nlev)))
         return rc;
+    // host arrays (the caller's *_Rloc fields) are staged through ws_H:
+    // inputs copied in here, outputs copied back after the last launch
+    const double *din[11];
+    double *dout[10];
+    bool hout[10];
+    {
+        const size_t arr = (size_t)nlev * ld_lm * 2;  // doubles per array
+        int nh_in = 0, nh_out = 0;
+        bool hin[11];
+        for (int i = 0; i < 11; ++i) nh_in += (hin[i] = !is_device_ptr(in[i]));
+        for (int i = 0; i < 10; ++i) nh_out += (hout[i] = !is_device_ptr(out[i]));
+        if (nh_in + nh_out &&
+            (rc = ensure_buf(P->ws_H, P->ws_H_bytes, (size_t)(nh_in + nh_out) * arr * sizeof(double))))
+            return rc;
+        int slot = 0;
+        for (int i = 0; i < 11; ++i) {
+            din[i] = in[i];
+            if (!hin[i]) continue;
+            double *d = P->ws_H + (size_t)slot++ * arr;
+            MHX_HIP_CHECK(hipMemcpyAsync(d, in[i], arr * sizeof(double), hipMemcpyHostToDevice,
+                                         P->stream));
+            din[i] = d;
+        }
+        for (int i = 0; i < 10; ++i) dout[i] = hout[i] ? P->ws_H + (size_t)slot++ * arr : out[i];
+    }
     std::vector<double> &rad = P->h_rad;
     rad.assign(3 * (size_t)nlev, 0.0);
     for (int i = 0; i < nlev; ++i) {
@@ -835,13 +863,54 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
     // inputs: 0 w, 1 dw, 2 ddw, 3 z, 4 dz, 5 s, 6 b, 7 db, 8 ddb, 9 aj, 10 dj
     enum { W = 0, DW, DDW, Z, DZ, S, B, DB, DDB, AJ, DJ };
     // s, ds/dtheta, ds/dphi -> comps 15..17
-    {
+    if (split) {
+        // FP32 Fourier data; F2 theta-blocked; compile-time column operations
+        NSynArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.src_ld = ld_lm;
+        a.fout = reinterpret_cast<float *>(P->ws_F);
+        a.fout_ld = (size_t)18 * P->n_m * P->n_theta;
+        a.lcut = P->d_lcut;
+        a.or2 = P->d_rad;
+        a.nlev = nlev;
+        a.src[0] = din[S];
+        a.comp0 = 15;
+        if ((rc = launch_nsyn16(P, NS_SG, a, s)) || (rc = mark())) return rc;
+        // u, curl u, du/dr, B, curl B (see the FP64 table below)
+        const int qsrc[5][4] = {{W, DW, Z, -1}, {Z, DZ, W, DDW}, {DW, DDW, DZ, -1},
+                                {B, DB, AJ, -1}, {AJ, DJ, B, DDB}};
+        for (int v = 0; v < 5; ++v) {
+            for (int k = 0; k < 4; ++k) a.src[k] = qsrc[v][k] >= 0 ? din[qsrc[v][k]] : nullptr;
+            a.comp0 = 3 * v;
+            const int kind = qsrc[v][3] >= 0 ? NS_QSTC : NS_QST;
+            if ((rc = launch_nsyn16(P, kind, a, s)) || (rc = mark())) return rc;
+        }
+        if ((rc = launch_nl_fused(P, nlev, P->ws_F, P->ws_F2, P->d_rad, LFfac, P->ws_G, grids, s)) ||
+            (rc = mark()))
+            return rc;
+        NAnaArgs b;
+        std::memset(&b, 0, sizeof(b));
+        b.fin = reinterpret_cast<const float *>(P->ws_F2);
+        b.fin_ld = (size_t)10 * P->n_m * ((P->n_theta + 15) / 16 * 16);
+        b.out_ld = ld_lm;
+        b.lcut = P->d_lcut;
+        b.nlev = nlev;
+        for (int v = 0; v < 4; ++v) {
+            const int nout = v < 3 ? 3 : 1;
+            for (int k = 0; k < 3; ++k) {
+                b.in_comp[k] = k < nout ? 3 * v + k : -1;
+                b.out[k] = k < nout ? dout[3 * v + k] : nullptr;
+            }
+            if ((rc = launch_nana16(P, v < 3 ? NA_QST : NA_SH, b, s)) || (rc = mark())) return rc;
+        }
+    }
+    if (!split) {
         SynJob j = syn_job(P, SYN_SG, nlev, ld_lm);
-        j.src[0] = in[S];
+        j.src[0] = din[S];
         j.col[0] = op(0);
         j.col[1] = op(0, G_ONE, 1, 1);
         j.out_comp[0] = 15; j.out_comp[1] = 16; j.out_comp[2] = 17;
-        if ((rc = split ? launch_syn16(P, j, s) : launch_syn(P, j, s)) || (rc = mark())) return rc;
+        if ((rc = launch_syn(P, j, s)) || (rc = mark())) return rc;
     }
     struct QSpec { int q, sidx, t0, t1; bool curl; int comp0; };
     // torpol_to_spat(a, b, c): Q = l(l+1) a, S = b, T = c
@@ -854,36 +923,42 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
         {AJ, DJ, B, DDB, true, 12},   // curl B
     };
     for (const QSpec &q : qs) {
+        if (split) break;
         SynJob j = syn_job(P, SYN_QST, nlev, ld_lm);
-        j.src[0] = in[q.q];
-        j.src[1] = in[q.sidx];
-        j.src[2] = in[q.t0];
-        j.src[3] = q.t1 >= 0 ? in[q.t1] : nullptr;
+        j.src[0] = din[q.q];
+        j.src[1] = din[q.sidx];
+        j.src[2] = din[q.t0];
+        j.src[3] = q.t1 >= 0 ? din[q.t1] : nullptr;
         ColOp T = q.curl ? op(2, G_OR2LL, 0, 0, 1.0, 3, -1.0) : op(2);
         qst_cols(j, op(0, G_LL), op(1), T);
         j.out_comp[0] = q.comp0; j.out_comp[1] = q.comp0 + 1; j.out_comp[2] = q.comp0 + 2;
-        if ((rc = split ? launch_syn16(P, j, s) : launch_syn(P, j, s)) || (rc = mark())) return rc;
+        if ((rc = launch_syn(P, j, s)) || (rc = mark())) return rc;
     }
     // fused: inverse FFTs, grid products, forward FFTs; du/dr grids to ws_G
-    if ((rc = launch_nl_fused(P, nlev, P->ws_F, P->ws_F2, P->d_rad, LFfac, P->ws_G, grids, s)) ||
-        (rc = mark()))
+    if (!split && ((rc = launch_nl_fused(P, nlev, P->ws_F, P->ws_F2, P->d_rad, LFfac, P->ws_G,
+                                         grids, s)) ||
+                   (rc = mark())))
         return rc;
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < 3 && !split; ++v) {
         AnaJob j = ana_job(P, ANA_QST, nlev, ld_lm);
         j.in_comp[0] = 3 * v;
         j.in_comp[1] = 3 * v + 1;
         j.in_comp[2] = 3 * v + 2;
-        j.out[0] = out[3 * v];
-        j.out[1] = out[3 * v + 1];
-        j.out[2] = out[3 * v + 2];
-        if ((rc = split ? launch_ana16(P, j, s) : launch_ana(P, j, s)) || (rc = mark())) return rc;
+        j.out[0] = dout[3 * v];
+        j.out[1] = dout[3 * v + 1];
+        j.out[2] = dout[3 * v + 2];
+        if ((rc = launch_ana(P, j, s)) || (rc = mark())) return rc;
     }
-    {
+    if (!split) {
         AnaJob j = ana_job(P, ANA_SH, nlev, ld_lm);
         j.in_comp[0] = 9;
-        j.out[0] = out[9];
-        if ((rc = split ? launch_ana16(P, j, s) : launch_ana(P, j, s)) || (rc = mark())) return rc;
+        j.out[0] = dout[9];
+        if ((rc = launch_ana(P, j, s)) || (rc = mark())) return rc;
     }
+    for (int i = 0; i < 10; ++i)
+        if (hout[i])
+            MHX_HIP_CHECK(hipMemcpyAsync(out[i], dout[i], (size_t)nlev * ld_lm * 2 * sizeof(double),
+                                         hipMemcpyDeviceToHost, P->stream));
     if (P->timing) {
         // launches: 6 synthesis, fused grid stage, 4 analysis
         MHX_HIP_CHECK(hipEventSynchronize(P->ev[nev - 1]));
