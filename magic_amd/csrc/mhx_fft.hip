@@ -105,6 +105,34 @@ struct FFTArgs {
     const void *twr;  // exp(-2 pi i k / N), k <= M
 };
 
+// 12-point DFT as 4 x 3 with the prime-factor (Good-Thomas) index maps
+// n = 3 n1 + 4 n2, k = 9 k1 + 4 k2 (mod 12): no internal twiddles
+template <typename C> struct Dft<12, C> {
+    __device__ __forceinline__ static void run(C *x, int sign)
+    {
+        C y[4][3];
+#pragma unroll
+        for (int n2 = 0; n2 < 3; ++n2) {
+            C a[4];
+#pragma unroll
+            for (int n1 = 0; n1 < 4; ++n1) a[n1] = x[(3 * n1 + 4 * n2) % 12];
+            Dft<4, C>::run(a, sign);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; ++k1) y[k1][n2] = a[k1];
+        }
+        C out[12];
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            C b[3] = {y[k1][0], y[k1][1], y[k1][2]};
+            Dft<3, C>::run(b, sign);
+#pragma unroll
+            for (int k2 = 0; k2 < 3; ++k2) out[(9 * k1 + 4 * k2) % 12] = b[k2];
+        }
+#pragma unroll
+        for (int i = 0; i < 12; ++i) x[i] = out[i];
+    }
+};
+
 // one Stockham pass from `in` to `out` over G rows of M complex values
 template <int R, typename C>
 __device__ void pass_pp(const C *in, C *out, int G, int M, int L, const C *tw, int sign)
@@ -527,14 +555,14 @@ __global__ void k_axi(int l_max, int nh, const double *__restrict__ plm_syn,
 // NPT grid points of every component for the products.
 
 // compiler-level ordering of LDS accesses between the lanes of one wave
-__device__ inline void wave_order()
+__device__ __forceinline__ void wave_order()
 {
     __builtin_amdgcn_wave_barrier();
     asm volatile("" ::: "memory");
 }
 
 template <int R, int L, int M, typename C>
-__device__ inline void slot_pass(C *buf, int t, const C *tw, int sign)
+__device__ __forceinline__ void slot_pass(C *buf, int t, const C *tw, int sign)
 {
     constexpr int Q = M / 12, NB = 12 / R;  // lanes per slot, butterflies per lane
     constexpr int T = M / R, tstride = M / (L * R);
@@ -546,10 +574,12 @@ __device__ inline void slot_pass(C *buf, int t, const C *tw, int sign)
 #pragma unroll
         for (int s = 0; s < R; ++s) {
             C v = buf[j + s * T];
-            if (s > 0) {
-                C w = tw[s * k * tstride];
-                if (sign > 0) w = cconj(w);
-                v = cmul(v, w);
+            if constexpr (L > 1) {
+                if (s > 0) {
+                    C w = tw[s * k * tstride];
+                    if (sign > 0) w = cconj(w);
+                    v = cmul(v, w);
+                }
             }
             x[b][s] = v;
         }
@@ -567,23 +597,24 @@ __device__ inline void slot_pass(C *buf, int t, const C *tw, int sign)
     wave_order();
 }
 
-// M = 3 4^A 2^B: A radix-4 passes, B radix-2 passes, one radix-3 pass (the
-// order of the plan's factorisation)
+// M = 3 4^A 2^B = 12 4^(A-1) 2^B: a radix-12 pass (no twiddles), A - 1
+// radix-4 passes, B radix-2 passes
 constexpr int pow4(int a) { return a == 0 ? 1 : 4 * pow4(a - 1); }
 template <int A, int B> struct SlotFFT {
     static constexpr int M = 3 * pow4(A) * (B ? 2 : 1);
-    template <int I, typename C> __device__ static void radix4(C *buf, int t, const C *tw, int sign)
+    template <int I, typename C>
+    __device__ __forceinline__ static void radix4(C *buf, int t, const C *tw, int sign)
     {
         if constexpr (I < A) {
-            slot_pass<4, pow4(I), M>(buf, t, tw, sign);
+            slot_pass<4, 12 * pow4(I - 1), M>(buf, t, tw, sign);
             radix4<I + 1>(buf, t, tw, sign);
         }
     }
-    template <typename C> __device__ static void run(C *buf, int t, const C *tw, int sign)
+    template <typename C> __device__ __forceinline__ static void run(C *buf, int t, const C *tw, int sign)
     {
-        radix4<0>(buf, t, tw, sign);
-        if constexpr (B) slot_pass<2, pow4(A), M>(buf, t, tw, sign);
-        slot_pass<3, pow4(A) * (B ? 2 : 1), M>(buf, t, tw, sign);
+        slot_pass<12, 1, M>(buf, t, tw, sign);
+        radix4<1>(buf, t, tw, sign);
+        if constexpr (B) slot_pass<2, 12 * pow4(A - 1), M>(buf, t, tw, sign);
     }
 };
 
@@ -650,20 +681,26 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
             FFT::run(z, t, s_tw, +1);
         }
         __syncthreads();
+        // thread owns the point pairs (2q, 2q+1), q = tid + p2 nthr
 #pragma unroll
-        for (int p = 0; p < NPT; ++p) {
-            const int j = tid + p * nthr;
+        for (int p2 = 0; p2 < NPT / 2; ++p2) {
+            const int q = tid + p2 * nthr;
 #pragma unroll
             for (int r = 0; r < 6; ++r) {
-                T v = T(0);
-                if (j < N) {
-                    C zz = rows[r * M + (j >> 1)];
-                    v = (j & 1) ? zz.y : zz.x;
+                C zz = mk<C>(T(0), T(0));
+                if (q < M) {
+                    zz = rows[r * M + q];
                     const int comp = round == 0 ? r : (round == 1 ? 9 + r : (r < 3 ? 6 + r : 12 + r));
-                    if (full_grids) full_grids[(((size_t)lev * 18 + comp) * N + j) * n_theta + th] = v;
-                    if (round == 2 && r < 3) dgrid[(((size_t)lev * 3 + r) * n_theta + th) * N + j] = v;
+                    if (full_grids) {
+                        T *fg = full_grids + (((size_t)lev * 18 + comp) * N + 2 * q) * n_theta + th;
+                        fg[0] = zz.x;
+                        fg[n_theta] = zz.y;
+                    }
+                    if (round == 2 && r < 3)
+                        *reinterpret_cast<C *>(dgrid + (((size_t)lev * 3 + r) * n_theta + th) * N + 2 * q) = zz;
                 }
-                val[r][p] = v;
+                val[r][2 * p2] = zz.x;
+                val[r][2 * p2 + 1] = zz.y;
             }
         }
         __syncthreads();
@@ -675,15 +712,15 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
     auto forward6 = [&](int round) {
         const int nc = round == 0 ? 6 : 4;
 #pragma unroll
-        for (int p = 0; p < NPT; ++p) {
-            const int j = tid + p * nthr;
-            if (j < N) {
+        for (int p2 = 0; p2 < NPT / 2; ++p2) {
+            const int q = tid + p2 * nthr;
+            if (q < M) {
 #pragma unroll
                 for (int r = 0; r < 6; ++r) {
                     if (r >= nc) break;
                     const bool horiz = (r % 3 != 0) && !(round == 1 && r == 3);
-                    const T v = horiz ? val[r][p] * o2 : val[r][p];
-                    reinterpret_cast<T *>(&rows[r * M + (j >> 1)])[j & 1] = v;
+                    const T sc = horiz ? o2 : T(1);
+                    rows[r * M + q] = mk<C>(val[r][2 * p2] * sc, val[r][2 * p2 + 1] * sc);
                 }
             }
         }
