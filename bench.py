@@ -26,34 +26,55 @@ METRIC = "SHT pairs/s at l_max=255 (split-prec vs FP64 GPU) at ≤1e-6 rel err v
 PEAK = {"fp64": 78.6, "fp16": 2500.0}   # dense MFMA TFLOP/s
 PEAK_HBM_GBS = 8000.0
 
-# launch order of mhx_nl_step and the algorithmic work of each launch:
+# launch order of mhx_nl_step: (role, kind, Legendre weight, sources / comps)
 # Legendre weights are flops per (N_h * lm_max) per level (FMA = 2), charged
-# the FP64-equivalent count in both forms; the fused grid stage counts bytes.
+# the FP64-equivalent count in both forms.
 LAUNCHES = [
-    ("syn_scal_grad", "k_syn<SYN_SG>", "flop", 12),
-    ("syn_qst_u", "k_syn<SYN_QST>", "flop", 20),
-    ("syn_qst_curl_u", "k_syn<SYN_QST>", "flop", 20),
-    ("syn_qst_du", "k_syn<SYN_QST>", "flop", 20),
-    ("syn_qst_b", "k_syn<SYN_QST>", "flop", 20),
-    ("syn_qst_curl_b", "k_syn<SYN_QST>", "flop", 20),
-    ("grid_stage", "k_nl_fused", "fused", 0),
-    ("ana_qst_adv", "k_ana<ANA_QST>", "flop", 20),
-    ("ana_qst_vs", "k_ana<ANA_QST>", "flop", 20),
-    ("ana_qst_vxb", "k_ana<ANA_QST>", "flop", 20),
-    ("ana_sh_ugs", "k_ana<ANA_SH>", "flop", 4),
+    ("syn_sg", "syn", 12, 1),
+    ("syn_qst", "syn", 20, 3),   # u: w dw z
+    ("syn_qst", "syn", 20, 4),   # curl u: z dz w ddw
+    ("syn_qst", "syn", 20, 3),   # du/dr: dw ddw dz
+    ("syn_qst", "syn", 20, 3),   # B: b db aj
+    ("syn_qst", "syn", 20, 4),   # curl B: aj dj b ddb
+    ("grid", "grid", 0, 0),
+    ("ana_qst", "ana", 20, 3),
+    ("ana_qst", "ana", 20, 3),
+    ("ana_qst", "ana", 20, 3),
+    ("ana_sh", "ana", 4, 1),
 ]
+# kernels behind each role (prefix of the rocprofv3 kernel name)
+KERNELS = {
+    "split16": {"syn_sg": "k_nsyn16w<0>", "syn_qst": "k_nsyn16w<", "grid": "k_nl_slots<float",
+                "ana_qst": "k_nana16w<0>", "ana_sh": "k_nana16w<1>"},
+    "fp64": {"syn_sg": "k_syn<2>", "syn_qst": "k_syn<3>", "grid": "k_nl_slots<double",
+             "ana_qst": "k_ana<2>", "ana_sh": "k_ana<0>"},
+}
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "nl_step_l255_traffic.json")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic(kind, w, nh, lm_max, n_theta, n_phi, n_m, nlev, word):
-    """flops, or bytes for the fused grid stage: 18 Fourier comps in, 10 out
-    (n_theta x n_m complex each) and the 3 du/dr grids (n_theta x n_phi)."""
-    if kind == "flop":
-        return w * nh * lm_max * nlev
-    return ((18 + 10) * n_theta * n_m * 2 * word + 3 * n_theta * n_phi * word) * nlev
+def algorithmic(kind, w, nsrc, nh, lm_max, n_theta, n_phi, n_m, nlev, word, tword):
+    """(flops, HBM bytes) of one launch.  Legendre: w N_h lm_max flops per
+    level; bytes = coefficient arrays (16 B per lm) + Fourier comps (n_theta x
+    n_m complex of 2 word bytes) + the tables read once (P and dP, tword
+    bytes per entry: 8 FP64, 4 for the FP16 hi/lo pair).  Grid stage: 18
+    Fourier comps in, 10 out, 3 du/dr grids (n_theta x n_phi) out."""
+    four = n_theta * n_m * 2 * word
+    coef = 16 * lm_max
+    if kind == "grid":
+        return 0.0, float(((18 + 10) * four + 3 * n_theta * n_phi * word) * nlev)
+    flops = float(w * nh * lm_max * nlev)
+    ntab = 1 if w == 4 else 2
+    if kind == "syn":
+        nout = 3
+        byts = (nsrc * coef + nout * four) * nlev + ntab * tword * nh * lm_max
+    else:
+        nout = 3 if w == 20 else 1
+        byts = (nsrc * four + nout * coef) * nlev + ntab * tword * nh * lm_max
+    return flops, float(byts)
 
 
 def cpu_baseline(l_max, nlev_sample):
@@ -112,22 +133,27 @@ def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut):
     dt = time.perf_counter() - t0
     p.set_timing(False)
     lt /= steps
-    word = 4 if precision == sht.PREC_SPLIT16 else 8
+    split = precision == sht.PREC_SPLIT16
+    word, tword = (4, 4) if split else (8, 8)
     nh, lm_max, n_m = nt // 2, p.lm_max, p.n_m_max
     per_kernel = {}
-    for (name, kern, kind, w), t in zip(LAUNCHES, lt):
-        a = algorithmic(kind, w, nh, lm_max, nt, nph, n_m, nlev, word)
-        d = per_kernel.setdefault(kern, {"ms": 0.0, "n": 0, "work": 0.0, "kind": kind})
+    total_flop = flop_ms = 0.0
+    for (role, kind, w, nsrc), t in zip(LAUNCHES, lt):
+        fl, by = algorithmic(kind, w, nsrc, nh, lm_max, nt, nph, n_m, nlev, word, tword)
+        d = per_kernel.setdefault(role, {"ms": 0.0, "n": 0, "flop": 0.0, "bytes": 0.0})
         d["ms"] += t
         d["n"] += 1
-        d["work"] += a
-    kernels = {k: {"avg_ms": v["ms"] / v["n"], "launches": v["n"],
-                   ("TFLOP/s" if v["kind"] == "flop" else "GB/s"):
-                       (v["work"] / (v["ms"] * 1e-3) / (1e12 if v["kind"] == "flop" else 1e9))}
-               for k, v in per_kernel.items()}
-    flop_ms = sum(t for (_, _, k, _), t in zip(LAUNCHES, lt) if k == "flop")
-    total_flop = sum(algorithmic(k, w, nh, lm_max, nt, nph, n_m, nlev, word)
-                     for _, _, k, w in LAUNCHES if k == "flop")
+        d["flop"] += fl
+        d["bytes"] += by
+        if kind != "grid":
+            total_flop += fl
+            flop_ms += t
+    kernels = {}
+    for role, v in per_kernel.items():
+        ms = v["ms"] / v["n"]
+        kernels[role] = {"avg_ms": ms, "launches": v["n"],
+                         "TFLOP/s": v["flop"] / v["n"] / (ms * 1e-3) / 1e12,
+                         "GB/s": v["bytes"] / v["n"] / (ms * 1e-3) / 1e9}
     outs = [t.cpu().numpy() for t in tout]
     p.close()
     return {"pairs": nlev * steps / dt, "ms_per_step": 1e3 * dt / steps,
@@ -135,21 +161,41 @@ def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut):
             "legendre_tflops": total_flop / (flop_ms * 1e-3) / 1e12, "outs": outs}
 
 
-def roofline(form, peak_key):
+def measured_traffic(form_key, role):
+    """HBM bytes per launch of the role's kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE and WRITE_SIZE scaled by a plain copy), or None."""
+    try:
+        data = json.load(open(TRAFFIC_JSON))
+    except (OSError, ValueError):
+        return None
+    pref = KERNELS[form_key][role]
+    hits = [v for k, v in data.get("kernels", {}).items() if k.startswith(pref)]
+    if not hits:
+        return None
+    return sum(h["bytes"] for h in hits) / len(hits)
+
+
+def roofline(form, form_key):
+    """Roofline of the dominant kernel (largest total time): MFMA-bound when
+    its arithmetic intensity exceeds the ridge of its datapath, else HBM."""
+    peak_key = "fp16" if form_key == "split16" else "fp64"
     pk = form["per_kernel"]
     dom = max(pk, key=lambda k: pk[k]["ms"])
     d = pk[dom]
     avg_ms = d["ms"] / d["n"]
-    work = d["work"] / d["n"]
-    if d["kind"] == "flop":
-        achieved = work / (avg_ms * 1e-3) / 1e12
+    flop, byts = d["flop"] / d["n"], d["bytes"] / d["n"]
+    ridge = PEAK[peak_key] * 1e12 / (PEAK_HBM_GBS * 1e9)
+    if flop > 0 and flop / byts >= ridge:
+        achieved = flop / (avg_ms * 1e-3) / 1e12
         r = {"bound": "mfma", "achieved": achieved, "peak": PEAK[peak_key], "unit": "TFLOP/s",
              "frac": achieved / PEAK[peak_key]}
     else:
-        achieved = work / (avg_ms * 1e-3) / 1e9
+        achieved = byts / (avg_ms * 1e-3) / 1e9
         r = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
              "frac": achieved / PEAK_HBM_GBS}
-    r.update({"kernel": dom, "avg_launch_ms": avg_ms, "traffic": None})
+    r.update({"kernel": KERNELS[form_key][dom], "role": dom, "avg_launch_ms": avg_ms,
+              "algorithmic_bytes": byts, "algorithmic_flop": flop,
+              "traffic": measured_traffic(form_key, dom)})
     return r
 
 
@@ -237,7 +283,7 @@ def main():
                    "precision": main_key},
         "legendre_tflops_fp64_equiv": main["legendre_tflops"],
         "kernels": main["kernels"],
-        "roofline": roofline(main, "fp16" if main_key == "split16" else "fp64"),
+        "roofline": roofline(main, main_key),
         "cpu_baseline": cpu,
     }
     if "fp64" in forms and main_key != "fp64":

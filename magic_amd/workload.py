@@ -74,12 +74,14 @@ def coefficients(l_max, config, field, level, kind="l+1^-2", potential=True, m_m
 NL_FIELD_ORDER = ("w", "dw", "ddw", "z", "dz", "s", "b", "db", "ddb", "aj", "dj")
 
 
-def nl_inputs(l_max, nlev, config=2, kind="l^-5/3", minc=1):
+def nl_inputs(l_max, nlev, config=2, kind="l^-5/3", minc=1, dipole=1.0):
     """Coefficient inputs of the nonlinear step whose synthesised fields
     (velocity, vorticity, magnetic field, current density, temperature) have
     amplitude spectrum `kind` with random phases: each potential is drawn with
     that spectrum and divided by the l-factor its transform applies (l(l+1)
-    for the poloidal potentials w and b, l for the others, 1 for s)."""
+    for the poloidal potentials w and b, l for the others, 1 for s).
+    `dipole` multiplies l = 1 of the poloidal magnetic potential b and its
+    radial derivatives db, ddb (a dipole-dominated field)."""
     ls, _ = lm_index(l_max, None, minc)
     lsafe = np.maximum(ls, 1).astype(np.float64)
     out = []
@@ -90,6 +92,8 @@ def nl_inputs(l_max, nlev, config=2, kind="l^-5/3", minc=1):
             fac = np.ones_like(lsafe)
         else:
             fac = 1.0 / lsafe
+        if dipole != 1.0 and name in ("b", "db", "ddb"):
+            fac = fac * np.where(ls == 1, dipole, 1.0)
         out.append(np.stack([coefficients(l_max, config, f, lev, kind, potential=(name != "s"),
                                           minc=minc) * fac for lev in range(nlev)]))
     return out

@@ -65,3 +65,25 @@ def test_split_full_config_vs_fp64():
     o1 = _run(p16, sub, subrad, [L])
     for k in range(10):
         assert rel_l2(o1[k][0], outs[k][5]) < 1e-12
+
+
+def test_split_l511_dipole_vs_fp64():
+    """The large-shell case: l_max=511, 768 x 1536, l^-5/3 spectra with the
+    l = 1 poloidal magnetic field 100x larger, on 8 levels (one with a lower
+    lcut); every output within 1e-6 of the FP64 form."""
+    L, nlev = 511, 8
+    nt, nph = workload.grid_size(L)
+    inputs = workload.nl_inputs(L, nlev, config=4, dipole=100.0)
+    rad = radial(nlev)
+    lcut = [L] * nlev
+    lcut[2] = L - 40
+    p64 = sht.Plan(L, nt, nph, max_levels=nlev)
+    ref = _run(p64, inputs, rad, lcut)
+    p64.close()
+    p16 = sht.Plan(L, nt, nph, precision=sht.PREC_SPLIT16, max_levels=nlev)
+    outs = _run(p16, inputs, rad, lcut)
+    p16.close()
+    errs = [rel_l2(outs[k], ref[k]) for k in range(10)]
+    print("split vs FP64 l_max=511 x 8, dipole x100:", " ".join("%.2e" % e for e in errs), flush=True)
+    for k, e in enumerate(errs):
+        assert e < SPLIT_TOL, (sht.NL_OUTPUTS[k], e)
