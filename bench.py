@@ -44,8 +44,8 @@ LAUNCHES = [
 ]
 # kernels behind each role (prefix of the rocprofv3 kernel name)
 KERNELS = {
-    "split16": {"syn_sg": "k_nsyn16w<0>", "syn_qst": "k_nsyn16w<", "grid": "k_nl_slots<float",
-                "ana_qst": "k_nana16w<0>", "ana_sh": "k_nana16w<1>"},
+    "split16": {"syn_sg": "k_nsyn16p<0>", "syn_qst": "k_nsyn16p<", "grid": "k_nl_slots<float",
+                "ana_qst": "k_nana16p<0>", "ana_sh": "k_nana16w<1>"},
     "fp64": {"syn_sg": "k_syn<2>", "syn_qst": "k_syn<3>", "grid": "k_nl_slots<double",
              "ana_qst": "k_ana<2>", "ana_sh": "k_ana<0>"},
 }

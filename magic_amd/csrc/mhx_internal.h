@@ -153,6 +153,12 @@ struct Plan {
     _Float16 *d_ana16_P = nullptr, *d_ana16_dP = nullptr;
     size_t *d_syn16_off = nullptr, *d_ana16_off = nullptr, *d_ana16_soff = nullptr;
     float *d_syn16_rs = nullptr, *d_ana16_rs = nullptr;
+    // P-only split tables with l <= l_max + 1 (dP products folded onto P by
+    // sin(theta) dP_l = l c_{l+1} P_{l+1} - (l+1) c_l P_{l-1}); same layouts
+    // with K'[par] = #{l in [m, l_max+1] : l - m = par mod 2}
+    _Float16 *d_syn16x = nullptr, *d_ana16x = nullptr;
+    size_t *d_syn16x_off = nullptr, *d_ana16x_off = nullptr, *d_ana16x_soff = nullptr;
+    float *d_syn16x_rs = nullptr, *d_ana16x_rs = nullptr;
 
     // workspaces
     int ws_levels = 0;

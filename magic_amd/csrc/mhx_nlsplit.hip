@@ -625,6 +625,324 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16w(NSynArgs a)
     }
 }
 
+// ----------------------------------- synthesis, whole-K form on P only
+// The dP products are folded onto P by
+//   sin(theta) dP_l = l c_{l+1} P_{l+1} - (l+1) c_l P_{l-1},
+//   c_l = sqrt((l^2 - m^2) / (4 l^2 - 1))   (src/plms.f90:81-97, :142-162),
+// so sum_l dP_l x_l = sum_l P_l x'_l with
+//   x'_l = (l-1) c_l x_{l-1} - (l+2) c_{l+1} x_{l+1},   l = m .. l_max + 1.
+// Columns against P (table to l_max + 1):
+//   QST: C0 = Q, C1 = S' + i m T (-> vt), C2 = i m S - T' (-> vp)
+//   SG : C0 = s, C1 = s' (-> ds/dtheta_c); ds/dphi = i m (P C0) in the epilogue
+// Staging: 1. Q, S, T (SG: s) in FP32 into the LDS rows of the columns;
+// 2. the neighbour combinations in place (all reads before all writes);
+// 3. one power of two per (column, level); rows split in place to hi/lo;
+// 4. contraction over the k steps of both parities, the table fragments
+//    loaded PFD steps ahead (the step's MFMA work is shorter than an L2 hit);
+// 5. epilogue: outputs staged per level in LDS, written as theta runs.
+constexpr int PKP = 160;                 // k per parity held (l_max + 1 - m)/2 + 1 <= 160
+constexpr int PROW = 4 * PKP + 16;       // bytes per LDS row (padded)
+
+__device__ inline float rec_cf(int l, int m)
+{
+    const float dl = (float)l, dm = (float)m;
+    return l > m ? sqrtf((dl * dl - dm * dm) / (4.f * dl * dl - 1.f)) : 0.f;
+}
+
+template <int V>
+__global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
+{
+    constexpr int NSRC = V == NS_SG ? 1 : (V == NS_QSTC ? 4 : 3);
+    constexpr int NCOL = V == NS_SG ? 2 : 3, NROW = NCOL * 2 * 16;
+    constexpr int SB = NROW * PROW > WLG * (2 * WNT + 4) * 4 ? NROW * PROW : WLG * (2 * WNT + 4) * 4;
+    __shared__ __align__(16) unsigned char sb[SB];
+    __shared__ unsigned lvmax[NCOL * WLG];
+    __shared__ float lvsc[NCOL * WLG];
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int ng = (a.nlev + WLG - 1) / WLG;
+    int mc, rest;
+    xcd_deal(blockIdx.x, ng * a.ngz, mc, rest);
+    if (mc >= a.n_m) return;
+    const int gy = rest % ng, gz = rest / ng;
+    const int m = mc * a.minc;
+    const float fm = (float)m;
+    const int lev0 = gy * WLG, nlh = min(WLG, a.nlev - lev0);
+    const int l0 = a.lstart[mc];
+    const int K0 = (a.l_max + 1 - m) / 2 + 1, K1 = (a.l_max + 2 - m) / 2;
+    const int nks0 = (K0 + KS - 1) / KS, nks1 = (K1 + KS - 1) / KS;
+    if (tid < NCOL * WLG) lvmax[tid] = 0u;
+    // float slot of (column c, l, level levl, re/im ri)
+    auto slot = [&](int c, int l, int levl, int ri) -> float * {
+        const int par = (l - m) & 1, k = (l - m) >> 1;
+        return reinterpret_cast<float *>(sb + ((c * 2 + par) * 16 + levl * 2 + ri) * PROW) + k;
+    };
+    const int levl = tid >> 5, g0 = tid & 31;   // staging: level, group of 4 l
+    const int lev = lev0 + levl;
+    const bool lok = levl < nlh;
+    const int lhi = lok ? min(a.l_max, a.lcut[lev]) : -1;
+    __syncthreads();
+
+    // ---- 1. Q, S, T (SG: s) in FP32, l = m + 4 g + j over 3 rounds of 128 l
+    {
+        const double o2 = (V == NS_QSTC && lok) ? a.or2[lev] : 0.0;
+        float mx0 = 0.f;
+#pragma unroll
+        for (int it = 0; it < 3; ++it) {
+            const int g = g0 + 32 * it;
+            double2 x[4][NSRC];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int l = m + 4 * g + j;
+                const size_t o = ((size_t)(lok ? lev : 0) * a.src_ld + (size_t)(l0 + l - m)) * 2;
+#pragma unroll
+                for (int s2 = 0; s2 < NSRC; ++s2)
+                    x[j][s2] = (l <= lhi) ? *reinterpret_cast<const double2 *>(a.src[s2] + o)
+                                          : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int l = m + 4 * g + j;
+                if (((l - m) >> 1) >= PKP) continue;
+                const double ll = (double)l * (double)(l + 1);
+                float cr[3], ci[3];
+                if constexpr (V == NS_SG) {
+                    cr[0] = cr[1] = cr[2] = (float)x[j][0].x;
+                    ci[0] = ci[1] = ci[2] = (float)x[j][0].y;
+                } else {
+                    cr[0] = (float)(ll * x[j][0].x);
+                    ci[0] = (float)(ll * x[j][0].y);
+                    cr[1] = (float)x[j][1].x;
+                    ci[1] = (float)x[j][1].y;
+                    if constexpr (V == NS_QSTC) {
+                        cr[2] = (float)(o2 * ll * x[j][2].x - x[j][3].x);
+                        ci[2] = (float)(o2 * ll * x[j][2].y - x[j][3].y);
+                    } else {
+                        cr[2] = (float)x[j][2].x;
+                        ci[2] = (float)x[j][2].y;
+                    }
+                }
+                mx0 = fmaxf(mx0, fmaxf(fabsf(cr[0]), fabsf(ci[0])));
+#pragma unroll
+                for (int c = 0; c < NCOL; ++c) {
+                    *slot(c, l, levl, 0) = cr[c];
+                    *slot(c, l, levl, 1) = ci[c];
+                }
+            }
+        }
+        if (lok) atomicMax(&lvmax[levl], __float_as_uint(mx0));
+    }
+    __syncthreads();
+    // ---- 2. C1 (and C2) from the neighbours; all reads, then all writes.
+    // Column 1 holds S (SG: s), column 2 holds T.
+    {
+        float sr[3][6], si[3][6], tr[3][6], ti[3][6];  // [round][l - 1 .. l + 4]
+#pragma unroll
+        for (int it = 0; it < 3; ++it) {
+            const int lb = m + 4 * (g0 + 32 * it);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const int l = lb - 1 + q;
+                const bool in = l >= m && ((l - m) >> 1) < PKP;
+                sr[it][q] = in ? *slot(1, l, levl, 0) : 0.f;
+                si[it][q] = in ? *slot(1, l, levl, 1) : 0.f;
+                if constexpr (NCOL == 3) {
+                    tr[it][q] = in ? *slot(2, l, levl, 0) : 0.f;
+                    ti[it][q] = in ? *slot(2, l, levl, 1) : 0.f;
+                }
+            }
+        }
+        __syncthreads();
+        float mx1 = 0.f, mx2 = 0.f;
+#pragma unroll
+        for (int it = 0; it < 3; ++it) {
+            const int lb = m + 4 * (g0 + 32 * it);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int l = lb + j;
+                if (((l - m) >> 1) >= PKP) continue;
+                const float am = (float)(l - 1) * rec_cf(l, m), ap = (float)(l + 2) * rec_cf(l + 1, m);
+                // x' = am x_{l-1} - ap x_{l+1}: neighbours at q = j and j + 2
+                const float spr = am * sr[it][j] - ap * sr[it][j + 2];
+                const float spi = am * si[it][j] - ap * si[it][j + 2];
+                if constexpr (NCOL == 2) {
+                    mx1 = fmaxf(mx1, fmaxf(fabsf(spr), fabsf(spi)));
+                    *slot(1, l, levl, 0) = spr;
+                    *slot(1, l, levl, 1) = spi;
+                } else {
+                    const float tpr = am * tr[it][j] - ap * tr[it][j + 2];
+                    const float tpi = am * ti[it][j] - ap * ti[it][j + 2];
+                    // C1 = S' + i m T, C2 = i m S - T'
+                    const float c1r = spr - fm * ti[it][j + 1], c1i = spi + fm * tr[it][j + 1];
+                    const float c2r = -fm * si[it][j + 1] - tpr, c2i = fm * sr[it][j + 1] - tpi;
+                    mx1 = fmaxf(mx1, fmaxf(fabsf(c1r), fabsf(c1i)));
+                    mx2 = fmaxf(mx2, fmaxf(fabsf(c2r), fabsf(c2i)));
+                    *slot(1, l, levl, 0) = c1r;
+                    *slot(1, l, levl, 1) = c1i;
+                    *slot(2, l, levl, 0) = c2r;
+                    *slot(2, l, levl, 1) = c2i;
+                }
+            }
+        }
+        if (lok) {
+            atomicMax(&lvmax[WLG + levl], __float_as_uint(mx1));
+            if constexpr (NCOL == 3) atomicMax(&lvmax[2 * WLG + levl], __float_as_uint(mx2));
+        }
+    }
+    __syncthreads();
+    if (tid < NCOL * WLG) lvsc[tid] = bscale((double)__uint_as_float(lvmax[tid]));
+    __syncthreads();
+    // ---- 3. rows in place: [hi 160][lo 160]; five segments of 32 floats per
+    // row, all of a row's segments in the same pass
+    for (int base = 0; base < 5 * NROW; base += 255) {
+        const int seg = base + tid;
+        const bool act = tid < 255 && seg < 5 * NROW;
+        const int row = seg / 5, part = seg % 5;
+        float v[32];
+        if (act) {
+            const f4 *src = reinterpret_cast<const f4 *>(sb + row * PROW + 128 * part);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const f4 w = src[i];
+                v[4 * i] = w[0]; v[4 * i + 1] = w[1]; v[4 * i + 2] = w[2]; v[4 * i + 3] = w[3];
+            }
+        }
+        __syncthreads();
+        if (act) {
+            const int c = row / 32, lv = (row & 15) >> 1;
+            const float sc = lvsc[c * WLG + lv];
+            h8 *hp = reinterpret_cast<h8 *>(sb + row * PROW + 64 * part);
+            h8 *lp = reinterpret_cast<h8 *>(sb + row * PROW + 2 * PKP + 64 * part);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                h8 h, lo;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    _Float16 a1, b1;
+                    split_h(v[8 * i + e] * sc, a1, b1);
+                    h[e] = a1;
+                    lo[e] = b1;
+                }
+                hp[i] = h;
+                lp[i] = lo;
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- 4. contraction: NCOL products against P per k step; steps
+    // j < nks0 are parity 0 (ks = j), the rest parity 1 (ks = j - nks0)
+    f4 acc[NCOL][2][WTT];  // [column][Sym/Anti][theta tile]
+#pragma unroll
+    for (int o = 0; o < NCOL; ++o)
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+            for (int t = 0; t < WTT; ++t) acc[o][s2][t] = f4{0.f, 0.f, 0.f, 0.f};
+    const int tile0 = gz * 4 * WTT + wave * WTT;
+    bool tok[WTT];
+#pragma unroll
+    for (int t = 0; t < WTT; ++t) tok[t] = tile0 + t < a.nt16;
+    const int nst = nks0 + nks1;
+    struct Frag { h8 h[WTT], l[WTT]; };
+    auto load_a = [&](int j, Frag &f) {
+        if (j >= nst) return;
+        const int par = j >= nks0, ks = j - par * nks0;
+#pragma unroll
+        for (int tt = 0; tt < WTT; ++tt) {
+            if (tok[tt]) {
+                const size_t to = a.off[2 * mc + par] + (((size_t)ks * a.nt16 + tile0 + tt) * 64 + lane) * 16;
+                f.h[tt] = *reinterpret_cast<const h8 *>(a.tP + to);
+                f.l[tt] = *reinterpret_cast<const h8 *>(a.tP + to + 8);
+            }
+        }
+    };
+    auto step = [&](auto parc, int ks, const Frag &f) {
+        constexpr int PAR = decltype(parc)::value;
+        const int kb = 2 * (ks * KS + 8 * (lane >> 4));
+#pragma unroll
+        for (int c = 0; c < NCOL; ++c) {
+            const int row = (c * 2 + PAR) * 16 + (lane & 15);
+            const h8 bh = *reinterpret_cast<const h8 *>(sb + row * PROW + kb);
+            const h8 bl = *reinterpret_cast<const h8 *>(sb + row * PROW + 2 * PKP + kb);
+#pragma unroll
+            for (int tt = 0; tt < WTT; ++tt) {
+                if (!tok[tt]) continue;
+                f4 t = acc[c][PAR][tt];
+                t = mfma16(f.h[tt], bl, t);
+                t = mfma16(f.l[tt], bh, t);
+                t = mfma16(f.h[tt], bh, t);
+                acc[c][PAR][tt] = t;
+            }
+        }
+    };
+    auto run = [&](int j, const Frag &f) {
+        if (j < nks0) step(std::integral_constant<int, 0>(), j, f);
+        else step(std::integral_constant<int, 1>(), j - nks0, f);
+    };
+    // ring of four fragment sets: step j's loads are issued at step j - 3
+    Frag f0, f1, f2, f3;
+    load_a(0, f0);
+    load_a(1, f1);
+    load_a(2, f2);
+    for (int j = 0; j < nst; j += 4) {
+        load_a(j + 3, f3);
+        run(j, f0);
+        if (j + 1 >= nst) break;
+        load_a(j + 4, f0);
+        run(j + 1, f1);
+        if (j + 2 >= nst) break;
+        load_a(j + 5, f1);
+        run(j + 2, f2);
+        if (j + 3 >= nst) break;
+        load_a(j + 6, f2);
+        run(j + 3, f3);
+    }
+
+    // ---- 5. epilogue, one output at a time (contiguous theta runs)
+    float *so = reinterpret_cast<float *>(sb);
+    const int rowf = 2 * a.n_theta + 4;
+    const int n = lane & 15, ol = n >> 1, ri = n & 1;
+    const int r0 = 2 * 16 * (gz * 4 * WTT), r1 = min(a.n_theta, 2 * 16 * (gz + 1) * 4 * WTT);
+    const int nq = (r1 - r0) / 2;
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+        __syncthreads();  // B rows / previous output consumed
+        const int oc = (NCOL == 2 && o == 2) ? 0 : o;  // SG: i m (P C0)
+        const float inv = 1.f / lvsc[oc * WLG + ol];
+#pragma unroll
+        for (int tt = 0; tt < WTT; ++tt) {
+            if (!tok[tt]) continue;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int th = (tile0 + tt) * 16 + 4 * (lane >> 4) + j;
+                float sy = acc[oc][0][tt][j], an = acc[oc][1][tt][j];
+                if (NCOL == 2 && o == 2) {
+                    // (i m) X: re = -m X.im, im = m X.re, from lane n ^ 1
+                    const float osy = __shfl_xor(sy, 1), oan = __shfl_xor(an, 1);
+                    sy = ri ? fm * osy : -fm * osy;
+                    an = ri ? fm * oan : -fm * oan;
+                }
+                if (th >= a.nh) continue;
+                const float rs = inv / a.rowscale[(size_t)mc * a.nh + th];
+                float *d = so + ol * rowf + 4 * th + ri;
+                d[0] = (sy + an) * rs;
+                d[2] = (sy - an) * rs;
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < WLG * nq; i += 256) {
+            const int q = i % nq, lv = i / nq;
+            if (lv >= nlh) continue;
+            const f4 w = *reinterpret_cast<const f4 *>(so + lv * rowf + 2 * r0 + 4 * q);
+            float *fo = a.fout + ((size_t)(lev0 + lv) * a.fout_ld +
+                                  ((size_t)(a.comp0 + o) * a.n_m + mc) * a.n_theta + r0) * 2;
+            *reinterpret_cast<f4 *>(fo + 4 * q) = w;
+        }
+    }
+}
+
 // -------------------------------------------------------------- analysis
 template <int V> struct NAna;
 // planes: 0 f, 1 vt, 2 -i m vp, 3 -vp, 4 -i m vt
@@ -1169,6 +1487,276 @@ __global__ __launch_bounds__(256, 2) void k_nana16w(NAnaArgs a)
     }
 }
 
+// ------------------------- analysis, whole-theta form on wP only (QST)
+// The wdP products are folded onto wP by the same identity as in synthesis:
+// with G_l = sum_theta wP_l v,
+//   sum_theta wdP_l v = l c_{l+1} G_{l+1} - (l+1) c_l G_{l-1},
+// so three products G^f, G^vt, G^vp (l = m .. l_max + 1) replace five, and
+//   Q = G^f, S = (l c_{l+1} G^vt_{l+1} - (l+1) c_l G^vt_{l-1}) - i m G^vp_l,
+//   T = -(l c_{l+1} G^vp_{l+1} - (l+1) c_l G^vp_{l-1}) - i m G^vt_l,
+// S, T / l(l+1) (src/shtransforms.f90:731-872 with the crossed vt/vp).
+// Staging and contraction as k_nana16w; the wave's l tiles are
+// wave, wave + 4, wave + 8.  The epilogue keeps G (FP32, row factors
+// removed) in LDS as [group][level][l - m] and forms the outputs per (level, l).
+constexpr int PTT = 3;  // l tiles per wave
+
+template <int V>
+__global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
+{
+    constexpr int NCOL = 3, NROW = NCOL * 2 * 16;
+    constexpr int SB = NROW * AROW;
+    __shared__ __align__(16) unsigned char sb[SB];
+    __shared__ unsigned lvmax[2 * WLG];
+    __shared__ float lvsc[2 * WLG];
+    constexpr int cgrp[3] = {0, 1, 1};
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int ng = (a.nlev + WLG - 1) / WLG;
+    int mc, rest;
+    xcd_deal(blockIdx.x, ng, mc, rest);
+    if (mc >= a.n_m) return;
+    const int gy = rest;
+    const int m = mc * a.minc;
+    const int lev0 = gy * WLG, nlh = min(WLG, a.nlev - lev0);
+    const int l0 = a.lstart[mc];
+    const int K0 = (a.l_max + 1 - m) / 2 + 1;
+    const int ntl = (K0 + 15) / 16;
+    bool tok[PTT];
+#pragma unroll
+    for (int i = 0; i < PTT; ++i) tok[i] = wave + 4 * i < ntl;
+    if (tid < 2 * WLG) lvmax[tid] = 0u;
+    __syncthreads();
+
+    // ---- 1. FP32 Sym/Anti columns (as k_nana16w)
+    {
+        const int nent = WLG * a.ntb * 4;
+        constexpr int NE = (WLG * (AKP / 8) * 4 + 255) / 256;  // 3
+        f4 xv[NE][NCOL][2];
+#pragma unroll
+        for (int i = 0; i < NE; ++i) {
+            const int e = tid + 256 * i;
+            const int q = e & 3, tb = (e >> 2) % a.ntb, levl = (e >> 2) / a.ntb;
+            const int lev = lev0 + levl;
+            const int th0 = 8 * tb + 2 * q;
+            bool valid = e < nent && levl < nlh && th0 < a.nh;
+            if (valid) {
+                const int lc = a.lcut[lev];
+                valid = m <= (lc < a.m_max ? lc : a.m_max);
+            }
+            const bool ok1 = th0 + 1 < a.nh;
+#pragma unroll
+            for (int k = 0; k < NCOL; ++k) {
+                xv[i][k][0] = f4{0.f, 0.f, 0.f, 0.f};
+                xv[i][k][1] = f4{0.f, 0.f, 0.f, 0.f};
+                if (valid) {
+                    const size_t o = ((size_t)lev * a.fin_ld +
+                                      (((size_t)a.in_comp[k] * a.ntb + tb) * a.n_m + mc) * 16 + 4 * q) * 2;
+                    xv[i][k][0] = *reinterpret_cast<const f4 *>(a.fin + o);
+                    if (ok1) xv[i][k][1] = *reinterpret_cast<const f4 *>(a.fin + o + 4);
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NE; ++i) {
+            const int e = tid + 256 * i;
+            const int q = e & 3, tb = (e >> 2) % a.ntb, levl = (e >> 2) / a.ntb;
+            const int th0 = 8 * tb + 2 * q;
+            if (e >= nent || th0 >= a.nh) continue;
+            float mx0 = 0.f, mx1 = 0.f;
+#pragma unroll
+            for (int sa = 0; sa < 2; ++sa)
+#pragma unroll
+                for (int k = 0; k < NCOL; ++k) {
+                    float cr[2], ci[2];
+#pragma unroll
+                    for (int p = 0; p < 2; ++p) {
+                        const f4 w = xv[i][k][p];  // N re, N im, S re, S im
+                        cr[p] = sa ? w[0] - w[2] : w[0] + w[2];
+                        ci[p] = sa ? w[1] - w[3] : w[1] + w[3];
+                    }
+                    float &mx = cgrp[k] ? mx1 : mx0;
+                    mx = fmaxf(mx, fmaxf(fmaxf(fabsf(cr[0]), fabsf(cr[1])), fmaxf(fabsf(ci[0]), fabsf(ci[1]))));
+#pragma unroll
+                    for (int ri = 0; ri < 2; ++ri) {
+                        const int row = (k * 2 + sa) * 16 + levl * 2 + ri;
+                        const float2 v = ri ? make_float2(ci[0], ci[1]) : make_float2(cr[0], cr[1]);
+                        *reinterpret_cast<float2 *>(sb + row * AROW + 4 * th0) = v;
+                    }
+                }
+            if (levl < nlh) {
+                atomicMax(&lvmax[levl], __float_as_uint(mx0));
+                atomicMax(&lvmax[WLG + levl], __float_as_uint(mx1));
+            }
+        }
+        for (int e = tid; e < NROW * (AKP / 2); e += 256) {
+            const int row = e / (AKP / 2), k2 = 2 * (e % (AKP / 2));
+            if (k2 >= a.nh)
+                *reinterpret_cast<float2 *>(sb + row * AROW + 4 * k2) = make_float2(0.f, 0.f);
+        }
+    }
+    __syncthreads();
+    if (tid < 2 * WLG) lvsc[tid] = bscale((double)__uint_as_float(lvmax[tid]));
+    __syncthreads();
+    // ---- 2. rows in place: [hi 192][lo 192]
+    for (int base = 0; base < 3 * NROW; base += 255) {
+        const int seg = base + tid;
+        const bool act = tid < 255 && seg < 3 * NROW;
+        const int row = seg / 3, part = seg % 3;
+        float v[64];
+        if (act) {
+            const f4 *src = reinterpret_cast<const f4 *>(sb + row * AROW + 256 * part);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const f4 w = src[i];
+                v[4 * i] = w[0]; v[4 * i + 1] = w[1]; v[4 * i + 2] = w[2]; v[4 * i + 3] = w[3];
+            }
+        }
+        __syncthreads();
+        if (act) {
+            const int c = row / 32, levl = (row & 15) >> 1;
+            const float sc = lvsc[(cgrp[c] ? WLG : 0) + levl];
+            h8 *hp = reinterpret_cast<h8 *>(sb + row * AROW + 128 * part);
+            h8 *lp = reinterpret_cast<h8 *>(sb + row * AROW + 2 * AKP + 128 * part);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                h8 h, lo;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    _Float16 a1, b1;
+                    split_h(v[8 * i + e] * sc, a1, b1);
+                    h[e] = a1;
+                    lo[e] = b1;
+                }
+                hp[i] = h;
+                lp[i] = lo;
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- 3. contraction over theta_h: G^c (parity par) += wP(par) x col(c, Sym/Anti = par)
+    f4 acc[NCOL][2][PTT];
+#pragma unroll
+    for (int o = 0; o < NCOL; ++o)
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+            for (int t = 0; t < PTT; ++t) acc[o][s2][t] = f4{0.f, 0.f, 0.f, 0.f};
+    // half step h: theta chunk kc = h >> 1, parity h & 1; the fragments of
+    // half step h are loaded at half step h - 3 (ring of four)
+    const int nhs = 2 * a.nq32;
+    struct AF { h8 h[PTT], l[PTT]; };
+    auto load_h = [&](int h, AF &f) {
+        if (h >= nhs) return;
+        const int kc = h >> 1, par = h & 1;
+#pragma unroll
+        for (int tt = 0; tt < PTT; ++tt) {
+            if (tok[tt]) {
+                const size_t to = a.off[2 * mc + par] +
+                                  (((size_t)(wave + 4 * tt) * a.nq32 + kc) * 64 + lane) * 16;
+                f.h[tt] = *reinterpret_cast<const h8 *>(a.tP + to);
+                f.l[tt] = *reinterpret_cast<const h8 *>(a.tP + to + 8);
+            }
+        }
+    };
+    auto half = [&](auto parc, int kc, const AF &f) {
+        constexpr int PAR = decltype(parc)::value;
+        const int kb = 2 * (kc * KS + 8 * (lane >> 4));
+#pragma unroll
+        for (int c = 0; c < NCOL; ++c) {
+            const int row = (c * 2 + PAR) * 16 + (lane & 15);
+            const h8 bh = *reinterpret_cast<const h8 *>(sb + row * AROW + kb);
+            const h8 bl = *reinterpret_cast<const h8 *>(sb + row * AROW + 2 * AKP + kb);
+#pragma unroll
+            for (int tt = 0; tt < PTT; ++tt) {
+                if (!tok[tt]) continue;
+                f4 t = acc[c][PAR][tt];
+                t = mfma16(f.h[tt], bl, t);
+                t = mfma16(f.l[tt], bh, t);
+                t = mfma16(f.h[tt], bh, t);
+                acc[c][PAR][tt] = t;
+            }
+        }
+    };
+    using P0 = std::integral_constant<int, 0>;
+    using P1 = std::integral_constant<int, 1>;
+    AF f0, f1, f2, f3;
+    load_h(0, f0);
+    load_h(1, f1);
+    load_h(2, f2);
+    for (int h = 0; h < nhs; h += 4) {  // nhs is even
+        load_h(h + 3, f3);
+        half(P0(), h >> 1, f0);
+        load_h(h + 4, f0);
+        half(P1(), h >> 1, f1);
+        if (h + 2 >= nhs) break;
+        load_h(h + 5, f1);
+        half(P0(), (h >> 1) + 1, f2);
+        load_h(h + 6, f2);
+        half(P1(), (h >> 1) + 1, f3);
+    }
+
+    // ---- 4. G (row factors removed) into LDS: [c][level][l - m] complex float
+    float *sg = reinterpret_cast<float *>(sb);
+    const int rowg = 2 * (a.l_max + 2) + 2;
+    const int kmax = a.l_max + 1 - m;  // l - m at most
+    __syncthreads();  // B rows consumed
+    {
+        const int n = lane & 15, levl = n >> 1, ri = n & 1;
+#pragma unroll
+        for (int tt = 0; tt < PTT; ++tt) {
+            if (!tok[tt]) continue;
+#pragma unroll
+            for (int par = 0; par < 2; ++par)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int kk = 16 * (wave + 4 * tt) + 4 * (lane >> 4) + j;
+                    const int d = par + 2 * kk;
+                    if (d > kmax) continue;
+                    const float rr = 1.f / a.rowscale[a.soff[2 * mc + par] + kk];
+#pragma unroll
+                    for (int c = 0; c < NCOL; ++c)
+                        sg[(c * WLG + levl) * rowg + 2 * d + ri] = acc[c][par][tt][j] * rr;
+                }
+        }
+    }
+    __syncthreads();
+    // ---- 5. Q, S, T per (level, l), coalesced double2 stores
+    const int nl = a.l_max - m + 1;
+    for (int i = tid; i < WLG * nl; i += 256) {
+        const int d = i % nl, lv = i / nl;
+        if (lv >= nlh) continue;
+        const int lev = lev0 + lv;
+        const int l = m + d;
+        const int lc = a.lcut[lev];
+        const bool keep = m <= (lc < a.m_max ? lc : a.m_max) && l <= lc;
+        double2 q = make_double2(0.0, 0.0), s = q, t = q;
+        if (keep) {
+            const double inv0 = 1.0 / (double)lvsc[lv], inv1 = 1.0 / (double)lvsc[WLG + lv];
+            const double dm = (double)m;
+            const double cp = (double)l * (double)rec_cf(l + 1, m);   // l c_{l+1}
+            const double cm = (double)(l + 1) * (double)rec_cf(l, m); // (l+1) c_l
+            const float *gf = sg + (0 * WLG + lv) * rowg + 2 * d;
+            const float *gt = sg + (1 * WLG + lv) * rowg + 2 * d;
+            const float *gp = sg + (2 * WLG + lv) * rowg + 2 * d;
+            const double tmr = d > 0 ? (double)gt[-2] : 0.0, tmi = d > 0 ? (double)gt[-1] : 0.0;
+            const double pmr = d > 0 ? (double)gp[-2] : 0.0, pmi = d > 0 ? (double)gp[-1] : 0.0;
+            const double dtr = cp * gt[2] - cm * tmr, dti = cp * gt[3] - cm * tmi;
+            const double dpr = cp * gp[2] - cm * pmr, dpi = cp * gp[3] - cm * pmi;
+            // -i m X: re = m X.im, im = -m X.re
+            const double oll = l > 0 ? 1.0 / ((double)l * (double)(l + 1)) : 1.0;
+            q = make_double2((double)gf[0] * inv0, (double)gf[1] * inv0);
+            s = make_double2((dtr + dm * gp[1]) * inv1 * oll, (dti - dm * gp[0]) * inv1 * oll);
+            t = make_double2((-dpr + dm * gt[1]) * inv1 * oll, (-dpi - dm * gt[0]) * inv1 * oll);
+        }
+        const size_t o = ((size_t)lev * a.out_ld + (size_t)(l0 + d)) * 2;
+        *reinterpret_cast<double2 *>(a.out[0] + o) = q;
+        *reinterpret_cast<double2 *>(a.out[1] + o) = s;
+        *reinterpret_cast<double2 *>(a.out[2] + o) = t;
+    }
+}
+
 }  // namespace
 
 int launch_nsyn16(const Plan *p, int kind, const NSynArgs &args, hipStream_t s)
@@ -1186,6 +1774,24 @@ int launch_nsyn16(const Plan *p, int kind, const NSynArgs &args, hipStream_t s)
     a.nh = p->nh;
     a.n_theta = p->n_theta;
     a.nt16 = p->nt16;
+    if ((p->l_max + 1) / 2 + 1 <= PKP && p->n_theta <= WNT) {
+        // whole-K form on P only: 12 theta tiles per block
+        a.tP = p->d_syn16x;
+        a.off = p->d_syn16x_off;
+        a.rowscale = p->d_syn16x_rs;
+        a.tpb = 4 * WTT;
+        a.ngz = (p->nt16 + a.tpb - 1) / a.tpb;
+        const int ng = (a.nlev + WLG - 1) / WLG;
+        dim3 grid((unsigned)(8 * ((p->n_m + 7) / 8) * ng * a.ngz));
+        switch (kind) {
+        case NS_SG: k_nsyn16p<NS_SG><<<grid, 256, 0, s>>>(a); break;
+        case NS_QST: k_nsyn16p<NS_QST><<<grid, 256, 0, s>>>(a); break;
+        case NS_QSTC: k_nsyn16p<NS_QSTC><<<grid, 256, 0, s>>>(a); break;
+        default: return fail(MHX_E_INVALID, "unknown nonlinear-step synthesis kind");
+        }
+        MHX_HIP_CHECK(hipGetLastError());
+        return MHX_OK;
+    }
     if (p->l_max / 2 + 1 <= WKP && p->n_theta <= WNT) {
         // whole-K form: 12 theta tiles per block
         a.tpb = 4 * WTT;
@@ -1237,6 +1843,21 @@ int launch_nana16(const Plan *p, int kind, const NAnaArgs &args, hipStream_t s)
     a.nq32 = p->nq32;
     const int ntl0 = (p->l_max / 2 + 1 + 15) / 16;
     a.ngz = (ntl0 + 7) / 8;
+    const int ntlx = ((p->l_max + 1) / 2 + 1 + 15) / 16;  // l tiles with l <= l_max + 1
+    const bool gfit = 3 * WLG * (2 * (p->l_max + 2) + 2) * 4 <= 6 * 16 * AROW;
+    if (kind == NA_QST && p->nh <= AKP && ntlx <= 4 * PTT && gfit) {
+        // whole-theta form on wP only
+        a.tP = p->d_ana16x;
+        a.off = p->d_ana16x_off;
+        a.rowscale = p->d_ana16x_rs;
+        a.soff = p->d_ana16x_soff;
+        a.ngz = 1;
+        const int ng = (a.nlev + WLG - 1) / WLG;
+        dim3 grid((unsigned)(8 * ((p->n_m + 7) / 8) * ng));
+        k_nana16p<NA_QST><<<grid, 256, 0, s>>>(a);
+        MHX_HIP_CHECK(hipGetLastError());
+        return MHX_OK;
+    }
     if (p->nh <= AKP && p->l_max / 2 + 1 <= WKP && ntl0 <= 8) {
         // whole-theta form
         const int ng = (a.nlev + WLG - 1) / WLG;

@@ -464,6 +464,13 @@ int mhx_plan_destroy(mhx_plan *h)
     free_dev(p->d_ana16_soff);
     free_dev(p->d_syn16_rs);
     free_dev(p->d_ana16_rs);
+    free_dev(p->d_syn16x);
+    free_dev(p->d_ana16x);
+    free_dev(p->d_syn16x_off);
+    free_dev(p->d_ana16x_off);
+    free_dev(p->d_ana16x_soff);
+    free_dev(p->d_syn16x_rs);
+    free_dev(p->d_ana16x_rs);
     free_dev(p->ws_F);
     free_dev(p->ws_G);
     free_dev(p->ws_F2);

@@ -592,6 +592,117 @@ __global__ void k_split_ana(int l_max, int minc, int n_m, int nh, int nq4, int n
     }
 }
 
+// ---- P-only tables with l <= l_max + 1 --------------------------------
+// c_l of the three-term recurrence cos(theta) P_l = c_{l+1} P_{l+1} + c_l P_{l-1}
+// of orthonormal P_lm (src/plms.f90:81-97)
+__device__ inline double rec_c(int l, int m)
+{
+    const double dl = (double)l, dm = (double)m;
+    return sqrt((dl * dl - dm * dm) / (4.0 * dl * dl - 1.0));
+}
+
+// P_l(theta_h) for l in [m, l_max + 1] from the FP64 synthesis table, the
+// last one by one step of the recurrence
+__device__ inline double syn_p(const double *sP, const size_t *off, int mc, int m, int l,
+                               int l_max, int th, int nt16, double ct)
+{
+    auto at = [&](int ll) {
+        const int par = (ll - m) & 1, kk = (ll - m) >> 1;
+        return sP[off[2 * mc + par] + ((size_t)(kk >> 2) * nt16 + (th >> 4)) * 64 + (th & 15) + 16 * (kk & 3)];
+    };
+    if (l <= l_max) return at(l);
+    const double pl = at(l_max), plm1 = (l_max - 1 >= m) ? at(l_max - 1) : 0.0;
+    return (ct * pl - rec_c(l_max, m) * plm1) / rec_c(l_max + 1, m);
+}
+
+__device__ inline double ana_p(const double *aP, const size_t *off, int mc, int m, int l,
+                               int l_max, int th, int nq4, double ct)
+{
+    auto at = [&](int ll) {
+        const int par = (ll - m) & 1, kk = (ll - m) >> 1;
+        return aP[off[2 * mc + par] + ((size_t)(kk >> 4) * nq4 + (th >> 2)) * 64 + (kk & 15) + 16 * (th & 3)];
+    };
+    if (l <= l_max) return at(l);
+    const double pl = at(l_max), plm1 = (l_max - 1 >= m) ? at(l_max - 1) : 0.0;
+    return (ct * pl - rec_c(l_max, m) * plm1) / rec_c(l_max + 1, m);
+}
+
+// synthesis rows (mc, theta): factor from max over l of |P|, then split
+__global__ void k_split_synx(int l_max, int minc, int n_m, int nh, int nt16,
+                             const double *__restrict__ sP, const size_t *__restrict__ off,
+                             const size_t *__restrict__ offx, const double *__restrict__ theta,
+                             float *__restrict__ rs, _Float16 *__restrict__ hP)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_m * nh) return;
+    int mc = idx / nh, th = idx - mc * nh;
+    int m = mc * minc;
+    const double ct = cos(theta[th]);
+    double mx = 0.0;
+    for (int l = m; l <= l_max + 1; ++l) mx = fmax(mx, fabs(syn_p(sP, off, mc, m, l, l_max, th, nt16, ct)));
+    const float sc = pow2_scale((float)mx);
+    rs[idx] = sc;
+    for (int l = m; l <= l_max + 1; ++l) {
+        int par = (l - m) & 1, kk = (l - m) >> 1;
+        int kc = kk >> 5, kin = kk & 31;
+        int lane = (th & 15) + 16 * (kin >> 3);
+        size_t o16 = offx[2 * mc + par] + (((size_t)kc * nt16 + (th >> 4)) * 64 + lane) * 16 + (kin & 7);
+        _Float16 h, lo;
+        split_h((float)(syn_p(sP, off, mc, m, l, l_max, th, nt16, ct) * sc), h, lo);
+        hP[o16] = h;
+        hP[o16 + 8] = lo;
+    }
+}
+
+// analysis rows (mc, par, kk): factor from max over theta of |wP|
+__global__ void k_rowscale_anax(int l_max, int minc, int n_m, int nh, int nq4,
+                                const double *__restrict__ aP, const size_t *__restrict__ off,
+                                const size_t *__restrict__ soffx, const double *__restrict__ theta,
+                                float *__restrict__ rs, int nrows_total)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nrows_total) return;
+    int lo = 0, hi = 2 * n_m - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if (soffx[mid] <= (size_t)idx) lo = mid; else hi = mid - 1;
+    }
+    int blk = lo, mc = blk / 2, par = blk % 2;
+    int kk = idx - (int)soffx[blk];
+    int m = mc * minc;
+    int l = m + par + 2 * kk;
+    if (l > l_max + 1) { rs[idx] = 1.f; return; }
+    double mx = 0.0;
+    for (int th = 0; th < nh; ++th)
+        mx = fmax(mx, fabs(ana_p(aP, off, mc, m, l, l_max, th, nq4, cos(theta[th]))));
+    rs[idx] = pow2_scale((float)mx);
+}
+
+__global__ void k_split_anax(int l_max, int minc, int n_m, int nh, int nq4, int nq32,
+                             const double *__restrict__ aP, const size_t *__restrict__ off,
+                             const size_t *__restrict__ offx, const size_t *__restrict__ soffx,
+                             const double *__restrict__ theta, const float *__restrict__ rs,
+                             _Float16 *__restrict__ hP)
+{
+    // one thread per (mc, theta)
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_m * nh) return;
+    int mc = idx / nh, th = idx - mc * nh;
+    int m = mc * minc;
+    const double ct = cos(theta[th]);
+    for (int l = m; l <= l_max + 1; ++l) {
+        int par = (l - m) & 1, kk = (l - m) >> 1;
+        double s = (double)rs[soffx[2 * mc + par] + kk];
+        int kc = th >> 5, kin = th & 31;
+        int lane = (kk & 15) + 16 * (kin >> 3);
+        size_t o16 = offx[2 * mc + par] + (((size_t)(kk >> 4) * nq32 + kc) * 64 + lane) * 16 + (kin & 7);
+        _Float16 h, lo;
+        split_h((float)(ana_p(aP, off, mc, m, l, l_max, th, nq4, ct) * s), h, lo);
+        hP[o16] = h;
+        hP[o16 + 8] = lo;
+    }
+}
+
 }  // namespace
 
 int build_split_tables(Plan *p)
@@ -645,6 +756,46 @@ int build_split_tables(Plan *p)
                                           p->d_ana_wP, p->d_ana_wdP, p->d_ana_off, p->d_ana16_off,
                                           p->d_ana16_soff, p->d_ana16_rs, p->d_ana16_P,
                                           p->d_ana16_dP);
+    MHX_HIP_CHECK(hipGetLastError());
+
+    // P-only tables with l <= l_max + 1
+    std::vector<size_t> offx(2 * p->n_m), aoffx(2 * p->n_m), soffx(2 * p->n_m + 1);
+    size_t sox = 0, aox = 0, srx = 0;
+    for (int mc = 0; mc < p->n_m; ++mc) {
+        int m = mc * p->minc;
+        int K[2] = {(p->l_max + 1 - m) / 2 + 1, (p->l_max + 2 - m) / 2};
+        int Kt0 = (K[0] + 15) / 16;
+        for (int par = 0; par < 2; ++par) {
+            offx[2 * mc + par] = sox;
+            sox += (size_t)((K[par] + 31) / 32) * p->nt16 * 64 * 16;
+            aoffx[2 * mc + par] = aox;
+            aox += (size_t)Kt0 * p->nq32 * 64 * 16;
+            soffx[2 * mc + par] = srx;
+            srx += (size_t)Kt0 * 16;
+        }
+    }
+    soffx[2 * p->n_m] = srx;
+    MHX_HIP_CHECK(hipMalloc(&p->d_syn16x, sox * 2));
+    MHX_HIP_CHECK(hipMalloc(&p->d_ana16x, aox * 2));
+    MHX_HIP_CHECK(hipMemset(p->d_syn16x, 0, sox * 2));
+    MHX_HIP_CHECK(hipMemset(p->d_ana16x, 0, aox * 2));
+    MHX_HIP_CHECK(hipMalloc(&p->d_syn16x_off, offx.size() * sizeof(size_t)));
+    MHX_HIP_CHECK(hipMalloc(&p->d_ana16x_off, aoffx.size() * sizeof(size_t)));
+    MHX_HIP_CHECK(hipMalloc(&p->d_ana16x_soff, soffx.size() * sizeof(size_t)));
+    MHX_HIP_CHECK(hipMemcpy(p->d_syn16x_off, offx.data(), offx.size() * sizeof(size_t), hipMemcpyHostToDevice));
+    MHX_HIP_CHECK(hipMemcpy(p->d_ana16x_off, aoffx.data(), aoffx.size() * sizeof(size_t), hipMemcpyHostToDevice));
+    MHX_HIP_CHECK(hipMemcpy(p->d_ana16x_soff, soffx.data(), soffx.size() * sizeof(size_t), hipMemcpyHostToDevice));
+    MHX_HIP_CHECK(hipMalloc(&p->d_syn16x_rs, (size_t)p->n_m * nh * sizeof(float)));
+    MHX_HIP_CHECK(hipMalloc(&p->d_ana16x_rs, srx * sizeof(float)));
+    k_split_synx<<<(n + 127) / 128, 128>>>(p->l_max, p->minc, p->n_m, nh, p->nt16, p->d_syn_P,
+                                           p->d_syn_off, p->d_syn16x_off, p->d_theta, p->d_syn16x_rs,
+                                           p->d_syn16x);
+    k_rowscale_anax<<<(int)((srx + 127) / 128), 128>>>(p->l_max, p->minc, p->n_m, nh, p->nq4,
+                                                       p->d_ana_wP, p->d_ana_off, p->d_ana16x_soff,
+                                                       p->d_theta, p->d_ana16x_rs, (int)srx);
+    k_split_anax<<<(n + 127) / 128, 128>>>(p->l_max, p->minc, p->n_m, nh, p->nq4, p->nq32,
+                                           p->d_ana_wP, p->d_ana_off, p->d_ana16x_off,
+                                           p->d_ana16x_soff, p->d_theta, p->d_ana16x_rs, p->d_ana16x);
     MHX_HIP_CHECK(hipGetLastError());
     MHX_HIP_CHECK(hipDeviceSynchronize());
     return MHX_OK;
