@@ -101,6 +101,7 @@ struct NSynArgs {
     const size_t *off;
     const float *rowscale;
     const int *lstart;
+    const float *crec;      // c_l per (m, l), filled by the launcher
     int l_max, minc, n_m, nh, n_theta, nt16, tpb, ngz;
 };
 
@@ -118,6 +119,7 @@ struct NAnaArgs {
     const float *rowscale;
     const size_t *soff;
     const int *lstart;
+    const float *crec;      // c_l per (m, l), filled by the launcher
     int l_max, m_max, minc, n_m, nh, n_theta, ntb, nq32, ngz;
 };
 
@@ -159,6 +161,8 @@ struct Plan {
     _Float16 *d_syn16x = nullptr, *d_ana16x = nullptr;
     size_t *d_syn16x_off = nullptr, *d_ana16x_off = nullptr, *d_ana16x_soff = nullptr;
     float *d_syn16x_rs = nullptr, *d_ana16x_rs = nullptr;
+    // c_l = sqrt((l^2 - m^2) / (4 l^2 - 1)) for l = m .. l_max + 2 at lstart[mc] + 2 mc
+    float *d_crec = nullptr;
 
     // workspaces
     int ws_levels = 0;

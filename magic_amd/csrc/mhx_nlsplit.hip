@@ -643,12 +643,6 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16w(NSynArgs a)
 constexpr int PKP = 160;                 // k per parity held (l_max + 1 - m)/2 + 1 <= 160
 constexpr int PROW = 4 * PKP + 16;       // bytes per LDS row (padded)
 
-__device__ inline float rec_cf(int l, int m)
-{
-    const float dl = (float)l, dm = (float)m;
-    return l > m ? sqrtf((dl * dl - dm * dm) / (4.f * dl * dl - 1.f)) : 0.f;
-}
-
 template <int V>
 __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
 {
@@ -684,6 +678,18 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
     const int lhi = lok ? min(a.l_max, a.lcut[lev]) : -1;
     __syncthreads();
 
+    // c_l for the thread's l = lb .. lb + 4 of each round (0 past l_max + 2)
+    float cv[3][5];
+    {
+        const float *cr = a.crec + a.lstart[mc] + 2 * mc;
+#pragma unroll
+        for (int it = 0; it < 3; ++it)
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                const int d = 4 * (g0 + 32 * it) + q;
+                cv[it][q] = d <= a.l_max + 2 - m ? cr[d] : 0.f;
+            }
+    }
     // ---- 1. Q, S, T (SG: s) in FP32, l = m + 4 g + j over 3 rounds of 128 l
     {
         const double o2 = (V == NS_QSTC && lok) ? a.or2[lev] : 0.0;
@@ -731,7 +737,10 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
                 }
             }
         }
-        if (lok) atomicMax(&lvmax[levl], __float_as_uint(mx0));
+        // max over the 32 lanes of this level, then one atomic
+#pragma unroll
+        for (int off = 16; off >= 1; off >>= 1) mx0 = fmaxf(mx0, __shfl_xor(mx0, off));
+        if (lok && g0 == 0) atomicMax(&lvmax[levl], __float_as_uint(mx0));
     }
     __syncthreads();
     // ---- 2. C1 (and C2) from the neighbours; all reads, then all writes.
@@ -762,7 +771,7 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
             for (int j = 0; j < 4; ++j) {
                 const int l = lb + j;
                 if (((l - m) >> 1) >= PKP) continue;
-                const float am = (float)(l - 1) * rec_cf(l, m), ap = (float)(l + 2) * rec_cf(l + 1, m);
+                const float am = (float)(l - 1) * cv[it][j], ap = (float)(l + 2) * cv[it][j + 1];
                 // x' = am x_{l-1} - ap x_{l+1}: neighbours at q = j and j + 2
                 const float spr = am * sr[it][j] - ap * sr[it][j + 2];
                 const float spi = am * si[it][j] - ap * si[it][j + 2];
@@ -785,7 +794,12 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
                 }
             }
         }
-        if (lok) {
+#pragma unroll
+        for (int off = 16; off >= 1; off >>= 1) {
+            mx1 = fmaxf(mx1, __shfl_xor(mx1, off));
+            if constexpr (NCOL == 3) mx2 = fmaxf(mx2, __shfl_xor(mx2, off));
+        }
+        if (lok && g0 == 0) {
             atomicMax(&lvmax[WLG + levl], __float_as_uint(mx1));
             if constexpr (NCOL == 3) atomicMax(&lvmax[2 * WLG + levl], __float_as_uint(mx2));
         }
@@ -844,6 +858,15 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
     bool tok[WTT];
 #pragma unroll
     for (int t = 0; t < WTT; ++t) tok[t] = tile0 + t < a.nt16;
+    // row factors of the wave's theta rows, for the epilogue
+    float rsv[WTT][4];
+#pragma unroll
+    for (int tt = 0; tt < WTT; ++tt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int th = (tile0 + tt) * 16 + 4 * (lane >> 4) + j;
+            rsv[tt][j] = (tok[tt] && th < a.nh) ? a.rowscale[(size_t)mc * a.nh + th] : 1.f;
+        }
     const int nst = nks0 + nks1;
     struct Frag { h8 h[WTT], l[WTT]; };
     auto load_a = [&](int j, Frag &f) {
@@ -925,7 +948,7 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
                     an = ri ? fm * oan : -fm * oan;
                 }
                 if (th >= a.nh) continue;
-                const float rs = inv / a.rowscale[(size_t)mc * a.nh + th];
+                const float rs = inv / rsv[tt][j];
                 float *d = so + ol * rowf + 4 * th + ri;
                 d[0] = (sy + an) * rs;
                 d[2] = (sy - an) * rs;
@@ -1509,6 +1532,10 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
     __shared__ unsigned lvmax[2 * WLG];
     __shared__ float lvsc[2 * WLG];
     constexpr int cgrp[3] = {0, 1, 1};
+    // per block: lcut of the levels, c_l (l - m = 0 .. l_max + 2 - m), row factors
+    __shared__ int slc[WLG];
+    __shared__ float scr[4 * 16 * PTT * 2];
+    __shared__ float srs[2][16 * 4 * PTT];
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -1526,6 +1553,15 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
 #pragma unroll
     for (int i = 0; i < PTT; ++i) tok[i] = wave + 4 * i < ntl;
     if (tid < 2 * WLG) lvmax[tid] = 0u;
+    if (tid < WLG) slc[tid] = tid < nlh ? a.lcut[lev0 + tid] : -1;
+    {
+        const float *cr = a.crec + a.lstart[mc] + 2 * mc;
+        for (int d = tid; d <= a.l_max + 2 - m; d += 256) scr[d] = cr[d];
+        for (int e = tid; e < 2 * 16 * ntl; e += 256) {
+            const int par = e / (16 * ntl), kk = e - par * 16 * ntl;
+            srs[par][kk] = a.rowscale[a.soff[2 * mc + par] + kk];
+        }
+    }
     __syncthreads();
 
     // ---- 1. FP32 Sym/Anti columns (as k_nana16w)
@@ -1541,7 +1577,7 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
             const int th0 = 8 * tb + 2 * q;
             bool valid = e < nent && levl < nlh && th0 < a.nh;
             if (valid) {
-                const int lc = a.lcut[lev];
+                const int lc = slc[levl];
                 valid = m <= (lc < a.m_max ? lc : a.m_max);
             }
             const bool ok1 = th0 + 1 < a.nh;
@@ -1714,7 +1750,7 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
                     const int kk = 16 * (wave + 4 * tt) + 4 * (lane >> 4) + j;
                     const int d = par + 2 * kk;
                     if (d > kmax) continue;
-                    const float rr = 1.f / a.rowscale[a.soff[2 * mc + par] + kk];
+                    const float rr = 1.f / srs[par][kk];
 #pragma unroll
                     for (int c = 0; c < NCOL; ++c)
                         sg[(c * WLG + levl) * rowg + 2 * d + ri] = acc[c][par][tt][j] * rr;
@@ -1729,14 +1765,14 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
         if (lv >= nlh) continue;
         const int lev = lev0 + lv;
         const int l = m + d;
-        const int lc = a.lcut[lev];
+        const int lc = slc[lv];
         const bool keep = m <= (lc < a.m_max ? lc : a.m_max) && l <= lc;
         double2 q = make_double2(0.0, 0.0), s = q, t = q;
         if (keep) {
             const double inv0 = 1.0 / (double)lvsc[lv], inv1 = 1.0 / (double)lvsc[WLG + lv];
             const double dm = (double)m;
-            const double cp = (double)l * (double)rec_cf(l + 1, m);   // l c_{l+1}
-            const double cm = (double)(l + 1) * (double)rec_cf(l, m); // (l+1) c_l
+            const double cp = (double)l * (double)scr[d + 1];   // l c_{l+1}
+            const double cm = (double)(l + 1) * (double)scr[d]; // (l+1) c_l
             const float *gf = sg + (0 * WLG + lv) * rowg + 2 * d;
             const float *gt = sg + (1 * WLG + lv) * rowg + 2 * d;
             const float *gp = sg + (2 * WLG + lv) * rowg + 2 * d;
@@ -1763,6 +1799,7 @@ int launch_nsyn16(const Plan *p, int kind, const NSynArgs &args, hipStream_t s)
 {
     if (args.nlev <= 0) return MHX_OK;
     NSynArgs a = args;
+    a.crec = p->d_crec;
     a.tP = p->d_syn16_P;
     a.tdP = p->d_syn16_dP;
     a.off = p->d_syn16_off;
@@ -1827,6 +1864,7 @@ int launch_nana16(const Plan *p, int kind, const NAnaArgs &args, hipStream_t s)
 {
     if (args.nlev <= 0) return MHX_OK;
     NAnaArgs a = args;
+    a.crec = p->d_crec;
     a.tP = p->d_ana16_P;
     a.tdP = p->d_ana16_dP;
     a.off = p->d_ana16_off;

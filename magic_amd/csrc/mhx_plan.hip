@@ -471,6 +471,7 @@ int mhx_plan_destroy(mhx_plan *h)
     free_dev(p->d_ana16x_soff);
     free_dev(p->d_syn16x_rs);
     free_dev(p->d_ana16x_rs);
+    free_dev(p->d_crec);
     free_dev(p->ws_F);
     free_dev(p->ws_G);
     free_dev(p->ws_F2);

@@ -703,6 +703,17 @@ __global__ void k_split_anax(int l_max, int minc, int n_m, int nh, int nq4, int 
     }
 }
 
+// c_l = sqrt((l^2 - m^2) / (4 l^2 - 1)) (0 for l = m) in FP64, stored FP32;
+// one thread per (mc, l - m), l <= l_max + 2
+__global__ void k_crec(int l_max, int minc, int n_m, const int *__restrict__ lstart, float *__restrict__ c)
+{
+    const int mc = blockIdx.y;
+    const int m = mc * minc, d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mc >= n_m || d > l_max + 2 - m) return;
+    const int l = m + d;
+    c[lstart[mc] + 2 * mc + d] = l > m ? (float)rec_c(l, m) : 0.f;
+}
+
 }  // namespace
 
 int build_split_tables(Plan *p)
@@ -796,6 +807,13 @@ int build_split_tables(Plan *p)
     k_split_anax<<<(n + 127) / 128, 128>>>(p->l_max, p->minc, p->n_m, nh, p->nq4, p->nq32,
                                            p->d_ana_wP, p->d_ana_off, p->d_ana16x_off,
                                            p->d_ana16x_soff, p->d_theta, p->d_ana16x_rs, p->d_ana16x);
+    {
+        const size_t ncr = (size_t)p->lstart[p->n_m - 1] + 2 * (p->n_m - 1) +
+                           (p->l_max + 3 - (p->n_m - 1) * p->minc);
+        MHX_HIP_CHECK(hipMalloc(&p->d_crec, ncr * sizeof(float)));
+        dim3 g((p->l_max + 3 + 127) / 128, p->n_m);
+        k_crec<<<g, 128>>>(p->l_max, p->minc, p->n_m, p->d_lstart, p->d_crec);
+    }
     MHX_HIP_CHECK(hipGetLastError());
     MHX_HIP_CHECK(hipDeviceSynchronize());
     return MHX_OK;
