@@ -24,6 +24,7 @@
 // Blocks are dealt to the 8 XCDs by m modulo 8 (balanced work per XCD; the
 // level groups of one m run back to back on one XCD and share its L2).
 #include "mhx_internal.h"
+#include <utility>
 
 #include <cmath>
 #include <type_traits>
@@ -625,6 +626,17 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16w(NSynArgs a)
     }
 }
 
+// compile-time loop: f(integral_constant<int, S>) for S = 0 .. N - 1
+template <class F, int... S>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, S...>)
+{
+    (f(std::integral_constant<int, S>()), ...);
+}
+template <int N, class F> __device__ __forceinline__ void static_for(F &&f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>());
+}
+
 // ----------------------------------- synthesis, whole-K form on P only
 // The dP products are folded onto P by
 //   sin(theta) dP_l = l c_{l+1} P_{l+1} - (l+1) c_l P_{l-1},
@@ -858,32 +870,44 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
     bool tok[WTT];
 #pragma unroll
     for (int t = 0; t < WTT; ++t) tok[t] = tile0 + t < a.nt16;
-    // row factors of the wave's theta rows, for the epilogue
-    float rsv[WTT][4];
+    // The contraction is straight-line code: a fixed 2 NCH steps (parity
+    // 0 chunks, then parity 1 chunks), every global load unconditional, so
+    // that the fragments of the next three steps stay in flight (a load under
+    // a branch makes the wait before the next MFMA cover every outstanding
+    // load).  Chunks past a parity's K hold zero B rows and read a clamped,
+    // finite table chunk; tiles past nt16 compute on a copy of the last tile
+    // and are dropped in the epilogue.
+    constexpr int NCH = PKP / KS, NSTP = 2 * NCH;
+    float rsv[WTT][4];  // row factors of the wave's theta rows, for the epilogue
 #pragma unroll
     for (int tt = 0; tt < WTT; ++tt)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int th = (tile0 + tt) * 16 + 4 * (lane >> 4) + j;
-            rsv[tt][j] = (tok[tt] && th < a.nh) ? a.rowscale[(size_t)mc * a.nh + th] : 1.f;
+            const int th = min((tile0 + tt) * 16 + 4 * (lane >> 4) + j, a.nh - 1);
+            rsv[tt][j] = a.rowscale[(size_t)mc * a.nh + th];
         }
-    const int nst = nks0 + nks1;
     struct Frag { h8 h[WTT], l[WTT]; };
-    auto load_a = [&](int j, Frag &f) {
-        if (j >= nst) return;
-        const int par = j >= nks0, ks = j - par * nks0;
+    Frag fr[4];
+    const size_t toff0 = a.off[2 * mc], toff1 = a.off[2 * mc + 1];
+    auto load_s = [&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        if constexpr (S < NSTP) {
+            constexpr int PAR = S / NCH;
+            const int ks = min(S % NCH, (PAR ? nks1 : nks0) - 1);
 #pragma unroll
-        for (int tt = 0; tt < WTT; ++tt) {
-            if (tok[tt]) {
-                const size_t to = a.off[2 * mc + par] + (((size_t)ks * a.nt16 + tile0 + tt) * 64 + lane) * 16;
-                f.h[tt] = *reinterpret_cast<const h8 *>(a.tP + to);
-                f.l[tt] = *reinterpret_cast<const h8 *>(a.tP + to + 8);
+            for (int tt = 0; tt < WTT; ++tt) {
+                const int tile = min(tile0 + tt, a.nt16 - 1);
+                const size_t to = (PAR ? toff1 : toff0) + (((size_t)ks * a.nt16 + tile) * 64 + lane) * 16;
+                fr[S % 4].h[tt] = *reinterpret_cast<const h8 *>(a.tP + to);
+                fr[S % 4].l[tt] = *reinterpret_cast<const h8 *>(a.tP + to + 8);
             }
         }
     };
-    auto step = [&](auto parc, int ks, const Frag &f) {
-        constexpr int PAR = decltype(parc)::value;
-        const int kb = 2 * (ks * KS + 8 * (lane >> 4));
+    auto comp_s = [&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        constexpr int PAR = S / NCH, KC = S % NCH;
+        const Frag &f = fr[S % 4];
+        const int kb = 2 * (KC * KS + 8 * (lane >> 4));
 #pragma unroll
         for (int c = 0; c < NCOL; ++c) {
             const int row = (c * 2 + PAR) * 16 + (lane & 15);
@@ -891,7 +915,6 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
             const h8 bl = *reinterpret_cast<const h8 *>(sb + row * PROW + 2 * PKP + kb);
 #pragma unroll
             for (int tt = 0; tt < WTT; ++tt) {
-                if (!tok[tt]) continue;
                 f4 t = acc[c][PAR][tt];
                 t = mfma16(f.h[tt], bl, t);
                 t = mfma16(f.l[tt], bh, t);
@@ -900,28 +923,17 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
             }
         }
     };
-    auto run = [&](int j, const Frag &f) {
-        if (j < nks0) step(std::integral_constant<int, 0>(), j, f);
-        else step(std::integral_constant<int, 1>(), j - nks0, f);
-    };
-    // ring of four fragment sets: step j's loads are issued at step j - 3
-    Frag f0, f1, f2, f3;
-    load_a(0, f0);
-    load_a(1, f1);
-    load_a(2, f2);
-    for (int j = 0; j < nst; j += 4) {
-        load_a(j + 3, f3);
-        run(j, f0);
-        if (j + 1 >= nst) break;
-        load_a(j + 4, f0);
-        run(j + 1, f1);
-        if (j + 2 >= nst) break;
-        load_a(j + 5, f1);
-        run(j + 2, f2);
-        if (j + 3 >= nst) break;
-        load_a(j + 6, f2);
-        run(j + 3, f3);
-    }
+    load_s(std::integral_constant<int, 0>());
+    load_s(std::integral_constant<int, 1>());
+    load_s(std::integral_constant<int, 2>());
+    __builtin_amdgcn_sched_barrier(0);
+    static_for<NSTP>([&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        load_s(std::integral_constant<int, S + 3>());
+        __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of this step
+        comp_s(sc);
+        __builtin_amdgcn_sched_barrier(0);
+    });
 
     // ---- 5. epilogue, one output at a time (contiguous theta runs)
     float *so = reinterpret_cast<float *>(sb);
@@ -1679,26 +1691,34 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
         for (int s2 = 0; s2 < 2; ++s2)
 #pragma unroll
             for (int t = 0; t < PTT; ++t) acc[o][s2][t] = f4{0.f, 0.f, 0.f, 0.f};
-    // half step h: theta chunk kc = h >> 1, parity h & 1; the fragments of
-    // half step h are loaded at half step h - 3 (ring of four)
-    const int nhs = 2 * a.nq32;
+    // Straight-line contraction over a fixed 2 AKP / KS half steps (theta
+    // chunk H / 2, parity H % 2), all loads unconditional, the fragments of
+    // half step H loaded at H - 3 (ring of four).  Chunks past nq32 read a
+    // clamped table chunk against zero B rows (theta_h >= nh is zero in
+    // LDS); tiles past ntl are dropped in the epilogue.
+    constexpr int NHS = 2 * (AKP / KS);
     struct AF { h8 h[PTT], l[PTT]; };
-    auto load_h = [&](int h, AF &f) {
-        if (h >= nhs) return;
-        const int kc = h >> 1, par = h & 1;
+    AF fr[4];
+    const size_t toff0 = a.off[2 * mc], toff1 = a.off[2 * mc + 1];
+    auto load_h = [&](auto hc) {
+        constexpr int H = decltype(hc)::value;
+        if constexpr (H < NHS) {
+            constexpr int PAR = H % 2;
+            const int kc = min(H / 2, a.nq32 - 1);
 #pragma unroll
-        for (int tt = 0; tt < PTT; ++tt) {
-            if (tok[tt]) {
-                const size_t to = a.off[2 * mc + par] +
-                                  (((size_t)(wave + 4 * tt) * a.nq32 + kc) * 64 + lane) * 16;
-                f.h[tt] = *reinterpret_cast<const h8 *>(a.tP + to);
-                f.l[tt] = *reinterpret_cast<const h8 *>(a.tP + to + 8);
+            for (int tt = 0; tt < PTT; ++tt) {
+                const int tile = min(wave + 4 * tt, ntl - 1);
+                const size_t to = (PAR ? toff1 : toff0) + (((size_t)tile * a.nq32 + kc) * 64 + lane) * 16;
+                fr[H % 4].h[tt] = *reinterpret_cast<const h8 *>(a.tP + to);
+                fr[H % 4].l[tt] = *reinterpret_cast<const h8 *>(a.tP + to + 8);
             }
         }
     };
-    auto half = [&](auto parc, int kc, const AF &f) {
-        constexpr int PAR = decltype(parc)::value;
-        const int kb = 2 * (kc * KS + 8 * (lane >> 4));
+    auto comp_h = [&](auto hc) {
+        constexpr int H = decltype(hc)::value;
+        constexpr int PAR = H % 2, KC = H / 2;
+        const AF &f = fr[H % 4];
+        const int kb = 2 * (KC * KS + 8 * (lane >> 4));
 #pragma unroll
         for (int c = 0; c < NCOL; ++c) {
             const int row = (c * 2 + PAR) * 16 + (lane & 15);
@@ -1706,7 +1726,6 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
             const h8 bl = *reinterpret_cast<const h8 *>(sb + row * AROW + 2 * AKP + kb);
 #pragma unroll
             for (int tt = 0; tt < PTT; ++tt) {
-                if (!tok[tt]) continue;
                 f4 t = acc[c][PAR][tt];
                 t = mfma16(f.h[tt], bl, t);
                 t = mfma16(f.l[tt], bh, t);
@@ -1715,23 +1734,17 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
             }
         }
     };
-    using P0 = std::integral_constant<int, 0>;
-    using P1 = std::integral_constant<int, 1>;
-    AF f0, f1, f2, f3;
-    load_h(0, f0);
-    load_h(1, f1);
-    load_h(2, f2);
-    for (int h = 0; h < nhs; h += 4) {  // nhs is even
-        load_h(h + 3, f3);
-        half(P0(), h >> 1, f0);
-        load_h(h + 4, f0);
-        half(P1(), h >> 1, f1);
-        if (h + 2 >= nhs) break;
-        load_h(h + 5, f1);
-        half(P0(), (h >> 1) + 1, f2);
-        load_h(h + 6, f2);
-        half(P1(), (h >> 1) + 1, f3);
-    }
+    load_h(std::integral_constant<int, 0>());
+    load_h(std::integral_constant<int, 1>());
+    load_h(std::integral_constant<int, 2>());
+    __builtin_amdgcn_sched_barrier(0);
+    static_for<NHS>([&](auto hc) {
+        constexpr int H = decltype(hc)::value;
+        load_h(std::integral_constant<int, H + 3>());
+        __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of this step
+        comp_h(hc);
+        __builtin_amdgcn_sched_barrier(0);
+    });
 
     // ---- 4. G (row factors removed) into LDS: [c][level][l - m] complex float
     float *sg = reinterpret_cast<float *>(sb);
