@@ -706,19 +706,30 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
     {
         const double o2 = (V == NS_QSTC && lok) ? a.or2[lev] : 0.0;
         float mx0 = 0.f;
+        // all loads of the three rounds first, unconditional at a clamped l
+        // (a load under a branch waits for itself before the next one);
+        // values past lhi are zeroed after
+        double2 xa[3][4][NSRC];
+#pragma unroll
+        for (int it = 0; it < 3; ++it)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int l = m + 4 * (g0 + 32 * it) + j;
+                const int lc = min(l, max(lhi, m));
+                const size_t o = ((size_t)(lok ? lev : lev0) * a.src_ld + (size_t)(l0 + lc - m)) * 2;
+#pragma unroll
+                for (int s2 = 0; s2 < NSRC; ++s2) xa[it][j][s2] = *reinterpret_cast<const double2 *>(a.src[s2] + o);
+            }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int it = 0; it < 3; ++it) {
             const int g = g0 + 32 * it;
-            double2 x[4][NSRC];
+            double2 (&x)[4][NSRC] = xa[it];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int l = m + 4 * g + j;
-                const size_t o = ((size_t)(lok ? lev : 0) * a.src_ld + (size_t)(l0 + l - m)) * 2;
+            for (int j = 0; j < 4; ++j)
 #pragma unroll
                 for (int s2 = 0; s2 < NSRC; ++s2)
-                    x[j][s2] = (l <= lhi) ? *reinterpret_cast<const double2 *>(a.src[s2] + o)
-                                          : make_double2(0.0, 0.0);
-            }
+                    if (m + 4 * g + j > lhi) x[j][s2] = make_double2(0.0, 0.0);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int l = m + 4 * g + j;
