@@ -1549,7 +1549,7 @@ constexpr int PTT = 3;  // l tiles per wave
 template <int V>
 __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
 {
-    constexpr int NCOL = 3, NROW = NCOL * 2 * 16;
+    constexpr int NCOL = V == NA_SH ? 1 : 3, NROW = NCOL * 2 * 16;
     constexpr int SB = NROW * AROW;
     __shared__ __align__(16) unsigned char sb[SB];
     __shared__ unsigned lvmax[2 * WLG];
@@ -1793,27 +1793,32 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
         const bool keep = m <= (lc < a.m_max ? lc : a.m_max) && l <= lc;
         double2 q = make_double2(0.0, 0.0), s = q, t = q;
         if (keep) {
-            const double inv0 = 1.0 / (double)lvsc[lv], inv1 = 1.0 / (double)lvsc[WLG + lv];
-            const double dm = (double)m;
-            const double cp = (double)l * (double)scr[d + 1];   // l c_{l+1}
-            const double cm = (double)(l + 1) * (double)scr[d]; // (l+1) c_l
+            const double inv0 = 1.0 / (double)lvsc[lv];
             const float *gf = sg + (0 * WLG + lv) * rowg + 2 * d;
-            const float *gt = sg + (1 * WLG + lv) * rowg + 2 * d;
-            const float *gp = sg + (2 * WLG + lv) * rowg + 2 * d;
-            const double tmr = d > 0 ? (double)gt[-2] : 0.0, tmi = d > 0 ? (double)gt[-1] : 0.0;
-            const double pmr = d > 0 ? (double)gp[-2] : 0.0, pmi = d > 0 ? (double)gp[-1] : 0.0;
-            const double dtr = cp * gt[2] - cm * tmr, dti = cp * gt[3] - cm * tmi;
-            const double dpr = cp * gp[2] - cm * pmr, dpi = cp * gp[3] - cm * pmi;
-            // -i m X: re = m X.im, im = -m X.re
-            const double oll = l > 0 ? 1.0 / ((double)l * (double)(l + 1)) : 1.0;
             q = make_double2((double)gf[0] * inv0, (double)gf[1] * inv0);
-            s = make_double2((dtr + dm * gp[1]) * inv1 * oll, (dti - dm * gp[0]) * inv1 * oll);
-            t = make_double2((-dpr + dm * gt[1]) * inv1 * oll, (-dpi - dm * gt[0]) * inv1 * oll);
+            if constexpr (V == NA_QST) {
+                const double inv1 = 1.0 / (double)lvsc[WLG + lv];
+                const double dm = (double)m;
+                const double cp = (double)l * (double)scr[d + 1];   // l c_{l+1}
+                const double cm = (double)(l + 1) * (double)scr[d]; // (l+1) c_l
+                const float *gt = sg + (1 * WLG + lv) * rowg + 2 * d;
+                const float *gp = sg + (2 * WLG + lv) * rowg + 2 * d;
+                const double tmr = d > 0 ? (double)gt[-2] : 0.0, tmi = d > 0 ? (double)gt[-1] : 0.0;
+                const double pmr = d > 0 ? (double)gp[-2] : 0.0, pmi = d > 0 ? (double)gp[-1] : 0.0;
+                const double dtr = cp * gt[2] - cm * tmr, dti = cp * gt[3] - cm * tmi;
+                const double dpr = cp * gp[2] - cm * pmr, dpi = cp * gp[3] - cm * pmi;
+                // -i m X: re = m X.im, im = -m X.re
+                const double oll = l > 0 ? 1.0 / ((double)l * (double)(l + 1)) : 1.0;
+                s = make_double2((dtr + dm * gp[1]) * inv1 * oll, (dti - dm * gp[0]) * inv1 * oll);
+                t = make_double2((-dpr + dm * gt[1]) * inv1 * oll, (-dpi - dm * gt[0]) * inv1 * oll);
+            }
         }
         const size_t o = ((size_t)lev * a.out_ld + (size_t)(l0 + d)) * 2;
         *reinterpret_cast<double2 *>(a.out[0] + o) = q;
-        *reinterpret_cast<double2 *>(a.out[1] + o) = s;
-        *reinterpret_cast<double2 *>(a.out[2] + o) = t;
+        if constexpr (V == NA_QST) {
+            *reinterpret_cast<double2 *>(a.out[1] + o) = s;
+            *reinterpret_cast<double2 *>(a.out[2] + o) = t;
+        }
     }
 }
 
@@ -1907,7 +1912,7 @@ int launch_nana16(const Plan *p, int kind, const NAnaArgs &args, hipStream_t s)
     a.ngz = (ntl0 + 7) / 8;
     const int ntlx = ((p->l_max + 1) / 2 + 1 + 15) / 16;  // l tiles with l <= l_max + 1
     const bool gfit = 3 * WLG * (2 * (p->l_max + 2) + 2) * 4 <= 6 * 16 * AROW;
-    if (kind == NA_QST && p->nh <= AKP && ntlx <= 4 * PTT && gfit) {
+    if ((kind == NA_QST || kind == NA_SH) && p->nh <= AKP && ntlx <= 4 * PTT && gfit) {
         // whole-theta form on wP only
         a.tP = p->d_ana16x;
         a.off = p->d_ana16x_off;
@@ -1916,7 +1921,8 @@ int launch_nana16(const Plan *p, int kind, const NAnaArgs &args, hipStream_t s)
         a.ngz = 1;
         const int ng = (a.nlev + WLG - 1) / WLG;
         dim3 grid((unsigned)(8 * ((p->n_m + 7) / 8) * ng));
-        k_nana16p<NA_QST><<<grid, 256, 0, s>>>(a);
+        if (kind == NA_QST) k_nana16p<NA_QST><<<grid, 256, 0, s>>>(a);
+        else k_nana16p<NA_SH><<<grid, 256, 0, s>>>(a);
         MHX_HIP_CHECK(hipGetLastError());
         return MHX_OK;
     }
