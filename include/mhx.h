@@ -154,7 +154,11 @@ int mhx_spat_to_qst_batch(mhx_plan *p, int nlev, const double *f,
  * out[10] = AdvrLM AdvtLM AdvpLM VSrLM VStLM VSpLM VxBrLM VxBtLM VxBpLM
  *           uGsLM (coefficients, ld_lm)
  * radial: per-level or2, or4, orho1 (nlev each) and LFfac; lcut[nlev].
- * All arrays device memory except lcut/radial (host). */
+ * Coefficient arrays may be host or device memory; lcut/radial are host.
+ * With a non-null stream and device outputs the call returns without
+ * waiting: the work is ordered after `stream` and `stream` after the work
+ * (the host arrays lcut/radial are copied before the call returns).
+ * Otherwise the call returns when the outputs are written. */
 int mhx_nl_step(mhx_plan *p, int nlev, const double *const *in, size_t ld_lm,
                 double *const *out, const double *or2, const double *or4,
                 const double *orho1, double LFfac, const int *lcut,

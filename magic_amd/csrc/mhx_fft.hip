@@ -877,9 +877,14 @@ int launch_slots(const Plan *p, const FFTArgs &f, int nlev, const void *Fin, voi
     nthr = (nthr + 63) / 64 * 64;
     if (NPT * nthr < p->n_phi) return fail(MHX_E_UNSUPPORTED, "n_phi too large for the fused stage");
     const size_t shm = ((size_t)2 * M + 1 + 6 * (size_t)M) * 2 * sizeof(T);
-    if (shm > 64 * 1024)
+    // set once per kernel (the attribute persists), so that a call under
+    // graph capture makes no attribute call
+    static int shm_set = 0;
+    if (shm > 64 * 1024 && (int)shm > shm_set) {
         MHX_HIP_CHECK(hipFuncSetAttribute((const void *)k_nl_slots<T, A, B>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        shm_set = (int)shm;
+    }
     dim3 grid((unsigned)(p->n_theta * nlev));
     k_nl_slots<T, A, B><<<grid, nthr, shm, s>>>(f, static_cast<const T *>(Fin), static_cast<T *>(Fout),
                                                 p->n_m, p->n_theta, nlev, p->d_osin2, rad, LFfac,

@@ -175,8 +175,18 @@ struct Plan {
     int *d_lcut = nullptr;      int d_lcut_n = 0;
     double *d_rad = nullptr;    int d_rad_n = 0;          // or2, or4, orho1
     std::vector<double> h_rad;  // host copy of the per-level radial factors
+    // pinned staging of the nonlinear step's per-level factors and lcut
+    void *h_pin = nullptr;      size_t h_pin_bytes = 0;
+    // the split nonlinear step's launches as a replayed graph, keyed by the
+    // call's arrays and sizes (captured on the second call with one key)
+    hipGraphExec_t gexec = nullptr;
+    std::vector<unsigned long long> gkey, gkey_seen;
     hipStream_t stream = nullptr;
     hipEvent_t ev_user = nullptr;  // orders the plan stream after a caller's stream
+    hipEvent_t ev_done = nullptr;  // orders a caller's stream after the plan stream
+    hipEvent_t ev_pin = nullptr;   // after the uploads from h_pin
+    bool pin_pending = false;      // ev_pin recorded and not yet waited for
+    bool last_host_out = false;    // the last nonlinear step copied outputs to host
 
     bool timing = false;
     static constexpr int NEV = 16;

@@ -1,5 +1,6 @@
 // Plan management and the C ABI of include/mhx.h.
 #include "mhx_internal.h"
+#include <cstdint>
 
 #include <algorithm>
 #include <cmath>
@@ -431,6 +432,8 @@ int mhx_plan_create(int l_max, int m_max, int minc, int n_theta, int n_phi, int 
     }
     for (int i = 0; i < Plan::NEV; ++i) (void)hipEventCreate(&p->ev[i]);
     (void)hipEventCreateWithFlags(&p->ev_user, hipEventDisableTiming);
+    (void)hipEventCreateWithFlags(&p->ev_done, hipEventDisableTiming);
+    (void)hipEventCreateWithFlags(&p->ev_pin, hipEventDisableTiming);
     *plan = h;
     return MHX_OK;
 }
@@ -482,6 +485,10 @@ int mhx_plan_destroy(mhx_plan *h)
     for (int i = 0; i < Plan::NEV; ++i)
         if (p->ev[i]) (void)hipEventDestroy(p->ev[i]);
     if (p->ev_user) (void)hipEventDestroy(p->ev_user);
+    if (p->ev_done) (void)hipEventDestroy(p->ev_done);
+    if (p->ev_pin) (void)hipEventDestroy(p->ev_pin);
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->h_pin) (void)hipHostFree(p->h_pin);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete h;
     return MHX_OK;
@@ -820,9 +827,9 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
     for (int i = 0; i < 10; ++i)
         if (!out[i]) return fail(MHX_E_INVALID, "null output array " + std::to_string(i));
     int rc;
-    if ((rc = order_after(P, stream)) || (rc = ensure_ws(P, nlev)) ||
-        (rc = upload_lcut(P, lcut, nlev)))
-        return rc;
+    if ((rc = order_after(P, stream)) || (rc = ensure_ws(P, nlev))) return rc;
+    for (int i = 0; i < nlev; ++i)
+        if ((rc = check_lcut(P, lcut[i]))) return rc;
     // host arrays (the caller's *_Rloc fields) are staged through ws_H:
     // inputs copied in here, outputs copied back after the last launch
     const double *din[11];
@@ -847,6 +854,7 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
             din[i] = d;
         }
         for (int i = 0; i < 10; ++i) dout[i] = hout[i] ? P->ws_H + (size_t)slot++ * arr : out[i];
+        P->last_host_out = nh_out > 0;
     }
     std::vector<double> &rad = P->h_rad;
     rad.assign(3 * (size_t)nlev, 0.0);
@@ -855,8 +863,31 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
         rad[nlev + i] = or4[i];
         rad[2 * nlev + i] = orho1[i];
     }
-    MHX_HIP_CHECK(hipMemcpyAsync(P->d_rad, rad.data(), rad.size() * sizeof(double),
-                                 hipMemcpyHostToDevice, P->stream));
+    {
+        // per-level factors and lcut through a pinned buffer: the copies are
+        // then asynchronous (the previous call's copies completed before its
+        // end-of-call synchronisation, so the buffer is free to refill)
+        const size_t need = rad.size() * sizeof(double) + (size_t)nlev * sizeof(int);
+        if (P->h_pin_bytes < need) {
+            if (P->h_pin) (void)hipHostFree(P->h_pin);
+            P->h_pin = nullptr;
+            P->h_pin_bytes = 0;
+            MHX_HIP_CHECK(hipHostMalloc(&P->h_pin, need, hipHostMallocDefault));
+            P->h_pin_bytes = need;
+        }
+        if (P->pin_pending) MHX_HIP_CHECK(hipEventSynchronize(P->ev_pin));  // previous copies read
+        P->pin_pending = false;
+        double *hr = static_cast<double *>(P->h_pin);
+        int *hl = reinterpret_cast<int *>(hr + rad.size());
+        std::memcpy(hr, rad.data(), rad.size() * sizeof(double));
+        std::memcpy(hl, lcut, (size_t)nlev * sizeof(int));
+        MHX_HIP_CHECK(hipMemcpyAsync(P->d_rad, hr, rad.size() * sizeof(double), hipMemcpyHostToDevice,
+                                     P->stream));
+        MHX_HIP_CHECK(hipMemcpyAsync(P->d_lcut, hl, (size_t)nlev * sizeof(int), hipMemcpyHostToDevice,
+                                     P->stream));
+        MHX_HIP_CHECK(hipEventRecord(P->ev_pin, P->stream));
+        P->pin_pending = true;
+    }
     hipStream_t s = P->stream;
     // a split-precision plan runs the Legendre stage in scaled FP16 x3 and
     // keeps Fourier data and grids in FP32; the per-call API stays FP64
@@ -871,97 +902,145 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
     // inputs: 0 w, 1 dw, 2 ddw, 3 z, 4 dz, 5 s, 6 b, 7 db, 8 ddb, 9 aj, 10 dj
     enum { W = 0, DW, DDW, Z, DZ, S, B, DB, DDB, AJ, DJ };
     // s, ds/dtheta, ds/dphi -> comps 15..17
-    if (split) {
-        // FP32 Fourier data; F2 theta-blocked; compile-time column operations
-        NSynArgs a;
-        std::memset(&a, 0, sizeof(a));
-        a.src_ld = ld_lm;
-        a.fout = reinterpret_cast<float *>(P->ws_F);
-        a.fout_ld = (size_t)18 * P->n_m * P->n_theta;
-        a.lcut = P->d_lcut;
-        a.or2 = P->d_rad;
-        a.nlev = nlev;
-        a.src[0] = din[S];
-        a.comp0 = 15;
-        if ((rc = launch_nsyn16(P, NS_SG, a, s)) || (rc = mark())) return rc;
-        // u, curl u, du/dr, B, curl B (see the FP64 table below)
-        const int qsrc[5][4] = {{W, DW, Z, -1}, {Z, DZ, W, DDW}, {DW, DDW, DZ, -1},
-                                {B, DB, AJ, -1}, {AJ, DJ, B, DDB}};
-        for (int v = 0; v < 5; ++v) {
-            for (int k = 0; k < 4; ++k) a.src[k] = qsrc[v][k] >= 0 ? din[qsrc[v][k]] : nullptr;
-            a.comp0 = 3 * v;
-            const int kind = qsrc[v][3] >= 0 ? NS_QSTC : NS_QST;
-            if ((rc = launch_nsyn16(P, kind, a, s)) || (rc = mark())) return rc;
-        }
-        if ((rc = launch_nl_fused(P, nlev, P->ws_F, P->ws_F2, P->d_rad, LFfac, P->ws_G, grids, s)) ||
-            (rc = mark()))
-            return rc;
-        NAnaArgs b;
-        std::memset(&b, 0, sizeof(b));
-        b.fin = reinterpret_cast<const float *>(P->ws_F2);
-        b.fin_ld = (size_t)10 * P->n_m * ((P->n_theta + 15) / 16 * 16);
-        b.out_ld = ld_lm;
-        b.lcut = P->d_lcut;
-        b.nlev = nlev;
-        for (int v = 0; v < 4; ++v) {
-            const int nout = v < 3 ? 3 : 1;
-            for (int k = 0; k < 3; ++k) {
-                b.in_comp[k] = k < nout ? 3 * v + k : -1;
-                b.out[k] = k < nout ? dout[3 * v + k] : nullptr;
+    {
+        auto launch_split = [&]() -> int {
+            int rc2;
+            // FP32 Fourier data; F2 theta-blocked; compile-time column operations
+            NSynArgs a;
+            std::memset(&a, 0, sizeof(a));
+            a.src_ld = ld_lm;
+            a.fout = reinterpret_cast<float *>(P->ws_F);
+            a.fout_ld = (size_t)18 * P->n_m * P->n_theta;
+            a.lcut = P->d_lcut;
+            a.or2 = P->d_rad;
+            a.nlev = nlev;
+            a.src[0] = din[S];
+            a.comp0 = 15;
+            if ((rc2 = launch_nsyn16(P, NS_SG, a, s)) || (rc2 = mark())) return rc2;
+            // u, curl u, du/dr, B, curl B (see the FP64 table below)
+            const int qsrc[5][4] = {{W, DW, Z, -1}, {Z, DZ, W, DDW}, {DW, DDW, DZ, -1},
+                                    {B, DB, AJ, -1}, {AJ, DJ, B, DDB}};
+            for (int v = 0; v < 5; ++v) {
+                for (int k = 0; k < 4; ++k) a.src[k] = qsrc[v][k] >= 0 ? din[qsrc[v][k]] : nullptr;
+                a.comp0 = 3 * v;
+                const int kind = qsrc[v][3] >= 0 ? NS_QSTC : NS_QST;
+                if ((rc2 = launch_nsyn16(P, kind, a, s)) || (rc2 = mark())) return rc2;
             }
-            if ((rc = launch_nana16(P, v < 3 ? NA_QST : NA_SH, b, s)) || (rc = mark())) return rc;
+            if ((rc2 = launch_nl_fused(P, nlev, P->ws_F, P->ws_F2, P->d_rad, LFfac, P->ws_G, grids, s)) ||
+                (rc2 = mark()))
+                return rc2;
+            NAnaArgs b;
+            std::memset(&b, 0, sizeof(b));
+            b.fin = reinterpret_cast<const float *>(P->ws_F2);
+            b.fin_ld = (size_t)10 * P->n_m * ((P->n_theta + 15) / 16 * 16);
+            b.out_ld = ld_lm;
+            b.lcut = P->d_lcut;
+            b.nlev = nlev;
+            for (int v = 0; v < 4; ++v) {
+                const int nout = v < 3 ? 3 : 1;
+                for (int k = 0; k < 3; ++k) {
+                    b.in_comp[k] = k < nout ? 3 * v + k : -1;
+                    b.out[k] = k < nout ? dout[3 * v + k] : nullptr;
+                }
+                if ((rc2 = launch_nana16(P, v < 3 ? NA_QST : NA_SH, b, s)) || (rc2 = mark())) return rc2;
+            }
+            return MHX_OK;
+        };
+        auto launch_fp64 = [&]() -> int {
+            int rc2;
+            {
+                SynJob j = syn_job(P, SYN_SG, nlev, ld_lm);
+                j.src[0] = din[S];
+                j.col[0] = op(0);
+                j.col[1] = op(0, G_ONE, 1, 1);
+                j.out_comp[0] = 15; j.out_comp[1] = 16; j.out_comp[2] = 17;
+                if ((rc2 = launch_syn(P, j, s)) || (rc2 = mark())) return rc2;
+            }
+            struct QSpec { int q, sidx, t0, t1; bool curl; int comp0; };
+            // torpol_to_spat(a, b, c): Q = l(l+1) a, S = b, T = c
+            // torpol_to_curl_spat(or2, B, ddB, J, dJ): Q = l(l+1) J, S = dJ, T = or2 l(l+1) B - ddB
+            const QSpec qs[5] = {
+                {W, DW, Z, -1, false, 0},     // u
+                {Z, DZ, W, DDW, true, 3},     // curl u
+                {DW, DDW, DZ, -1, false, 6},  // du/dr
+                {B, DB, AJ, -1, false, 9},    // B
+                {AJ, DJ, B, DDB, true, 12},   // curl B
+            };
+            for (const QSpec &q : qs) {
+                SynJob j = syn_job(P, SYN_QST, nlev, ld_lm);
+                j.src[0] = din[q.q];
+                j.src[1] = din[q.sidx];
+                j.src[2] = din[q.t0];
+                j.src[3] = q.t1 >= 0 ? din[q.t1] : nullptr;
+                ColOp T = q.curl ? op(2, G_OR2LL, 0, 0, 1.0, 3, -1.0) : op(2);
+                qst_cols(j, op(0, G_LL), op(1), T);
+                j.out_comp[0] = q.comp0; j.out_comp[1] = q.comp0 + 1; j.out_comp[2] = q.comp0 + 2;
+                if ((rc2 = launch_syn(P, j, s)) || (rc2 = mark())) return rc2;
+            }
+            // fused: inverse FFTs, grid products, forward FFTs; du/dr grids to ws_G
+            if (((rc2 = launch_nl_fused(P, nlev, P->ws_F, P->ws_F2, P->d_rad, LFfac, P->ws_G,
+                                                 grids, s)) ||
+                           (rc2 = mark())))
+                return rc2;
+            for (int v = 0; v < 3; ++v) {
+                AnaJob j = ana_job(P, ANA_QST, nlev, ld_lm);
+                j.in_comp[0] = 3 * v;
+                j.in_comp[1] = 3 * v + 1;
+                j.in_comp[2] = 3 * v + 2;
+                j.out[0] = dout[3 * v];
+                j.out[1] = dout[3 * v + 1];
+                j.out[2] = dout[3 * v + 2];
+                if ((rc2 = launch_ana(P, j, s)) || (rc2 = mark())) return rc2;
+            }
+            {
+                AnaJob j = ana_job(P, ANA_SH, nlev, ld_lm);
+                j.in_comp[0] = 9;
+                j.out[0] = dout[9];
+                if ((rc2 = launch_ana(P, j, s)) || (rc2 = mark())) return rc2;
+            }
+            return MHX_OK;
+        };
+        auto launch_all = [&]() -> int { return split ? launch_split() : launch_fp64(); };
+        if (P->timing) {
+            if ((rc = launch_all())) return rc;
+        } else {
+            std::vector<unsigned long long> key = {(unsigned long long)nlev, (unsigned long long)ld_lm,
+                                                   (unsigned long long)(uintptr_t)grids};
+            unsigned long long lf;
+            std::memcpy(&lf, &LFfac, sizeof(lf));
+            key.push_back(lf);
+            for (int i = 0; i < 11; ++i) key.push_back((unsigned long long)(uintptr_t)din[i]);
+            for (int i = 0; i < 10; ++i) key.push_back((unsigned long long)(uintptr_t)dout[i]);
+            // workspaces the launches address (reallocated when nlev grows)
+            for (const void *w : {(const void *)P->ws_F, (const void *)P->ws_F2, (const void *)P->ws_G,
+                                  (const void *)P->d_lcut, (const void *)P->d_rad})
+                key.push_back((unsigned long long)(uintptr_t)w);
+            if (P->gexec && key == P->gkey) {
+                MHX_HIP_CHECK(hipGraphLaunch(P->gexec, s));
+            } else if (key == P->gkey_seen) {
+                // second call with these arrays: capture the launches once
+                hipGraph_t g = nullptr;
+                MHX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                const int rcl = launch_all();
+                const hipError_t ec = hipStreamEndCapture(s, &g);
+                if (rcl) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rcl;
+                }
+                if (ec != hipSuccess) return fail(MHX_E_HIP, "graph capture of the nonlinear step failed");
+                hipGraphExec_t ex = nullptr;
+                const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ei != hipSuccess) return fail(MHX_E_HIP, "graph instantiation failed");
+                if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
+                P->gexec = ex;
+                P->gkey = key;
+                MHX_HIP_CHECK(hipGraphLaunch(P->gexec, s));
+            } else {
+                if ((rc = launch_all())) return rc;
+                P->gkey_seen = key;
+            }
         }
-    }
-    if (!split) {
-        SynJob j = syn_job(P, SYN_SG, nlev, ld_lm);
-        j.src[0] = din[S];
-        j.col[0] = op(0);
-        j.col[1] = op(0, G_ONE, 1, 1);
-        j.out_comp[0] = 15; j.out_comp[1] = 16; j.out_comp[2] = 17;
-        if ((rc = launch_syn(P, j, s)) || (rc = mark())) return rc;
-    }
-    struct QSpec { int q, sidx, t0, t1; bool curl; int comp0; };
-    // torpol_to_spat(a, b, c): Q = l(l+1) a, S = b, T = c
-    // torpol_to_curl_spat(or2, B, ddB, J, dJ): Q = l(l+1) J, S = dJ, T = or2 l(l+1) B - ddB
-    const QSpec qs[5] = {
-        {W, DW, Z, -1, false, 0},     // u
-        {Z, DZ, W, DDW, true, 3},     // curl u
-        {DW, DDW, DZ, -1, false, 6},  // du/dr
-        {B, DB, AJ, -1, false, 9},    // B
-        {AJ, DJ, B, DDB, true, 12},   // curl B
-    };
-    for (const QSpec &q : qs) {
-        if (split) break;
-        SynJob j = syn_job(P, SYN_QST, nlev, ld_lm);
-        j.src[0] = din[q.q];
-        j.src[1] = din[q.sidx];
-        j.src[2] = din[q.t0];
-        j.src[3] = q.t1 >= 0 ? din[q.t1] : nullptr;
-        ColOp T = q.curl ? op(2, G_OR2LL, 0, 0, 1.0, 3, -1.0) : op(2);
-        qst_cols(j, op(0, G_LL), op(1), T);
-        j.out_comp[0] = q.comp0; j.out_comp[1] = q.comp0 + 1; j.out_comp[2] = q.comp0 + 2;
-        if ((rc = launch_syn(P, j, s)) || (rc = mark())) return rc;
-    }
-    // fused: inverse FFTs, grid products, forward FFTs; du/dr grids to ws_G
-    if (!split && ((rc = launch_nl_fused(P, nlev, P->ws_F, P->ws_F2, P->d_rad, LFfac, P->ws_G,
-                                         grids, s)) ||
-                   (rc = mark())))
-        return rc;
-    for (int v = 0; v < 3 && !split; ++v) {
-        AnaJob j = ana_job(P, ANA_QST, nlev, ld_lm);
-        j.in_comp[0] = 3 * v;
-        j.in_comp[1] = 3 * v + 1;
-        j.in_comp[2] = 3 * v + 2;
-        j.out[0] = dout[3 * v];
-        j.out[1] = dout[3 * v + 1];
-        j.out[2] = dout[3 * v + 2];
-        if ((rc = launch_ana(P, j, s)) || (rc = mark())) return rc;
-    }
-    if (!split) {
-        AnaJob j = ana_job(P, ANA_SH, nlev, ld_lm);
-        j.in_comp[0] = 9;
-        j.out[0] = dout[9];
-        if ((rc = launch_ana(P, j, s)) || (rc = mark())) return rc;
     }
     for (int i = 0; i < 10; ++i)
         if (hout[i])
@@ -986,14 +1065,28 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
     return MHX_OK;
 }
 
+// end of a nonlinear step: with a caller's stream and device outputs the
+// caller's stream is ordered after the step (no host wait); otherwise the
+// host waits for the step
+static int finish_step(mhx_plan *h, void *stream)
+{
+    Plan *P = &h->p;
+    if (stream && !P->last_host_out) {
+        MHX_HIP_CHECK(hipEventRecord(P->ev_done, P->stream));
+        MHX_HIP_CHECK(hipStreamWaitEvent((hipStream_t)stream, P->ev_done, 0));
+        return MHX_OK;
+    }
+    MHX_HIP_CHECK(hipStreamSynchronize(P->stream));
+    return MHX_OK;
+}
+
 int mhx_nl_step(mhx_plan *h, int nlev, const double *const *in, size_t ld_lm, double *const *out,
                 const double *or2, const double *or4, const double *orho1, double LFfac,
                 const int *lcut, void *stream)
 {
     int rc = nl_step_impl(h, nlev, in, ld_lm, out, or2, or4, orho1, LFfac, lcut, nullptr, stream);
     if (rc) return rc;
-    MHX_HIP_CHECK(hipStreamSynchronize(h->p.stream));
-    return MHX_OK;
+    return finish_step(h, stream);
 }
 
 int mhx_nl_step_grids(mhx_plan *h, int nlev, const double *const *in, size_t ld_lm,
@@ -1004,8 +1097,7 @@ int mhx_nl_step_grids(mhx_plan *h, int nlev, const double *const *in, size_t ld_
     if (!grids) return fail(MHX_E_INVALID, "null grids pointer");
     int rc = nl_step_impl(h, nlev, in, ld_lm, out, or2, or4, orho1, LFfac, lcut, grids, stream);
     if (rc) return rc;
-    MHX_HIP_CHECK(hipStreamSynchronize(h->p.stream));
-    return MHX_OK;
+    return finish_step(h, stream);
 }
 
 int mhx_set_timing(mhx_plan *h, int on)

@@ -302,21 +302,25 @@ NL_GRIDS = ("vr", "vt", "vp", "cvr", "cvt", "cvp", "dvr", "dvt", "dvp", "br", "b
             "cbr", "cbt", "cbp", "s", "dsdt", "dsdp")
 
 
-def nl_step(plan, inputs, outputs, or2, or4, orho1, LFfac, lcut, grids=None):
+def nl_step(plan, inputs, outputs, or2, or4, orho1, LFfac, lcut, grids=None, stream=None):
     """The nonlinear-term path of one radial loop for all levels at once
     (src/rIter.f90:466-712, bulk levels).  ``inputs``: 11 device tensors
     (nlev, lm_max) complex128 in NL_INPUTS order; ``outputs``: 10 such tensors
     (NL_OUTPUTS order), written in place.  ``grids`` (optional): a float64
-    device tensor (nlev, 18, n_phi, n_theta) that receives the inverse grids."""
+    device tensor (nlev, 18, n_phi, n_theta) that receives the inverse grids.
+    ``stream`` (optional, a torch.cuda.Stream): the step is ordered on it and
+    the call returns without waiting; without it the call returns when the
+    outputs are written."""
     L = plan._L
     nlev = inputs[0].shape[0]
     ins = (ctypes.c_void_p * 11)(*[t.data_ptr() for t in inputs])
     outs = (ctypes.c_void_p * 10)(*[t.data_ptr() for t in outputs])
     d = lambda v: (ctypes.c_double * nlev)(*[float(x) for x in v])  # noqa: E731
     lc = (ctypes.c_int * nlev)(*[int(x) for x in lcut])
+    st = ctypes.c_void_p(stream.cuda_stream) if stream is not None else None
     if grids is None:
         _check(L.mhx_nl_step(plan.h, nlev, ins, plan.lm_max, outs, d(or2), d(or4), d(orho1),
-                             float(LFfac), lc, None))
+                             float(LFfac), lc, st))
     else:
         _check(L.mhx_nl_step_grids(plan.h, nlev, ins, plan.lm_max, outs, d(or2), d(or4),
-                                   d(orho1), float(LFfac), lc, _ptr(grids), None))
+                                   d(orho1), float(LFfac), lc, _ptr(grids), st))

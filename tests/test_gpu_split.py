@@ -87,3 +87,49 @@ def test_split_l511_dipole_vs_fp64():
     print("split vs FP64 l_max=511 x 8, dipole x100:", " ".join("%.2e" % e for e in errs), flush=True)
     for k, e in enumerate(errs):
         assert e < SPLIT_TOL, (sht.NL_OUTPUTS[k], e)
+
+
+def test_split_graph_replay_matches_eager():
+    """The split step is captured into a graph on its second call with the same
+    arrays and replayed on later calls.  Every replay, with the outputs zeroed
+    and other device work and synchronisations in between, equals the eager
+    first call bit for bit; after the inputs change in place, a replay equals
+    the eager call of a fresh plan on the new inputs."""
+    import torch
+    L, nlev = 127, 24
+    nt, nph = workload.grid_size(L)
+    inputs = make_inputs(L, nlev)
+    rad = radial(nlev)
+    lcut = [L] * nlev
+    lcut[5] = L - 11
+    dev = torch.device("cuda")
+    tin = [torch.from_numpy(x).to(dev) for x in inputs]
+    p16 = sht.Plan(L, nt, nph, precision=sht.PREC_SPLIT16, max_levels=nlev)
+    tout = [torch.zeros((nlev, p16.lm_max), dtype=torch.complex128, device=dev) for _ in range(10)]
+
+    def step(plan, outs):
+        sht.nl_step(plan, tin, outs, rad[0], rad[1], rad[2], rad[3], lcut)
+        torch.cuda.synchronize()
+        return [t.cpu().numpy() for t in outs]
+
+    ref = step(p16, tout)
+    for rep in range(4):
+        a = torch.rand(512, 512, device=dev)
+        (a @ a).sum().item()
+        for t in tout:
+            t.zero_()
+        torch.cuda.synchronize()
+        got = step(p16, tout)
+        for k in range(10):
+            assert np.array_equal(got[k], ref[k]), (rep, sht.NL_OUTPUTS[k])
+    for t in tin:
+        t.mul_(0.5).add_(t.roll(1, dims=1))
+    got = step(p16, tout)
+    fresh = sht.Plan(L, nt, nph, precision=sht.PREC_SPLIT16, max_levels=nlev)
+    tout2 = [torch.zeros_like(t) for t in tout]
+    want = step(fresh, tout2)
+    fresh.close()
+    p16.close()
+    for k in range(10):
+        assert np.array_equal(got[k], want[k]), sht.NL_OUTPUTS[k]
+        assert not np.array_equal(got[k], ref[k]), sht.NL_OUTPUTS[k]
