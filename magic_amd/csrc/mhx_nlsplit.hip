@@ -679,11 +679,6 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
     const int K0 = (a.l_max + 1 - m) / 2 + 1, K1 = (a.l_max + 2 - m) / 2;
     const int nks0 = (K0 + KS - 1) / KS, nks1 = (K1 + KS - 1) / KS;
     if (tid < NCOL * WLG) lvmax[tid] = 0u;
-    // float slot of (column c, l, level levl, re/im ri)
-    auto slot = [&](int c, int l, int levl, int ri) -> float * {
-        const int par = (l - m) & 1, k = (l - m) >> 1;
-        return reinterpret_cast<float *>(sb + ((c * 2 + par) * 16 + levl * 2 + ri) * PROW) + k;
-    };
     const int levl = tid >> 5, g0 = tid & 31;   // staging: level, group of 4 l
     const int lev = lev0 + levl;
     const bool lok = levl < nlh;
@@ -702,10 +697,10 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
                 cv[it][q] = d <= a.l_max + 2 - m ? cr[d] : 0.f;
             }
     }
-    // ---- 1. Q, S, T (SG: s) in FP32, l = m + 4 g + j over 3 rounds of 128 l
+    // ---- 1. Q, S, T (SG: s) in FP32 registers, l = m + 4 (g0 + 32 it) + j
+    float2 cq[3][4], cs[3][4], ct[3][4];  // column 0 (Q or s), S (or s), T
     {
         const double o2 = (V == NS_QSTC && lok) ? a.or2[lev] : 0.0;
-        float mx0 = 0.f;
         // all loads of the three rounds first, unconditional at a clamped l
         // (a load under a branch waits for itself before the next one);
         // values past lhi are zeroed after
@@ -722,151 +717,124 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
             }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int it = 0; it < 3; ++it) {
-            const int g = g0 + 32 * it;
-            double2 (&x)[4][NSRC] = xa[it];
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int s2 = 0; s2 < NSRC; ++s2)
-                    if (m + 4 * g + j > lhi) x[j][s2] = make_double2(0.0, 0.0);
+        for (int it = 0; it < 3; ++it)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int l = m + 4 * g + j;
-                if (((l - m) >> 1) >= PKP) continue;
+                const int l = m + 4 * (g0 + 32 * it) + j;
+                const bool in = l <= lhi;
+                const double2 *x = xa[it][j];
                 const double ll = (double)l * (double)(l + 1);
-                float cr[3], ci[3];
                 if constexpr (V == NS_SG) {
-                    cr[0] = cr[1] = cr[2] = (float)x[j][0].x;
-                    ci[0] = ci[1] = ci[2] = (float)x[j][0].y;
+                    const float2 v = in ? make_float2((float)x[0].x, (float)x[0].y) : make_float2(0.f, 0.f);
+                    cq[it][j] = v;
+                    cs[it][j] = v;
                 } else {
-                    cr[0] = (float)(ll * x[j][0].x);
-                    ci[0] = (float)(ll * x[j][0].y);
-                    cr[1] = (float)x[j][1].x;
-                    ci[1] = (float)x[j][1].y;
-                    if constexpr (V == NS_QSTC) {
-                        cr[2] = (float)(o2 * ll * x[j][2].x - x[j][3].x);
-                        ci[2] = (float)(o2 * ll * x[j][2].y - x[j][3].y);
-                    } else {
-                        cr[2] = (float)x[j][2].x;
-                        ci[2] = (float)x[j][2].y;
-                    }
-                }
-                mx0 = fmaxf(mx0, fmaxf(fabsf(cr[0]), fabsf(ci[0])));
-#pragma unroll
-                for (int c = 0; c < NCOL; ++c) {
-                    *slot(c, l, levl, 0) = cr[c];
-                    *slot(c, l, levl, 1) = ci[c];
+                    cq[it][j] = in ? make_float2((float)(ll * x[0].x), (float)(ll * x[0].y)) : make_float2(0.f, 0.f);
+                    cs[it][j] = in ? make_float2((float)x[1].x, (float)x[1].y) : make_float2(0.f, 0.f);
+                    if constexpr (V == NS_QSTC)
+                        ct[it][j] = in ? make_float2((float)(o2 * ll * x[2].x - x[3].x),
+                                                     (float)(o2 * ll * x[2].y - x[3].y))
+                                       : make_float2(0.f, 0.f);
+                    else
+                        ct[it][j] = in ? make_float2((float)x[2].x, (float)x[2].y) : make_float2(0.f, 0.f);
                 }
             }
-        }
-        // max over the 32 lanes of this level, then one atomic
-#pragma unroll
-        for (int off = 16; off >= 1; off >>= 1) mx0 = fmaxf(mx0, __shfl_xor(mx0, off));
-        if (lok && g0 == 0) atomicMax(&lvmax[levl], __float_as_uint(mx0));
     }
-    __syncthreads();
-    // ---- 2. C1 (and C2) from the neighbours; all reads, then all writes.
-    // Column 1 holds S (SG: s), column 2 holds T.
+    // ---- 2. C1 (and C2) from the neighbours l - 1, l + 1: within a round
+    // from the thread itself or the lanes g0 -+ 1 of its level (32-wide
+    // shuffles); across rounds from lane 31 of round it - 1 / lane 0 of it + 1
+    float2 c1[3][4], c2[3][4];
+    float mx0 = 0.f, mx1 = 0.f, mx2 = 0.f;
     {
-        float sr[3][6], si[3][6], tr[3][6], ti[3][6];  // [round][l - 1 .. l + 4]
+        auto sh = [&](float2 v, int src) {
+            return make_float2(__shfl(v.x, src, 32), __shfl(v.y, src, 32));
+        };
+        float2 sl[3], sr[3], tl[3], tr[3];  // S, T at l = lb - 1 and lb + 4
 #pragma unroll
         for (int it = 0; it < 3; ++it) {
-            const int lb = m + 4 * (g0 + 32 * it);
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                const int l = lb - 1 + q;
-                const bool in = l >= m && ((l - m) >> 1) < PKP;
-                sr[it][q] = in ? *slot(1, l, levl, 0) : 0.f;
-                si[it][q] = in ? *slot(1, l, levl, 1) : 0.f;
-                if constexpr (NCOL == 3) {
-                    tr[it][q] = in ? *slot(2, l, levl, 0) : 0.f;
-                    ti[it][q] = in ? *slot(2, l, levl, 1) : 0.f;
-                }
+            const float2 z = make_float2(0.f, 0.f);
+            const float2 a1 = sh(cs[it][3], (g0 + 31) & 31), b1 = it > 0 ? sh(cs[it - 1][3], 31) : z;
+            const float2 a2 = sh(cs[it][0], (g0 + 1) & 31), b2 = it < 2 ? sh(cs[it + 1][0], 0) : z;
+            sl[it] = g0 > 0 ? a1 : b1;
+            sr[it] = g0 < 31 ? a2 : b2;
+            if constexpr (NCOL == 3) {
+                const float2 a3 = sh(ct[it][3], (g0 + 31) & 31), b3 = it > 0 ? sh(ct[it - 1][3], 31) : z;
+                const float2 a4 = sh(ct[it][0], (g0 + 1) & 31), b4 = it < 2 ? sh(ct[it + 1][0], 0) : z;
+                tl[it] = g0 > 0 ? a3 : b3;
+                tr[it] = g0 < 31 ? a4 : b4;
             }
         }
-        __syncthreads();
-        float mx1 = 0.f, mx2 = 0.f;
 #pragma unroll
-        for (int it = 0; it < 3; ++it) {
-            const int lb = m + 4 * (g0 + 32 * it);
+        for (int it = 0; it < 3; ++it)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int l = lb + j;
-                if (((l - m) >> 1) >= PKP) continue;
+                const int l = m + 4 * (g0 + 32 * it) + j;
                 const float am = (float)(l - 1) * cv[it][j], ap = (float)(l + 2) * cv[it][j + 1];
-                // x' = am x_{l-1} - ap x_{l+1}: neighbours at q = j and j + 2
-                const float spr = am * sr[it][j] - ap * sr[it][j + 2];
-                const float spi = am * si[it][j] - ap * si[it][j + 2];
+                const float2 sm = j > 0 ? cs[it][j - 1] : sl[it], sp = j < 3 ? cs[it][j + 1] : sr[it];
+                // x' = am x_{l-1} - ap x_{l+1}
+                const float spr = am * sm.x - ap * sp.x, spi = am * sm.y - ap * sp.y;
                 if constexpr (NCOL == 2) {
-                    mx1 = fmaxf(mx1, fmaxf(fabsf(spr), fabsf(spi)));
-                    *slot(1, l, levl, 0) = spr;
-                    *slot(1, l, levl, 1) = spi;
+                    c1[it][j] = make_float2(spr, spi);
                 } else {
-                    const float tpr = am * tr[it][j] - ap * tr[it][j + 2];
-                    const float tpi = am * ti[it][j] - ap * ti[it][j + 2];
+                    const float2 tm = j > 0 ? ct[it][j - 1] : tl[it], tp = j < 3 ? ct[it][j + 1] : tr[it];
+                    const float tpr = am * tm.x - ap * tp.x, tpi = am * tm.y - ap * tp.y;
                     // C1 = S' + i m T, C2 = i m S - T'
-                    const float c1r = spr - fm * ti[it][j + 1], c1i = spi + fm * tr[it][j + 1];
-                    const float c2r = -fm * si[it][j + 1] - tpr, c2i = fm * sr[it][j + 1] - tpi;
-                    mx1 = fmaxf(mx1, fmaxf(fabsf(c1r), fabsf(c1i)));
-                    mx2 = fmaxf(mx2, fmaxf(fabsf(c2r), fabsf(c2i)));
-                    *slot(1, l, levl, 0) = c1r;
-                    *slot(1, l, levl, 1) = c1i;
-                    *slot(2, l, levl, 0) = c2r;
-                    *slot(2, l, levl, 1) = c2i;
+                    c1[it][j] = make_float2(spr - fm * ct[it][j].y, spi + fm * ct[it][j].x);
+                    c2[it][j] = make_float2(-fm * cs[it][j].y - tpr, fm * cs[it][j].x - tpi);
+                    mx2 = fmaxf(mx2, fmaxf(fabsf(c2[it][j].x), fabsf(c2[it][j].y)));
                 }
+                mx0 = fmaxf(mx0, fmaxf(fabsf(cq[it][j].x), fabsf(cq[it][j].y)));
+                mx1 = fmaxf(mx1, fmaxf(fabsf(c1[it][j].x), fabsf(c1[it][j].y)));
             }
-        }
+    }
+    // ---- 3. one power of two per (column, level) from the exact maxima,
+    // then hi/lo pairs (k = 2 g, 2 g + 1 of one parity) written once
 #pragma unroll
-        for (int off = 16; off >= 1; off >>= 1) {
-            mx1 = fmaxf(mx1, __shfl_xor(mx1, off));
-            if constexpr (NCOL == 3) mx2 = fmaxf(mx2, __shfl_xor(mx2, off));
-        }
-        if (lok && g0 == 0) {
-            atomicMax(&lvmax[WLG + levl], __float_as_uint(mx1));
-            if constexpr (NCOL == 3) atomicMax(&lvmax[2 * WLG + levl], __float_as_uint(mx2));
-        }
+    for (int off = 16; off >= 1; off >>= 1) {
+        mx0 = fmaxf(mx0, __shfl_xor(mx0, off));
+        mx1 = fmaxf(mx1, __shfl_xor(mx1, off));
+        if constexpr (NCOL == 3) mx2 = fmaxf(mx2, __shfl_xor(mx2, off));
+    }
+    if (lok && g0 == 0) {
+        atomicMax(&lvmax[levl], __float_as_uint(mx0));
+        atomicMax(&lvmax[WLG + levl], __float_as_uint(mx1));
+        if constexpr (NCOL == 3) atomicMax(&lvmax[2 * WLG + levl], __float_as_uint(mx2));
     }
     __syncthreads();
     if (tid < NCOL * WLG) lvsc[tid] = bscale((double)__uint_as_float(lvmax[tid]));
     __syncthreads();
-    // ---- 3. rows in place: [hi 160][lo 160]; five segments of 32 floats per
-    // row, all of a row's segments in the same pass
-    for (int base = 0; base < 5 * NROW; base += 255) {
-        const int seg = base + tid;
-        const bool act = tid < 255 && seg < 5 * NROW;
-        const int row = seg / 5, part = seg % 5;
-        float v[32];
-        if (act) {
-            const f4 *src = reinterpret_cast<const f4 *>(sb + row * PROW + 128 * part);
+    {
+        float scl[NCOL];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const f4 w = src[i];
-                v[4 * i] = w[0]; v[4 * i + 1] = w[1]; v[4 * i + 2] = w[2]; v[4 * i + 3] = w[3];
+        for (int c = 0; c < NCOL; ++c) scl[c] = lvsc[c * WLG + levl];
+#pragma unroll
+        for (int it = 0; it < 3; ++it) {
+            const int g = g0 + 32 * it;
+            if (2 * g >= PKP) continue;
+#pragma unroll
+            for (int c = 0; c < NCOL; ++c) {
+                const float2 *v = c == 0 ? cq[it] : (c == 1 ? c1[it] : c2[it]);
+#pragma unroll
+                for (int par = 0; par < 2; ++par)
+#pragma unroll
+                    for (int ri = 0; ri < 2; ++ri) {
+                        const float v0 = (ri ? v[par].y : v[par].x) * scl[c];
+                        const float v1 = (ri ? v[par + 2].y : v[par + 2].x) * scl[c];
+                        _Float16 h0, l0h, h1, l1h;
+                        split_h(v0, h0, l0h);
+                        split_h(v1, h1, l1h);
+                        unsigned char *row = sb + ((c * 2 + par) * 16 + levl * 2 + ri) * PROW;
+                        typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+                        h2 hv, lv;
+                        hv[0] = h0; hv[1] = h1;
+                        lv[0] = l0h; lv[1] = l1h;
+                        *reinterpret_cast<h2 *>(row + 4 * g) = hv;
+                        *reinterpret_cast<h2 *>(row + 2 * PKP + 4 * g) = lv;
+                    }
             }
         }
-        __syncthreads();
-        if (act) {
-            const int c = row / 32, lv = (row & 15) >> 1;
-            const float sc = lvsc[c * WLG + lv];
-            h8 *hp = reinterpret_cast<h8 *>(sb + row * PROW + 64 * part);
-            h8 *lp = reinterpret_cast<h8 *>(sb + row * PROW + 2 * PKP + 64 * part);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                h8 h, lo;
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    _Float16 a1, b1;
-                    split_h(v[8 * i + e] * sc, a1, b1);
-                    h[e] = a1;
-                    lo[e] = b1;
-                }
-                hp[i] = h;
-                lp[i] = lo;
-            }
-        }
-        __syncthreads();
     }
+    __syncthreads();
 
     // ---- 4. contraction: NCOL products against P per k step; steps
     // j < nks0 are parity 0 (ks = j), the rest parity 1 (ks = j - nks0)
