@@ -1555,11 +1555,13 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
     }
     __syncthreads();
 
-    // ---- 1. FP32 Sym/Anti columns (as k_nana16w)
+    // ---- 1. Sym/Anti of the theta pairs in registers; exact maxima per
+    // (level, group); after the scales, hi/lo pairs (two theta_h) written once
+    // to the LDS rows [hi 192][lo 192] (theta_h >= nh zero)
+    const int nent = WLG * a.ntb * 4;
+    constexpr int NE = (WLG * (AKP / 8) * 4 + 255) / 256;  // 3
+    f4 xv[NE][NCOL][2];
     {
-        const int nent = WLG * a.ntb * 4;
-        constexpr int NE = (WLG * (AKP / 8) * 4 + 255) / 256;  // 3
-        f4 xv[NE][NCOL][2];
 #pragma unroll
         for (int i = 0; i < NE; ++i) {
             const int e = tid + 256 * i;
@@ -1592,75 +1594,69 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
             if (e >= nent || th0 >= a.nh) continue;
             float mx0 = 0.f, mx1 = 0.f;
 #pragma unroll
-            for (int sa = 0; sa < 2; ++sa)
+            for (int k = 0; k < NCOL; ++k)
 #pragma unroll
-                for (int k = 0; k < NCOL; ++k) {
-                    float cr[2], ci[2];
-#pragma unroll
-                    for (int p = 0; p < 2; ++p) {
-                        const f4 w = xv[i][k][p];  // N re, N im, S re, S im
-                        cr[p] = sa ? w[0] - w[2] : w[0] + w[2];
-                        ci[p] = sa ? w[1] - w[3] : w[1] + w[3];
-                    }
+                for (int p = 0; p < 2; ++p) {
+                    const f4 w = xv[i][k][p];  // N re, N im, S re, S im; |N +- S| <= |N| + |S|
                     float &mx = cgrp[k] ? mx1 : mx0;
-                    mx = fmaxf(mx, fmaxf(fmaxf(fabsf(cr[0]), fabsf(cr[1])), fmaxf(fabsf(ci[0]), fabsf(ci[1]))));
-#pragma unroll
-                    for (int ri = 0; ri < 2; ++ri) {
-                        const int row = (k * 2 + sa) * 16 + levl * 2 + ri;
-                        const float2 v = ri ? make_float2(ci[0], ci[1]) : make_float2(cr[0], cr[1]);
-                        *reinterpret_cast<float2 *>(sb + row * AROW + 4 * th0) = v;
-                    }
+                    mx = fmaxf(mx, fmaxf(fmaxf(fabsf(w[0] + w[2]), fabsf(w[0] - w[2])),
+                                         fmaxf(fabsf(w[1] + w[3]), fabsf(w[1] - w[3]))));
                 }
             if (levl < nlh) {
                 atomicMax(&lvmax[levl], __float_as_uint(mx0));
                 atomicMax(&lvmax[WLG + levl], __float_as_uint(mx1));
             }
         }
+        typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+        const h2 hz = {(_Float16)0.f, (_Float16)0.f};
         for (int e = tid; e < NROW * (AKP / 2); e += 256) {
             const int row = e / (AKP / 2), k2 = 2 * (e % (AKP / 2));
-            if (k2 >= a.nh)
-                *reinterpret_cast<float2 *>(sb + row * AROW + 4 * k2) = make_float2(0.f, 0.f);
+            if (k2 >= a.nh) {
+                *reinterpret_cast<h2 *>(sb + row * AROW + 2 * k2) = hz;
+                *reinterpret_cast<h2 *>(sb + row * AROW + 2 * AKP + 2 * k2) = hz;
+            }
         }
     }
     __syncthreads();
     if (tid < 2 * WLG) lvsc[tid] = bscale((double)__uint_as_float(lvmax[tid]));
     __syncthreads();
-    // ---- 2. rows in place: [hi 192][lo 192]
-    for (int base = 0; base < 3 * NROW; base += 255) {
-        const int seg = base + tid;
-        const bool act = tid < 255 && seg < 3 * NROW;
-        const int row = seg / 3, part = seg % 3;
-        float v[64];
-        if (act) {
-            const f4 *src = reinterpret_cast<const f4 *>(sb + row * AROW + 256 * part);
+    {
+        typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const f4 w = src[i];
-                v[4 * i] = w[0]; v[4 * i + 1] = w[1]; v[4 * i + 2] = w[2]; v[4 * i + 3] = w[3];
-            }
-        }
-        __syncthreads();
-        if (act) {
-            const int c = row / 32, levl = (row & 15) >> 1;
-            const float sc = lvsc[(cgrp[c] ? WLG : 0) + levl];
-            h8 *hp = reinterpret_cast<h8 *>(sb + row * AROW + 128 * part);
-            h8 *lp = reinterpret_cast<h8 *>(sb + row * AROW + 2 * AKP + 128 * part);
+        for (int i = 0; i < NE; ++i) {
+            const int e = tid + 256 * i;
+            const int q = e & 3, tb = (e >> 2) % a.ntb, levl = (e >> 2) / a.ntb;
+            const int th0 = 8 * tb + 2 * q;
+            if (e >= nent || th0 >= a.nh) continue;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                h8 h, lo;
+            for (int k = 0; k < NCOL; ++k) {
+                const float sc = lvsc[(cgrp[k] ? WLG : 0) + levl];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    _Float16 a1, b1;
-                    split_h(v[8 * i + e] * sc, a1, b1);
-                    h[e] = a1;
-                    lo[e] = b1;
+                for (int sa = 0; sa < 2; ++sa) {
+                    float cr[2], ci[2];
+#pragma unroll
+                    for (int p = 0; p < 2; ++p) {
+                        const f4 w = xv[i][k][p];
+                        cr[p] = (sa ? w[0] - w[2] : w[0] + w[2]) * sc;
+                        ci[p] = (sa ? w[1] - w[3] : w[1] + w[3]) * sc;
+                    }
+#pragma unroll
+                    for (int ri = 0; ri < 2; ++ri) {
+                        _Float16 h0, l0h, h1, l1h;
+                        split_h(ri ? ci[0] : cr[0], h0, l0h);
+                        split_h(ri ? ci[1] : cr[1], h1, l1h);
+                        h2 hv, lv;
+                        hv[0] = h0; hv[1] = h1;
+                        lv[0] = l0h; lv[1] = l1h;
+                        unsigned char *rowp = sb + ((k * 2 + sa) * 16 + levl * 2 + ri) * AROW;
+                        *reinterpret_cast<h2 *>(rowp + 2 * th0) = hv;
+                        *reinterpret_cast<h2 *>(rowp + 2 * AKP + 2 * th0) = lv;
+                    }
                 }
-                hp[i] = h;
-                lp[i] = lo;
             }
         }
-        __syncthreads();
     }
+    __syncthreads();
 
     // ---- 3. contraction over theta_h: G^c (parity par) += wP(par) x col(c, Sym/Anti = par)
     f4 acc[NCOL][2][PTT];
