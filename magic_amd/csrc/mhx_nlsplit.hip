@@ -866,53 +866,64 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
             rsv[tt][j] = a.rowscale[(size_t)mc * a.nh + th];
         }
     struct Frag { h8 h[WTT], l[WTT]; };
-    Frag fr[4];
     const size_t toff0 = a.off[2 * mc], toff1 = a.off[2 * mc + 1];
-    auto load_s = [&](auto sc) {
-        constexpr int S = decltype(sc)::value;
-        if constexpr (S < NSTP) {
-            constexpr int PAR = S / NCH;
-            const int ks = min(S % NCH, (PAR ? nks1 : nks0) - 1);
+    // NC chunks per parity (nks0 >= nks1); one straight-line body per NC
+    auto contract = [&](auto ncc) {
+        constexpr int NC = decltype(ncc)::value, NS = 2 * NC;
+        Frag fr[4];
+        auto load_s = [&](auto sc) {
+            constexpr int S = decltype(sc)::value;
+            if constexpr (S < NS) {
+                constexpr int PAR = S / NC;
+                const int ks = min(S % NC, (PAR ? nks1 : nks0) - 1);
 #pragma unroll
-            for (int tt = 0; tt < WTT; ++tt) {
-                const int tile = min(tile0 + tt, a.nt16 - 1);
-                const size_t to = (PAR ? toff1 : toff0) + (((size_t)ks * a.nt16 + tile) * 64 + lane) * 16;
-                fr[S % 4].h[tt] = *reinterpret_cast<const h8 *>(a.tP + to);
-                fr[S % 4].l[tt] = *reinterpret_cast<const h8 *>(a.tP + to + 8);
+                for (int tt = 0; tt < WTT; ++tt) {
+                    const int tile = min(tile0 + tt, a.nt16 - 1);
+                    const size_t to = (PAR ? toff1 : toff0) + (((size_t)ks * a.nt16 + tile) * 64 + lane) * 16;
+                    fr[S % 4].h[tt] = *reinterpret_cast<const h8 *>(a.tP + to);
+                    fr[S % 4].l[tt] = *reinterpret_cast<const h8 *>(a.tP + to + 8);
+                }
             }
-        }
-    };
-    auto comp_s = [&](auto sc) {
-        constexpr int S = decltype(sc)::value;
-        constexpr int PAR = S / NCH, KC = S % NCH;
-        const Frag &f = fr[S % 4];
-        const int kb = 2 * (KC * KS + 8 * (lane >> 4));
+        };
+        auto comp_s = [&](auto sc) {
+            constexpr int S = decltype(sc)::value;
+            constexpr int PAR = S / NC, KC = S % NC;
+            const Frag &f = fr[S % 4];
+            const int kb = 2 * (KC * KS + 8 * (lane >> 4));
 #pragma unroll
-        for (int c = 0; c < NCOL; ++c) {
-            const int row = (c * 2 + PAR) * 16 + (lane & 15);
-            const h8 bh = *reinterpret_cast<const h8 *>(sb + row * PROW + kb);
-            const h8 bl = *reinterpret_cast<const h8 *>(sb + row * PROW + 2 * PKP + kb);
+            for (int c = 0; c < NCOL; ++c) {
+                const int row = (c * 2 + PAR) * 16 + (lane & 15);
+                const h8 bh = *reinterpret_cast<const h8 *>(sb + row * PROW + kb);
+                const h8 bl = *reinterpret_cast<const h8 *>(sb + row * PROW + 2 * PKP + kb);
 #pragma unroll
-            for (int tt = 0; tt < WTT; ++tt) {
-                f4 t = acc[c][PAR][tt];
-                t = mfma16(f.h[tt], bl, t);
-                t = mfma16(f.l[tt], bh, t);
-                t = mfma16(f.h[tt], bh, t);
-                acc[c][PAR][tt] = t;
+                for (int tt = 0; tt < WTT; ++tt) {
+                    f4 t = acc[c][PAR][tt];
+                    t = mfma16(f.h[tt], bl, t);
+                    t = mfma16(f.l[tt], bh, t);
+                    t = mfma16(f.h[tt], bh, t);
+                    acc[c][PAR][tt] = t;
+                }
             }
-        }
-    };
-    load_s(std::integral_constant<int, 0>());
-    load_s(std::integral_constant<int, 1>());
-    load_s(std::integral_constant<int, 2>());
-    __builtin_amdgcn_sched_barrier(0);
-    static_for<NSTP>([&](auto sc) {
-        constexpr int S = decltype(sc)::value;
-        load_s(std::integral_constant<int, S + 3>());
-        __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of this step
-        comp_s(sc);
+        };
+        load_s(std::integral_constant<int, 0>());
+        load_s(std::integral_constant<int, 1>());
+        load_s(std::integral_constant<int, 2>());
         __builtin_amdgcn_sched_barrier(0);
-    });
+        static_for<NS>([&](auto sc) {
+            constexpr int S = decltype(sc)::value;
+            load_s(std::integral_constant<int, S + 3>());
+            __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of this step
+            comp_s(sc);
+            __builtin_amdgcn_sched_barrier(0);
+        });
+    };
+    switch (nks0) {
+    case 1: contract(std::integral_constant<int, 1>()); break;
+    case 2: contract(std::integral_constant<int, 2>()); break;
+    case 3: contract(std::integral_constant<int, 3>()); break;
+    case 4: contract(std::integral_constant<int, 4>()); break;
+    default: contract(std::integral_constant<int, NCH>()); break;
+    }
 
     // ---- 5. epilogue, one output at a time (contiguous theta runs)
     float *so = reinterpret_cast<float *>(sb);
@@ -1701,6 +1712,7 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
             const h8 bl = *reinterpret_cast<const h8 *>(sb + row * AROW + 2 * AKP + kb);
 #pragma unroll
             for (int tt = 0; tt < PTT; ++tt) {
+                if (!tok[tt]) continue;  // wave-uniform, no loads inside
                 f4 t = acc[c][PAR][tt];
                 t = mfma16(f.h[tt], bl, t);
                 t = mfma16(f.l[tt], bh, t);
