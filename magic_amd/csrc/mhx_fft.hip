@@ -576,7 +576,8 @@ __device__ __forceinline__ void slot_pass(C *buf, int t, const C *tw, int sign)
             C v = buf[j + s * T];
             if constexpr (L > 1) {
                 if (s > 0) {
-                    C w = tw[s * k * tstride];
+                    const int ti = s * k * tstride;
+                    C w = tw[ti + (ti >> 3)];  // table padded by one entry per 8 (banks)
                     if (sign > 0) w = cconj(w);
                     v = cmul(v, w);
                 }
@@ -632,8 +633,8 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
     extern __shared__ char smem_raw[];
     typedef SlotFFT<A, B> FFT;
     constexpr int M = FFT::M, N = 2 * M, Q = M / 12;
-    C *s_tw = reinterpret_cast<C *>(smem_raw);
-    C *s_twr = s_tw + M;
+    C *s_tw = reinterpret_cast<C *>(smem_raw);  // M + M / 8 (padded)
+    C *s_twr = s_tw + M + M / 8;
     C *rows = s_twr + M + 1;  // 6 slot rows of M
     int lb;
     {
@@ -647,7 +648,12 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
     const T o2 = T(osin2[th]);
     const T or2 = T(rad[lev]), or4 = T(rad[nlev + lev]), orho1 = T(rad[2 * nlev + lev]);
     const T LFfac = T(LFfac_d);
-    load_twiddles(s_tw, s_twr, f);
+    {
+        const C *tw = reinterpret_cast<const C *>(f.tw);
+        const C *twr = reinterpret_cast<const C *>(f.twr);
+        for (int e = threadIdx.x; e < M; e += blockDim.x) s_tw[e + (e >> 3)] = tw[e];
+        for (int e = threadIdx.x; e <= M; e += blockDim.x) s_twr[e] = twr[e];
+    }
     __syncthreads();
 
     const int ntb = (n_theta + 15) / 16;
@@ -876,7 +882,7 @@ int launch_slots(const Plan *p, const FFTArgs &f, int nlev, const void *Fin, voi
     if (nthr < 64) nthr = 64;
     nthr = (nthr + 63) / 64 * 64;
     if (NPT * nthr < p->n_phi) return fail(MHX_E_UNSUPPORTED, "n_phi too large for the fused stage");
-    const size_t shm = ((size_t)2 * M + 1 + 6 * (size_t)M) * 2 * sizeof(T);
+    const size_t shm = ((size_t)2 * M + M / 8 + 1 + 6 * (size_t)M) * 2 * sizeof(T);
     // set once per kernel (the attribute persists), so that a call under
     // graph capture makes no attribute call
     static int shm_set = 0;
