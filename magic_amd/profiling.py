@@ -27,7 +27,7 @@ def short(name):
 
 def stats(path, title, cmd):
     rows = list(csv.DictReader(open(path)))
-    out = ["# %s" % title, "", "Command (from the repo root on the GPU box, after "
+    out = ["# %s" % title, "", "Command (after "
            "`cd /tmp && export TMPDIR=/tmp`):", "", "    %s" % cmd, "",
            "rocprofv3 kernel-trace summary (durations in microseconds):", "",
            "| kernel | calls | total us | average us | % |", "|---|---|---|---|---|"]
