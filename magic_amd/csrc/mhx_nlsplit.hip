@@ -313,318 +313,12 @@ __global__ __launch_bounds__(384, 2) void k_nsyn16(NSynArgs a)
     }
 }
 
-// ------------------------------------------- synthesis, whole-K form
-// For l_max <= 255 (at most 128 l of one parity per order) the block stages
-// the raw B columns of all l at once and keeps the i m factors out of them:
-//   1. Q = l(l+1) a, S = b, T = c or or2 l(l+1) c - d (SG: s) in FP32 into
-//      LDS rows [column][parity][16 cols][128 k], maxima per (group, level);
-//   2. in place, each row becomes [hi: 128 halves][lo: 128 halves];
-//   3. five products accumulate separately, streaming the table fragments
-//      (the next k step's loaded during this one):
-//         A0 = P Q, A1 = dP S, A2 = P T, A3 = P S, A4 = dP T   (SG: P s, dP s)
-//   4. epilogue: vr = A0, vt = A1 + i m A2, vp = i m A3 - A4 (SG: s = A0,
-//      ds/dtheta_c = A1, ds/dphi = i m A0), the i m rotation exchanging re/im
-//      between neighbouring lanes; staged in LDS per output and written as
-//      contiguous runs of n_theta rows.
-// One block: 4 waves x 3 theta tiles, 8 levels; at most ~75 KB of LDS so two
-// blocks share a CU.
-constexpr int WKP = 128;                 // k per parity held
-constexpr int WROW = 4 * WKP + 16;       // bytes per LDS row (padded)
+// ------------------------- whole-K / whole-theta forms (l_max <= 255)
+// One block covers one order m, 8 radial levels and all theta rows (4 waves x
+// 3 theta tiles in synthesis); its B operand for the whole K range sits in LDS.
 constexpr int WLG = 8;                   // levels per block
 constexpr int WTT = 3;                   // theta tiles per wave
 constexpr int WNT = 384;                 // n_theta at most (one block covers all rows)
-
-template <int V> struct WSyn;
-template <> struct WSyn<NS_SG> {
-    static constexpr int NSRC = 1, NCOL = 1, NACC = 2;
-    static constexpr int prod[2][2] = {{0, 0}, {1, 0}};  // (table, column) per accumulator
-    static constexpr int cgrp[1] = {0};
-};
-template <> struct WSyn<NS_QST> {
-    static constexpr int NSRC = 3, NCOL = 3, NACC = 5;
-    static constexpr int prod[5][2] = {{0, 0}, {1, 1}, {0, 2}, {0, 1}, {1, 2}};
-    static constexpr int cgrp[3] = {0, 1, 1};
-};
-template <> struct WSyn<NS_QSTC> {
-    static constexpr int NSRC = 4, NCOL = 3, NACC = 5;
-    static constexpr int prod[5][2] = {{0, 0}, {1, 1}, {0, 2}, {0, 1}, {1, 2}};
-    static constexpr int cgrp[3] = {0, 1, 1};
-};
-
-template <int V>
-__global__ __launch_bounds__(256, 2) void k_nsyn16w(NSynArgs a)
-{
-    using S = WSyn<V>;
-    constexpr int NCOL = S::NCOL, NACC = S::NACC, NSRC = S::NSRC;
-    constexpr int NROW = NCOL * 2 * 16;
-    // B rows, later one output at a time: [level][theta row] complex
-    constexpr int SB = NROW * WROW > WLG * (2 * WNT + 4) * 4 ? NROW * WROW : WLG * (2 * WNT + 4) * 4;
-    __shared__ __align__(16) unsigned char sb[SB];
-    __shared__ unsigned lvmax[2 * WLG];
-    __shared__ float lvsc[2 * WLG];
-
-    const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
-    const int ng = (a.nlev + WLG - 1) / WLG;
-    int mc, rest;
-    xcd_deal(blockIdx.x, ng * a.ngz, mc, rest);
-    if (mc >= a.n_m) return;
-    const int gy = rest % ng, gz = rest / ng;
-    const int m = mc * a.minc;
-    const int lev0 = gy * WLG, nlh = min(WLG, a.nlev - lev0);
-    const int l0 = a.lstart[mc];
-    const int K0 = (a.l_max - m) / 2 + 1, K1 = (a.l_max - m + 1) / 2;
-    const int nks0 = (K0 + KS - 1) / KS, nks1 = (K1 + KS - 1) / KS;
-    if (tid < 2 * WLG) lvmax[tid] = 0u;
-    __syncthreads();
-
-    // ---- 1. FP32 columns; thread (level, two groups of 4 consecutive l);
-    // all loads issued before any use
-    {
-        const int levl = tid >> 5, g0 = tid & 31;
-        const int lev = lev0 + levl;
-        const bool lok = levl < nlh;
-        const int lhi = lok ? min(a.l_max, a.lcut[lev]) : -1;
-        const double o2 = (V == NS_QSTC && lok) ? a.or2[lev] : 0.0;
-        double2 x[2][4][NSRC];  // [group][j][source]
-#pragma unroll
-        for (int it = 0; it < 2; ++it)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int l = m + 4 * (g0 + 32 * it) + j;
-                const size_t o = ((size_t)(lok ? lev : 0) * a.src_ld + (size_t)(l0 + l - m)) * 2;
-#pragma unroll
-                for (int s2 = 0; s2 < NSRC; ++s2)
-                    x[it][j][s2] = (l <= lhi) ? *reinterpret_cast<const double2 *>(a.src[s2] + o)
-                                              : make_double2(0.0, 0.0);
-            }
-        float mx0 = 0.f, mx1 = 0.f;
-#pragma unroll
-        for (int it = 0; it < 2; ++it) {
-            const int g = g0 + 32 * it;  // l = m + 4 g + j, k = 2 g + (j >> 1)
-#pragma unroll
-            for (int par = 0; par < 2; ++par) {
-                float cr[NCOL][2], ci[NCOL][2];
-#pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    const int l = m + 4 * g + 2 * q + par;
-                    const double2 *xs = x[it][2 * q + par];
-                    const double ll = (double)l * (double)(l + 1);
-                    if constexpr (V == NS_SG) {
-                        cr[0][q] = (float)xs[0].x;
-                        ci[0][q] = (float)xs[0].y;
-                    } else {
-                        cr[0][q] = (float)(ll * xs[0].x);
-                        ci[0][q] = (float)(ll * xs[0].y);
-                        cr[1][q] = (float)xs[1].x;
-                        ci[1][q] = (float)xs[1].y;
-                        if constexpr (V == NS_QSTC) {
-                            cr[2][q] = (float)(o2 * ll * xs[2].x - xs[3].x);
-                            ci[2][q] = (float)(o2 * ll * xs[2].y - xs[3].y);
-                        } else {
-                            cr[2][q] = (float)xs[2].x;
-                            ci[2][q] = (float)xs[2].y;
-                        }
-                    }
-                }
-#pragma unroll
-                for (int c = 0; c < NCOL; ++c) {
-                    float &mx = S::cgrp[c] ? mx1 : mx0;
-                    mx = fmaxf(mx, fmaxf(fmaxf(fabsf(cr[c][0]), fabsf(cr[c][1])),
-                                         fmaxf(fabsf(ci[c][0]), fabsf(ci[c][1]))));
-#pragma unroll
-                    for (int ri = 0; ri < 2; ++ri) {
-                        const int row = (c * 2 + par) * 16 + levl * 2 + ri;
-                        const float2 v = ri ? make_float2(ci[c][0], ci[c][1]) : make_float2(cr[c][0], cr[c][1]);
-                        *reinterpret_cast<float2 *>(sb + row * WROW + 4 * (2 * g)) = v;
-                    }
-                }
-            }
-        }
-        if (lok) {
-            atomicMax(&lvmax[levl], __float_as_uint(mx0));
-            atomicMax(&lvmax[WLG + levl], __float_as_uint(mx1));
-        }
-    }
-    __syncthreads();
-    if (tid < 2 * WLG) lvsc[tid] = bscale((double)__uint_as_float(lvmax[tid]));
-    __syncthreads();
-    // ---- 2. rows in place: [hi k 0..127][lo k 0..127]; half-rows of 64
-    // floats per thread, all reads before any write of the same row
-    for (int base = 0; base < 2 * NROW; base += 256) {
-        const int seg = base + tid;
-        const bool act = seg < 2 * NROW;
-        const int row = seg >> 1, half = seg & 1;
-        float v[64];
-        if (act) {
-            const f4 *src = reinterpret_cast<const f4 *>(sb + row * WROW + 256 * half);
-#pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const f4 w = src[i];
-                v[4 * i] = w[0]; v[4 * i + 1] = w[1]; v[4 * i + 2] = w[2]; v[4 * i + 3] = w[3];
-            }
-        }
-        __syncthreads();
-        if (act) {
-            const int c = row / 32, levl = (row & 15) >> 1;
-            const float sc = lvsc[(S::cgrp[c] ? WLG : 0) + levl];
-            h8 *hp = reinterpret_cast<h8 *>(sb + row * WROW + 128 * half);
-            h8 *lp = reinterpret_cast<h8 *>(sb + row * WROW + 256 + 128 * half);
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                h8 h, lo;
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    _Float16 a1, b1;
-                    split_h(v[8 * i + e] * sc, a1, b1);
-                    h[e] = a1;
-                    lo[e] = b1;
-                }
-                hp[i] = h;
-                lp[i] = lo;
-            }
-        }
-        __syncthreads();
-    }
-
-    // ---- 3. contraction
-    f4 acc[NACC][2][WTT];  // [product][Sym/Anti][theta tile]
-#pragma unroll
-    for (int o = 0; o < NACC; ++o)
-#pragma unroll
-        for (int s2 = 0; s2 < 2; ++s2)
-#pragma unroll
-            for (int t = 0; t < WTT; ++t) acc[o][s2][t] = f4{0.f, 0.f, 0.f, 0.f};
-    const int tile0 = gz * 4 * WTT + wave * WTT;
-    bool tok[WTT];
-#pragma unroll
-    for (int t = 0; t < WTT; ++t) tok[t] = tile0 + t < a.nt16;
-    auto load_a = [&](int par, int ks, h8 *Ph, h8 *Pl, h8 *dPh, h8 *dPl) {
-#pragma unroll
-        for (int tt = 0; tt < WTT; ++tt) {
-            if (tok[tt]) {
-                const size_t to = a.off[2 * mc + par] + (((size_t)ks * a.nt16 + tile0 + tt) * 64 + lane) * 16;
-                Ph[tt] = *reinterpret_cast<const h8 *>(a.tP + to);
-                Pl[tt] = *reinterpret_cast<const h8 *>(a.tP + to + 8);
-                dPh[tt] = *reinterpret_cast<const h8 *>(a.tdP + to);
-                dPl[tt] = *reinterpret_cast<const h8 *>(a.tdP + to + 8);
-            }
-        }
-    };
-    h8 cPh[WTT], cPl[WTT], cdPh[WTT], cdPl[WTT];
-    // one k step of parity PAR: P products feed accumulator PAR (Sym for
-    // even l - m), dP products 1 - PAR
-    auto step = [&](auto parc, int ks) {
-        constexpr int PAR = decltype(parc)::value;
-#pragma unroll
-        for (int c = 0; c < NCOL; ++c) {
-            const int row = (c * 2 + PAR) * 16 + (lane & 15);
-            const int kb = 2 * (ks * KS + 8 * (lane >> 4));
-            const h8 bh = *reinterpret_cast<const h8 *>(sb + row * WROW + kb);
-            const h8 bl = *reinterpret_cast<const h8 *>(sb + row * WROW + 256 + kb);
-#pragma unroll
-            for (int pr = 0; pr < NACC; ++pr) {
-                if (S::prod[pr][1] != c) continue;
-                const int tab = S::prod[pr][0];
-                const int tgt = tab == 0 ? PAR : 1 - PAR;
-#pragma unroll
-                for (int tt = 0; tt < WTT; ++tt) {
-                    if (!tok[tt]) continue;
-                    const h8 ah = tab == 0 ? cPh[tt] : cdPh[tt];
-                    const h8 al = tab == 0 ? cPl[tt] : cdPl[tt];
-                    f4 t = acc[pr][tgt][tt];
-                    t = mfma16(ah, bl, t);
-                    t = mfma16(al, bh, t);
-                    t = mfma16(ah, bh, t);
-                    acc[pr][tgt][tt] = t;
-                }
-            }
-        }
-    };
-    auto advance = [&](bool has, h8 *nPh, h8 *nPl, h8 *ndPh, h8 *ndPl) {
-        if (!has) return;
-#pragma unroll
-        for (int tt = 0; tt < WTT; ++tt) {
-            cPh[tt] = nPh[tt]; cPl[tt] = nPl[tt]; cdPh[tt] = ndPh[tt]; cdPl[tt] = ndPl[tt];
-        }
-    };
-    if (nks0 > 0) load_a(0, 0, cPh, cPl, cdPh, cdPl);
-    for (int ks = 0; ks < nks0; ++ks) {
-        h8 nPh[WTT], nPl[WTT], ndPh[WTT], ndPl[WTT];
-        const bool has = ks + 1 < nks0 || nks1 > 0;
-        if (ks + 1 < nks0) load_a(0, ks + 1, nPh, nPl, ndPh, ndPl);
-        else if (nks1 > 0) load_a(1, 0, nPh, nPl, ndPh, ndPl);
-        step(std::integral_constant<int, 0>(), ks);
-        advance(has, nPh, nPl, ndPh, ndPl);
-    }
-    for (int ks = 0; ks < nks1; ++ks) {
-        h8 nPh[WTT], nPl[WTT], ndPh[WTT], ndPl[WTT];
-        const bool has = ks + 1 < nks1;
-        if (has) load_a(1, ks + 1, nPh, nPl, ndPh, ndPl);
-        step(std::integral_constant<int, 1>(), ks);
-        advance(has, nPh, nPl, ndPh, ndPl);
-    }
-
-    // ---- 4. epilogue, one output at a time
-    float *so = reinterpret_cast<float *>(sb);
-    const int rowf = 2 * a.n_theta + 4;  // floats per level run, padded
-    const int n = lane & 15, levl = n >> 1, ri = n & 1;
-    const float inv0 = 1.f / lvsc[levl], inv1 = 1.f / lvsc[WLG + levl];
-    const float fm = (float)m;
-    const int r0 = 2 * 16 * (gz * 4 * WTT), r1 = min(a.n_theta, 2 * 16 * (gz + 1) * 4 * WTT);
-    const int nq = (r1 - r0) / 2;  // float4 = two theta rows
-    constexpr int NOUT = 3;
-#pragma unroll
-    for (int o = 0; o < NOUT; ++o) {
-        __syncthreads();  // B rows / previous output consumed
-#pragma unroll
-        for (int tt = 0; tt < WTT; ++tt) {
-            if (!tok[tt]) continue;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int th = (tile0 + tt) * 16 + 4 * (lane >> 4) + j;
-                const float rrs = th < a.nh ? 1.f / a.rowscale[(size_t)mc * a.nh + th] : 0.f;
-                float v[2];  // Sym, Anti
-#pragma unroll
-                for (int sa = 0; sa < 2; ++sa) {
-                    // (i m) X: re = -m X.im, im = m X.re, X.im / X.re from the
-                    // neighbouring lane (n ^ 1)
-                    auto im_rot = [&](float xv) {
-                        const float other = __shfl_xor(xv, 1);
-                        return ri ? fm * other : -fm * other;
-                    };
-                    if constexpr (V == NS_SG) {
-                        const float a0 = acc[0][sa][tt][j], a1 = acc[1][sa][tt][j];
-                        const float r2 = im_rot(a0);
-                        v[sa] = (o == 0 ? a0 : (o == 1 ? a1 : r2)) * inv0;
-                    } else {
-                        if (o == 0) {
-                            v[sa] = acc[0][sa][tt][j] * inv0;
-                        } else if (o == 1) {
-                            v[sa] = (acc[1][sa][tt][j] + im_rot(acc[2][sa][tt][j])) * inv1;
-                        } else {
-                            v[sa] = (im_rot(acc[3][sa][tt][j]) - acc[4][sa][tt][j]) * inv1;
-                        }
-                    }
-                }
-                if (th < a.nh) {
-                    float *d = so + levl * rowf + 4 * th + ri;
-                    d[0] = (v[0] + v[1]) * rrs;
-                    d[2] = (v[0] - v[1]) * rrs;
-                }
-            }
-        }
-        __syncthreads();
-        for (int i = tid; i < WLG * nq; i += 256) {
-            const int q = i % nq, lv = i / nq;
-            if (lv >= nlh) continue;
-            const f4 w = *reinterpret_cast<const f4 *>(so + lv * rowf + 2 * r0 + 4 * q);
-            float *fo = a.fout + ((size_t)(lev0 + lv) * a.fout_ld +
-                                  ((size_t)(a.comp0 + o) * a.n_m + mc) * a.n_theta + r0) * 2;
-            *reinterpret_cast<f4 *>(fo + 4 * q) = w;
-        }
-    }
-}
 
 // compile-time loop: f(integral_constant<int, S>) for S = 0 .. N - 1
 template <class F, int... S>
@@ -856,7 +550,7 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
     // load).  Chunks past a parity's K hold zero B rows and read a clamped,
     // finite table chunk; tiles past nt16 compute on a copy of the last tile
     // and are dropped in the epilogue.
-    constexpr int NCH = PKP / KS, NSTP = 2 * NCH;
+    constexpr int NCH = PKP / KS;
     float rsv[WTT][4];  // row factors of the wave's theta rows, for the epilogue
 #pragma unroll
     for (int tt = 0; tt < WTT; ++tt)
@@ -1215,313 +909,21 @@ __global__ __launch_bounds__(256) void k_nana16(NAnaArgs a)
     }
 }
 
-// -------------------------------------------- analysis, whole-theta form
-// For n_theta <= 384 and l_max <= 255 the block stages Sym/Anti of all theta
-// pairs of the raw columns (f, vt, vp; SH: f) at once, in FP32 then in place
-// as [hi 192][lo 192] per LDS row; five products accumulate separately
-//   A0 = wP f, A1 = wdP vt, A2 = wP vp, A3 = wdP vp, A4 = wP vt
-// and the epilogue forms Q = A0, S = A1 - i m A2, T = -A3 - i m A4
-// (src/shtransforms.f90:731-872 with the crossed vt/vp), / l(l+1) for S, T.
-// One block: 4 waves x 2 l tiles (both parities), 8 levels; ~75 KB of LDS.
-constexpr int AKP = 192;                 // theta_h held
+constexpr int AKP = 192;                 // theta_h held by the whole-theta analysis
 constexpr int AROW = 4 * AKP + 16;       // bytes per LDS row (padded)
 
-template <int V> struct WAna;
-template <> struct WAna<NA_QST> {
-    static constexpr int NIN = 3, NACC = 5, NOUT = 3;
-    static constexpr int prod[5][2] = {{0, 0}, {1, 1}, {0, 2}, {1, 2}, {0, 1}};  // (table, column)
-    static constexpr int cgrp[3] = {0, 1, 1};
-};
-template <> struct WAna<NA_SH> {
-    static constexpr int NIN = 1, NACC = 1, NOUT = 1;
-    static constexpr int prod[1][2] = {{0, 0}};
-    static constexpr int cgrp[1] = {0};
-};
-
-template <int V>
-__global__ __launch_bounds__(256, 2) void k_nana16w(NAnaArgs a)
-{
-    using S = WAna<V>;
-    constexpr int NCOL = S::NIN, NACC = S::NACC, NOUT = S::NOUT;
-    constexpr int NROW = NCOL * 2 * 16;
-    // B rows, later one output at a time: [level][l - m] complex double
-    constexpr int SB = NROW * AROW > WLG * (4 * WKP + 2) * 8 ? NROW * AROW : WLG * (4 * WKP + 2) * 8;
-    __shared__ __align__(16) unsigned char sb[SB];
-    __shared__ unsigned lvmax[2 * WLG];
-    __shared__ float lvsc[2 * WLG];
-
-    const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
-    const int ng = (a.nlev + WLG - 1) / WLG;
-    int mc, rest;
-    xcd_deal(blockIdx.x, ng * a.ngz, mc, rest);
-    if (mc >= a.n_m) return;
-    const int gy = rest % ng, gz = rest / ng;
-    const int m = mc * a.minc;
-    const int lev0 = gy * WLG, nlh = min(WLG, a.nlev - lev0);
-    const int l0 = a.lstart[mc];
-    const int K0 = (a.l_max - m) / 2 + 1;
-    const int ntl = (K0 + 15) / 16;
-    const int t0 = gz * 8 + wave * 2;
-    bool tok[2];
-    tok[0] = t0 < ntl;
-    tok[1] = t0 + 1 < ntl;
-    if (tid < 2 * WLG) lvmax[tid] = 0u;
-    __syncthreads();
-
-    // ---- 1. FP32 Sym/Anti columns; entry (level, theta block, quad q):
-    // rows 16 tb + 4 q .. + 3 = theta_h 8 tb + 2 q, + 1 (N, S each); the
-    // thread's entries e = tid + 256 i (i < 3) are loaded before any use
-    {
-        const int nent = WLG * a.ntb * 4;
-        constexpr int NE = (WLG * (AKP / 8) * 4 + 255) / 256;  // 3
-        f4 xv[NE][NCOL][2];
-#pragma unroll
-        for (int i = 0; i < NE; ++i) {
-            const int e = tid + 256 * i;
-            const int q = e & 3, tb = (e >> 2) % a.ntb, levl = (e >> 2) / a.ntb;
-            const int lev = lev0 + levl;
-            const int th0 = 8 * tb + 2 * q;
-            bool valid = e < nent && levl < nlh && th0 < a.nh;
-            if (valid) {
-                const int lc = a.lcut[lev];
-                valid = m <= (lc < a.m_max ? lc : a.m_max);
-            }
-            const bool ok1 = th0 + 1 < a.nh;
-#pragma unroll
-            for (int k = 0; k < NCOL; ++k) {
-                xv[i][k][0] = f4{0.f, 0.f, 0.f, 0.f};
-                xv[i][k][1] = f4{0.f, 0.f, 0.f, 0.f};
-                if (valid) {
-                    const size_t o = ((size_t)lev * a.fin_ld +
-                                      (((size_t)a.in_comp[k] * a.ntb + tb) * a.n_m + mc) * 16 + 4 * q) * 2;
-                    xv[i][k][0] = *reinterpret_cast<const f4 *>(a.fin + o);
-                    if (ok1) xv[i][k][1] = *reinterpret_cast<const f4 *>(a.fin + o + 4);
-                }
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < NE; ++i) {
-            const int e = tid + 256 * i;
-            const int q = e & 3, tb = (e >> 2) % a.ntb, levl = (e >> 2) / a.ntb;
-            const int th0 = 8 * tb + 2 * q;
-            if (e >= nent || th0 >= a.nh) continue;  // past the last pair
-            float mx0 = 0.f, mx1 = 0.f;
-#pragma unroll
-            for (int sa = 0; sa < 2; ++sa)
-#pragma unroll
-                for (int k = 0; k < NCOL; ++k) {
-                    float cr[2], ci[2];
-#pragma unroll
-                    for (int p = 0; p < 2; ++p) {
-                        const f4 w = xv[i][k][p];  // N re, N im, S re, S im
-                        cr[p] = sa ? w[0] - w[2] : w[0] + w[2];
-                        ci[p] = sa ? w[1] - w[3] : w[1] + w[3];
-                    }
-                    float &mx = S::cgrp[k] ? mx1 : mx0;
-                    mx = fmaxf(mx, fmaxf(fmaxf(fabsf(cr[0]), fabsf(cr[1])), fmaxf(fabsf(ci[0]), fabsf(ci[1]))));
-#pragma unroll
-                    for (int ri = 0; ri < 2; ++ri) {
-                        const int row = (k * 2 + sa) * 16 + levl * 2 + ri;
-                        const float2 v = ri ? make_float2(ci[0], ci[1]) : make_float2(cr[0], cr[1]);
-                        *reinterpret_cast<float2 *>(sb + row * AROW + 4 * th0) = v;
-                    }
-                }
-            if (levl < nlh) {
-                atomicMax(&lvmax[levl], __float_as_uint(mx0));
-                atomicMax(&lvmax[WLG + levl], __float_as_uint(mx1));
-            }
-        }
-        // theta_h in [nh, 192): zero (the tables are zero there too, but the
-        // rows must hold finite values)
-        for (int e = tid; e < NROW * (AKP / 2); e += 256) {
-            const int row = e / (AKP / 2), k2 = 2 * (e % (AKP / 2));
-            if (k2 >= a.nh)
-                *reinterpret_cast<float2 *>(sb + row * AROW + 4 * k2) = make_float2(0.f, 0.f);
-        }
-    }
-    __syncthreads();
-    if (tid < 2 * WLG) lvsc[tid] = bscale((double)__uint_as_float(lvmax[tid]));
-    __syncthreads();
-    // ---- 2. rows in place: [hi 192][lo 192]; three segments of 64 floats
-    // per row, all of a row's segments in the same pass
-    for (int base = 0; base < 3 * NROW; base += 255) {
-        const int seg = base + tid;
-        const bool act = tid < 255 && seg < 3 * NROW;
-        const int row = seg / 3, part = seg % 3;
-        float v[64];
-        if (act) {
-            const f4 *src = reinterpret_cast<const f4 *>(sb + row * AROW + 256 * part);
-#pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const f4 w = src[i];
-                v[4 * i] = w[0]; v[4 * i + 1] = w[1]; v[4 * i + 2] = w[2]; v[4 * i + 3] = w[3];
-            }
-        }
-        __syncthreads();
-        if (act) {
-            const int c = row / 32, levl = (row & 15) >> 1;
-            const float sc = lvsc[(S::cgrp[c] ? WLG : 0) + levl];
-            h8 *hp = reinterpret_cast<h8 *>(sb + row * AROW + 128 * part);
-            h8 *lp = reinterpret_cast<h8 *>(sb + row * AROW + 2 * AKP + 128 * part);
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                h8 h, lo;
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    _Float16 a1, b1;
-                    split_h(v[8 * i + e] * sc, a1, b1);
-                    h[e] = a1;
-                    lo[e] = b1;
-                }
-                hp[i] = h;
-                lp[i] = lo;
-            }
-        }
-        __syncthreads();
-    }
-
-    // ---- 3. contraction over theta_h; the table fragments of chunk kc + 1
-    // are loaded during chunk kc
-    f4 acc[NACC][2][2];  // [product][parity][l tile]
-#pragma unroll
-    for (int o = 0; o < NACC; ++o)
-#pragma unroll
-        for (int s2 = 0; s2 < 2; ++s2)
-#pragma unroll
-            for (int t = 0; t < 2; ++t) acc[o][s2][t] = f4{0.f, 0.f, 0.f, 0.f};
-    constexpr bool HASD = V == NA_QST;
-    auto load_a = [&](int kc, h8 (*Ph)[2], h8 (*Pl)[2], h8 (*dPh)[2], h8 (*dPl)[2]) {
-#pragma unroll
-        for (int par = 0; par < 2; ++par)
-#pragma unroll
-            for (int tt = 0; tt < 2; ++tt) {
-                if (tok[tt]) {
-                    const size_t to = a.off[2 * mc + par] +
-                                      (((size_t)(t0 + tt) * a.nq32 + kc) * 64 + lane) * 16;
-                    Ph[par][tt] = *reinterpret_cast<const h8 *>(a.tP + to);
-                    Pl[par][tt] = *reinterpret_cast<const h8 *>(a.tP + to + 8);
-                    if constexpr (HASD) {
-                        dPh[par][tt] = *reinterpret_cast<const h8 *>(a.tdP + to);
-                        dPl[par][tt] = *reinterpret_cast<const h8 *>(a.tdP + to + 8);
-                    }
-                }
-            }
-    };
-    h8 cPh[2][2], cPl[2][2], cdPh[2][2], cdPl[2][2];
-    load_a(0, cPh, cPl, cdPh, cdPl);
-    for (int kc = 0; kc < a.nq32; ++kc) {
-        h8 nPh[2][2], nPl[2][2], ndPh[2][2], ndPl[2][2];
-        const bool has = kc + 1 < a.nq32;
-        if (has) load_a(kc + 1, nPh, nPl, ndPh, ndPl);
-        const int kb = 2 * (kc * KS + 8 * (lane >> 4));
-#pragma unroll
-        for (int c = 0; c < NCOL; ++c)
-#pragma unroll
-            for (int sa = 0; sa < 2; ++sa) {
-                const int row = (c * 2 + sa) * 16 + (lane & 15);
-                const h8 bh = *reinterpret_cast<const h8 *>(sb + row * AROW + kb);
-                const h8 bl = *reinterpret_cast<const h8 *>(sb + row * AROW + 2 * AKP + kb);
-#pragma unroll
-                for (int pr = 0; pr < NACC; ++pr) {
-                    if (S::prod[pr][1] != c) continue;
-                    const int tab = S::prod[pr][0];
-                    // output parity par uses Sym/Anti = par (wP) or 1 - par (wdP)
-                    const int par = tab == 0 ? sa : 1 - sa;
-#pragma unroll
-                    for (int tt = 0; tt < 2; ++tt) {
-                        if (!tok[tt]) continue;
-                        const h8 ah = tab == 0 ? cPh[par][tt] : cdPh[par][tt];
-                        const h8 al = tab == 0 ? cPl[par][tt] : cdPl[par][tt];
-                        f4 t = acc[pr][par][tt];
-                        t = mfma16(ah, bl, t);
-                        t = mfma16(al, bh, t);
-                        t = mfma16(ah, bh, t);
-                        acc[pr][par][tt] = t;
-                    }
-                }
-            }
-        if (has) {
-#pragma unroll
-            for (int par = 0; par < 2; ++par)
-#pragma unroll
-                for (int tt = 0; tt < 2; ++tt) {
-                    cPh[par][tt] = nPh[par][tt];
-                    cPl[par][tt] = nPl[par][tt];
-                    if constexpr (HASD) {
-                        cdPh[par][tt] = ndPh[par][tt];
-                        cdPl[par][tt] = ndPl[par][tt];
-                    }
-                }
-        }
-    }
-
-    // ---- 4. epilogue, one output at a time; staged as [level][l - m]
-    double *so = reinterpret_cast<double *>(sb);
-    const int rowd = 2 * (a.l_max + 1) + 2;  // doubles per level run, padded
-    const int n = lane & 15, levl = n >> 1, ri = n & 1;
-    const int lc = levl < nlh ? a.lcut[lev0 + levl] : -1;
-    const bool mkeep = m <= (lc < a.m_max ? lc : a.m_max);
-    const double inv0 = 1.0 / (double)lvsc[levl], inv1 = 1.0 / (double)lvsc[WLG + levl];
-    const float fm = (float)m;
-    const int kl0 = 16 * 8 * gz, kl1 = 16 * 8 * (gz + 1);
-    const int la = m + 2 * kl0, lb = min(a.l_max, m + 2 * kl1 - 1);
-    const int nl = lb - la + 1;
-#pragma unroll
-    for (int o = 0; o < NOUT; ++o) {
-        __syncthreads();  // B rows / previous output consumed
-#pragma unroll
-        for (int tt = 0; tt < 2; ++tt) {
-            if (!tok[tt]) continue;
-#pragma unroll
-            for (int par = 0; par < 2; ++par)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    // (-i m) X: re = m X.im, im = -m X.re, from lane n ^ 1
-                    auto mim_rot = [&](float xv) {
-                        const float other = __shfl_xor(xv, 1);
-                        return ri ? -fm * other : fm * other;
-                    };
-                    double v;
-                    if constexpr (V == NA_SH) {
-                        v = (double)acc[0][par][tt][j] * inv0;
-                    } else {
-                        if (o == 0) v = (double)acc[0][par][tt][j] * inv0;
-                        else if (o == 1)
-                            v = ((double)acc[1][par][tt][j] + (double)mim_rot(acc[2][par][tt][j])) * inv1;
-                        else
-                            v = ((double)mim_rot(acc[4][par][tt][j]) - (double)acc[3][par][tt][j]) * inv1;
-                    }
-                    const int kk = 16 * (t0 + tt) + 4 * (lane >> 4) + j;
-                    const int l = m + par + 2 * kk;
-                    if (l > a.l_max) continue;
-                    const bool keep = mkeep && l <= lc;
-                    v = keep ? v / (double)a.rowscale[a.soff[2 * mc + par] + kk] : 0.0;
-                    if (o > 0 && l > 0) v /= (double)l * (double)(l + 1);
-                    so[levl * rowd + 2 * (l - m) + ri] = v;
-                }
-        }
-        __syncthreads();
-        for (int i = tid; i < WLG * nl; i += 256) {
-            const int q = i % nl, lv = i / nl;
-            if (lv >= nlh) continue;
-            const int l = la + q;
-            const double2 w = *reinterpret_cast<const double2 *>(so + lv * rowd + 2 * (l - m));
-            *reinterpret_cast<double2 *>(a.out[o] + ((size_t)(lev0 + lv) * a.out_ld + (size_t)(l0 + l - m)) * 2) = w;
-        }
-    }
-}
-
-// ------------------------- analysis, whole-theta form on wP only (QST)
+// ------------------------- analysis, whole-theta form on wP only (QST, SH)
 // The wdP products are folded onto wP by the same identity as in synthesis:
 // with G_l = sum_theta wP_l v,
 //   sum_theta wdP_l v = l c_{l+1} G_{l+1} - (l+1) c_l G_{l-1},
 // so three products G^f, G^vt, G^vp (l = m .. l_max + 1) replace five, and
 //   Q = G^f, S = (l c_{l+1} G^vt_{l+1} - (l+1) c_l G^vt_{l-1}) - i m G^vp_l,
 //   T = -(l c_{l+1} G^vp_{l+1} - (l+1) c_l G^vp_{l-1}) - i m G^vt_l,
-// S, T / l(l+1) (src/shtransforms.f90:731-872 with the crossed vt/vp).
-// Staging and contraction as k_nana16w; the wave's l tiles are
-// wave, wave + 4, wave + 8.  The epilogue keeps G (FP32, row factors
+// S, T / l(l+1) (src/shtransforms.f90:731-872 with the crossed vt/vp); SH
+// keeps only G^f.  The block stages Sym/Anti of all theta pairs of its 8
+// levels (hi/lo rows [hi 192][lo 192] in LDS, one power of two per (level,
+// group) from the exact maxima), then contracts over theta with the wave's l
+// tiles wave, wave + 4, wave + 8.  The epilogue keeps G (FP32, row factors
 // removed) in LDS as [group][level][l - m] and forms the outputs per (level, l).
 constexpr int PTT = 3;  // l tiles per wave
 
@@ -1834,21 +1236,6 @@ int launch_nsyn16(const Plan *p, int kind, const NSynArgs &args, hipStream_t s)
         MHX_HIP_CHECK(hipGetLastError());
         return MHX_OK;
     }
-    if (p->l_max / 2 + 1 <= WKP && p->n_theta <= WNT) {
-        // whole-K form: 12 theta tiles per block
-        a.tpb = 4 * WTT;
-        a.ngz = (p->nt16 + a.tpb - 1) / a.tpb;
-        const int ng = (a.nlev + WLG - 1) / WLG;
-        dim3 grid((unsigned)(8 * ((p->n_m + 7) / 8) * ng * a.ngz));
-        switch (kind) {
-        case NS_SG: k_nsyn16w<NS_SG><<<grid, 256, 0, s>>>(a); break;
-        case NS_QST: k_nsyn16w<NS_QST><<<grid, 256, 0, s>>>(a); break;
-        case NS_QSTC: k_nsyn16w<NS_QSTC><<<grid, 256, 0, s>>>(a); break;
-        default: return fail(MHX_E_INVALID, "unknown nonlinear-step synthesis kind");
-        }
-        MHX_HIP_CHECK(hipGetLastError());
-        return MHX_OK;
-    }
     const int nw = 6;  // 12 theta tiles per block (all of l_max = 255); 256 threads stage B
     a.tpb = 2 * nw;
     a.ngz = (p->nt16 + a.tpb - 1) / a.tpb;
@@ -1899,18 +1286,6 @@ int launch_nana16(const Plan *p, int kind, const NAnaArgs &args, hipStream_t s)
         dim3 grid((unsigned)(8 * ((p->n_m + 7) / 8) * ng));
         if (kind == NA_QST) k_nana16p<NA_QST><<<grid, 256, 0, s>>>(a);
         else k_nana16p<NA_SH><<<grid, 256, 0, s>>>(a);
-        MHX_HIP_CHECK(hipGetLastError());
-        return MHX_OK;
-    }
-    if (p->nh <= AKP && p->l_max / 2 + 1 <= WKP && ntl0 <= 8) {
-        // whole-theta form
-        const int ng = (a.nlev + WLG - 1) / WLG;
-        dim3 grid((unsigned)(8 * ((p->n_m + 7) / 8) * ng * a.ngz));
-        switch (kind) {
-        case NA_QST: k_nana16w<NA_QST><<<grid, 256, 0, s>>>(a); break;
-        case NA_SH: k_nana16w<NA_SH><<<grid, 256, 0, s>>>(a); break;
-        default: return fail(MHX_E_INVALID, "unknown nonlinear-step analysis kind");
-        }
         MHX_HIP_CHECK(hipGetLastError());
         return MHX_OK;
     }
