@@ -171,6 +171,38 @@ int mhx_nl_step_grids(mhx_plan *p, int nlev, const double *const *in,
                       const double *or4, const double *orho1, double LFfac,
                       const int *lcut, double *grids, void *stream);
 
+/* ---- uneven batches of independent transform jobs (split form) ----
+ * A job is the round trip of nfields fields of one kind at one truncation
+ * over nlev radial levels: synthesis to grids and analysis back, i.e. per
+ * level and field the reference's scal_to_spat + scal_to_SH (scalar) or
+ * torpol_to_spat + spat_to_qst (vector) of src/sht_native.f90:41-123,
+ * 358-364, which its callers loop over (src/rIter.f90:483-707).  l_max odd,
+ * at most 255; m_max = l_max, minc = 1, n_theta = 3 (l_max + 1) / 2,
+ * n_phi = 2 n_theta, lcut = l_max.  Arrays are device memory laid out
+ * [comp][field * nlev + lev][lm_max] complex: scalar comp s; vector in =
+ * W, dW, Z and out = Q, S, T.  mhx_batch_create builds one plan per
+ * distinct l_max and the jobs' workspaces, chooses the kernel forms per
+ * l_max by timing them on its largest job, and captures all the jobs'
+ * launches into one graph (jobs on parallel branches); mhx_batch_execute
+ * replays it (with a stream: returns at once, ordered on the stream;
+ * without: returns when done).  The jobs' arrays must stay allocated. */
+#define MHX_JOB_SCALAR 0
+#define MHX_JOB_VECTOR 1
+typedef struct mhx_job {
+    int l_max, nlev, kind, nfields;
+    const double *in;
+    double *out;
+} mhx_job;
+typedef struct mhx_batch mhx_batch;
+int mhx_batch_create(const mhx_job *jobs, int njobs, int precision, mhx_batch **batch);
+int mhx_batch_execute(mhx_batch *batch, void *stream);
+/* the tuned forms per l_max group (n entries at most): synthesis and
+ * analysis form (1 whole-K / whole-theta, 2 chunked) and row-FFT rows per
+ * block; returns the number of groups */
+int mhx_batch_tuning(const mhx_batch *batch, int *l_max, int *syn_form, int *ana_form,
+                     int *fft_rows, int n);
+int mhx_batch_destroy(mhx_batch *batch);
+
 /* durations (ms) of the stages of the last mhx_nl_step on this plan,
  * measured with HIP events on the launch stream: [0] synthesis Legendre,
  * [1] fused grid stage (inverse FFTs, grid products, forward FFTs), [2] and

@@ -286,7 +286,8 @@ __global__ void k_ifft_rows(FFTArgs f, const T *__restrict__ F, size_t F_ld, int
 template <typename T>
 __global__ void k_fft_rows(FFTArgs f, const T *__restrict__ g, size_t g_ld_lev, size_t g_ld_comp,
                            int n_m, int n_theta, int ncomp, int TB, int ntb, unsigned osin2_mask,
-                           const double *__restrict__ osin2, T *__restrict__ F, size_t F_ld)
+                           const double *__restrict__ osin2, T *__restrict__ F, size_t F_ld,
+                           int blk)
 {
     typedef cpx<T> C;
     extern __shared__ char smem_raw[];
@@ -312,13 +313,19 @@ __global__ void k_fft_rows(FFTArgs f, const T *__restrict__ g, size_t g_ld_lev, 
     __syncthreads();
     C *res = fft_pp(A, B, TB, f, s_tw, -1);
     const bool os = (osin2_mask >> comp) & 1u;
-    C *Fp = reinterpret_cast<C *>(F) + (size_t)lev * F_ld + ((size_t)comp * n_theta + th0) * n_m;
+    // F[lev][comp][theta][m], or theta-blocked F[lev][comp][theta/16][m][theta%16]
+    // (blk; the split analysis reads 16 rows of one m as one line)
+    C *Fl = reinterpret_cast<C *>(F) + (size_t)lev * F_ld;
+    const int nt16 = (n_theta + 15) / 16;
     for (int idx = threadIdx.x; idx < n_m * TB; idx += blockDim.x) {
         int k = idx % n_m, r = idx / n_m;
         if (r >= nrows) continue;
         T sc = T(1.0 / (double)N);
         if (os) sc *= T(osin2[th0 + r]);
-        Fp[(size_t)r * n_m + k] = rfft_post(res + r * M, k, M, s_twr, sc);
+        const int th = th0 + r;
+        const size_t o = blk ? (((size_t)comp * nt16 + (th >> 4)) * n_m + k) * 16 + (th & 15)
+                             : ((size_t)comp * n_theta + th) * n_m + k;
+        Fl[o] = rfft_post(res + r * M, k, M, s_twr, sc);
     }
 }
 
@@ -848,7 +855,43 @@ int launch_fft(const Plan *p, const double *grid, size_t grid_ld_lev, size_t gri
     size_t shm = api_shm(p, TB);
     k_fft_rows<double><<<dim3((unsigned)(ntb * ncomp * nlev)), 256, shm, s>>>(
         f, grid, grid_ld_lev, grid_ld_comp, p->n_m, p->n_theta, ncomp, TB, ntb, osin2_mask,
-        p->d_osin2, F, F_ld_lev);
+        p->d_osin2, F, F_ld_lev, 0);
+    MHX_HIP_CHECK(hipGetLastError());
+    return MHX_OK;
+}
+
+// FP32 row FFTs of the batched transform jobs, TB rows per block (a tuned
+// choice); the forward side writes the theta-blocked layout of the split
+// analysis
+int launch_ifft_f32(const Plan *p, const float *F, size_t F_ld, int ncomp, int nrow, float *grid,
+                    size_t g_ld_row, size_t g_ld_comp, int TB, hipStream_t s)
+{
+    if (nrow <= 0) return MHX_OK;
+    FFTArgs f = make_args(p);
+    f.tw = p->d_tw32;
+    f.twr = p->d_twr32;
+    const int ntb = (p->n_theta + TB - 1) / TB;
+    const size_t shm = ((size_t)2 * TB * p->M + 2 * p->M + 1) * sizeof(float2);
+    if (shm > 64 * 1024 || 256 % TB) return fail(MHX_E_UNSUPPORTED, "rows per block too large");
+    k_ifft_rows<float><<<dim3((unsigned)(ntb * ncomp * nrow)), 256, shm, s>>>(
+        f, F, F_ld, ncomp, p->n_m, p->n_theta, ncomp, TB, ntb, grid, g_ld_row, g_ld_comp);
+    MHX_HIP_CHECK(hipGetLastError());
+    return MHX_OK;
+}
+
+int launch_fft_f32(const Plan *p, const float *grid, size_t g_ld_row, size_t g_ld_comp, int ncomp,
+                   int nrow, unsigned osin2_mask, float *F2, size_t F2_ld, int TB, hipStream_t s)
+{
+    if (nrow <= 0) return MHX_OK;
+    FFTArgs f = make_args(p);
+    f.tw = p->d_tw32;
+    f.twr = p->d_twr32;
+    const int ntb = (p->n_theta + TB - 1) / TB;
+    const size_t shm = ((size_t)2 * TB * p->M + 2 * p->M + 1) * sizeof(float2);
+    if (shm > 64 * 1024 || 256 % TB) return fail(MHX_E_UNSUPPORTED, "rows per block too large");
+    k_fft_rows<float><<<dim3((unsigned)(ntb * ncomp * nrow)), 256, shm, s>>>(
+        f, grid, g_ld_row, g_ld_comp, p->n_m, p->n_theta, ncomp, TB, ntb, osin2_mask, p->d_osin2,
+        F2, F2_ld, 1);
     MHX_HIP_CHECK(hipGetLastError());
     return MHX_OK;
 }

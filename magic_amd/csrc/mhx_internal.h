@@ -84,7 +84,8 @@ struct AnaJob {
 };
 
 // Legendre kernels of the nonlinear step in the split form (mhx_nlsplit.hip)
-enum NSynKind { NS_SG = 0, NS_QST = 1, NS_QSTC = 2 };
+// NS_S: the scalar alone (no gradient), split whole-K form only
+enum NSynKind { NS_SG = 0, NS_QST = 1, NS_QSTC = 2, NS_S = 3 };
 enum NAnaKind { NA_QST = 0, NA_SH = 1 };
 
 struct NSynArgs {
@@ -163,6 +164,9 @@ struct Plan {
     float *d_syn16x_rs = nullptr, *d_ana16x_rs = nullptr;
     // c_l = sqrt((l^2 - m^2) / (4 l^2 - 1)) for l = m .. l_max + 2 at lstart[mc] + 2 mc
     float *d_crec = nullptr;
+    // split Legendre forms (0: by shape, 1: whole-K / whole-theta, 2: chunked),
+    // set by the batch autotuner
+    int syn_form = 0, ana_form = 0;
 
     // workspaces
     int ws_levels = 0;
@@ -222,6 +226,10 @@ int launch_fft(const Plan *p, const double *grid, size_t grid_ld_lev, size_t gri
 // inverse grids in the reference layout to full_grids when not null
 // (FP32 Fourier data and grids for a split-precision plan; there Fout is
 // theta-blocked, F2[lev][comp][theta/16][m][theta%16])
+int launch_ifft_f32(const Plan *p, const float *F, size_t F_ld, int ncomp, int nrow, float *grid,
+                    size_t g_ld_row, size_t g_ld_comp, int TB, hipStream_t s);
+int launch_fft_f32(const Plan *p, const float *grid, size_t g_ld_row, size_t g_ld_comp, int ncomp,
+                   int nrow, unsigned osin2_mask, float *F2, size_t F2_ld, int TB, hipStream_t s);
 int launch_nl_fused(const Plan *p, int nlev, const void *Fin, void *Fout,
                     const double *rad, double LFfac, void *dgrid, void *full_grids,
                     hipStream_t s);

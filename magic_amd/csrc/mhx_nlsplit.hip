@@ -352,8 +352,10 @@ constexpr int PROW = 4 * PKP + 16;       // bytes per LDS row (padded)
 template <int V>
 __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
 {
-    constexpr int NSRC = V == NS_SG ? 1 : (V == NS_QSTC ? 4 : 3);
-    constexpr int NCOL = V == NS_SG ? 2 : 3, NROW = NCOL * 2 * 16;
+    constexpr bool SCAL = V == NS_SG || V == NS_S;
+    constexpr int NSRC = SCAL ? 1 : (V == NS_QSTC ? 4 : 3);
+    constexpr int NCOL = V == NS_S ? 1 : (V == NS_SG ? 2 : 3), NROW = NCOL * 2 * 16;
+    constexpr int NOUT = V == NS_S ? 1 : 3;
     constexpr int SB = NROW * PROW > WLG * (2 * WNT + 4) * 4 ? NROW * PROW : WLG * (2 * WNT + 4) * 4;
     __shared__ __align__(16) unsigned char sb[SB];
     __shared__ unsigned lvmax[NCOL * WLG];
@@ -418,7 +420,7 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
                 const bool in = l <= lhi;
                 const double2 *x = xa[it][j];
                 const double ll = (double)l * (double)(l + 1);
-                if constexpr (V == NS_SG) {
+                if constexpr (SCAL) {
                     const float2 v = in ? make_float2((float)x[0].x, (float)x[0].y) : make_float2(0.f, 0.f);
                     cq[it][j] = v;
                     cs[it][j] = v;
@@ -438,8 +440,14 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
     // from the thread itself or the lanes g0 -+ 1 of its level (32-wide
     // shuffles); across rounds from lane 31 of round it - 1 / lane 0 of it + 1
     float2 c1[3][4], c2[3][4];
+    (void)c1; (void)c2;
     float mx0 = 0.f, mx1 = 0.f, mx2 = 0.f;
-    {
+    if constexpr (NCOL == 1) {
+#pragma unroll
+        for (int it = 0; it < 3; ++it)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) mx0 = fmaxf(mx0, fmaxf(fabsf(cq[it][j].x), fabsf(cq[it][j].y)));
+    } else {
         auto sh = [&](float2 v, int src) {
             return make_float2(__shfl(v.x, src, 32), __shfl(v.y, src, 32));
         };
@@ -486,12 +494,12 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
 #pragma unroll
     for (int off = 16; off >= 1; off >>= 1) {
         mx0 = fmaxf(mx0, __shfl_xor(mx0, off));
-        mx1 = fmaxf(mx1, __shfl_xor(mx1, off));
+        if constexpr (NCOL >= 2) mx1 = fmaxf(mx1, __shfl_xor(mx1, off));
         if constexpr (NCOL == 3) mx2 = fmaxf(mx2, __shfl_xor(mx2, off));
     }
     if (lok && g0 == 0) {
         atomicMax(&lvmax[levl], __float_as_uint(mx0));
-        atomicMax(&lvmax[WLG + levl], __float_as_uint(mx1));
+        if constexpr (NCOL >= 2) atomicMax(&lvmax[WLG + levl], __float_as_uint(mx1));
         if constexpr (NCOL == 3) atomicMax(&lvmax[2 * WLG + levl], __float_as_uint(mx2));
     }
     __syncthreads();
@@ -626,7 +634,7 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
     const int r0 = 2 * 16 * (gz * 4 * WTT), r1 = min(a.n_theta, 2 * 16 * (gz + 1) * 4 * WTT);
     const int nq = (r1 - r0) / 2;
 #pragma unroll
-    for (int o = 0; o < 3; ++o) {
+    for (int o = 0; o < NOUT; ++o) {
         __syncthreads();  // B rows / previous output consumed
         const int oc = (NCOL == 2 && o == 2) ? 0 : o;  // SG: i m (P C0)
         const float inv = 1.f / lvsc[oc * WLG + ol];
@@ -1218,7 +1226,7 @@ int launch_nsyn16(const Plan *p, int kind, const NSynArgs &args, hipStream_t s)
     a.nh = p->nh;
     a.n_theta = p->n_theta;
     a.nt16 = p->nt16;
-    if ((p->l_max + 1) / 2 + 1 <= PKP && p->n_theta <= WNT) {
+    if (p->syn_form != 2 && (p->l_max + 1) / 2 + 1 <= PKP && p->n_theta <= WNT) {
         // whole-K form on P only: 12 theta tiles per block
         a.tP = p->d_syn16x;
         a.off = p->d_syn16x_off;
@@ -1231,6 +1239,7 @@ int launch_nsyn16(const Plan *p, int kind, const NSynArgs &args, hipStream_t s)
         case NS_SG: k_nsyn16p<NS_SG><<<grid, 256, 0, s>>>(a); break;
         case NS_QST: k_nsyn16p<NS_QST><<<grid, 256, 0, s>>>(a); break;
         case NS_QSTC: k_nsyn16p<NS_QSTC><<<grid, 256, 0, s>>>(a); break;
+        case NS_S: k_nsyn16p<NS_S><<<grid, 256, 0, s>>>(a); break;
         default: return fail(MHX_E_INVALID, "unknown nonlinear-step synthesis kind");
         }
         MHX_HIP_CHECK(hipGetLastError());
@@ -1275,7 +1284,7 @@ int launch_nana16(const Plan *p, int kind, const NAnaArgs &args, hipStream_t s)
     a.ngz = (ntl0 + 7) / 8;
     const int ntlx = ((p->l_max + 1) / 2 + 1 + 15) / 16;  // l tiles with l <= l_max + 1
     const bool gfit = 3 * WLG * (2 * (p->l_max + 2) + 2) * 4 <= 6 * 16 * AROW;
-    if ((kind == NA_QST || kind == NA_SH) && p->nh <= AKP && ntlx <= 4 * PTT && gfit) {
+    if (p->ana_form != 2 && (kind == NA_QST || kind == NA_SH) && p->nh <= AKP && ntlx <= 4 * PTT && gfit) {
         // whole-theta form on wP only
         a.tP = p->d_ana16x;
         a.off = p->d_ana16x_off;

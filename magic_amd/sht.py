@@ -52,6 +52,10 @@ _SIGS = {
     "mhx_set_timing": "Pi",
     "mhx_stage_times": "Pp",
     "mhx_launch_times": "Ppi",
+    "mhx_batch_create": "piip",
+    "mhx_batch_execute": "pp",
+    "mhx_batch_tuning": "pppppi",
+    "mhx_batch_destroy": "p",
 }
 _CT = {"P": ctypes.c_void_p, "p": ctypes.c_void_p, "i": ctypes.c_int, "d": ctypes.c_double,
        "z": ctypes.c_size_t}
@@ -324,3 +328,63 @@ def nl_step(plan, inputs, outputs, or2, or4, orho1, LFfac, lcut, grids=None, str
     else:
         _check(L.mhx_nl_step_grids(plan.h, nlev, ins, plan.lm_max, outs, d(or2), d(or4),
                                    d(orho1), float(LFfac), lc, _ptr(grids), st))
+
+
+JOB_SCALAR, JOB_VECTOR = 0, 1
+
+
+class _Job(ctypes.Structure):
+    _fields_ = [("l_max", ctypes.c_int), ("nlev", ctypes.c_int), ("kind", ctypes.c_int),
+                ("nfields", ctypes.c_int), ("inp", ctypes.c_void_p), ("out", ctypes.c_void_p)]
+
+
+class Batch:
+    """An uneven batch of independent transform jobs run from one captured
+    graph (mhx_batch_*).  ``jobs``: (l_max, kind, inp, out) with ``inp`` and
+    ``out`` complex128 device tensors of shape (ncomp, nfields, nlev, lm_max),
+    ncomp 1 (JOB_SCALAR: s) or 3 (JOB_VECTOR: W, dW, Z in; Q, S, T out).
+    Each job is the round trip synthesis -> grids -> analysis of its fields
+    (scal_to_spat + scal_to_SH, or torpol_to_spat + spat_to_qst)."""
+
+    def __init__(self, jobs):
+        L = lib()
+        self._keep = []
+        arr = (_Job * len(jobs))()
+        for i, (l_max, kind, inp, out) in enumerate(jobs):
+            nc, nf, nlev, lm = inp.shape
+            if out.shape != inp.shape or nc != (3 if kind == JOB_VECTOR else 1):
+                raise MhxError("job %d: bad array shapes" % i)
+            if lm != (l_max + 1) * (l_max + 2) // 2:
+                raise MhxError("job %d: lm_max does not match l_max" % i)
+            arr[i] = _Job(l_max, nlev, kind, nf, inp.data_ptr(), out.data_ptr())
+            self._keep += [inp, out]
+        h = ctypes.c_void_p()
+        _check(L.mhx_batch_create(arr, len(jobs), PREC_SPLIT16, ctypes.byref(h)))
+        self.h = h
+
+    def execute(self, stream=None):
+        """Replays the batch's graph; with ``stream`` (a torch.cuda.Stream) it
+        returns at once, ordered on the stream."""
+        st = ctypes.c_void_p(stream.cuda_stream) if stream is not None else None
+        _check(lib().mhx_batch_execute(self.h, st))
+
+    def tuning(self):
+        """[(l_max, synthesis form, analysis form, row-FFT rows per block)]
+        as chosen when the batch was made (forms: 1 whole, 2 chunked)."""
+        n = 16
+        a = [(ctypes.c_int * n)() for _ in range(4)]
+        k = lib().mhx_batch_tuning(self.h, *a, n)
+        if k < 0:
+            _check(k)
+        return [tuple(x[i] for x in a) for i in range(min(k, n))]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mhx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -122,3 +122,29 @@ def grid_size(l_max, nalias=20, minc=1):
         raise ValueError("l_max=%d, minc=%d gives an invalid grid (%d x %d)"
                          % (l_max, minc, n_theta, n_phi))
     return n_theta, n_phi
+
+
+def batch_specs(njobs=64, seed=3):
+    """The uneven batch of the batched-transform workload: per job (l_max,
+    kind, nfields, nlev) with l_max from {63, 127, 191, 255}, n_r uniform in
+    17..65, scalar or vector with equal odds, 1..4 fields."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(njobs):
+        L = int(rng.choice([63, 127, 191, 255]))
+        kind = int(rng.integers(0, 2))
+        out.append((L, kind, int(rng.integers(1, 5)), int(rng.integers(17, 66))))
+    return out
+
+
+def batch_coeffs(rng, L, ncomp, nfields, nlev):
+    """(ncomp, nfields, nlev, lm_max) complex coefficients, amplitude
+    (l+1)^-2, random phases, real at m = 0 (m-major, l fastest)."""
+    ls, ms = [], []
+    for m in range(L + 1):
+        for l in range(m, L + 1):
+            ls.append(l)
+            ms.append(m)
+    ls, ms = np.array(ls), np.array(ms)
+    x = (ls + 1.0) ** -2 * np.exp(2j * np.pi * rng.random((ncomp, nfields, nlev, ls.size)))
+    return np.where(ms == 0, x.real + 0j, x).astype(np.complex128)
