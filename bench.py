@@ -45,7 +45,10 @@ LAUNCHES = [
 # kernels behind each role (prefix of the rocprofv3 kernel name)
 KERNELS = {
     "split16": {"syn_sg": "k_nsyn16p<0>", "syn_qst": "k_nsyn16p<", "grid": "k_nl_slots<float",
-                "ana_qst": "k_nana16p<0>", "ana_sh": "k_nana16w<1>"},
+                "ana_qst": "k_nana16p<0>", "ana_sh": "k_nana16p<1>"},
+    # l_max > 255: the chunked split Legendre kernels
+    "split16_chunked": {"syn_sg": "k_nsyn16<0>", "syn_qst": "k_nsyn16<", "grid": "k_nl_slots<float",
+                        "ana_qst": "k_nana16<0>", "ana_sh": "k_nana16<1>"},
     "fp64": {"syn_sg": "k_syn<2>", "syn_qst": "k_syn<3>", "grid": "k_nl_slots<double",
              "ana_qst": "k_ana<2>", "ana_sh": "k_ana<0>"},
 }
@@ -56,18 +59,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic(kind, w, nsrc, nh, lm_max, n_theta, n_phi, n_m, nlev, word, tword):
+def algorithmic(kind, w, nsrc, nh, lm_max, n_theta, n_phi, n_m, nlev, word, tword, p_only=False):
     """(flops, HBM bytes) of one launch.  Legendre: w N_h lm_max flops per
     level; bytes = coefficient arrays (16 B per lm) + Fourier comps (n_theta x
-    n_m complex of 2 word bytes) + the tables read once (P and dP, tword
-    bytes per entry: 8 FP64, 4 for the FP16 hi/lo pair).  Grid stage: 18
-    Fourier comps in, 10 out, 3 du/dr grids (n_theta x n_phi) out."""
+    n_m complex of 2 word bytes) + the tables read once (P and dP, or P alone
+    for the split kernels that fold dP onto P; tword bytes per entry: 8 FP64,
+    4 for the FP16 hi/lo pair).  Grid stage: 18 Fourier comps in, 10 out, 3
+    du/dr grids (n_theta x n_phi) out."""
     four = n_theta * n_m * 2 * word
     coef = 16 * lm_max
     if kind == "grid":
         return 0.0, float(((18 + 10) * four + 3 * n_theta * n_phi * word) * nlev)
     flops = float(w * nh * lm_max * nlev)
-    ntab = 1 if w == 4 else 2
+    ntab = 1 if (w == 4 or p_only) else 2
     if kind == "syn":
         nout = 3
         byts = (nsrc * coef + nout * four) * nlev + ntab * tword * nh * lm_max
@@ -147,7 +151,8 @@ def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut):
     per_kernel = {}
     total_flop = flop_ms = 0.0
     for (role, kind, w, nsrc), t in zip(LAUNCHES, lt):
-        fl, by = algorithmic(kind, w, nsrc, nh, lm_max, nt, nph, n_m, nlev, word, tword)
+        fl, by = algorithmic(kind, w, nsrc, nh, lm_max, nt, nph, n_m, nlev, word, tword,
+                             p_only=split and L <= 255)
         d = per_kernel.setdefault(role, {"ms": 0.0, "n": 0, "flop": 0.0, "bytes": 0.0})
         d["ms"] += t
         d["n"] += 1
@@ -169,9 +174,15 @@ def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut):
             "legendre_tflops": total_flop / (flop_ms * 1e-3) / 1e12, "outs": outs}
 
 
-def measured_traffic(form_key, role):
+PROFILED = (255, 97)  # (l_max, n_r) of the committed PMC summary
+
+
+def measured_traffic(form_key, role, workload):
     """HBM bytes per launch of the role's kernel from the committed rocprofv3
-    PMC summary (FETCH_SIZE and WRITE_SIZE scaled by a plain copy), or None."""
+    PMC summary (FETCH_SIZE and WRITE_SIZE scaled by a plain copy) when it
+    covers this workload, else None."""
+    if workload != PROFILED:
+        return None
     try:
         data = json.load(open(TRAFFIC_JSON))
     except (OSError, ValueError):
@@ -183,10 +194,10 @@ def measured_traffic(form_key, role):
     return sum(h["bytes"] for h in hits) / len(hits)
 
 
-def roofline(form, form_key):
+def roofline(form, form_key, workload):
     """Roofline of the dominant kernel (largest total time): MFMA-bound when
     its arithmetic intensity exceeds the ridge of its datapath, else HBM."""
-    peak_key = "fp16" if form_key == "split16" else "fp64"
+    peak_key = "fp16" if form_key.startswith("split16") else "fp64"
     pk = form["per_kernel"]
     dom = max(pk, key=lambda k: pk[k]["ms"])
     d = pk[dom]
@@ -203,7 +214,7 @@ def roofline(form, form_key):
              "frac": achieved / PEAK_HBM_GBS}
     r.update({"kernel": KERNELS[form_key][dom], "role": dom, "avg_launch_ms": avg_ms,
               "algorithmic_bytes": byts, "algorithmic_flop": flop,
-              "traffic": measured_traffic(form_key, dom)})
+              "traffic": measured_traffic(form_key, dom, workload)})
     return r
 
 
@@ -296,9 +307,10 @@ def main():
     ap.add_argument("--cpu-levels", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["both", "split16", "fp64"], default="both")
-    ap.add_argument("--config", type=int, choices=[2, 3], default=2,
-                    help="2: the nonlinear step (headline); 3: the uneven batch of 64 "
-                         "transform jobs in one captured graph")
+    ap.add_argument("--config", type=int, choices=[2, 3, 4], default=2,
+                    help="2: the nonlinear step at l_max=255 x 97 (headline); 3: the uneven "
+                         "batch of 64 transform jobs in one captured graph; 4: the nonlinear "
+                         "step at l_max=511 x 161 with the l=1 poloidal field x100")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the bench, run one device-to-device copy of this many MiB "
                          "(a kernel of known traffic, for scaling hardware byte counters)")
@@ -321,9 +333,12 @@ def main():
         print(json.dumps(bench_batch(args, dev)))
         return
     L, nlev = args.l_max, args.nlev
+    dipole = 1.0
+    if args.config == 4:
+        L, nlev, dipole = 511, 161, 100.0
     nt, nph = workload.grid_size(L)
     log("l_max=%d %dx%d nlev=%d" % (L, nt, nph, nlev))
-    inputs = workload.nl_inputs(L, nlev)
+    inputs = workload.nl_inputs(L, nlev, config=args.config, dipole=dipole)
     r = np.linspace(0.55, 1.45, nlev)
     rad = (1.0 / r ** 2, 1.0 / r ** 4, 1.0 / (1.0 + 0.1 * r), 0.8)
     lcut = [L] * nlev
@@ -341,6 +356,7 @@ def main():
         log("split16: %.3f ms/step, %.1f pairs/s" % (forms["split16"]["ms_per_step"],
                                                      forms["split16"]["pairs"]))
     main_key = "split16" if "split16" in forms else "fp64"
+    kern_key = "split16_chunked" if main_key == "split16" and L > 255 else main_key
     main = forms[main_key]
     errs = None
     if "fp64" in forms and "split16" in forms:
@@ -373,17 +389,17 @@ def main():
         "config": {"workload": "full nonlinear step (5 QST + scalar + gradient inverse, "
                                "fused FFTs + grid products, 3 QST + scalar forward)",
                    "l_max": L, "n_theta": nt, "n_phi": nph, "n_r": nlev,
-                   "precision": main_key},
+                   "dipole_l1_factor": dipole, "precision": main_key},
         "legendre_tflops_fp64_equiv": main["legendre_tflops"],
         "kernels": main["kernels"],
-        "roofline": roofline(main, main_key),
+        "roofline": roofline(main, kern_key, (L, nlev)),
         "cpu_baseline": cpu,
     }
     if "fp64" in forms and main_key != "fp64":
         f = forms["fp64"]
         res["fp64_form"] = {"value": f["pairs"], "ms_per_step": f["ms_per_step"],
                             "legendre_tflops": f["legendre_tflops"], "kernels": f["kernels"],
-                            "roofline": roofline(f, "fp64")}
+                            "roofline": roofline(f, "fp64", (L, nlev))}
         res["speedup_vs_fp64"] = main["pairs"] / f["pairs"]
     if errs is not None:
         res["rel_l2_err_vs_fp64"] = errs
