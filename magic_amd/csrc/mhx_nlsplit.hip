@@ -356,7 +356,7 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
     constexpr int NSRC = SCAL ? 1 : (V == NS_QSTC ? 4 : 3);
     constexpr int NCOL = V == NS_S ? 1 : (V == NS_SG ? 2 : 3), NROW = NCOL * 2 * 16;
     constexpr int NOUT = V == NS_S ? 1 : 3;
-    constexpr int SB = NROW * PROW > WLG * (2 * WNT + 4) * 4 ? NROW * PROW : WLG * (2 * WNT + 4) * 4;
+    constexpr int SB = NROW * PROW;  // B rows [column][parity][16 columns][hi | lo]
     __shared__ __align__(16) unsigned char sb[SB];
     __shared__ unsigned lvmax[NCOL * WLG];
     __shared__ float lvsc[NCOL * WLG];
@@ -627,23 +627,24 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
     default: contract(std::integral_constant<int, NCH>()); break;
     }
 
-    // ---- 5. epilogue, one output at a time (contiguous theta runs)
-    float *so = reinterpret_cast<float *>(sb);
-    const int rowf = 2 * a.n_theta + 4;
+    // ---- 5. epilogue straight from the accumulators: lane (level, re/im)
+    // holds theta_h = 4 (lane / 16) + j of a tile; after one exchange with its
+    // re/im partner each lane owns two theta_h (j = p + 2 ri) and stores the
+    // complex (North, South) pair of each as 16 contiguous bytes
     const int n = lane & 15, ol = n >> 1, ri = n & 1;
-    const int r0 = 2 * 16 * (gz * 4 * WTT), r1 = min(a.n_theta, 2 * 16 * (gz + 1) * 4 * WTT);
-    const int nq = (r1 - r0) / 2;
+    const bool lvok = ol < nlh;
 #pragma unroll
     for (int o = 0; o < NOUT; ++o) {
-        __syncthreads();  // B rows / previous output consumed
         const int oc = (NCOL == 2 && o == 2) ? 0 : o;  // SG: i m (P C0)
         const float inv = 1.f / lvsc[oc * WLG + ol];
+        float *fo = a.fout + ((size_t)(lev0 + (lvok ? ol : 0)) * a.fout_ld +
+                              ((size_t)(a.comp0 + o) * a.n_m + mc) * a.n_theta) * 2;
 #pragma unroll
         for (int tt = 0; tt < WTT; ++tt) {
             if (!tok[tt]) continue;
+            float vn[4], vs[4];  // North, South of this lane's re or im part
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int th = (tile0 + tt) * 16 + 4 * (lane >> 4) + j;
                 float sy = acc[oc][0][tt][j], an = acc[oc][1][tt][j];
                 if (NCOL == 2 && o == 2) {
                     // (i m) X: re = -m X.im, im = m X.re, from lane n ^ 1
@@ -651,21 +652,25 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
                     sy = ri ? fm * osy : -fm * osy;
                     an = ri ? fm * oan : -fm * oan;
                 }
-                if (th >= a.nh) continue;
                 const float rs = inv / rsv[tt][j];
-                float *d = so + ol * rowf + 4 * th + ri;
-                d[0] = (sy + an) * rs;
-                d[2] = (sy - an) * rs;
+                vn[j] = (sy + an) * rs;
+                vs[j] = (sy - an) * rs;
             }
-        }
-        __syncthreads();
-        for (int i = tid; i < WLG * nq; i += 256) {
-            const int q = i % nq, lv = i / nq;
-            if (lv >= nlh) continue;
-            const f4 w = *reinterpret_cast<const f4 *>(so + lv * rowf + 2 * r0 + 4 * q);
-            float *fo = a.fout + ((size_t)(lev0 + lv) * a.fout_ld +
-                                  ((size_t)(a.comp0 + o) * a.n_m + mc) * a.n_theta + r0) * 2;
-            *reinterpret_cast<f4 *>(fo + 4 * q) = w;
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                // send the partner the part of the theta_h it stores (selects
+                // with fixed indices keep vn, vs in registers)
+                const float on = ri ? vn[p + 2] : vn[p], os = ri ? vs[p + 2] : vs[p];
+                const float gn = __shfl_xor(ri ? vn[p] : vn[p + 2], 1);
+                const float gs = __shfl_xor(ri ? vs[p] : vs[p + 2], 1);
+                const int th = (tile0 + tt) * 16 + 4 * (lane >> 4) + p + 2 * ri;
+                f4 w;
+                w[0] = ri ? gn : on;
+                w[1] = ri ? on : gn;
+                w[2] = ri ? gs : os;
+                w[3] = ri ? os : gs;
+                if (lvok && th < a.nh) *reinterpret_cast<f4 *>(fo + 4 * th) = w;
+            }
         }
     }
 }
