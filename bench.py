@@ -304,13 +304,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--l_max", type=int, default=255)
     ap.add_argument("--nlev", type=int, default=97)
-    ap.add_argument("--cpu-levels", type=int, default=24)
+    ap.add_argument("--cpu-levels", type=int, default=None,
+                    help="levels of the CPU baseline sample (default 24, 8 at l_max=511, "
+                         "3 at l_max=1023: about 10-30 s of CPU work)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["both", "split16", "fp64"], default="both")
-    ap.add_argument("--config", type=int, choices=[2, 3, 4], default=2,
+    ap.add_argument("--spectrum", default=None,
+                    help="amplitude spectrum of the synthesised fields: l^-5/3 (default), "
+                         "white-to-l^-3 (default of --config 5), slopes-0-3, l+1^-<s>")
+    ap.add_argument("--config", type=int, choices=[2, 3, 4, 5], default=2,
                     help="2: the nonlinear step at l_max=255 x 97 (headline); 3: the uneven "
                          "batch of 64 transform jobs in one captured graph; 4: the nonlinear "
-                         "step at l_max=511 x 161 with the l=1 poloidal field x100")
+                         "step at l_max=511 x 161 with the l=1 poloidal field x100; 5: the "
+                         "nonlinear step at l_max=1023 x 64 with white-to-l^-3 spectra")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the bench, run one device-to-device copy of this many MiB "
                          "(a kernel of known traffic, for scaling hardware byte counters)")
@@ -333,12 +339,15 @@ def main():
         print(json.dumps(bench_batch(args, dev)))
         return
     L, nlev = args.l_max, args.nlev
-    dipole = 1.0
+    dipole, kind = 1.0, "l^-5/3"
     if args.config == 4:
         L, nlev, dipole = 511, 161, 100.0
+    if args.config == 5:
+        L, nlev, kind = 1023, 64, "white-to-l^-3"
+    kind = args.spectrum or kind
     nt, nph = workload.grid_size(L)
-    log("l_max=%d %dx%d nlev=%d" % (L, nt, nph, nlev))
-    inputs = workload.nl_inputs(L, nlev, config=args.config, dipole=dipole)
+    log("l_max=%d %dx%d nlev=%d spectrum %s" % (L, nt, nph, nlev, kind))
+    inputs = workload.nl_inputs(L, nlev, config=args.config, kind=kind, dipole=dipole)
     r = np.linspace(0.55, 1.45, nlev)
     rad = (1.0 / r ** 2, 1.0 / r ** 4, 1.0 / (1.0 + 0.1 * r), 0.8)
     lcut = [L] * nlev
@@ -365,12 +374,24 @@ def main():
             a, b = forms["split16"]["outs"][k], forms["fp64"]["outs"][k]
             errs[name] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
         log("max rel L2 error split vs fp64: %.3e" % max(errs.values()))
+        # per l-band of 64 (the large-shell cases state their error by band)
+        ls = np.concatenate([np.arange(m, L + 1) for m in range(L + 1)])
+        band_errs = {}
+        for k, name in enumerate(sht.NL_OUTPUTS):
+            a, b = forms["split16"]["outs"][k], forms["fp64"]["outs"][k]
+            row = []
+            for b0 in range(0, L + 1, 64):
+                sel = (ls >= b0) & (ls < b0 + 64)
+                den = np.linalg.norm(b[:, sel])
+                row.append(float(np.linalg.norm(a[:, sel] - b[:, sel]) / den) if den > 0 else 0.0)
+            band_errs[name] = row
 
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline (%d levels)" % args.cpu_levels)
+        ncpu = args.cpu_levels or {511: 8, 1023: 3}.get(L, 24)
+        log("cpu baseline (%d levels)" % ncpu)
         try:
-            cpu = cpu_baseline(L, args.cpu_levels)
+            cpu = cpu_baseline(L, ncpu)
         except Exception as e:  # reported, not fatal for the GPU figure
             log("cpu baseline failed: %r" % (e,))
     res = {
@@ -389,7 +410,7 @@ def main():
         "config": {"workload": "full nonlinear step (5 QST + scalar + gradient inverse, "
                                "fused FFTs + grid products, 3 QST + scalar forward)",
                    "l_max": L, "n_theta": nt, "n_phi": nph, "n_r": nlev,
-                   "dipole_l1_factor": dipole, "precision": main_key},
+                   "dipole_l1_factor": dipole, "spectrum": kind, "precision": main_key},
         "legendre_tflops_fp64_equiv": main["legendre_tflops"],
         "kernels": main["kernels"],
         "roofline": roofline(main, kern_key, (L, nlev)),
@@ -403,6 +424,8 @@ def main():
         res["speedup_vs_fp64"] = main["pairs"] / f["pairs"]
     if errs is not None:
         res["rel_l2_err_vs_fp64"] = errs
+        if L > 255:
+            res["rel_l2_err_vs_fp64_per_l_band_of_64"] = band_errs
         res["max_rel_l2_err_vs_fp64"] = max(errs.values())
     if args.calib_copy_mib > 0:
         n = args.calib_copy_mib << 18  # float32 elements

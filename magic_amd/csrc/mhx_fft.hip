@@ -551,15 +551,16 @@ __global__ void k_axi(int l_max, int nh, const double *__restrict__ plm_syn,
 }
 
 // ---------------------------------------------------------------------------
-// Fused grid stage, slot form.  Each complex FFT of length M = 12 Q is done
+// Fused grid stage, slot form.  Each complex FFT of length M = P Q is done
 // by one "slot" of Q lanes inside a single wave (Q divides 64), every lane
-// holding 12 points; the Stockham passes (radices 4, 2, 3 dividing 12) run
-// in place in the slot's LDS row with no block barrier: within a wave the
-// LDS operations execute in program order, and each pass loads all of its
-// inputs before it stores any output.  Six slots work on one theta row:
-// the 18 inverse FFTs in three rounds of six, the 10 forward FFTs in two.
-// Between rounds the block redistributes the row so that each thread holds
-// NPT grid points of every component for the products.
+// holding P points (P = 12, or 24 for M = 1536 where 12 points per lane
+// would need 128 lanes); the Stockham passes (radices 12, 4 and 2, each
+// dividing P) run in place in the slot's LDS row with no block barrier:
+// within a wave the LDS operations execute in program order, and each pass
+// loads all of its inputs before it stores any output.  Six slots work on one
+// theta row: the 18 inverse FFTs in three rounds of six, the 10 forward FFTs
+// in two.  Between rounds the block redistributes the row so that each thread
+// holds P / 3 grid points of every component for the products.
 
 // compiler-level ordering of LDS accesses between the lanes of one wave
 __device__ __forceinline__ void wave_order()
@@ -568,10 +569,11 @@ __device__ __forceinline__ void wave_order()
     asm volatile("" ::: "memory");
 }
 
-template <int R, int L, int M, typename C>
+template <int R, int L, int M, int P, typename C>
 __device__ __forceinline__ void slot_pass(C *buf, int t, const C *tw, int sign)
 {
-    constexpr int Q = M / 12, NB = 12 / R;  // lanes per slot, butterflies per lane
+    constexpr int Q = M / P, NB = P / R;  // lanes per slot, butterflies per lane
+    static_assert(P % R == 0 && Q * P == M, "pass does not tile the slot");
     constexpr int T = M / R, tstride = M / (L * R);
     C x[NB][R];
 #pragma unroll
@@ -606,27 +608,27 @@ __device__ __forceinline__ void slot_pass(C *buf, int t, const C *tw, int sign)
 }
 
 // M = 3 4^A 2^B = 12 4^(A-1) 2^B: a radix-12 pass (no twiddles), A - 1
-// radix-4 passes, B radix-2 passes
+// radix-4 passes, B radix-2 passes; Q = min(M / 12, 64) lanes of P points
 constexpr int pow4(int a) { return a == 0 ? 1 : 4 * pow4(a - 1); }
 template <int A, int B> struct SlotFFT {
     static constexpr int M = 3 * pow4(A) * (B ? 2 : 1);
+    static constexpr int Q = M / 12 <= 64 ? M / 12 : 64, P = M / Q;
+    static_assert(Q * P == M && P % 12 == 0 && 64 % Q == 0, "unsupported slot length");
     template <int I, typename C>
     __device__ __forceinline__ static void radix4(C *buf, int t, const C *tw, int sign)
     {
         if constexpr (I < A) {
-            slot_pass<4, 12 * pow4(I - 1), M>(buf, t, tw, sign);
+            slot_pass<4, 12 * pow4(I - 1), M, P>(buf, t, tw, sign);
             radix4<I + 1>(buf, t, tw, sign);
         }
     }
     template <typename C> __device__ __forceinline__ static void run(C *buf, int t, const C *tw, int sign)
     {
-        slot_pass<12, 1, M>(buf, t, tw, sign);
+        slot_pass<12, 1, M, P>(buf, t, tw, sign);
         radix4<1>(buf, t, tw, sign);
-        if constexpr (B) slot_pass<2, 12 * pow4(A - 1), M>(buf, t, tw, sign);
+        if constexpr (B) slot_pass<2, 12 * pow4(A - 1), M, P>(buf, t, tw, sign);
     }
 };
-
-constexpr int NPT = 4;  // grid points per thread (N <= 4 * threads)
 
 template <typename T, int A, int B>
 __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict__ Fin,
@@ -639,7 +641,8 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
     constexpr bool BLK = sizeof(T) == 4;  // split form: theta-blocked F2
     extern __shared__ char smem_raw[];
     typedef SlotFFT<A, B> FFT;
-    constexpr int M = FFT::M, N = 2 * M, Q = M / 12;
+    constexpr int M = FFT::M, N = 2 * M, Q = FFT::Q;
+    constexpr int NPT = FFT::P / 3;  // grid points per thread: N = NPT * 6 Q
     C *s_tw = reinterpret_cast<C *>(smem_raw);  // M + M / 8 (padded)
     C *s_twr = s_tw + M + M / 8;
     C *rows = s_twr + M + 1;  // 6 slot rows of M
@@ -903,7 +906,8 @@ int fused_threads(int n_phi)
 }
 
 // the slot form applies when M = 3 4^A 2^B with 1 <= A <= 4, B <= 1 and
-// M / 12 lanes per slot dividing 64: M = 12 .. 768 (n_phi = 24 .. 1536)
+// min(M / 12, 64) lanes per slot dividing 64: M = 12 .. 1536 (n_phi = 24 ..
+// 3072)
 bool slot_form(const Plan *p, int &A, int &B)
 {
     int m = p->M;
@@ -913,14 +917,14 @@ bool slot_form(const Plan *p, int &A, int &B)
     while (m % 4 == 0) { m /= 4; ++A; }
     B = (m == 2);
     if (m != 1 && m != 2) return false;
-    return A >= 1 && A <= 4 && !(A == 4 && B);
+    return A >= 1 && A <= 4;
 }
 
 template <typename T, int A, int B>
 int launch_slots(const Plan *p, const FFTArgs &f, int nlev, const void *Fin, void *Fout,
                  const double *rad, double LFfac, void *dgrid, void *full_grids, hipStream_t s)
 {
-    constexpr int M = SlotFFT<A, B>::M, Q = M / 12;
+    constexpr int M = SlotFFT<A, B>::M, Q = SlotFFT<A, B>::Q, NPT = SlotFFT<A, B>::P / 3;
     int nthr = 6 * Q;
     if (nthr < 64) nthr = 64;
     nthr = (nthr + 63) / 64 * 64;
@@ -962,24 +966,41 @@ int launch_fused_t(const Plan *p, int nlev, const void *Fin, void *Fout, const d
         case 6: return launch_slots<T, 3, 0>(p, f, nlev, Fin, Fout, rad, LFfac, dgrid, full_grids, s);
         case 7: return launch_slots<T, 3, 1>(p, f, nlev, Fin, Fout, rad, LFfac, dgrid, full_grids, s);
         case 8: return launch_slots<T, 4, 0>(p, f, nlev, Fin, Fout, rad, LFfac, dgrid, full_grids, s);
+        case 9:
+            // M = 1536 (the l_max=1023 grid): six rows of 1536 take 98 KB of
+            // LDS in FP32; in FP64 they do not fit and the generic form runs
+            if constexpr (sizeof(T) == 4)
+                return launch_slots<T, 4, 1>(p, f, nlev, Fin, Fout, rad, LFfac, dgrid, full_grids, s);
+            break;
         default: break;
         }
     }
     int nthr = fused_threads(p->n_phi);
     if (nthr * PMAX < p->n_phi) return fail(MHX_E_UNSUPPORTED, "n_phi too large for the fused stage");
     const size_t cs = 2 * sizeof(T);
+    // three rows per FFT pass while they fit the default 64 KB, else one row
+    // in up to the 160 KB a workgroup may hold on gfx950 (n_phi = 3072 in FP64)
     int G = ((size_t)(6 * p->M + 2 * p->M + 1) * cs <= 64 * 1024) ? 3 : 1;
     size_t shm = ((size_t)2 * G * p->M + 2 * p->M + 1) * cs;
-    if (shm > 64 * 1024) return fail(MHX_E_UNSUPPORTED, "n_phi too large for the fused stage (LDS)");
+    if (shm > 160 * 1024) return fail(MHX_E_UNSUPPORTED, "n_phi too large for the fused stage (LDS)");
     dim3 grid((unsigned)(p->n_theta * nlev));
     const T *fi = static_cast<const T *>(Fin);
     T *fo = static_cast<T *>(Fout), *dg = static_cast<T *>(dgrid), *fg = static_cast<T *>(full_grids);
-    if (G == 3)
+    if (G == 3) {
         k_nl_fused<T, 3><<<grid, nthr, shm, s>>>(f, fi, fo, p->n_m, p->n_theta, nlev, p->d_osin2,
                                                   rad, LFfac, dg, fg);
-    else
+    } else {
+        // set once per instance and raised only (the attribute persists), so
+        // that a call under graph capture makes no attribute call
+        static int shm_set = 0;
+        if (shm > 64 * 1024 && (int)shm > shm_set) {
+            MHX_HIP_CHECK(hipFuncSetAttribute((const void *)k_nl_fused<T, 1>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+            shm_set = (int)shm;
+        }
         k_nl_fused<T, 1><<<grid, nthr, shm, s>>>(f, fi, fo, p->n_m, p->n_theta, nlev, p->d_osin2,
                                                   rad, LFfac, dg, fg);
+    }
     MHX_HIP_CHECK(hipGetLastError());
     return MHX_OK;
 }

@@ -52,6 +52,9 @@ def spectrum(l, kind):
         return (l + 1.0) ** -2
     if kind == "l^-5/3":
         return np.maximum(l, 1.0) ** (-5.0 / 3.0)
+    if kind == "white-to-l^-3":
+        # flat for l << 64, l^-3 for l >> 64
+        return (1.0 + (l / 64.0) ** 2) ** -1.5
     if kind.startswith("l+1^-"):
         return (l + 1.0) ** (-float(kind[5:]))
     raise ValueError(kind)
@@ -86,6 +89,8 @@ def nl_inputs(l_max, nlev, config=2, kind="l^-5/3", minc=1, dipole=1.0):
     lsafe = np.maximum(ls, 1).astype(np.float64)
     out = []
     for f, name in enumerate(NL_FIELD_ORDER):
+        # "slopes-0-3": field f has amplitude (l+1)^-s with s = f mod 4
+        kf = "l+1^-%d" % (f % 4) if kind == "slopes-0-3" else kind
         if name in ("w", "b"):
             fac = 1.0 / (lsafe * (lsafe + 1.0))
         elif name == "s":
@@ -94,7 +99,7 @@ def nl_inputs(l_max, nlev, config=2, kind="l^-5/3", minc=1, dipole=1.0):
             fac = 1.0 / lsafe
         if dipole != 1.0 and name in ("b", "db", "ddb"):
             fac = fac * np.where(ls == 1, dipole, 1.0)
-        out.append(np.stack([coefficients(l_max, config, f, lev, kind, potential=(name != "s"),
+        out.append(np.stack([coefficients(l_max, config, f, lev, kf, potential=(name != "s"),
                                           minc=minc) * fac for lev in range(nlev)]))
     return out
 

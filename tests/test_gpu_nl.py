@@ -64,6 +64,36 @@ def test_nl_step_matches_oracle(L, nlev):
         assert err < OUT_TOL, ("out", sht.NL_OUTPUTS[k], err)
 
 
+@pytest.mark.parametrize("n_phi", [200, 3072])
+def test_nl_step_generic_grid_stage(n_phi):
+    """M = n_phi/2 = 100 (no factor 3) takes the generic fused grid stage in
+    both forms; M = 1536 (the l_max=1023 grid) takes the slot form with 24
+    points per lane in FP32 and the generic form with one FFT row per pass
+    (98 KB of LDS) in FP64: FP64 within 1e-12 of the oracle, the split form
+    within 1e-6."""
+    from oracle import oracle
+    L, nt, nlev = 64, 96, 3
+    print("nl_step generic grid stage", n_phi, flush=True)
+    o = oracle.Plan(L, nt, n_phi)
+    inputs = make_inputs(L, nlev)
+    rad = radial(nlev)
+    lcut = [L] * nlev
+    Gr, outr = oracle_step(o, inputs, *rad, lcut)
+    p = sht.Plan(L, nt, n_phi, max_levels=nlev)
+    G, outs = _run(p, inputs, rad, lcut)
+    p.close()
+    for c in range(18):
+        assert rel_l2(G[:, c], Gr[:, c]) < 1e-12, ("grid", sht.NL_GRIDS[c])
+    for k in range(10):
+        assert rel_l2(outs[k], outr[k]) < OUT_TOL, ("out", sht.NL_OUTPUTS[k])
+    p16 = sht.Plan(L, nt, n_phi, precision=sht.PREC_SPLIT16, max_levels=nlev)
+    _, outs = _run(p16, inputs, rad, lcut, with_grids=False)
+    p16.close()
+    for k in range(10):
+        e = rel_l2(outs[k], outr[k])
+        assert e < 1e-6, ("split out", sht.NL_OUTPUTS[k], e)
+
+
 def test_nl_step_full_size_properties():
     """l_max=255, 384x768, 97 levels: level independence (a level's result
     does not depend on the batch it runs in) and the linear grid-to-spectrum
