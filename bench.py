@@ -305,8 +305,8 @@ def main():
     ap.add_argument("--l_max", type=int, default=255)
     ap.add_argument("--nlev", type=int, default=97)
     ap.add_argument("--cpu-levels", type=int, default=None,
-                    help="levels of the CPU baseline sample (default 24, 8 at l_max=511, "
-                         "3 at l_max=1023: about 10-30 s of CPU work)")
+                    help="levels of the CPU baseline sample (default: all 97 at l_max=255, 8 at "
+                         "l_max=511, 3 at l_max=1023, else 24)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["both", "split16", "fp64"], default="both")
     ap.add_argument("--spectrum", default=None,
@@ -388,7 +388,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        ncpu = args.cpu_levels or {511: 8, 1023: 3}.get(L, 24)
+        ncpu = args.cpu_levels or {255: 97, 511: 8, 1023: 3}.get(L, 24)
         log("cpu baseline (%d levels)" % ncpu)
         try:
             cpu = cpu_baseline(L, ncpu)

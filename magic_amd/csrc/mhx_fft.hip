@@ -613,6 +613,8 @@ constexpr int pow4(int a) { return a == 0 ? 1 : 4 * pow4(a - 1); }
 template <int A, int B> struct SlotFFT {
     static constexpr int M = 3 * pow4(A) * (B ? 2 : 1);
     static constexpr int Q = M / 12 <= 64 ? M / 12 : 64, P = M / Q;
+    // threads of the grid-stage block: six slots, at least one wave
+    static constexpr int NTHR = (6 * Q < 64 ? 64 : 6 * Q + 63) / 64 * 64;
     static_assert(Q * P == M && P % 12 == 0 && 64 % Q == 0, "unsupported slot length");
     template <int I, typename C>
     __device__ __forceinline__ static void radix4(C *buf, int t, const C *tw, int sign)
@@ -681,17 +683,48 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
             const int comp = round == 0 ? slot : (round == 1 ? 9 + slot : (slot < 3 ? 6 + slot : 12 + slot));
             const C *X = Fi + (size_t)comp * n_m * n_theta;
             // Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/N} (X_k - conj X_{M-k})
-            for (int k = t; k <= M / 2; k += Q) {
-                const int k2 = M - k;
-                C a = (k < n_m) ? X[(size_t)k * n_theta] : mk<C>(T(0), T(0));
-                C c = (k2 < n_m) ? X[(size_t)k2 * n_theta] : mk<C>(T(0), T(0));
-                if (k == 0) {
-                    a.y = T(0);
-                    c.y = T(0);
+            if constexpr (FFT::P == 24) {
+                // M = 1536: the lane's k = t + i Q <= M/2 take NK = 13 steps;
+                // all their loads are issued first, at clamped addresses
+                // (measured faster there; at 12 points per lane the guarded
+                // loads one step at a time are faster: they skip the
+                // coefficients past n_m)
+                constexpr int NK = M / (2 * Q) + 1;
+                C xa[NK], xc[NK];
+#pragma unroll
+                for (int i = 0; i < NK; ++i) {
+                    const int k = t + i * Q;
+                    xa[i] = X[(size_t)min(k, n_m - 1) * n_theta];
+                    xc[i] = X[(size_t)min(M - k, n_m - 1) * n_theta];
                 }
-                z[k] = cadd(cadd(a, cconj(c)), cmuli(cmul(cconj(s_twr[k]), csub(a, cconj(c))), 1));
-                if (k2 < M && k2 != k)
-                    z[k2] = cadd(cadd(c, cconj(a)), cmuli(cmul(cconj(s_twr[k2]), csub(c, cconj(a))), 1));
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int i = 0; i < NK; ++i) {
+                    const int k = t + i * Q, k2 = M - k;
+                    if (k > M / 2) continue;
+                    C a = (k < n_m) ? xa[i] : mk<C>(T(0), T(0));
+                    C c = (k2 < n_m) ? xc[i] : mk<C>(T(0), T(0));
+                    if (k == 0) {
+                        a.y = T(0);
+                        c.y = T(0);
+                    }
+                    z[k] = cadd(cadd(a, cconj(c)), cmuli(cmul(cconj(s_twr[k]), csub(a, cconj(c))), 1));
+                    if (k2 < M && k2 != k)
+                        z[k2] = cadd(cadd(c, cconj(a)), cmuli(cmul(cconj(s_twr[k2]), csub(c, cconj(a))), 1));
+                }
+            } else {
+                for (int k = t; k <= M / 2; k += Q) {
+                    const int k2 = M - k;
+                    C a = (k < n_m) ? X[(size_t)k * n_theta] : mk<C>(T(0), T(0));
+                    C c = (k2 < n_m) ? X[(size_t)k2 * n_theta] : mk<C>(T(0), T(0));
+                    if (k == 0) {
+                        a.y = T(0);
+                        c.y = T(0);
+                    }
+                    z[k] = cadd(cadd(a, cconj(c)), cmuli(cmul(cconj(s_twr[k]), csub(a, cconj(c))), 1));
+                    if (k2 < M && k2 != k)
+                        z[k2] = cadd(cadd(c, cconj(a)), cmuli(cmul(cconj(s_twr[k2]), csub(c, cconj(a))), 1));
+                }
             }
             wave_order();
             FFT::run(z, t, s_tw, +1);
@@ -924,10 +957,7 @@ template <typename T, int A, int B>
 int launch_slots(const Plan *p, const FFTArgs &f, int nlev, const void *Fin, void *Fout,
                  const double *rad, double LFfac, void *dgrid, void *full_grids, hipStream_t s)
 {
-    constexpr int M = SlotFFT<A, B>::M, Q = SlotFFT<A, B>::Q, NPT = SlotFFT<A, B>::P / 3;
-    int nthr = 6 * Q;
-    if (nthr < 64) nthr = 64;
-    nthr = (nthr + 63) / 64 * 64;
+    constexpr int M = SlotFFT<A, B>::M, NPT = SlotFFT<A, B>::P / 3, nthr = SlotFFT<A, B>::NTHR;
     if (NPT * nthr < p->n_phi) return fail(MHX_E_UNSUPPORTED, "n_phi too large for the fused stage");
     const size_t shm = ((size_t)2 * M + M / 8 + 1 + 6 * (size_t)M) * 2 * sizeof(T);
     // set once per kernel (the attribute persists), so that a call under

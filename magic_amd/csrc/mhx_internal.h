@@ -188,6 +188,13 @@ struct Plan {
     hipStream_t stream = nullptr;
     hipEvent_t ev_user = nullptr;  // orders the plan stream after a caller's stream
     hipEvent_t ev_done = nullptr;  // orders a caller's stream after the plan stream
+    // branch streams of the nonlinear step: its six synthesis launches and
+    // its four analysis launches are mutually independent and are dealt
+    // over nbranch streams (the plan stream and nbranch - 1 of these)
+    static constexpr int MAXBR = 4;
+    int nbranch = 3;
+    hipStream_t br[MAXBR - 1] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[MAXBR - 1] = {};
     hipEvent_t ev_pin = nullptr;   // after the uploads from h_pin
     bool pin_pending = false;      // ev_pin recorded and not yet waited for
     bool last_host_out = false;    // the last nonlinear step copied outputs to host

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace mhx {
@@ -434,6 +435,15 @@ int mhx_plan_create(int l_max, int m_max, int minc, int n_theta, int n_phi, int 
     (void)hipEventCreateWithFlags(&p->ev_user, hipEventDisableTiming);
     (void)hipEventCreateWithFlags(&p->ev_done, hipEventDisableTiming);
     (void)hipEventCreateWithFlags(&p->ev_pin, hipEventDisableTiming);
+    if (const char *e = std::getenv("MHX_BRANCHES")) p->nbranch = std::max(1, std::min(Plan::MAXBR, std::atoi(e)));
+    (void)hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
+    for (int i = 0; i + 1 < p->nbranch; ++i) {
+        if (hipStreamCreateWithFlags(&p->br[i], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&p->ev_join[i], hipEventDisableTiming) != hipSuccess) {
+            mhx_plan_destroy(h);
+            return fail(MHX_E_HIP, "branch stream creation failed");
+        }
+    }
     *plan = h;
     return MHX_OK;
 }
@@ -487,6 +497,11 @@ int mhx_plan_destroy(mhx_plan *h)
     if (p->ev_user) (void)hipEventDestroy(p->ev_user);
     if (p->ev_done) (void)hipEventDestroy(p->ev_done);
     if (p->ev_pin) (void)hipEventDestroy(p->ev_pin);
+    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+    for (int i = 0; i + 1 < Plan::MAXBR; ++i) {
+        if (p->ev_join[i]) (void)hipEventDestroy(p->ev_join[i]);
+        if (p->br[i]) (void)hipStreamDestroy(p->br[i]);
+    }
     if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     if (p->h_pin) (void)hipHostFree(p->h_pin);
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -898,6 +913,23 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
         return MHX_OK;
     };
     if ((rc = mark())) return rc;
+    // independent launches dealt over the branch streams (one stream when
+    // timing, so that the events between launches measure each launch)
+    const int nbr = P->timing ? 1 : P->nbranch;
+    auto fork = [&]() -> int {
+        if (nbr < 2) return MHX_OK;
+        MHX_HIP_CHECK(hipEventRecord(P->ev_fork, s));
+        for (int i = 0; i + 1 < nbr; ++i) MHX_HIP_CHECK(hipStreamWaitEvent(P->br[i], P->ev_fork, 0));
+        return MHX_OK;
+    };
+    auto join = [&]() -> int {
+        for (int i = 0; i + 1 < nbr; ++i) {
+            MHX_HIP_CHECK(hipEventRecord(P->ev_join[i], P->br[i]));
+            MHX_HIP_CHECK(hipStreamWaitEvent(s, P->ev_join[i], 0));
+        }
+        return MHX_OK;
+    };
+    auto on = [&](int i) -> hipStream_t { return i % nbr == 0 ? s : P->br[i % nbr - 1]; };
 
     // inputs: 0 w, 1 dw, 2 ddw, 3 z, 4 dz, 5 s, 6 b, 7 db, 8 ddb, 9 aj, 10 dj
     enum { W = 0, DW, DDW, Z, DZ, S, B, DB, DDB, AJ, DJ };
@@ -916,7 +948,8 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
             a.nlev = nlev;
             a.src[0] = din[S];
             a.comp0 = 15;
-            if ((rc2 = launch_nsyn16(P, NS_SG, a, s)) || (rc2 = mark())) return rc2;
+            if ((rc2 = fork())) return rc2;
+            if ((rc2 = launch_nsyn16(P, NS_SG, a, on(5))) || (rc2 = mark())) return rc2;
             // u, curl u, du/dr, B, curl B (see the FP64 table below)
             const int qsrc[5][4] = {{W, DW, Z, -1}, {Z, DZ, W, DDW}, {DW, DDW, DZ, -1},
                                     {B, DB, AJ, -1}, {AJ, DJ, B, DDB}};
@@ -924,8 +957,9 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
                 for (int k = 0; k < 4; ++k) a.src[k] = qsrc[v][k] >= 0 ? din[qsrc[v][k]] : nullptr;
                 a.comp0 = 3 * v;
                 const int kind = qsrc[v][3] >= 0 ? NS_QSTC : NS_QST;
-                if ((rc2 = launch_nsyn16(P, kind, a, s)) || (rc2 = mark())) return rc2;
+                if ((rc2 = launch_nsyn16(P, kind, a, on(v))) || (rc2 = mark())) return rc2;
             }
+            if ((rc2 = join())) return rc2;
             if ((rc2 = launch_nl_fused(P, nlev, P->ws_F, P->ws_F2, P->d_rad, LFfac, P->ws_G, grids, s)) ||
                 (rc2 = mark()))
                 return rc2;
@@ -936,15 +970,17 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
             b.out_ld = ld_lm;
             b.lcut = P->d_lcut;
             b.nlev = nlev;
+            if ((rc2 = fork())) return rc2;
             for (int v = 0; v < 4; ++v) {
                 const int nout = v < 3 ? 3 : 1;
                 for (int k = 0; k < 3; ++k) {
                     b.in_comp[k] = k < nout ? 3 * v + k : -1;
                     b.out[k] = k < nout ? dout[3 * v + k] : nullptr;
                 }
-                if ((rc2 = launch_nana16(P, v < 3 ? NA_QST : NA_SH, b, s)) || (rc2 = mark())) return rc2;
+                if ((rc2 = launch_nana16(P, v < 3 ? NA_QST : NA_SH, b, on(v))) || (rc2 = mark()))
+                    return rc2;
             }
-            return MHX_OK;
+            return join();
         };
         auto launch_fp64 = [&]() -> int {
             int rc2;
@@ -954,7 +990,8 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
                 j.col[0] = op(0);
                 j.col[1] = op(0, G_ONE, 1, 1);
                 j.out_comp[0] = 15; j.out_comp[1] = 16; j.out_comp[2] = 17;
-                if ((rc2 = launch_syn(P, j, s)) || (rc2 = mark())) return rc2;
+                if ((rc2 = fork())) return rc2;
+                if ((rc2 = launch_syn(P, j, on(5))) || (rc2 = mark())) return rc2;
             }
             struct QSpec { int q, sidx, t0, t1; bool curl; int comp0; };
             // torpol_to_spat(a, b, c): Q = l(l+1) a, S = b, T = c
@@ -966,7 +1003,8 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
                 {B, DB, AJ, -1, false, 9},    // B
                 {AJ, DJ, B, DDB, true, 12},   // curl B
             };
-            for (const QSpec &q : qs) {
+            for (int v = 0; v < 5; ++v) {
+                const QSpec &q = qs[v];
                 SynJob j = syn_job(P, SYN_QST, nlev, ld_lm);
                 j.src[0] = din[q.q];
                 j.src[1] = din[q.sidx];
@@ -975,13 +1013,15 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
                 ColOp T = q.curl ? op(2, G_OR2LL, 0, 0, 1.0, 3, -1.0) : op(2);
                 qst_cols(j, op(0, G_LL), op(1), T);
                 j.out_comp[0] = q.comp0; j.out_comp[1] = q.comp0 + 1; j.out_comp[2] = q.comp0 + 2;
-                if ((rc2 = launch_syn(P, j, s)) || (rc2 = mark())) return rc2;
+                if ((rc2 = launch_syn(P, j, on(v))) || (rc2 = mark())) return rc2;
             }
+            if ((rc2 = join())) return rc2;
             // fused: inverse FFTs, grid products, forward FFTs; du/dr grids to ws_G
             if (((rc2 = launch_nl_fused(P, nlev, P->ws_F, P->ws_F2, P->d_rad, LFfac, P->ws_G,
                                                  grids, s)) ||
                            (rc2 = mark())))
                 return rc2;
+            if ((rc2 = fork())) return rc2;
             for (int v = 0; v < 3; ++v) {
                 AnaJob j = ana_job(P, ANA_QST, nlev, ld_lm);
                 j.in_comp[0] = 3 * v;
@@ -990,15 +1030,15 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
                 j.out[0] = dout[3 * v];
                 j.out[1] = dout[3 * v + 1];
                 j.out[2] = dout[3 * v + 2];
-                if ((rc2 = launch_ana(P, j, s)) || (rc2 = mark())) return rc2;
+                if ((rc2 = launch_ana(P, j, on(v))) || (rc2 = mark())) return rc2;
             }
             {
                 AnaJob j = ana_job(P, ANA_SH, nlev, ld_lm);
                 j.in_comp[0] = 9;
                 j.out[0] = dout[9];
-                if ((rc2 = launch_ana(P, j, s)) || (rc2 = mark())) return rc2;
+                if ((rc2 = launch_ana(P, j, on(3))) || (rc2 = mark())) return rc2;
             }
-            return MHX_OK;
+            return join();
         };
         auto launch_all = [&]() -> int { return split ? launch_split() : launch_fp64(); };
         if (P->timing) {
