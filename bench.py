@@ -59,12 +59,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic(kind, w, nsrc, nh, lm_max, n_theta, n_phi, n_m, nlev, word, tword, p_only=False):
+def algorithmic(kind, w, nsrc, nh, lm_max, n_theta, n_phi, n_m, nlev, word, tword, p_only=False,
+                rec=False):
     """(flops, HBM bytes) of one launch.  Legendre: w N_h lm_max flops per
     level; bytes = coefficient arrays (16 B per lm) + Fourier comps (n_theta x
     n_m complex of 2 word bytes) + the tables read once (P and dP, or P alone
     for the split kernels that fold dP onto P; tword bytes per entry: 8 FP64,
-    4 for the FP16 hi/lo pair).  Grid stage: 18 Fourier comps in, 10 out, 3
+    4 for the FP16 hi/lo pair; none for a synthesis that generates its tables
+    by recurrence).  Grid stage: 18 Fourier comps in, 10 out, 3
     du/dr grids (n_theta x n_phi) out."""
     four = n_theta * n_m * 2 * word
     coef = 16 * lm_max
@@ -72,6 +74,8 @@ def algorithmic(kind, w, nsrc, nh, lm_max, n_theta, n_phi, n_m, nlev, word, twor
         return 0.0, float(((18 + 10) * four + 3 * n_theta * n_phi * word) * nlev)
     flops = float(w * nh * lm_max * nlev)
     ntab = 1 if (w == 4 or p_only) else 2
+    if kind == "syn" and rec:
+        ntab = 0
     if kind == "syn":
         nout = 3
         byts = (nsrc * coef + nout * four) * nlev + ntab * tword * nh * lm_max
@@ -145,14 +149,15 @@ def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut):
         lt += np.array(p.launch_times())
     p.set_timing(False)
     lt /= nt_steps
-    split = precision == sht.PREC_SPLIT16
+    split = precision in (sht.PREC_SPLIT16, sht.PREC_SPLIT16_REC)
     word, tword = (4, 4) if split else (8, 8)
     nh, lm_max, n_m = nt // 2, p.lm_max, p.n_m_max
     per_kernel = {}
     total_flop = flop_ms = 0.0
     for (role, kind, w, nsrc), t in zip(LAUNCHES, lt):
         fl, by = algorithmic(kind, w, nsrc, nh, lm_max, nt, nph, n_m, nlev, word, tword,
-                             p_only=split and L <= 255)
+                             p_only=split and L <= 255,
+                             rec=precision == sht.PREC_SPLIT16_REC)
         d = per_kernel.setdefault(role, {"ms": 0.0, "n": 0, "flop": 0.0, "bytes": 0.0})
         d["ms"] += t
         d["n"] += 1
@@ -309,14 +314,19 @@ def main():
                          "l_max=511, 3 at l_max=1023, else 24)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["both", "split16", "fp64"], default="both")
+    ap.add_argument("--tables", choices=["recurrence", "stored"], default="recurrence",
+                    help="--config 5: split-form Legendre tables generated in-kernel by the FP64 "
+                         "recurrence (default) or read from stored FP16 tables")
     ap.add_argument("--spectrum", default=None,
                     help="amplitude spectrum of the synthesised fields: l^-5/3 (default), "
-                         "white-to-l^-3 (default of --config 5), slopes-0-3, l+1^-<s>")
+                         "slopes-0-3 (default of --config 5: field f with (l+1)^-(f mod 4)), "
+                         "white-to-l^-3 ((1+(l/64)^2)^-3/2), l+1^-<s>")
     ap.add_argument("--config", type=int, choices=[2, 3, 4, 5], default=2,
                     help="2: the nonlinear step at l_max=255 x 97 (headline); 3: the uneven "
                          "batch of 64 transform jobs in one captured graph; 4: the nonlinear "
                          "step at l_max=511 x 161 with the l=1 poloidal field x100; 5: the "
-                         "nonlinear step at l_max=1023 x 64 with white-to-l^-3 spectra")
+                         "nonlinear step at l_max=1023 x 64, spectral slopes 0-3 per field, "
+                         "tables generated in-kernel")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the bench, run one device-to-device copy of this many MiB "
                          "(a kernel of known traffic, for scaling hardware byte counters)")
@@ -343,7 +353,7 @@ def main():
     if args.config == 4:
         L, nlev, dipole = 511, 161, 100.0
     if args.config == 5:
-        L, nlev, kind = 1023, 64, "white-to-l^-3"
+        L, nlev, kind = 1023, 64, "slopes-0-3"
     kind = args.spectrum or kind
     nt, nph = workload.grid_size(L)
     log("l_max=%d %dx%d nlev=%d spectrum %s" % (L, nt, nph, nlev, kind))
@@ -360,7 +370,11 @@ def main():
         log("fp64: %.3f ms/step, %.1f pairs/s" % (forms["fp64"]["ms_per_step"],
                                                   forms["fp64"]["pairs"]))
     if args.form in ("both", "split16"):
-        forms["split16"] = run_form(sht.PREC_SPLIT16, L, nlev, args.steps, args.warmup, tin,
+        # the l_max=1023 case generates its tables in-kernel (--tables stored
+        # runs the stored-table kernels instead)
+        rec = args.config == 5 and args.tables == "recurrence"
+        forms["split16"] = run_form(sht.PREC_SPLIT16_REC if rec else sht.PREC_SPLIT16, L, nlev,
+                                    args.steps, args.warmup, tin,
                                     rad, lcut)
         log("split16: %.3f ms/step, %.1f pairs/s" % (forms["split16"]["ms_per_step"],
                                                      forms["split16"]["pairs"]))
@@ -410,7 +424,9 @@ def main():
         "config": {"workload": "full nonlinear step (5 QST + scalar + gradient inverse, "
                                "fused FFTs + grid products, 3 QST + scalar forward)",
                    "l_max": L, "n_theta": nt, "n_phi": nph, "n_r": nlev,
-                   "dipole_l1_factor": dipole, "spectrum": kind, "precision": main_key},
+                   "dipole_l1_factor": dipole, "spectrum": kind, "precision": main_key,
+                   "tables": ("recurrence" if args.config == 5 and args.tables == "recurrence"
+                              else "stored")},
         "legendre_tflops_fp64_equiv": main["legendre_tflops"],
         "kernels": main["kernels"],
         "roofline": roofline(main, kern_key, (L, nlev)),

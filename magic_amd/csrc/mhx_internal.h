@@ -103,6 +103,8 @@ struct NSynArgs {
     const float *rowscale;
     const int *lstart;
     const float *crec;      // c_l per (m, l), filled by the launcher
+    const double2 *seed;    // recurrence seeds per (mc, theta_h)
+    const double *cost;     // cos(theta_h)
     int l_max, minc, n_m, nh, n_theta, nt16, tpb, ngz;
 };
 
@@ -152,6 +154,11 @@ struct Plan {
     // 16x16x32 MFMA operand layout, with power-of-2 row factors
     int nq32 = 0;                   // theta chunks of 32 (northern half)
     size_t syn16_size = 0, ana16_size = 0;   // halves
+    // recurrence-generated synthesis tables (rec): FP64 seeds P_mm(theta) x
+    // row scale as (mantissa, power-of-two exponent) per (mc, theta_h)
+    bool rec = false;
+    double2 *d_rec_seed = nullptr;
+    double *d_cost = nullptr;  // cos(theta_h)
     _Float16 *d_syn16_P = nullptr, *d_syn16_dP = nullptr;
     _Float16 *d_ana16_P = nullptr, *d_ana16_dP = nullptr;
     size_t *d_syn16_off = nullptr, *d_ana16_off = nullptr, *d_ana16_soff = nullptr;

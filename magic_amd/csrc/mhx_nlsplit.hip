@@ -93,8 +93,22 @@ template <> struct NSyn<NS_QSTC> {
     static constexpr int pgrp[5] = {0, 1, 1, 1, 1}, ogrp[3] = {0, 1, 1};
 };
 
-template <int V>
-__global__ __launch_bounds__(384, 2) void k_nsyn16(NSynArgs a)
+// REC: the table fragments are not read but generated per chunk by the FP64
+// three-term recurrence in l (src/plms.f90:81-97 in the form
+// P_{l+1} = A_{l+1} (cos(theta) P_l - eps_l P_{l-1}), A = 1/eps,
+// eps_l = sqrt((l^2 - m^2) / (4 l^2 - 1))) and the identity
+// sin(theta) dP_l/dtheta = l eps_{l+1} P_{l+1} - (l + 1) eps_l P_{l-1}
+// (src/plms.f90:142-162).  Lane (row r, group g) of a wave owns theta rows
+// r of its two tiles and, in chunk c, l = m + 64 c + 16 g + j (j < 16): it
+// walks 64 steps per chunk, keeping the first 16, from seeds P_mm(theta)
+// times the row scale held as FP64 mantissa and power-of-two exponent (the
+// exponent is renormalised once per chunk, so no value leaves the double
+// range however small P_mm is).  The coefficients of the block's order sit
+// in LDS.
+constexpr int RLMAX = 1152;  // coefficient entries l - m (l_max <= 1023)
+
+template <int V, bool REC>
+__global__ __launch_bounds__(REC ? 256 : 384, 2) void k_nsyn16(NSynArgs a)
 {
     using S = NSyn<V>;
     constexpr int NPL = S::NPL, NOUT = S::NOUT, NPROD = S::NPROD, NSRC = S::NSRC;
@@ -169,6 +183,49 @@ __global__ __launch_bounds__(384, 2) void k_nsyn16(NSynArgs a)
     tok[0] = tile0 < a.nt16;
     tok[1] = tile0 + 1 < a.nt16;
 
+    // REC: recurrence coefficients (A_l, eps_l), l = m + i, zero past l_max + 1
+    __shared__ double2 rco[REC ? RLMAX : 1];
+    const int rg = lane >> 4;  // lane group: l offset 16 rg within a chunk
+    double rp[2], rq[2], rct[2], reps = 0.0;  // P_{l-1}, P_l (x 2^-E), cos(theta), eps_l
+    int rE[2];
+    if constexpr (REC) {
+        for (int i = tid; i < RLMAX; i += nthr) {
+            const int l = m + i;
+            double A = 0.0, e = 0.0;
+            if (i > 0 && l <= a.l_max + 1) {
+                const double dl = (double)l;
+                const double num = (dl - dm) * (dl + dm), den = (2.0 * dl - 1.0) * (2.0 * dl + 1.0);
+                A = sqrt(den / num);
+                e = sqrt(num / den);
+            }
+            rco[i] = make_double2(A, e);
+        }
+#pragma unroll
+        for (int tt = 0; tt < 2; ++tt) {
+            const int th = (tile0 + tt) * 16 + (lane & 15);
+            const bool ok = tok[tt] && th < a.nh;
+            const double2 sd = ok ? a.seed[(size_t)mc * a.nh + th] : make_double2(0.0, 0.0);
+            rq[tt] = sd.x;
+            rp[tt] = 0.0;
+            rE[tt] = (int)sd.y;
+            rct[tt] = ok ? a.cost[th] : 0.0;
+        }
+        __syncthreads();
+        // group rg starts at l = m + 16 rg
+        for (int j = 0; j < 48; ++j) {
+            if (j < 16 * rg) {
+                const double2 ce = rco[j + 1];
+#pragma unroll
+                for (int tt = 0; tt < 2; ++tt) {
+                    const double pn = ce.x * (rct[tt] * rq[tt] - reps * rp[tt]);
+                    rp[tt] = rq[tt];
+                    rq[tt] = pn;
+                }
+                reps = ce.y;
+            }
+        }
+    }
+
     for (int c = 0; c < nks0; ++c) {
         __syncthreads();  // lvsc ready / previous chunk consumed
         // ---- stage l = m + 64 c + 4 g + j (j < 4) for LGS levels: 16 LGS
@@ -240,6 +297,87 @@ __global__ __launch_bounds__(384, 2) void k_nsyn16(NSynArgs a)
             }
         }
         __syncthreads();
+        if constexpr (REC) {
+            // ---- per theta tile: generate the chunk's fragments of both
+            // parities (element e of parity par is l = l0 + 2 e + par,
+            // l0 = m + 64 c + 16 rg), then contract them
+            const int i0 = 64 * c + 16 * rg;
+            const double dl0 = (double)(m + i0), reps0 = reps, dlmax = (double)a.l_max;
+#pragma unroll
+            for (int tt = 0; tt < 2; ++tt) {
+                if (!tok[tt]) continue;
+                // renormalise: |P| within [2^-256, 2^256] x 2^E at the chunk start
+                {
+                    const double v = fabs(rq[tt]) + fabs(rp[tt]);
+                    const bool lo = v != 0.0 && v < 0x1p-256, hi = v > 0x1p256;
+                    const double f = lo ? 0x1p256 : (hi ? 0x1p-256 : 1.0);
+                    rq[tt] *= f;
+                    rp[tt] *= f;
+                    rE[tt] += lo ? -256 : (hi ? 256 : 0);
+                }
+                const double sc = rE[tt] >= -1000 ? ldexp(1.0, rE[tt]) : 0.0;
+                h8 gPh[2], gPl[2], gdPh[2], gdPl[2];  // [parity]
+                double ep = reps0, q = rq[tt], pv = rp[tt];
+                const double x = rct[tt];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const double2 ce = rco[i0 + j + 1];  // A_{l+1}, eps_{l+1}, l = l0 + j
+                    const double dl = dl0 + (double)j;
+                    const double pn = ce.x * (x * q - ep * pv);
+                    const double dp = dl * ce.y * pn - (dl + 1.0) * ep * pv;
+                    // entries past l_max are zero, as in the stored tables
+                    // (P_{l_max+1} is outside the row scale's range)
+                    const bool in = dl <= dlmax;
+                    _Float16 h, l;
+                    split_h(in ? (float)(q * sc) : 0.f, h, l);
+                    gPh[j & 1][j >> 1] = h;
+                    gPl[j & 1][j >> 1] = l;
+                    split_h(in ? (float)(dp * sc) : 0.f, h, l);
+                    gdPh[j & 1][j >> 1] = h;
+                    gdPl[j & 1][j >> 1] = l;
+                    pv = q;
+                    q = pn;
+                    ep = ce.y;
+                    if (j & 1) __builtin_amdgcn_sched_barrier(0);
+                }
+#pragma unroll 4
+                for (int j = 16; j < 64; ++j) {
+                    const double2 ce = rco[i0 + j + 1];
+                    const double pn = ce.x * (x * q - ep * pv);
+                    pv = q;
+                    q = pn;
+                    ep = ce.y;
+                }
+                rq[tt] = q;
+                rp[tt] = pv;
+                reps = ep;
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int par = 0; par < 2; ++par) {
+                    if (c >= (par ? nks1 : nks0)) continue;
+#pragma unroll
+                    for (int tn = 0; tn < TNC; ++tn) {
+                        if (tn >= ntn) break;
+#pragma unroll
+                        for (int pr = 0; pr < NPROD; ++pr) {
+                            const int o = S::prod[pr][0], tab = S::prod[pr][1], pl = S::prod[pr][2];
+                            const int tgt = tab == 0 ? par : 1 - par;
+                            const int bo = pl * PLANE + (par * 32 + tn * 16 + (lane & 15)) * ROWH + 8 * (lane >> 4);
+                            const h8 bh = *reinterpret_cast<const h8 *>(bhi + bo);
+                            const h8 bl = *reinterpret_cast<const h8 *>(blo + bo);
+                            const h8 ah = tab == 0 ? gPh[par] : gdPh[par];
+                            const h8 al = tab == 0 ? gPl[par] : gdPl[par];
+                            f4 t = acc[o][tgt][tt][tn];
+                            t = mfma16(ah, bl, t);
+                            t = mfma16(al, bh, t);
+                            t = mfma16(ah, bh, t);
+                            acc[o][tgt][tt][tn] = t;
+                        }
+                    }
+                }
+            }
+            continue;
+        }
         // ---- contraction: this wave's two theta tiles x ntn column tiles
 #pragma unroll
         for (int par = 0; par < 2; ++par) {
@@ -1250,17 +1388,33 @@ int launch_nsyn16(const Plan *p, int kind, const NSynArgs &args, hipStream_t s)
         MHX_HIP_CHECK(hipGetLastError());
         return MHX_OK;
     }
-    const int nw = 6;  // 12 theta tiles per block (all of l_max = 255); 256 threads stage B
+    // 12 theta tiles per block from stored tables (all of l_max = 255); 8 with
+    // the recurrence (its fragments of both parities live in registers);
+    // 256 threads stage B
+    const int nw = p->rec ? 4 : 6;
     a.tpb = 2 * nw;
     a.ngz = (p->nt16 + a.tpb - 1) / a.tpb;
     const int ng = (a.nlev + LGS - 1) / LGS;
     dim3 grid((unsigned)(8 * ((p->n_m + 7) / 8) * ng * a.ngz));
     const int nthr = 64 * nw;
-    switch (kind) {
-    case NS_SG: k_nsyn16<NS_SG><<<grid, nthr, 0, s>>>(a); break;
-    case NS_QST: k_nsyn16<NS_QST><<<grid, nthr, 0, s>>>(a); break;
-    case NS_QSTC: k_nsyn16<NS_QSTC><<<grid, nthr, 0, s>>>(a); break;
-    default: return fail(MHX_E_INVALID, "unknown nonlinear-step synthesis kind");
+    if (p->rec) {
+        if (p->l_max > 1023 || !p->d_rec_seed)
+            return fail(MHX_E_UNSUPPORTED, "recurrence tables need l_max <= 1023");
+        a.seed = p->d_rec_seed;
+        a.cost = p->d_cost;
+        switch (kind) {
+        case NS_SG: k_nsyn16<NS_SG, true><<<grid, nthr, 0, s>>>(a); break;
+        case NS_QST: k_nsyn16<NS_QST, true><<<grid, nthr, 0, s>>>(a); break;
+        case NS_QSTC: k_nsyn16<NS_QSTC, true><<<grid, nthr, 0, s>>>(a); break;
+        default: return fail(MHX_E_INVALID, "unknown nonlinear-step synthesis kind");
+        }
+    } else {
+        switch (kind) {
+        case NS_SG: k_nsyn16<NS_SG, false><<<grid, nthr, 0, s>>>(a); break;
+        case NS_QST: k_nsyn16<NS_QST, false><<<grid, nthr, 0, s>>>(a); break;
+        case NS_QSTC: k_nsyn16<NS_QSTC, false><<<grid, nthr, 0, s>>>(a); break;
+        default: return fail(MHX_E_INVALID, "unknown nonlinear-step synthesis kind");
+        }
     }
     MHX_HIP_CHECK(hipGetLastError());
     return MHX_OK;

@@ -353,8 +353,12 @@ int mhx_plan_create(int l_max, int m_max, int minc, int n_theta, int n_phi, int 
     m_max = (m_max / minc) * minc;
     if (n_theta < 4 || n_theta % 4) return fail(MHX_E_INVALID, "n_theta must be a multiple of 4");
     if (n_phi < 8 || n_phi % 4) return fail(MHX_E_INVALID, "n_phi must be a multiple of 4");
-    if (precision != MHX_PREC_FP64 && precision != MHX_PREC_SPLIT16)
+    if (precision != MHX_PREC_FP64 && precision != MHX_PREC_SPLIT16 && precision != MHX_PREC_SPLIT16_REC)
         return fail(MHX_E_INVALID, "unknown precision");
+    const bool rec = precision == MHX_PREC_SPLIT16_REC;
+    if (rec && (l_max <= 255 || l_max > 1023))
+        return fail(MHX_E_UNSUPPORTED, "recurrence-generated tables need 255 < l_max <= 1023");
+    if (rec) precision = MHX_PREC_SPLIT16;
     std::vector<int> fac;
     if (!factorize(n_phi / 2, fac)) return fail(MHX_E_INVALID, "n_phi/2 must factor into 2, 3, 5");
     int n_m = m_max / minc + 1;
@@ -371,6 +375,7 @@ int mhx_plan_create(int l_max, int m_max, int minc, int n_theta, int n_phi, int 
     p->nh = n_theta / 2;
     p->n_m = n_m;
     p->precision = precision;
+    p->rec = rec;
     p->nt16 = (p->nh + 15) / 16;
     p->nq4 = (p->nh + 3) / 4;
     p->M = n_phi / 2;
@@ -476,6 +481,8 @@ int mhx_plan_destroy(mhx_plan *h)
     free_dev(p->d_ana16_off);
     free_dev(p->d_ana16_soff);
     free_dev(p->d_syn16_rs);
+    free_dev(p->d_rec_seed);
+    free_dev(p->d_cost);
     free_dev(p->d_ana16_rs);
     free_dev(p->d_syn16x);
     free_dev(p->d_ana16x);

@@ -716,6 +716,43 @@ __global__ void k_crec(int l_max, int minc, int n_m, const int *__restrict__ lst
 
 }  // namespace
 
+// recurrence seeds: P_mm(theta) times the synthesis row scale, FP64 mantissa
+// and power-of-two exponent (P_mm = sqrt(prod_{j=3,5..2m+1} j/(j-1)) / sqrt(4
+// pi) sin^m theta, src/plms.f90:55-65, which leaves the double range for large
+// m near the poles); one thread per (mc, theta_h)
+__global__ void k_rec_seed(int minc, int n_m, int nh, const double *__restrict__ theta,
+                           const float *__restrict__ rs, double2 *__restrict__ seed,
+                           double *__restrict__ cost)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_m * nh) return;
+    int mc = idx / nh, th = idx - mc * nh;
+    int m = mc * minc;
+    const double t = theta[th];
+    const double st = sin(t);
+    if (mc == 0) cost[th] = cos(t);
+    double fac = 1.0;
+    for (int j = 3; j <= 2 * m + 1; j += 2) fac *= (double)j / (double)(j - 1);
+    // st^m = f^m 2^(k m) with st = f 2^k, f in [0.5, 1); f^m in steps of 256
+    int k;
+    const double f = frexp(st, &k);
+    double pm = 0.28209479177387814 * sqrt(fac);
+    long e = (long)k * m;
+    int left = m;
+    while (left > 0) {
+        const int n = left < 256 ? left : 256;
+        int k2;
+        pm = frexp(pm * pow(f, (double)n), &k2);
+        e += k2;
+        left -= n;
+    }
+    // times the row scale (a power of two)
+    int ks;
+    frexpf(rs[idx], &ks);
+    e += ks - 1;
+    seed[idx] = make_double2(pm, (double)e);
+}
+
 int build_split_tables(Plan *p)
 {
     const int nh = p->nh;
@@ -738,12 +775,14 @@ int build_split_tables(Plan *p)
     soff[2 * p->n_m] = sr;
     p->syn16_size = so;
     p->ana16_size = ao;
-    MHX_HIP_CHECK(hipMalloc(&p->d_syn16_P, so * 2));
-    MHX_HIP_CHECK(hipMalloc(&p->d_syn16_dP, so * 2));
+    if (!p->rec) {
+        MHX_HIP_CHECK(hipMalloc(&p->d_syn16_P, so * 2));
+        MHX_HIP_CHECK(hipMalloc(&p->d_syn16_dP, so * 2));
+        MHX_HIP_CHECK(hipMemset(p->d_syn16_P, 0, so * 2));
+        MHX_HIP_CHECK(hipMemset(p->d_syn16_dP, 0, so * 2));
+    }
     MHX_HIP_CHECK(hipMalloc(&p->d_ana16_P, ao * 2));
     MHX_HIP_CHECK(hipMalloc(&p->d_ana16_dP, ao * 2));
-    MHX_HIP_CHECK(hipMemset(p->d_syn16_P, 0, so * 2));
-    MHX_HIP_CHECK(hipMemset(p->d_syn16_dP, 0, so * 2));
     MHX_HIP_CHECK(hipMemset(p->d_ana16_P, 0, ao * 2));
     MHX_HIP_CHECK(hipMemset(p->d_ana16_dP, 0, ao * 2));
     MHX_HIP_CHECK(hipMalloc(&p->d_syn16_off, off16.size() * sizeof(size_t)));
@@ -757,9 +796,17 @@ int build_split_tables(Plan *p)
     int n = p->n_m * nh;
     k_rowscale_syn<<<(n + 127) / 128, 128>>>(p->l_max, p->minc, p->n_m, nh, p->nt16, p->d_syn_P,
                                              p->d_syn_dP, p->d_syn_off, p->d_syn16_rs);
-    k_split_syn<<<(n + 127) / 128, 128>>>(p->l_max, p->minc, p->n_m, nh, p->nt16, p->d_syn_P,
-                                          p->d_syn_dP, p->d_syn_off, p->d_syn16_off, p->d_syn16_rs,
-                                          p->d_syn16_P, p->d_syn16_dP);
+    if (p->rec) {
+        // no stored synthesis tables: FP64 seeds for the in-kernel recurrence
+        MHX_HIP_CHECK(hipMalloc(&p->d_rec_seed, (size_t)n * sizeof(double2)));
+        MHX_HIP_CHECK(hipMalloc(&p->d_cost, (size_t)nh * sizeof(double)));
+        k_rec_seed<<<(n + 127) / 128, 128>>>(p->minc, p->n_m, nh, p->d_theta, p->d_syn16_rs,
+                                             p->d_rec_seed, p->d_cost);
+    } else {
+        k_split_syn<<<(n + 127) / 128, 128>>>(p->l_max, p->minc, p->n_m, nh, p->nt16, p->d_syn_P,
+                                              p->d_syn_dP, p->d_syn_off, p->d_syn16_off, p->d_syn16_rs,
+                                              p->d_syn16_P, p->d_syn16_dP);
+    }
     k_rowscale_ana<<<(int)((sr + 127) / 128), 128>>>(p->l_max, p->minc, p->n_m, nh, p->nq4,
                                                      p->d_ana_wP, p->d_ana_wdP, p->d_ana_off,
                                                      p->d_ana16_soff, p->d_ana16_rs, (int)sr);

@@ -20,6 +20,9 @@ LIB_PATH = os.path.join(HERE, "libmhx.so")
 
 PREC_FP64 = 0
 PREC_SPLIT16 = 1
+# the split form with the nonlinear step's tables for l_max > 255 recomputed
+# in-kernel (FP64 recurrence from FP64 seeds); 255 < l_max <= 1023
+PREC_SPLIT16_REC = 2
 
 _lib = None
 

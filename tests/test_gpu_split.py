@@ -89,6 +89,45 @@ def test_split_l511_dipole_vs_fp64():
         assert e < SPLIT_TOL, (sht.NL_OUTPUTS[k], e)
 
 
+@pytest.mark.parametrize("L,nlev", [(511, 6), (1023, 2)])
+def test_split_recurrence_tables(L, nlev):
+    """PREC_SPLIT16_REC: the synthesis tables generated in-kernel by the FP64
+    recurrence from FP64 seeds (no stored FP16 synthesis tables) give every
+    output within 1e-6 of the FP64 form, one level with a lower lcut; at
+    l_max=511 they also agree with the stored-table split form to 5e-7 (the
+    two differ only by the rounding of the table entries: each is ~3e-7 from
+    FP64)."""
+    print("split recurrence tables", L, nlev, flush=True)
+    nt, nph = workload.grid_size(L)
+    inputs = make_inputs(L, nlev)
+    rad = radial(nlev)
+    lcut = [L] * nlev
+    lcut[-1] = L - 33
+    p64 = sht.Plan(L, nt, nph, max_levels=nlev)
+    ref = _run(p64, inputs, rad, lcut)
+    p64.close()
+    pr = sht.Plan(L, nt, nph, precision=sht.PREC_SPLIT16_REC, max_levels=nlev)
+    outs = _run(pr, inputs, rad, lcut)
+    pr.close()
+    errs = [rel_l2(outs[k], ref[k]) for k in range(10)]
+    print("recurrence split vs FP64:", " ".join("%.2e" % e for e in errs), flush=True)
+    for k, e in enumerate(errs):
+        assert e < SPLIT_TOL, (sht.NL_OUTPUTS[k], e)
+    if L <= 511:
+        ps = sht.Plan(L, nt, nph, precision=sht.PREC_SPLIT16, max_levels=nlev)
+        st = _run(ps, inputs, rad, lcut)
+        ps.close()
+        d = [rel_l2(outs[k], st[k]) for k in range(10)]
+        print("recurrence vs stored tables:", " ".join("%.2e" % e for e in d), flush=True)
+        assert max(d) < 5e-7
+
+
+def test_split_recurrence_plan_limits():
+    """the recurrence form is refused outside 255 < l_max <= 1023"""
+    with pytest.raises(sht.MhxError):
+        sht.Plan(127, 192, 384, precision=sht.PREC_SPLIT16_REC)
+
+
 def test_split_graph_replay_matches_eager():
     """The split step is captured into a graph on its second call with the same
     arrays and replayed on later calls.  Every replay, with the outputs zeroed
