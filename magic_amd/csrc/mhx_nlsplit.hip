@@ -144,22 +144,33 @@ __global__ __launch_bounds__(REC ? 256 : 384, 2) void k_nsyn16(NSynArgs a)
             const int lhi = min(a.l_max, a.lcut[lev]);
             const double o2 = (V == NS_QSTC) ? a.or2[lev] : 0.0;
             const double mf = dm > 1.0 ? dm : 1.0;
-            for (int l = m + t; l <= lhi; l += tpl) {
-                const size_t o = ((size_t)lev * a.src_ld + (size_t)(l0 + l - m)) * 2;
-                double x[NSRC];
+            // four l per trip, their loads issued before use (clamped to lhi;
+            // a clamped repeat does not change the maxima)
+            for (int lb = m + t; lb <= lhi; lb += 4 * tpl) {
+                double2 v[4][NSRC];
 #pragma unroll
-                for (int s = 0; s < NSRC; ++s) {
-                    double2 v = *reinterpret_cast<const double2 *>(a.src[s] + o);
-                    x[s] = fmax(fabs(v.x), fabs(v.y));
+                for (int u = 0; u < 4; ++u) {
+                    const int l = min(lb + u * tpl, lhi);
+                    const size_t o = ((size_t)lev * a.src_ld + (size_t)(l0 + l - m)) * 2;
+#pragma unroll
+                    for (int s = 0; s < NSRC; ++s) v[u][s] = *reinterpret_cast<const double2 *>(a.src[s] + o);
                 }
-                const double ll = (double)l * (double)(l + 1);
-                if constexpr (V == NS_SG) {
-                    mx0 = fmax(mx0, x[0]);
-                    mx1 = fmax(mx1, mf * x[0]);
-                } else {
-                    double tb = (V == NS_QSTC) ? o2 * ll * x[2] + x[3] : x[2];
-                    mx0 = fmax(mx0, ll * x[0]);
-                    mx1 = fmax(mx1, mf * fmax(x[1], tb));
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int l = min(lb + u * tpl, lhi);
+                    double x[NSRC];
+#pragma unroll
+                    for (int s = 0; s < NSRC; ++s) x[s] = fmax(fabs(v[u][s].x), fabs(v[u][s].y));
+                    const double ll = (double)l * (double)(l + 1);
+                    if constexpr (V == NS_SG) {
+                        mx0 = fmax(mx0, x[0]);
+                        mx1 = fmax(mx1, mf * x[0]);
+                    } else {
+                        double tb = (V == NS_QSTC) ? o2 * ll * x[2] + x[3] : x[2];
+                        mx0 = fmax(mx0, ll * x[0]);
+                        mx1 = fmax(mx1, mf * fmax(x[1], tb));
+                    }
                 }
             }
             atomicMax(&lvmax[levl], __float_as_uint((float)mx0));
@@ -238,6 +249,20 @@ __global__ __launch_bounds__(REC ? 256 : 384, 2) void k_nsyn16(NSynArgs a)
             const double o2 = (V == NS_QSTC && lok) ? a.or2[lev] : 0.0;
             const float sc0 = lvsc[levl], sc1 = lvsc[LGS + levl];
             const float fm = (float)dm;
+            // the four l of this thread, all loads issued first at addresses
+            // clamped into the order's range, then masked
+            double2 xv[2][2][NSRC];  // [par][q][source]
+#pragma unroll
+            for (int par = 0; par < 2; ++par)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const int l = min(m + 64 * c + 4 * g + 2 * q + par, a.l_max);
+                    const size_t o = ((size_t)(lok ? lev : 0) * a.src_ld + (size_t)(l0 + l - m)) * 2;
+#pragma unroll
+                    for (int s2 = 0; s2 < NSRC; ++s2)
+                        xv[par][q][s2] = *reinterpret_cast<const double2 *>(a.src[s2] + o);
+                }
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int par = 0; par < 2; ++par) {
                 // planes of the two l of this parity, FP32, scaled
@@ -246,14 +271,11 @@ __global__ __launch_bounds__(REC ? 256 : 384, 2) void k_nsyn16(NSynArgs a)
                 for (int q = 0; q < 2; ++q) {
                     const int l = m + 64 * c + 4 * g + 2 * q + par;
                     const bool valid = l <= lhi;
-                    const size_t o = ((size_t)(lok ? lev : 0) * a.src_ld + (size_t)(l0 + l - m)) * 2;
                     double xr[NSRC], xi[NSRC];
 #pragma unroll
                     for (int s2 = 0; s2 < NSRC; ++s2) {
-                        double2 v = make_double2(0.0, 0.0);
-                        if (valid) v = *reinterpret_cast<const double2 *>(a.src[s2] + o);
-                        xr[s2] = v.x;
-                        xi[s2] = v.y;
+                        xr[s2] = valid ? xv[par][q][s2].x : 0.0;
+                        xi[s2] = valid ? xv[par][q][s2].y : 0.0;
                     }
                     const double ll = (double)l * (double)(l + 1);
                     if constexpr (V == NS_SG) {
@@ -876,11 +898,15 @@ __global__ __launch_bounds__(256) void k_nana16(NAnaArgs a)
                 const f4 *src = reinterpret_cast<const f4 *>(
                     fin + ((size_t)lev * a.fin_ld + (((size_t)a.in_comp[k] * a.ntb + tb) * a.n_m + mc) * 16) * 2);
                 const int nrow = min(16, a.n_theta - 16 * tb);  // rows past n_theta are padding
+                // all eight row pairs loaded first (clamped to the block's
+                // rows; a repeat does not change the maximum)
+                f4 w[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) w[q] = src[min(q, nrow / 2 - 1)];
                 float v = 0.f;
-                for (int q = 0; q < nrow / 2; ++q) {
-                    const f4 w = src[q];
-                    v = fmaxf(v, fmaxf(fmaxf(fabsf(w[0]), fabsf(w[1])), fmaxf(fabsf(w[2]), fabsf(w[3]))));
-                }
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    v = fmaxf(v, fmaxf(fmaxf(fabsf(w[q][0]), fabsf(w[q][1])), fmaxf(fabsf(w[q][2]), fabsf(w[q][3]))));
                 mx[k] = v;
             }
             atomicMax(&lvmax[levl], __float_as_uint(mx[0]));
@@ -903,6 +929,16 @@ __global__ __launch_bounds__(256) void k_nana16(NAnaArgs a)
 #pragma unroll
                 for (int n = 0; n < 2; ++n) acc[o][s][t][n] = f4{0.f, 0.f, 0.f, 0.f};
 
+    // this thread's staging level (fixed over the chunks): in range and m kept
+    bool stage_lev_ok;
+    {
+        const int levl = tid >> 4;
+        stage_lev_ok = false;
+        if (levl < nlh) {
+            const int lc = a.lcut[lev0 + levl];
+            stage_lev_ok = m <= (lc < a.m_max ? lc : a.m_max);
+        }
+    }
     // F2 layout: [lev][comp][theta block of 16 rows][m][16] complex
     for (int kc = 0; kc < a.nq32; ++kc) {
         __syncthreads();
@@ -912,24 +948,28 @@ __global__ __launch_bounds__(256) void k_nana16(NAnaArgs a)
             const int lev = lev0 + levl;
             const int thb = kc * 4 + tb;        // theta block (16 rows)
             const int th0 = 32 * kc + 8 * tb + 2 * q;
-            bool valid = levl < nlh && th0 < a.nh;
+            const bool valid = stage_lev_ok && th0 < a.nh;
             const bool ok1 = th0 + 1 < a.nh;  // rows past n_theta are padding
-            if (valid) {
-                const int lc = a.lcut[lev];
-                valid = m <= (lc < a.m_max ? lc : a.m_max);
-            }
             const float sc0 = lvsc[levl], sc1 = lvsc[LGS + levl];
             // rows 2 th, 2 th + 1 for th = th0, th0 + 1: 4 complex, 32 bytes
             float x[NIN][8];
+            // loads issued unconditionally at clamped addresses (level and
+            // theta block in range; the second row pair of a block is padding
+            // past n_theta), then masked
+            f4 xv0[NIN], xv1[NIN];
 #pragma unroll
             for (int k = 0; k < NIN; ++k) {
-                f4 v0 = f4{0.f, 0.f, 0.f, 0.f}, v1 = v0;
-                if (valid) {
-                    const size_t o = ((size_t)lev * a.fin_ld +
-                                      (((size_t)a.in_comp[k] * a.ntb + thb) * a.n_m + mc) * 16 + 4 * q) * 2;
-                    v0 = *reinterpret_cast<const f4 *>(fin + o);
-                    if (ok1) v1 = *reinterpret_cast<const f4 *>(fin + o + 4);
-                }
+                const size_t o = ((size_t)(levl < nlh ? lev : 0) * a.fin_ld +
+                                  (((size_t)a.in_comp[k] * a.ntb + min(thb, a.ntb - 1)) * a.n_m + mc) * 16 +
+                                  4 * q) * 2;
+                xv0[k] = *reinterpret_cast<const f4 *>(fin + o);
+                xv1[k] = *reinterpret_cast<const f4 *>(fin + o + 4);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < NIN; ++k) {
+                const f4 z4 = f4{0.f, 0.f, 0.f, 0.f};
+                const f4 v0 = valid ? xv0[k] : z4, v1 = (valid && ok1) ? xv1[k] : z4;
                 // x = N0re N0im S0re S0im N1re N1im S1re S1im
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
