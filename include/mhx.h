@@ -50,8 +50,9 @@ extern "C" {
 #define MHX_PREC_SPLIT16 1   /* scaled FP16 hi/lo split, FP32 accumulation */
 /* the split form with the nonlinear step's P_lm and dP_lm, for l_max > 255,
  * recomputed inside the Legendre kernels by an FP64 three-term recurrence
- * from FP64 seeds P_mm(theta) with a separate power-of-two exponent, instead
- * of read from stored FP16 tables; 255 < l_max <= 1023 */
+ * from FP64 seeds (P_mm(theta) for the synthesis, every 32nd degree for the
+ * analysis) held as mantissa and power-of-two exponent, instead of read from
+ * stored FP16 tables; 255 < l_max <= 1023 */
 #define MHX_PREC_SPLIT16_REC 2
 
 typedef struct mhx_plan mhx_plan;

@@ -123,6 +123,10 @@ struct NAnaArgs {
     const size_t *soff;
     const int *lstart;
     const float *crec;      // c_l per (m, l), filled by the launcher
+    const double2 *aseed;   // recurrence seeds per (segment, theta_h)
+    const int *aexp;
+    const size_t *segoff;
+    const double *cost;
     int l_max, m_max, minc, n_m, nh, n_theta, ntb, nq32, ngz;
 };
 
@@ -159,6 +163,11 @@ struct Plan {
     bool rec = false;
     double2 *d_rec_seed = nullptr;
     double *d_cost = nullptr;  // cos(theta_h)
+    // analysis: 2 pi w(theta) (P_{l-1}, P_l) at l = m + 32 s, mantissas and a
+    // common power-of-two exponent, per (mc, segment s, theta_h)
+    double2 *d_rec_aseed = nullptr;
+    int *d_rec_aexp = nullptr;
+    size_t *d_rec_segoff = nullptr;  // first segment of each order
     _Float16 *d_syn16_P = nullptr, *d_syn16_dP = nullptr;
     _Float16 *d_ana16_P = nullptr, *d_ana16_dP = nullptr;
     size_t *d_syn16_off = nullptr, *d_ana16_off = nullptr, *d_ana16_soff = nullptr;

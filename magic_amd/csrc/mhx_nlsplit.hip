@@ -852,46 +852,84 @@ template <> struct NAna<NA_SH> {
     static constexpr int pgrp[1] = {0}, ogrp[1] = {0};
 };
 
-template <int V>
+// REC: the table fragments are generated per theta chunk instead of read:
+// thread (theta row thl of the chunk, segment sg) runs the FP64 recurrence
+// over the 32 degrees l = m + 256 gz + 32 sg + j of its segment from the
+// stored FP64 seeds 2 pi w (P_{l-1}, P_l) (common power-of-two exponent)
+// and writes w P and w sin(theta) dP/dtheta (the identity of the synthesis),
+// times the row factor of l, as FP16 hi/lo to LDS [parity][k][theta]; the
+// contraction reads its fragments there.
+constexpr int ROWA = 32 + 8;   // halves per LDS row of generated fragments (padded)
+constexpr int RNG = 1;         // level groups per block with generated fragments
+constexpr int RLB = 256 + 8;   // recurrence coefficients held per block
+
+template <int V, bool REC>
 __global__ __launch_bounds__(256) void k_nana16(NAnaArgs a)
 {
     using S = NAna<V>;
     constexpr int NPL = S::NPL, NOUT = S::NOUT, NPROD = S::NPROD, NIN = S::NIN;
+    constexpr bool DP = V == NA_QST;  // dP fragments needed
+    // level groups of LGS per block: REC blocks run RNG groups on each
+    // generated chunk (the generation is then shared by RNG x 16 levels)
+    constexpr int NG = REC ? RNG : 1;
     __shared__ __align__(16) _Float16 bhi[NPL * PLANE];
     __shared__ __align__(16) _Float16 blo[NPL * PLANE];
-    __shared__ unsigned lvmax[2 * LGS];  // [column group][level]
-    __shared__ float lvsc[2 * LGS];
+    __shared__ unsigned lvmax[2 * LGS * NG];  // [level group][column group][level]
+    __shared__ float lvsc[2 * LGS * NG];
+    // REC: generated fragments [P hi, P lo, dP hi, dP lo][parity][k of the block][theta]
+    __shared__ __align__(16) _Float16 gA[REC ? (DP ? 4 : 2) * 2 * 128 * ROWA : 8];
+    __shared__ double2 rcb[REC ? RLB : 1];  // (A_l, eps_l), l = m + 256 gz + i
+    __shared__ float grs[REC ? 2 * 128 : 1];  // row factors [parity][k of the block]
 
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const int ng = (a.nlev + LGS - 1) / LGS;
+    const int ng = (a.nlev + LGS * NG - 1) / (LGS * NG);
     int mc, rest;
     xcd_deal(blockIdx.x, ng * a.ngz, mc, rest);
     if (mc >= a.n_m) return;
     const int gy = rest % ng, gz = rest / ng;
     const int m = mc * a.minc;
     const double dm = (double)m;
-    const int lev0 = gy * LGS, nlh = min(LGS, a.nlev - lev0);
-    const int ntn = nlh > 8 ? 2 : 1;
+    const int levb = gy * LGS * NG;  // first level of the block
+    auto nlh_of = [&](int s) { return max(0, min(LGS, a.nlev - levb - LGS * s)); };
     const int l0 = a.lstart[mc];
     const int K0 = (a.l_max - m) / 2 + 1;
     const int ntl = (K0 + 15) / 16;           // l tiles per parity
     const int t0 = gz * 8 + wave * 2;         // this wave's l tiles t0, t0+1
+    if (gz * 8 >= ntl) return;                // no l tile of this order here
     bool tok[2];
     tok[0] = t0 < ntl;
     tok[1] = t0 + 1 < ntl;
+    if constexpr (REC) {
+        for (int i = tid; i < RLB; i += nthr) {
+            const int l = m + 256 * gz + i;
+            double A = 0.0, e = 0.0;
+            if (l > m && l <= a.l_max + 1) {
+                const double dl = (double)l;
+                const double num = (dl - dm) * (dl + dm), den = (2.0 * dl - 1.0) * (2.0 * dl + 1.0);
+                A = sqrt(den / num);
+                e = sqrt(num / den);
+            }
+            rcb[i] = make_double2(A, e);
+        }
+        for (int i = tid; i < 256; i += nthr) {
+            const int par = i >> 7, kk = 128 * gz + (i & 127);
+            grs[i] = kk < ntl * 16 ? a.rowscale[a.soff[2 * mc + par] + kk] : 0.f;
+        }
+    }
 
     // ---- B scales per (level, column group) from this order's F2 column:
     // 2 max|f| (N +- S) and 2 max(1, m) max(|vt|, |vp|)
     const float *fin = a.fin;
-    if (tid < 2 * LGS) lvmax[tid] = 0u;
+    if (tid < 2 * LGS * NG) lvmax[tid] = 0u;
     __syncthreads();
     {
         // (level, theta block) pairs; 128 bytes = 16 rows per comp
-        for (int idx = tid; idx < LGS * a.ntb; idx += nthr) {
-            const int levl = idx % LGS, tb = idx / LGS;
-            if (levl >= nlh) continue;
-            const int lev = lev0 + levl;
+        for (int idx = tid; idx < LGS * NG * a.ntb; idx += nthr) {
+            const int levg = idx % (LGS * NG), tb = idx / (LGS * NG);
+            const int lev = levb + levg;
+            if (lev >= a.nlev) continue;
+            const int sgrp = levg / LGS, levl = levg % LGS;
             float mx[NIN];
 #pragma unroll
             for (int k = 0; k < NIN; ++k) {
@@ -909,48 +947,116 @@ __global__ __launch_bounds__(256) void k_nana16(NAnaArgs a)
                     v = fmaxf(v, fmaxf(fmaxf(fabsf(w[q][0]), fabsf(w[q][1])), fmaxf(fabsf(w[q][2]), fabsf(w[q][3]))));
                 mx[k] = v;
             }
-            atomicMax(&lvmax[levl], __float_as_uint(mx[0]));
-            if constexpr (NIN == 3) atomicMax(&lvmax[LGS + levl], __float_as_uint(fmaxf(mx[1], mx[2])));
+            atomicMax(&lvmax[sgrp * 2 * LGS + levl], __float_as_uint(mx[0]));
+            if constexpr (NIN == 3)
+                atomicMax(&lvmax[sgrp * 2 * LGS + LGS + levl], __float_as_uint(fmaxf(mx[1], mx[2])));
         }
     }
     __syncthreads();
-    if (tid < 2 * LGS) {
-        const double f = tid < LGS ? 2.0 : 2.0 * (dm > 1.0 ? dm : 1.0);
+    if (tid < 2 * LGS * NG) {
+        const double f = (tid % (2 * LGS)) < LGS ? 2.0 : 2.0 * (dm > 1.0 ? dm : 1.0);
         lvsc[tid] = bscale(f * (double)__uint_as_float(lvmax[tid]));
     }
 
-    f4 acc[NOUT][2][2][2];  // [out][parity][l tile][column tile]
+    f4 acc[NG][NOUT][2][2][2];  // [level group][out][parity][l tile][column tile]
 #pragma unroll
-    for (int o = 0; o < NOUT; ++o)
+    for (int g = 0; g < NG; ++g)
 #pragma unroll
-        for (int s = 0; s < 2; ++s)
+        for (int o = 0; o < NOUT; ++o)
 #pragma unroll
-            for (int t = 0; t < 2; ++t)
+            for (int s = 0; s < 2; ++s)
 #pragma unroll
-                for (int n = 0; n < 2; ++n) acc[o][s][t][n] = f4{0.f, 0.f, 0.f, 0.f};
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int n = 0; n < 2; ++n) acc[g][o][s][t][n] = f4{0.f, 0.f, 0.f, 0.f};
 
-    // this thread's staging level (fixed over the chunks): in range and m kept
-    bool stage_lev_ok;
-    {
+    // this thread's staging level in each group (fixed over the chunks): in
+    // range and m kept
+    bool stage_lev_ok[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
         const int levl = tid >> 4;
-        stage_lev_ok = false;
-        if (levl < nlh) {
-            const int lc = a.lcut[lev0 + levl];
-            stage_lev_ok = m <= (lc < a.m_max ? lc : a.m_max);
+        stage_lev_ok[g] = false;
+        if (levl < nlh_of(g)) {
+            const int lc = a.lcut[levb + LGS * g + levl];
+            stage_lev_ok[g] = m <= (lc < a.m_max ? lc : a.m_max);
         }
+    }
+    // REC: seeds of the first chunk (segment of this thread, theta = thl)
+    double2 nsd = make_double2(0.0, 0.0);
+    int nE = 0;
+    double nx = 0.0;
+    size_t seg_base = 0;
+    if constexpr (REC) {
+        const int segi = 8 * gz + (tid >> 5);
+        seg_base = (a.segoff[mc] + (size_t)min(segi, (a.l_max - m) >> 5)) * a.nh;
+        const int th0 = min(tid & 31, a.nh - 1);
+        nsd = a.aseed[seg_base + th0];
+        nE = a.aexp[seg_base + th0];
+        nx = a.cost[th0];
     }
     // F2 layout: [lev][comp][theta block of 16 rows][m][16] complex
     for (int kc = 0; kc < a.nq32; ++kc) {
         __syncthreads();
+        if constexpr (REC) {
+            // ---- generate this chunk's fragments (see above); the seeds of
+            // the next chunk are loaded now, used one chunk later
+            const int thl = tid & 31, seg = tid >> 5;
+            const int th = 32 * kc + thl;
+            const int segi = 8 * gz + seg;
+            const int ls = m + 32 * segi;
+            const bool gen = th < a.nh && ls <= a.l_max;
+            const double2 sd = nsd;
+            const int E = nE;
+            const double x = nx;
+            if (kc + 1 < a.nq32) {
+                const int th1 = min(th + 32, a.nh - 1);
+                const size_t so = seg_base + th1;
+                nsd = a.aseed[so];
+                nE = a.aexp[so];
+                nx = a.cost[th1];
+            }
+            const double sc = (gen && E >= -1000) ? ldexp(1.0, E) : 0.0;
+            double pv = sd.x, q = sd.y;
+            double eps = rcb[32 * seg].y;  // eps_{ls}
+            const double dls = (double)ls, dlmax = (double)a.l_max;
+#pragma unroll 4
+            for (int j = 0; j < 32; ++j) {
+                const double2 ce = rcb[32 * seg + j + 1];  // A_{l+1}, eps_{l+1}, l = ls + j
+                const double dl = dls + (double)j;
+                const double pn = ce.x * (x * q - eps * pv);
+                const int par = j & 1, kl = 16 * seg + (j >> 1);
+                const double r = dl <= dlmax ? sc * (double)grs[par * 128 + kl] : 0.0;
+                _Float16 h, lo;
+                split_h((float)(q * r), h, lo);
+                const int ai = (par * 128 + kl) * ROWA + thl;
+                gA[ai] = h;
+                gA[2 * 128 * ROWA + ai] = lo;
+                if constexpr (DP) {
+                    const double dp = dl * ce.y * pn - (dl + 1.0) * eps * pv;
+                    split_h((float)(dp * r), h, lo);
+                    gA[4 * 128 * ROWA + ai] = h;
+                    gA[6 * 128 * ROWA + ai] = lo;
+                }
+                pv = q;
+                q = pn;
+                eps = ce.y;
+            }
+        }
+#pragma unroll
+      for (int sg = 0; sg < NG; ++sg) {
+        if (sg > 0) __syncthreads();  // the previous group's contraction read B
+        const int lev0 = levb + LGS * sg, nlh = nlh_of(sg);
+        const int ntn = nlh > 8 ? 2 : 1;
         // ---- stage theta_h = 32 kc + 8 tb + 2 q + (0, 1): rows 64 kc + 16 tb + 4 q .. +3
         {
             const int q = tid & 3, tb = (tid >> 2) & 3, levl = tid >> 4;
             const int lev = lev0 + levl;
             const int thb = kc * 4 + tb;        // theta block (16 rows)
             const int th0 = 32 * kc + 8 * tb + 2 * q;
-            const bool valid = stage_lev_ok && th0 < a.nh;
+            const bool valid = stage_lev_ok[sg] && th0 < a.nh;
             const bool ok1 = th0 + 1 < a.nh;  // rows past n_theta are padding
-            const float sc0 = lvsc[levl], sc1 = lvsc[LGS + levl];
+            const float sc0 = lvsc[sg * 2 * LGS + levl], sc1 = lvsc[sg * 2 * LGS + LGS + levl];
             // rows 2 th, 2 th + 1 for th = th0, th0 + 1: 4 complex, 32 bytes
             float x[NIN][8];
             // loads issued unconditionally at clamped addresses (level and
@@ -1027,7 +1133,15 @@ __global__ __launch_bounds__(256) void k_nana16(NAnaArgs a)
             h8 aPh[2], aPl[2], adPh[2], adPl[2];
 #pragma unroll
             for (int tt = 0; tt < 2; ++tt) {
-                if (tok[tt]) {
+                if constexpr (REC) {
+                    const int ai = (par * 128 + 16 * (wave * 2 + tt) + (lane & 15)) * ROWA + 8 * (lane >> 4);
+                    aPh[tt] = *reinterpret_cast<const h8 *>(gA + ai);
+                    aPl[tt] = *reinterpret_cast<const h8 *>(gA + 2 * 128 * ROWA + ai);
+                    if constexpr (DP) {
+                        adPh[tt] = *reinterpret_cast<const h8 *>(gA + 4 * 128 * ROWA + ai);
+                        adPl[tt] = *reinterpret_cast<const h8 *>(gA + 6 * 128 * ROWA + ai);
+                    }
+                } else if (tok[tt]) {
                     const size_t to = a.off[2 * mc + par] +
                                       (((size_t)(t0 + tt) * a.nq32 + kc) * 64 + lane) * 16;
                     aPh[tt] = *reinterpret_cast<const h8 *>(a.tP + to);
@@ -1053,28 +1167,34 @@ __global__ __launch_bounds__(256) void k_nana16(NAnaArgs a)
                         if (!tok[tt]) continue;
                         const h8 ah = tab == 0 ? aPh[tt] : adPh[tt];
                         const h8 al = tab == 0 ? aPl[tt] : adPl[tt];
-                        f4 t = acc[o][par][tt][tn];
+                        f4 t = acc[sg][o][par][tt][tn];
                         t = mfma16(ah, bl, t);
                         t = mfma16(al, bh, t);
                         t = mfma16(ah, bh, t);
-                        acc[o][par][tt][tn] = t;
+                        acc[sg][o][par][tt][tn] = t;
                     }
                 }
             }
         }
+      }
     }
 
     // ---- epilogue: row r of tile t -> l = m + par + 2 (16 t + r); remove the
     // row and level factors, / l(l+1) for S and T, lcut masks
 #pragma unroll
-    for (int tn = 0; tn < 2; ++tn) {
+    for (int sg = 0; sg < NG; ++sg) {
+      const int lev0 = levb + LGS * sg, nlh = nlh_of(sg);
+      const int ntn = nlh > 8 ? 2 : 1;
+#pragma unroll
+      for (int tn = 0; tn < 2; ++tn) {
         if (tn >= ntn) break;
         const int n = tn * 16 + (lane & 15), levl = n >> 1, ri = n & 1;
         if (levl >= nlh) continue;
         const int lev = lev0 + levl;
         const int lc = a.lcut[lev];
         const bool mkeep = m <= (lc < a.m_max ? lc : a.m_max);
-        const double inv0 = 1.0 / (double)lvsc[levl], inv1 = 1.0 / (double)lvsc[LGS + levl];
+        const double inv0 = 1.0 / (double)lvsc[sg * 2 * LGS + levl];
+        const double inv1 = 1.0 / (double)lvsc[sg * 2 * LGS + LGS + levl];
 #pragma unroll
         for (int tt = 0; tt < 2; ++tt) {
             if (!tok[tt]) continue;
@@ -1091,12 +1211,13 @@ __global__ __launch_bounds__(256) void k_nana16(NAnaArgs a)
 #pragma unroll
                     for (int o = 0; o < NOUT; ++o) {
                         const double rs = rrs * (S::ogrp[o] ? inv1 : inv0);
-                        double v = keep ? (double)acc[o][par][tt][tn][j] * rs : 0.0;
+                        double v = keep ? (double)acc[sg][o][par][tt][tn][j] * rs : 0.0;
                         if (S::divll[o] && l > 0) v /= dll;
                         a.out[o][((size_t)lev * a.out_ld + (size_t)(l0 + l - m)) * 2 + ri] = v;
                     }
                 }
         }
+      }
     }
 }
 
@@ -1497,12 +1618,26 @@ int launch_nana16(const Plan *p, int kind, const NAnaArgs &args, hipStream_t s)
         MHX_HIP_CHECK(hipGetLastError());
         return MHX_OK;
     }
-    const int ng = (a.nlev + LGS - 1) / LGS;
+    const int ng = (a.nlev + LGS * (p->rec ? RNG : 1) - 1) / (LGS * (p->rec ? RNG : 1));
     dim3 grid((unsigned)(8 * ((p->n_m + 7) / 8) * ng * a.ngz));
-    switch (kind) {
-    case NA_QST: k_nana16<NA_QST><<<grid, 256, 0, s>>>(a); break;
-    case NA_SH: k_nana16<NA_SH><<<grid, 256, 0, s>>>(a); break;
-    default: return fail(MHX_E_INVALID, "unknown nonlinear-step analysis kind");
+    if (p->rec) {
+        if (p->l_max > 1023 || !p->d_rec_aseed)
+            return fail(MHX_E_UNSUPPORTED, "recurrence tables need l_max <= 1023");
+        a.aseed = p->d_rec_aseed;
+        a.aexp = p->d_rec_aexp;
+        a.segoff = p->d_rec_segoff;
+        a.cost = p->d_cost;
+        switch (kind) {
+        case NA_QST: k_nana16<NA_QST, true><<<grid, 256, 0, s>>>(a); break;
+        case NA_SH: k_nana16<NA_SH, true><<<grid, 256, 0, s>>>(a); break;
+        default: return fail(MHX_E_INVALID, "unknown nonlinear-step analysis kind");
+        }
+    } else {
+        switch (kind) {
+        case NA_QST: k_nana16<NA_QST, false><<<grid, 256, 0, s>>>(a); break;
+        case NA_SH: k_nana16<NA_SH, false><<<grid, 256, 0, s>>>(a); break;
+        default: return fail(MHX_E_INVALID, "unknown nonlinear-step analysis kind");
+        }
     }
     MHX_HIP_CHECK(hipGetLastError());
     return MHX_OK;

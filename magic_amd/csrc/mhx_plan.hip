@@ -482,6 +482,9 @@ int mhx_plan_destroy(mhx_plan *h)
     free_dev(p->d_ana16_soff);
     free_dev(p->d_syn16_rs);
     free_dev(p->d_rec_seed);
+    free_dev(p->d_rec_aseed);
+    free_dev(p->d_rec_aexp);
+    free_dev(p->d_rec_segoff);
     free_dev(p->d_cost);
     free_dev(p->d_ana16_rs);
     free_dev(p->d_syn16x);

@@ -753,6 +753,60 @@ __global__ void k_rec_seed(int minc, int n_m, int nh, const double *__restrict__
     seed[idx] = make_double2(pm, (double)e);
 }
 
+// analysis seeds: for each (mc, theta_h), the FP64 recurrence from P_mm
+// (src/plms.f90:55-97) with a separate power-of-two exponent, times the
+// quadrature weight 2 pi w(theta) (src/shtransforms.f90:75-76); at every
+// l = m + 32 s <= l_max the pair (P_{l-1}, P_l) is stored with its exponent
+__global__ void k_rec_ana_seed(int l_max, int minc, int n_m, int nh, const double *__restrict__ theta,
+                               const double *__restrict__ gauss, const size_t *__restrict__ segoff,
+                               double2 *__restrict__ seed, int *__restrict__ sexp)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_m * nh) return;
+    int mc = idx / nh, th = idx - mc * nh;
+    int m = mc * minc;
+    const double t = theta[th];
+    const double st = sin(t), ct = cos(t);
+    double fac = 1.0;
+    for (int j = 3; j <= 2 * m + 1; j += 2) fac *= (double)j / (double)(j - 1);
+    int k;
+    const double f = frexp(st, &k);
+    double q = 0.28209479177387814 * sqrt(fac) * 6.283185307179586 * gauss[th];
+    long e = (long)k * m;
+    for (int left = m; left > 0;) {
+        const int n = left < 256 ? left : 256;
+        int k2;
+        q = frexp(q * pow(f, (double)n), &k2);
+        e += k2;
+        left -= n;
+    }
+    double pv = 0.0, eps = 0.0;
+    const double dm = (double)m;
+    for (int l = m; l <= l_max; ++l) {
+        if (((l - m) & 31) == 0) {
+            const size_t o = (segoff[mc] + (size_t)((l - m) >> 5)) * nh + th;
+            seed[o] = make_double2(pv, q);
+            sexp[o] = (int)(e < -2000000000L ? -2000000000L : e);
+        }
+        const double dl = (double)(l + 1);
+        const double num = (dl - dm) * (dl + dm), den = (2.0 * dl - 1.0) * (2.0 * dl + 1.0);
+        const double pn = sqrt(den / num) * (ct * q - eps * pv);
+        pv = q;
+        q = pn;
+        eps = sqrt(num / den);
+        const double v = fabs(q) + fabs(pv);
+        if (v != 0.0 && v < 0x1p-256) {
+            q *= 0x1p256;
+            pv *= 0x1p256;
+            e -= 256;
+        } else if (v > 0x1p256) {
+            q *= 0x1p-256;
+            pv *= 0x1p-256;
+            e += 256;
+        }
+    }
+}
+
 int build_split_tables(Plan *p)
 {
     const int nh = p->nh;
@@ -781,10 +835,12 @@ int build_split_tables(Plan *p)
         MHX_HIP_CHECK(hipMemset(p->d_syn16_P, 0, so * 2));
         MHX_HIP_CHECK(hipMemset(p->d_syn16_dP, 0, so * 2));
     }
-    MHX_HIP_CHECK(hipMalloc(&p->d_ana16_P, ao * 2));
-    MHX_HIP_CHECK(hipMalloc(&p->d_ana16_dP, ao * 2));
-    MHX_HIP_CHECK(hipMemset(p->d_ana16_P, 0, ao * 2));
-    MHX_HIP_CHECK(hipMemset(p->d_ana16_dP, 0, ao * 2));
+    if (!p->rec) {
+        MHX_HIP_CHECK(hipMalloc(&p->d_ana16_P, ao * 2));
+        MHX_HIP_CHECK(hipMalloc(&p->d_ana16_dP, ao * 2));
+        MHX_HIP_CHECK(hipMemset(p->d_ana16_P, 0, ao * 2));
+        MHX_HIP_CHECK(hipMemset(p->d_ana16_dP, 0, ao * 2));
+    }
     MHX_HIP_CHECK(hipMalloc(&p->d_syn16_off, off16.size() * sizeof(size_t)));
     MHX_HIP_CHECK(hipMalloc(&p->d_ana16_off, aoff16.size() * sizeof(size_t)));
     MHX_HIP_CHECK(hipMalloc(&p->d_ana16_soff, soff.size() * sizeof(size_t)));
@@ -810,10 +866,27 @@ int build_split_tables(Plan *p)
     k_rowscale_ana<<<(int)((sr + 127) / 128), 128>>>(p->l_max, p->minc, p->n_m, nh, p->nq4,
                                                      p->d_ana_wP, p->d_ana_wdP, p->d_ana_off,
                                                      p->d_ana16_soff, p->d_ana16_rs, (int)sr);
-    k_split_ana<<<(n + 127) / 128, 128>>>(p->l_max, p->minc, p->n_m, nh, p->nq4, p->nq32,
-                                          p->d_ana_wP, p->d_ana_wdP, p->d_ana_off, p->d_ana16_off,
-                                          p->d_ana16_soff, p->d_ana16_rs, p->d_ana16_P,
-                                          p->d_ana16_dP);
+    if (p->rec) {
+        std::vector<size_t> segoff(p->n_m + 1);
+        size_t ns = 0;
+        for (int mc = 0; mc < p->n_m; ++mc) {
+            segoff[mc] = ns;
+            ns += (size_t)((p->l_max - mc * p->minc) / 32 + 1);
+        }
+        segoff[p->n_m] = ns;
+        MHX_HIP_CHECK(hipMalloc(&p->d_rec_segoff, segoff.size() * sizeof(size_t)));
+        MHX_HIP_CHECK(hipMemcpy(p->d_rec_segoff, segoff.data(), segoff.size() * sizeof(size_t),
+                                hipMemcpyHostToDevice));
+        MHX_HIP_CHECK(hipMalloc(&p->d_rec_aseed, ns * nh * sizeof(double2)));
+        MHX_HIP_CHECK(hipMalloc(&p->d_rec_aexp, ns * nh * sizeof(int)));
+        k_rec_ana_seed<<<(n + 127) / 128, 128>>>(p->l_max, p->minc, p->n_m, nh, p->d_theta, p->d_gauss,
+                                                 p->d_rec_segoff, p->d_rec_aseed, p->d_rec_aexp);
+    } else {
+        k_split_ana<<<(n + 127) / 128, 128>>>(p->l_max, p->minc, p->n_m, nh, p->nq4, p->nq32,
+                                              p->d_ana_wP, p->d_ana_wdP, p->d_ana_off, p->d_ana16_off,
+                                              p->d_ana16_soff, p->d_ana16_rs, p->d_ana16_P,
+                                              p->d_ana16_dP);
+    }
     MHX_HIP_CHECK(hipGetLastError());
 
     // P-only tables with l <= l_max + 1

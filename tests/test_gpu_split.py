@@ -91,8 +91,8 @@ def test_split_l511_dipole_vs_fp64():
 
 @pytest.mark.parametrize("L,nlev", [(511, 6), (1023, 2)])
 def test_split_recurrence_tables(L, nlev):
-    """PREC_SPLIT16_REC: the synthesis tables generated in-kernel by the FP64
-    recurrence from FP64 seeds (no stored FP16 synthesis tables) give every
+    """PREC_SPLIT16_REC: the synthesis and analysis tables generated in-kernel
+    by the FP64 recurrence from FP64 seeds (no stored FP16 tables) give every
     output within 1e-6 of the FP64 form, one level with a lower lcut; at
     l_max=511 they also agree with the stored-table split form to 5e-7 (the
     two differ only by the rounding of the table entries: each is ~3e-7 from
