@@ -226,10 +226,8 @@ struct Plan {
 // ------------------------------------------------------------- launchers
 int build_tables(Plan *p);
 int build_split_tables(Plan *p);
-int launch_syn16(const Plan *p, const SynJob &job, hipStream_t s);
 int launch_nsyn16(const Plan *p, int kind, const NSynArgs &args, hipStream_t s);
 int launch_nana16(const Plan *p, int kind, const NAnaArgs &args, hipStream_t s);
-int launch_ana16(const Plan *p, const AnaJob &job, hipStream_t s);
 int launch_syn(const Plan *p, const SynJob &job, hipStream_t s);
 int launch_ana(const Plan *p, const AnaJob &job, hipStream_t s);
 // Fourier data (complex, m < n_m): inverse side F[lev][comp][m][theta],

@@ -1,22 +1,14 @@
-// Split-precision Legendre stage: scaled FP16 hi/lo operands on
-// v_mfma_f32_16x16x32_f16 with FP32 accumulation (three products per term:
-// hi*hi + hi*lo + lo*hi), batched over radial levels.
+// Tables of the split-precision form (MHX_PREC_SPLIT16): the FP64 P_lm and
+// sin(theta) dP_lm/dtheta tables (mhx_tables.hip) scaled per row by a power
+// of two and split into FP16 hi/lo pairs in the MFMA fragment layouts the
+// split Legendre kernels of the nonlinear step read (mhx_nlsplit.hip), the
+// c_l coefficients of the dP identity, and the FP64 seeds of the in-kernel
+// recurrence (MHX_PREC_SPLIT16_REC).
 //
-// The contraction structure is that of the FP64 kernels (mhx_legendre.hip):
-// per order m, GEMM rows are theta (synthesis) or l (analysis), K is l or
-// theta of one parity, columns are 8 radial levels x re/im, with the
-// equatorial-symmetry accumulators Sym/Anti of src/shtransforms.f90:172-219.
-//
-// Scaling keeps every operand inside FP16's normal range:
-//   - tables: one power-of-2 factor per output row (per (m, theta) for the
-//     synthesis, per (m, l) for the analysis), common to P and dP, chosen so
-//     that the row maximum lies in [2^14, 2^15); generated in FP64 then split;
-//   - B operand: one power-of-2 factor per (chunk of K, radial level), common
-//     to all columns of that level, found while staging; each chunk is
-//     accumulated in fresh FP32 registers and added to the running FP32 sum
-//     after multiplying by that factor.
-// Row factors are applied in the epilogue; results are stored in FP32
-// (Fourier values) or FP64 (spectral coefficients).
+// Scaling keeps every table operand inside FP16's normal range: one
+// power-of-2 factor per output row (per (m, theta) for the synthesis, per
+// (m, l) for the analysis), common to P and dP, chosen so that the row
+// maximum lies in [2^14, 2^15); the kernels remove it in their epilogues.
 //
 // f16 16x16x32 lane layouts (checked on gfx950): A lane l holds row l%16,
 // k = 8(l/16)..8(l/16)+7; B lane l holds column l%16, the same k; C element
@@ -28,27 +20,6 @@
 namespace mhx {
 
 namespace {
-
-typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-
-constexpr int WG = 256;
-constexpr int KS = 32;    // K per MFMA step
-constexpr int NKS = 2;    // MFMA steps per staged chunk
-constexpr int CK = KS * NKS;  // K per chunk (per parity for the synthesis)
-
-__device__ inline f4 mfma16(h8 a, h8 b, f4 c)
-{
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-}
-
-__device__ inline int xcd_logical(int b, int nb)
-{
-    const int nx = 8;
-    int per = nb / nx, rem = nb % nx;
-    int x = b % nx, i = b / nx;
-    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + i;
-}
 
 // power-of-2 factor s with max*s in [2^14, 2^15); 1 for max == 0; capped
 // at 2^115 so that rows of tiny values (P_lm near the poles) stay finite
@@ -65,431 +36,6 @@ __device__ inline void split_h(float v, _Float16 &hi, _Float16 &lo)
 {
     hi = (_Float16)v;
     lo = (_Float16)(v - (float)hi);
-}
-
-__device__ inline void apply_factor(double &re, double &im, int ipow, int mpow, double m, double scale)
-{
-    double f = scale;
-    for (int k = 0; k < mpow; ++k) f *= m;
-    re *= f;
-    im *= f;
-    for (int k = 0; k < (ipow & 3); ++k) {
-        double t = re;
-        re = -im;
-        im = t;
-    }
-}
-
-struct Prod {
-    int acc, tab, col;
-};
-
-// same product tables as the FP64 kernels
-template <int KIND> struct Syn16;
-template <> struct Syn16<SYN_SCAL> { static constexpr int NOUT = 1, NCOL = 1, NPROD = 1; static constexpr Prod prod[1] = {{0, 0, 0}}; };
-template <> struct Syn16<SYN_GRAD> { static constexpr int NOUT = 2, NCOL = 2, NPROD = 2; static constexpr Prod prod[2] = {{0, 1, 0}, {1, 0, 1}}; };
-template <> struct Syn16<SYN_SG> { static constexpr int NOUT = 3, NCOL = 2, NPROD = 3; static constexpr Prod prod[3] = {{0, 0, 0}, {1, 1, 0}, {2, 0, 1}}; };
-template <> struct Syn16<SYN_QST> { static constexpr int NOUT = 3, NCOL = 5, NPROD = 5; static constexpr Prod prod[5] = {{0, 0, 0}, {1, 1, 1}, {1, 0, 2}, {2, 0, 3}, {2, 1, 4}}; };
-template <> struct Syn16<SYN_SPHTOR> { static constexpr int NOUT = 2, NCOL = 4, NPROD = 4; static constexpr Prod prod[4] = {{0, 1, 0}, {0, 0, 1}, {1, 0, 2}, {1, 1, 3}}; };
-
-template <int KIND> struct Ana16;
-template <> struct Ana16<ANA_SH> { static constexpr int NOUT = 1, NCOL = 1, NPROD = 1; static constexpr Prod prod[1] = {{0, 0, 0}}; static constexpr bool divll[1] = {false}; };
-template <> struct Ana16<ANA_SPHTOR> { static constexpr int NOUT = 2, NCOL = 4, NPROD = 4; static constexpr Prod prod[4] = {{0, 1, 0}, {0, 0, 1}, {1, 1, 2}, {1, 0, 3}}; static constexpr bool divll[2] = {true, true}; };
-template <> struct Ana16<ANA_QST> { static constexpr int NOUT = 3, NCOL = 5, NPROD = 5; static constexpr Prod prod[5] = {{0, 0, 0}, {1, 1, 1}, {1, 0, 2}, {2, 1, 3}, {2, 0, 4}}; static constexpr bool divll[3] = {false, true, true}; };
-
-struct Syn16Args {
-    SynJob job;
-    const _Float16 *tP, *tdP;   // split tables (hi/lo interleaved per lane)
-    const size_t *off;          // [2 mc + par] offsets in halves
-    const float *rowscale;      // [mc][nh]
-    const int *lstart;
-    const double *osin2;
-    int l_max, minc, n_m, nh, n_theta, nt16;
-};
-
-struct Ana16Args {
-    AnaJob job;
-    const _Float16 *tP, *tdP;
-    const size_t *off;
-    const float *rowscale;      // [2 mc + par] blocks of K0 entries: aoff_s
-    const size_t *soff;
-    const int *lstart;
-    int l_max, m_max, minc, n_m, nh, n_theta, nq32;
-};
-
-// ------------------------------------------------------------- synthesis
-template <int KIND>
-__global__ __launch_bounds__(WG) void k_syn16(Syn16Args a)
-{
-    using S = Syn16<KIND>;
-    constexpr int NOUT = S::NOUT, NCOL = S::NCOL, NPROD = S::NPROD;
-    // B chunk: [col][par][16 cols][CK k] halves, hi and lo planes
-    __shared__ _Float16 bhi[NCOL * 2 * 16 * CK];
-    __shared__ _Float16 blo[NCOL * 2 * 16 * CK];
-    __shared__ unsigned lvmax[LEV_GROUP];
-    __shared__ float lvscale[LEV_GROUP];
-
-    const SynJob &job = a.job;
-    const int lb = xcd_logical(blockIdx.x, gridDim.x);
-    const int mc = lb % a.n_m;
-    const int ng = (job.nlev + LEV_GROUP - 1) / LEV_GROUP;
-    const int gy = (lb / a.n_m) % ng, gz = lb / (a.n_m * ng);
-    const int m = mc * a.minc;
-    const double dm = (double)m;
-    const int lev0 = gy * LEV_GROUP;
-    const int nlev_here = min(LEV_GROUP, job.nlev - lev0);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tt = gz * 4 + wave;
-    const bool active = tt < a.nt16;
-    const int K[2] = {(a.l_max - m) / 2 + 1, (a.l_max - m + 1) / 2};
-    const int nks[2] = {(K[0] + KS - 1) / KS, (K[1] + KS - 1) / KS};
-    const int l0 = a.lstart[mc];
-    const int nchunk = (K[0] + CK - 1) / CK;
-
-    f4 acc[NOUT][2], tmp[NOUT][2];
-#pragma unroll
-    for (int o = 0; o < NOUT; ++o)
-#pragma unroll
-        for (int s = 0; s < 2; ++s) acc[o][s] = f4{0.f, 0.f, 0.f, 0.f};
-
-    // staging entries: (par, kk in chunk, level): 2 * CK * 8 = 1024 -> 4/thread
-    constexpr int NE = (2 * CK * LEV_GROUP) / WG;
-    for (int c = 0; c < nchunk; ++c) {
-        if (threadIdx.x < LEV_GROUP) lvmax[threadIdx.x] = 0u;
-        __syncthreads();
-        double cre[NE][NCOL], cim[NE][NCOL];
-#pragma unroll
-        for (int e = 0; e < NE; ++e) {
-            int idx = threadIdx.x + e * WG;
-            int levl = idx % LEV_GROUP;
-            int kkl = (idx / LEV_GROUP) % CK;
-            int par = idx / (LEV_GROUP * CK);
-            int l = m + par + 2 * (c * CK + kkl);
-            int lev = lev0 + levl;
-            bool valid = (levl < nlev_here) && (l <= a.l_max);
-            if (valid) valid = l <= job.lcut[lev];
-            double xr[MAX_SRC], xi[MAX_SRC];
-            size_t o = ((size_t)lev * job.src_ld + (size_t)(l0 + l - m)) * 2;
-#pragma unroll
-            for (int sidx = 0; sidx < MAX_SRC; ++sidx) {
-                double2 v = make_double2(0.0, 0.0);
-                if (valid && job.src[sidx]) v = *reinterpret_cast<const double2 *>(job.src[sidx] + o);
-                xr[sidx] = v.x;
-                xi[sidx] = v.y;
-            }
-            double ll = (double)l * (double)(l + 1);
-            float mx = 0.f;
-#pragma unroll
-            for (int col = 0; col < NCOL; ++col) {
-                double re = 0.0, im = 0.0;
-                if (valid) {
-                    const ColOp &op = job.col[col];
-                    double g = op.gkind == G_ONE ? 1.0 : (op.gkind == G_LL ? ll : job.or2[lev] * ll);
-                    double x0r = 0, x0i = 0, x1r = 0, x1i = 0;
-#pragma unroll
-                    for (int sidx = 0; sidx < MAX_SRC; ++sidx) {
-                        if (op.src0 == sidx) { x0r = xr[sidx]; x0i = xi[sidx]; }
-                        if (op.src1 == sidx) { x1r = xr[sidx]; x1i = xi[sidx]; }
-                    }
-                    re = g * x0r + op.hcoef * x1r;
-                    im = g * x0i + op.hcoef * x1i;
-                    apply_factor(re, im, op.ipow, op.mpow, dm, op.scale);
-                }
-                cre[e][col] = re;
-                cim[e][col] = im;
-                mx = fmaxf(mx, fmaxf(fabsf((float)re), fabsf((float)im)));
-            }
-            atomicMax(&lvmax[levl], __float_as_uint(mx));
-        }
-        __syncthreads();
-        if (threadIdx.x < LEV_GROUP) lvscale[threadIdx.x] = pow2_scale(__uint_as_float(lvmax[threadIdx.x]));
-        __syncthreads();
-#pragma unroll
-        for (int e = 0; e < NE; ++e) {
-            int idx = threadIdx.x + e * WG;
-            int levl = idx % LEV_GROUP;
-            int kkl = (idx / LEV_GROUP) % CK;
-            int par = idx / (LEV_GROUP * CK);
-            double sc = (double)lvscale[levl];
-#pragma unroll
-            for (int col = 0; col < NCOL; ++col) {
-                _Float16 h, l;
-                int base = ((col * 2 + par) * 16 + levl * 2) * CK + kkl;
-                split_h((float)(cre[e][col] * sc), h, l);
-                bhi[base] = h;
-                blo[base] = l;
-                split_h((float)(cim[e][col] * sc), h, l);
-                bhi[base + CK] = h;
-                blo[base + CK] = l;
-            }
-        }
-        __syncthreads();
-        if (active) {
-#pragma unroll
-            for (int o = 0; o < NOUT; ++o)
-#pragma unroll
-                for (int s = 0; s < 2; ++s) tmp[o][s] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int par = 0; par < 2; ++par) {
-#pragma unroll
-                for (int q = 0; q < NKS; ++q) {
-                    int ks = c * NKS + q;
-                    if (ks >= nks[par]) break;
-                    size_t to = a.off[2 * mc + par] + (((size_t)ks * a.nt16 + tt) * 64 + lane) * 16;
-                    h8 aPh = *reinterpret_cast<const h8 *>(a.tP + to);
-                    h8 aPl = *reinterpret_cast<const h8 *>(a.tP + to + 8);
-                    h8 adPh = *reinterpret_cast<const h8 *>(a.tdP + to);
-                    h8 adPl = *reinterpret_cast<const h8 *>(a.tdP + to + 8);
-#pragma unroll
-                    for (int pr = 0; pr < NPROD; ++pr) {
-                        const Prod P = S::prod[pr];
-                        int bo = ((P.col * 2 + par) * 16 + (lane & 15)) * CK + q * KS + 8 * (lane >> 4);
-                        h8 bh = *reinterpret_cast<const h8 *>(bhi + bo);
-                        h8 bl = *reinterpret_cast<const h8 *>(blo + bo);
-                        int tgt = (P.tab == 0) ? par : (1 - par);
-                        h8 ah = P.tab == 0 ? aPh : adPh;
-                        h8 al = P.tab == 0 ? aPl : adPl;
-                        f4 t = tmp[P.acc][tgt];
-                        t = mfma16(ah, bl, t);
-                        t = mfma16(al, bh, t);
-                        t = mfma16(ah, bh, t);
-                        tmp[P.acc][tgt] = t;
-                    }
-                }
-            }
-            float inv = 1.f / lvscale[(lane & 15) >> 1];
-#pragma unroll
-            for (int o = 0; o < NOUT; ++o)
-#pragma unroll
-                for (int s = 0; s < 2; ++s)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) acc[o][s][j] += tmp[o][s][j] * inv;
-        }
-        __syncthreads();
-    }
-
-    // epilogue: north = Sym + Anti, south = Sym - Anti, times the row factor;
-    // float2 stores into F[lev][comp][m][theta]
-    if (active) {
-        const int col = lane & 15, levl = col >> 1, ri = col & 1;
-        if (levl < nlev_here) {
-            float *fo = reinterpret_cast<float *>(job.fout) + (size_t)(lev0 + levl) * job.fout_ld * 2;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                int th = tt * 16 + 4 * (lane >> 4) + j;
-                if (th >= a.nh) continue;
-                float rs = 1.f / a.rowscale[(size_t)mc * a.nh + th];
-#pragma unroll
-                for (int o = 0; o < NOUT; ++o) {
-                    float sc = rs;
-                    if (job.row_scale[o]) sc *= (float)a.osin2[2 * th];
-                    size_t base = (((size_t)job.out_comp[o] * a.n_m + mc) * a.n_theta + 2 * th) * 2;
-                    fo[base + ri] = (acc[o][0][j] + acc[o][1][j]) * sc;
-                    fo[base + 2 + ri] = (acc[o][0][j] - acc[o][1][j]) * sc;
-                }
-            }
-        }
-    }
-}
-
-// -------------------------------------------------------------- analysis
-template <int KIND>
-__global__ __launch_bounds__(WG) void k_ana16(Ana16Args a)
-{
-    using S = Ana16<KIND>;
-    constexpr int NOUT = S::NOUT, NCOL = S::NCOL, NPROD = S::NPROD;
-    // B chunk: [col][sym/anti][16 cols][CK theta] halves
-    __shared__ _Float16 bhi[NCOL * 2 * 16 * CK];
-    __shared__ _Float16 blo[NCOL * 2 * 16 * CK];
-    __shared__ unsigned lvmax[LEV_GROUP];
-    __shared__ float lvscale[LEV_GROUP];
-
-    const AnaJob &job = a.job;
-    const int lb = xcd_logical(blockIdx.x, gridDim.x);
-    const int mc = lb % a.n_m;
-    const int ng = (job.nlev + LEV_GROUP - 1) / LEV_GROUP;
-    const int gy = (lb / a.n_m) % ng, gz = lb / (a.n_m * ng);
-    const int m = mc * a.minc;
-    const double dm = (double)m;
-    const int lev0 = gy * LEV_GROUP;
-    const int nlev_here = min(LEV_GROUP, job.nlev - lev0);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int t = gz * 4 + wave;  // pair tile: l = m + 32 t + 0..31
-    const int K0 = (a.l_max - m) / 2 + 1;
-    const int ntile = (K0 + 15) / 16;
-    const bool active = t < ntile;
-    const int l0 = a.lstart[mc];
-    const int nchunk = (a.nh + CK - 1) / CK;
-
-    int csrc[NCOL], cip[NCOL], cmp[NCOL];
-    double csc[NCOL];
-    if constexpr (KIND == ANA_SH) {
-        csrc[0] = 0; cip[0] = 0; cmp[0] = 0; csc[0] = 1.0;
-    } else {
-        const int b = (KIND == ANA_QST) ? 1 : 0;
-        if constexpr (KIND == ANA_QST) { csrc[0] = 0; cip[0] = 0; cmp[0] = 0; csc[0] = 1.0; }
-        csrc[b + 0] = 1; cip[b + 0] = 0; cmp[b + 0] = 0; csc[b + 0] = 1.0;
-        csrc[b + 1] = 2; cip[b + 1] = 1; cmp[b + 1] = 1; csc[b + 1] = -1.0;
-        csrc[b + 2] = 2; cip[b + 2] = 0; cmp[b + 2] = 0; csc[b + 2] = -1.0;
-        csrc[b + 3] = 1; cip[b + 3] = 1; cmp[b + 3] = 1; csc[b + 3] = -1.0;
-    }
-    bool mok[LEV_GROUP];
-#pragma unroll
-    for (int i = 0; i < LEV_GROUP; ++i) mok[i] = false;
-
-    f4 acc[NOUT][2], tmp[NOUT][2];
-#pragma unroll
-    for (int o = 0; o < NOUT; ++o)
-#pragma unroll
-        for (int s = 0; s < 2; ++s) acc[o][s] = f4{0.f, 0.f, 0.f, 0.f};
-
-    constexpr int NE = (CK * LEV_GROUP) / WG;  // 2
-    const float *fin = reinterpret_cast<const float *>(job.fin);
-    for (int c = 0; c < nchunk; ++c) {
-        if (threadIdx.x < LEV_GROUP) lvmax[threadIdx.x] = 0u;
-        __syncthreads();
-        float sy[NE][NCOL][2], an[NE][NCOL][2];
-#pragma unroll
-        for (int e = 0; e < NE; ++e) {
-            int idx = threadIdx.x + e * WG;
-            int levl = idx % LEV_GROUP;
-            int thl = idx / LEV_GROUP;
-            int th = c * CK + thl;
-            int lev = lev0 + levl;
-            bool valid = (levl < nlev_here) && (th < a.nh);
-            if (valid) {
-                int lc = job.lcut[lev];
-                int mlim = lc < a.m_max ? lc : a.m_max;
-                valid = m <= mlim;
-            }
-            float nr[3], ni[3], sr[3], si[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) { nr[k] = ni[k] = sr[k] = si[k] = 0.f; }
-            if (valid) {
-                int nsrc = (KIND == ANA_SH) ? 1 : 3;
-                for (int k = 0; k < nsrc; ++k) {
-                    int comp = job.in_comp[k];
-                    if (comp < 0) continue;
-                    size_t o = ((size_t)lev * job.fin_ld + ((size_t)comp * a.n_theta + 2 * th) * a.n_m + mc) * 2;
-                    size_t o2 = o + (size_t)a.n_m * 2;
-                    nr[k] = fin[o];
-                    ni[k] = fin[o + 1];
-                    sr[k] = fin[o2];
-                    si[k] = fin[o2 + 1];
-                }
-            }
-            float mx = 0.f;
-#pragma unroll
-            for (int col = 0; col < NCOL; ++col) {
-                int k = csrc[col];
-                double syr = (double)nr[k] + sr[k], syi = (double)ni[k] + si[k];
-                double anr = (double)nr[k] - sr[k], ani = (double)ni[k] - si[k];
-                apply_factor(syr, syi, cip[col], cmp[col], dm, csc[col]);
-                apply_factor(anr, ani, cip[col], cmp[col], dm, csc[col]);
-                sy[e][col][0] = (float)syr; sy[e][col][1] = (float)syi;
-                an[e][col][0] = (float)anr; an[e][col][1] = (float)ani;
-                mx = fmaxf(mx, fmaxf(fmaxf(fabsf((float)syr), fabsf((float)syi)),
-                                     fmaxf(fabsf((float)anr), fabsf((float)ani))));
-            }
-            atomicMax(&lvmax[levl], __float_as_uint(mx));
-        }
-        __syncthreads();
-        if (threadIdx.x < LEV_GROUP) lvscale[threadIdx.x] = pow2_scale(__uint_as_float(lvmax[threadIdx.x]));
-        __syncthreads();
-#pragma unroll
-        for (int e = 0; e < NE; ++e) {
-            int idx = threadIdx.x + e * WG;
-            int levl = idx % LEV_GROUP;
-            int thl = idx / LEV_GROUP;
-            float sc = lvscale[levl];
-#pragma unroll
-            for (int col = 0; col < NCOL; ++col) {
-#pragma unroll
-                for (int ri = 0; ri < 2; ++ri) {
-                    _Float16 h, l;
-                    int b0 = ((col * 2 + 0) * 16 + levl * 2 + ri) * CK + thl;
-                    int b1 = ((col * 2 + 1) * 16 + levl * 2 + ri) * CK + thl;
-                    split_h(sy[e][col][ri] * sc, h, l);
-                    bhi[b0] = h;
-                    blo[b0] = l;
-                    split_h(an[e][col][ri] * sc, h, l);
-                    bhi[b1] = h;
-                    blo[b1] = l;
-                }
-            }
-        }
-        __syncthreads();
-        if (active) {
-#pragma unroll
-            for (int o = 0; o < NOUT; ++o)
-#pragma unroll
-                for (int s = 0; s < 2; ++s) tmp[o][s] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int q = 0; q < NKS; ++q) {
-                int kc = c * NKS + q;
-                if (kc >= a.nq32) break;
-#pragma unroll
-                for (int par = 0; par < 2; ++par) {
-                    size_t to = a.off[2 * mc + par] + (((size_t)t * a.nq32 + kc) * 64 + lane) * 16;
-                    h8 aPh = *reinterpret_cast<const h8 *>(a.tP + to);
-                    h8 aPl = *reinterpret_cast<const h8 *>(a.tP + to + 8);
-                    h8 adPh = *reinterpret_cast<const h8 *>(a.tdP + to);
-                    h8 adPl = *reinterpret_cast<const h8 *>(a.tdP + to + 8);
-#pragma unroll
-                    for (int pr = 0; pr < NPROD; ++pr) {
-                        const Prod P = S::prod[pr];
-                        int use = (P.tab == 0) ? par : (1 - par);
-                        int bo = ((P.col * 2 + use) * 16 + (lane & 15)) * CK + q * KS + 8 * (lane >> 4);
-                        h8 bh = *reinterpret_cast<const h8 *>(bhi + bo);
-                        h8 bl = *reinterpret_cast<const h8 *>(blo + bo);
-                        h8 ah = P.tab == 0 ? aPh : adPh;
-                        h8 al = P.tab == 0 ? aPl : adPl;
-                        f4 tt2 = tmp[P.acc][par];
-                        tt2 = mfma16(ah, bl, tt2);
-                        tt2 = mfma16(al, bh, tt2);
-                        tt2 = mfma16(ah, bh, tt2);
-                        tmp[P.acc][par] = tt2;
-                    }
-                }
-            }
-            float inv = 1.f / lvscale[(lane & 15) >> 1];
-#pragma unroll
-            for (int o = 0; o < NOUT; ++o)
-#pragma unroll
-                for (int s = 0; s < 2; ++s)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) acc[o][s][j] += tmp[o][s][j] * inv;
-        }
-        __syncthreads();
-    }
-
-    // epilogue: row r -> l = m + par + 2 (16 t + r), times 1/rowscale,
-    // divided by l(l+1) for the spheroidal/toroidal outputs
-    if (active) {
-        const int col = lane & 15, levl = col >> 1, ri = col & 1;
-        if (levl < nlev_here) {
-            int lev = lev0 + levl;
-            int lc = job.lcut[lev];
-            bool mkeep = m <= (lc < a.m_max ? lc : a.m_max);
-#pragma unroll
-            for (int par = 0; par < 2; ++par)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    int r = 4 * (lane >> 4) + j;
-                    int kk = 16 * t + r;
-                    int l = m + par + 2 * kk;
-                    if (l > a.l_max) continue;
-                    float rs = a.rowscale[a.soff[2 * mc + par] + kk];
-                    bool keep = mkeep && l <= lc;
-#pragma unroll
-                    for (int o = 0; o < NOUT; ++o) {
-                        double v = keep ? (double)acc[o][par][j] / (double)rs : 0.0;
-                        if (S::divll[o] && l > 0) v /= (double)l * (double)(l + 1);
-                        job.out[o][((size_t)lev * job.out_ld + (size_t)(l0 + l - m)) * 2 + ri] = v;
-                    }
-                }
-        }
-    }
 }
 
 // ------------------------------------------------------------ split tables
@@ -936,66 +482,6 @@ int build_split_tables(Plan *p)
     }
     MHX_HIP_CHECK(hipGetLastError());
     MHX_HIP_CHECK(hipDeviceSynchronize());
-    return MHX_OK;
-}
-
-int launch_syn16(const Plan *p, const SynJob &job, hipStream_t s)
-{
-    if (job.nlev <= 0) return MHX_OK;
-    Syn16Args a;
-    a.job = job;
-    a.tP = p->d_syn16_P;
-    a.tdP = p->d_syn16_dP;
-    a.off = p->d_syn16_off;
-    a.rowscale = p->d_syn16_rs;
-    a.lstart = p->d_lstart;
-    a.osin2 = p->d_osin2;
-    a.l_max = p->l_max;
-    a.minc = p->minc;
-    a.n_m = p->n_m;
-    a.nh = p->nh;
-    a.n_theta = p->n_theta;
-    a.nt16 = p->nt16;
-    dim3 grid((unsigned)(p->n_m * ((job.nlev + LEV_GROUP - 1) / LEV_GROUP) * ((p->nt16 + 3) / 4)));
-    switch (job.kind) {
-    case SYN_SCAL: k_syn16<SYN_SCAL><<<grid, WG, 0, s>>>(a); break;
-    case SYN_GRAD: k_syn16<SYN_GRAD><<<grid, WG, 0, s>>>(a); break;
-    case SYN_SG: k_syn16<SYN_SG><<<grid, WG, 0, s>>>(a); break;
-    case SYN_QST: k_syn16<SYN_QST><<<grid, WG, 0, s>>>(a); break;
-    case SYN_SPHTOR: k_syn16<SYN_SPHTOR><<<grid, WG, 0, s>>>(a); break;
-    default: return fail(MHX_E_INVALID, "unknown synthesis kind");
-    }
-    MHX_HIP_CHECK(hipGetLastError());
-    return MHX_OK;
-}
-
-int launch_ana16(const Plan *p, const AnaJob &job, hipStream_t s)
-{
-    if (job.nlev <= 0) return MHX_OK;
-    Ana16Args a;
-    a.job = job;
-    a.tP = p->d_ana16_P;
-    a.tdP = p->d_ana16_dP;
-    a.off = p->d_ana16_off;
-    a.rowscale = p->d_ana16_rs;
-    a.soff = p->d_ana16_soff;
-    a.lstart = p->d_lstart;
-    a.l_max = p->l_max;
-    a.m_max = p->m_max;
-    a.minc = p->minc;
-    a.n_m = p->n_m;
-    a.nh = p->nh;
-    a.n_theta = p->n_theta;
-    a.nq32 = p->nq32;
-    int ntile0 = ((p->l_max) / 2 + 1 + 15) / 16;
-    dim3 grid((unsigned)(p->n_m * ((job.nlev + LEV_GROUP - 1) / LEV_GROUP) * ((ntile0 + 3) / 4)));
-    switch (job.kind) {
-    case ANA_SH: k_ana16<ANA_SH><<<grid, WG, 0, s>>>(a); break;
-    case ANA_SPHTOR: k_ana16<ANA_SPHTOR><<<grid, WG, 0, s>>>(a); break;
-    case ANA_QST: k_ana16<ANA_QST><<<grid, WG, 0, s>>>(a); break;
-    default: return fail(MHX_E_INVALID, "unknown analysis kind");
-    }
-    MHX_HIP_CHECK(hipGetLastError());
     return MHX_OK;
 }
 
