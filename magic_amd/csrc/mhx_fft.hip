@@ -575,11 +575,17 @@ __device__ __forceinline__ void slot_pass(C *buf, int t, const C *tw, int sign)
     constexpr int Q = M / P, NB = P / R;  // lanes per slot, butterflies per lane
     static_assert(P % R == 0 && Q * P == M, "pass does not tile the slot");
     constexpr int T = M / R, tstride = M / (L * R);
+    // k = (t + Q b) mod L from t mod L (t < Q: no division when L >= Q)
+    const int tm = L >= Q ? t : t % L;
+    auto kof = [&](int b) {
+        const int k = tm + (Q * b) % L;
+        return k >= L ? k - L : k;
+    };
     C x[NB][R];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const int j = t + Q * b;
-        const int k = j % L;
+        const int k = kof(b);
 #pragma unroll
         for (int s = 0; s < R; ++s) {
             C v = buf[j + s * T];
@@ -598,7 +604,7 @@ __device__ __forceinline__ void slot_pass(C *buf, int t, const C *tw, int sign)
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const int j = t + Q * b;
-        const int k = j % L;
+        const int k = kof(b);
         Dft<R, C>::run(x[b], sign);
         C *op = buf + (j - k) * R + k;
 #pragma unroll
@@ -713,10 +719,13 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
                         z[k2] = cadd(cadd(c, cconj(a)), cmuli(cmul(cconj(s_twr[k2]), csub(c, cconj(a))), 1));
                 }
             } else {
-                for (int k = t; k <= M / 2; k += Q) {
+                // pointers stepped by Q rows (no index products per step)
+                const size_t step = (size_t)Q * n_theta;
+                const C *xa = X + (size_t)t * n_theta, *xc = X + (size_t)(M - t) * n_theta;
+                for (int k = t; k <= M / 2; k += Q, xa += step, xc -= step) {
                     const int k2 = M - k;
-                    C a = (k < n_m) ? X[(size_t)k * n_theta] : mk<C>(T(0), T(0));
-                    C c = (k2 < n_m) ? X[(size_t)k2 * n_theta] : mk<C>(T(0), T(0));
+                    C a = (k < n_m) ? *xa : mk<C>(T(0), T(0));
+                    C c = (k2 < n_m) ? *xc : mk<C>(T(0), T(0));
                     if (k == 0) {
                         a.y = T(0);
                         c.y = T(0);
@@ -779,13 +788,11 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
             FFT::run(z, t, s_tw, -1);
             const int comp = round == 0 ? (slot < 3 ? slot : slot + 3) : (slot < 3 ? slot + 3 : 9);
             const T invN = T(1.0 / (double)N);
-            for (int k = t; k < n_m; k += Q) {
-                C v = rfft_post(z, k, M, s_twr, invN);
-                if constexpr (BLK)
-                    Fo[(((size_t)comp * ntb + (th >> 4)) * n_m + k) * 16 + (th & 15)] = v;
-                else
-                    Fo[((size_t)comp * n_theta + th) * n_m + k] = v;
-            }
+            // F2 theta-blocked (split form): consecutive k are 16 apart
+            constexpr int FS = BLK ? 16 : 1;
+            C *fo = BLK ? Fo + (((size_t)comp * ntb + (th >> 4)) * n_m + t) * 16 + (th & 15)
+                        : Fo + ((size_t)comp * n_theta + th) * n_m + t;
+            for (int k = t; k < n_m; k += Q, fo += Q * FS) *fo = rfft_post(z, k, M, s_twr, invN);
         }
         __syncthreads();
     };
