@@ -118,6 +118,38 @@ def cpu_baseline(l_max, nlev_sample):
             "kind": "port", "sample": "1 radial level of the l_max=%d nonlinear step" % l_max}
 
 
+def run_td(L, nlev, tout, rad, lcut, reps=20):
+    """The lm-space assembly after the step (mhx_get_td) on the step's FP64
+    outputs and inputs, timed on a side stream; algorithmic bytes per level
+    and lm: 10 complex in, 7 out."""
+    import torch
+
+    from magic_amd import sht, workload
+    nt, nph = workload.grid_size(L)
+    p = sht.Plan(L, nt, nph, max_levels=nlev)
+    dev = tout[0].device
+    ins = [t.clone() for t in tout[:3]] + [tout[4].clone()] + [t.clone() for t in tout[6:9]] + \
+          [torch.ones_like(tout[0]) for _ in range(3)]
+    outs = [torch.zeros_like(tout[0]) for _ in range(7)]
+    st = torch.cuda.Stream()
+    st.wait_stream(torch.cuda.current_stream())
+    epsc = [0.5] * nlev
+    for _ in range(3):
+        sht.get_td(p, ins, outs, rad[0], rad[1], epsc, 1.0e3, lcut, stream=st)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        sht.get_td(p, ins, outs, rad[0], rad[1], epsc, 1.0e3, lcut, stream=st)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    p.close()
+    byts = 17.0 * 16.0 * p.lm_max * nlev
+    return {"kernel": "k_get_td", "avg_ms_per_call": ms, "bound": "hbm",
+            "algorithmic_bytes": byts, "GB/s": byts / (ms * 1e-3) / 1e9,
+            "frac_of_hbm_peak": byts / (ms * 1e-3) / 8.0e12}
+
+
 def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut):
     import numpy as np
     import torch
@@ -173,8 +205,9 @@ def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut):
                          "TFLOP/s": v["flop"] / v["n"] / (ms * 1e-3) / 1e12,
                          "GB/s": v["bytes"] / v["n"] / (ms * 1e-3) / 1e9}
     outs = [t.cpu().numpy() for t in tout]
+    td = run_td(L, nlev, tout, rad, lcut) if precision == sht.PREC_FP64 else None
     p.close()
-    return {"pairs": nlev * steps / dt, "ms_per_step": 1e3 * dt / steps,
+    return {"td": td, "pairs": nlev * steps / dt, "ms_per_step": 1e3 * dt / steps,
             "per_kernel": per_kernel, "kernels": kernels,
             "legendre_tflops": total_flop / (flop_ms * 1e-3) / 1e12, "outs": outs}
 
@@ -432,6 +465,11 @@ def main():
         "roofline": roofline(main, kern_key, (L, nlev)),
         "cpu_baseline": cpu,
     }
+    if "fp64" in forms and forms["fp64"].get("td"):
+        # the next stage of the time step (src/get_td.f90), FP64, not in `value`
+        res["td_assembly"] = forms["fp64"]["td"]
+        log("get_td: %.3f ms per call (%.0f GB/s)" % (res["td_assembly"]["avg_ms_per_call"],
+                                                       res["td_assembly"]["GB/s"]))
     if "fp64" in forms and main_key != "fp64":
         f = forms["fp64"]
         res["fp64_form"] = {"value": f["pairs"], "ms_per_step": f["ms_per_step"],

@@ -177,6 +177,25 @@ int mhx_nl_step_grids(mhx_plan *p, int nlev, const double *const *in,
                       const double *or4, const double *orho1, double LFfac,
                       const int *lcut, double *grids, void *stream);
 
+/* lm-space assembly of the explicit terms at interior radial levels, the
+ * step after the forward transforms of the nonlinear step: the reference's
+ * nonlinear_lm_t get_dwdt, get_dzdt, get_dpdt, get_dsdt and get_dbdt
+ * (src/get_td.f90:133-589, called per level at src/rIter.f90:412-429) for a
+ * rotating Boussinesq MHD run without the double-curl form (l_corr,
+ * l_conv_nl, l_mag_nl; not l_anel, l_double_curl, l_single_matrix), batched
+ * over nlev levels.
+ * in[10] = AdvrLM AdvtLM AdvpLM VStLM VxBrLM VxBtLM VxBpLM (the outputs of
+ *          mhx_nl_step) w dw z (the fields at the levels), ld_lm
+ * out[7] = dwdt dzdt dpdt dsdt dbdt djdt dVxBhLM, ld_lm; dpdt of l = 0 is
+ *          zero (the reference leaves it unset)
+ * per level (host arrays): or2, or4, epsc = epsc * epscProf(nR) (the l = 0
+ * term of dsdt), lcut = l_R(nR); CorFac as src/get_td.f90 reads it; or1 =
+ * sqrt(or2) and r^2 = 1/or2.  Arrays host or device memory; stream as in
+ * mhx_nl_step.  FP64 in every precision. */
+int mhx_get_td(mhx_plan *p, int nlev, const double *const *in, size_t ld_lm,
+               double *const *out, const double *or2, const double *or4,
+               const double *epsc, double CorFac, const int *lcut, void *stream);
+
 /* ---- uneven batches of independent transform jobs (split form) ----
  * A job is the round trip of nfields fields of one kind at one truncation
  * over nlev radial levels: synthesis to grids and analysis back, i.e. per

@@ -130,6 +130,20 @@ struct NAnaArgs {
     int l_max, m_max, minc, n_m, nh, n_theta, ntb, nq32, ngz;
 };
 
+// lm-space assembly of the explicit terms (mhx_td.hip)
+struct TdArgs {
+    const double *in[10];   // AdvrLM AdvtLM AdvpLM VStLM VxBrLM VxBtLM VxBpLM w dw z
+    double *out[7];         // dwdt dzdt dpdt dsdt dbdt djdt dVxBhLM
+    size_t ld_lm;           // complex elements per level
+    const double *rad;      // device: or2[nlev], or4[nlev], epsc[nlev]
+    const int *lcut;        // device: l_R per level
+    double CorFac;
+    int nlev;
+    // filled by the launcher
+    const int *lm2l, *lm2m;
+    int l_max, lm_max;
+};
+
 // ------------------------------------------------------------------ plan
 struct Plan {
     int l_max, m_max, minc, n_theta, n_phi, nh, n_m, lm_max, precision;
@@ -158,6 +172,12 @@ struct Plan {
     // 16x16x32 MFMA operand layout, with power-of-2 row factors
     int nq32 = 0;                   // theta chunks of 32 (northern half)
     size_t syn16_size = 0, ana16_size = 0;   // halves
+    // per-level factors of mhx_get_td (or2, or4, epsc; l_R)
+    double *d_td_rad = nullptr;
+    int *d_td_lcut = nullptr;
+    int td_nlev = 0;
+    std::vector<double> h_td;
+    std::vector<int> h_td_lcut;
     // recurrence-generated synthesis tables (rec): FP64 seeds P_mm(theta) x
     // row scale as (mantissa, power-of-two exponent) per (mc, theta_h)
     bool rec = false;
@@ -192,6 +212,7 @@ struct Plan {
     double *ws_C = nullptr;     size_t ws_C_bytes = 0;    // staged coefficients
     double *ws_H = nullptr;     size_t ws_H_bytes = 0;    // nl step host arrays, staged
     int *d_lm2l = nullptr;      // lm_max
+    int *d_lm2m = nullptr;      // lm_max
     int *d_lcut = nullptr;      int d_lcut_n = 0;
     double *d_rad = nullptr;    int d_rad_n = 0;          // or2, or4, orho1
     std::vector<double> h_rad;  // host copy of the per-level radial factors
@@ -227,6 +248,7 @@ struct Plan {
 int build_tables(Plan *p);
 int build_split_tables(Plan *p);
 int launch_nsyn16(const Plan *p, int kind, const NSynArgs &args, hipStream_t s);
+int launch_get_td(const Plan *p, const TdArgs &args, hipStream_t s);
 int launch_nana16(const Plan *p, int kind, const NAnaArgs &args, hipStream_t s);
 int launch_syn(const Plan *p, const SynJob &job, hipStream_t s);
 int launch_ana(const Plan *p, const AnaJob &job, hipStream_t s);

@@ -381,11 +381,11 @@ int mhx_plan_create(int l_max, int m_max, int minc, int n_theta, int n_phi, int 
     p->M = n_phi / 2;
     p->fac = fac;
     p->lstart.resize(n_m);
-    std::vector<int> lm2l;
+    std::vector<int> lm2l, lm2m;
     int lm = 0;
     for (int mc = 0; mc < n_m; ++mc) {
         p->lstart[mc] = lm;
-        for (int l = mc * minc; l <= l_max; ++l) lm2l.push_back(l), ++lm;
+        for (int l = mc * minc; l <= l_max; ++l) lm2l.push_back(l), lm2m.push_back(mc * minc), ++lm;
     }
     p->lm_max = lm;
 
@@ -421,7 +421,7 @@ int mhx_plan_create(int l_max, int m_max, int minc, int n_theta, int n_phi, int 
     if ((rc = up(p->d_lstart, p->lstart)) || (rc = up(p->d_theta, p->theta_ord)) ||
         (rc = up(p->d_gauss, p->gauss)) || (rc = up(p->d_osin2, osin2)) ||
         (rc = up(p->d_tw, tw)) || (rc = up(p->d_twr, twr)) || (rc = up(d_lm2l, lm2l)) ||
-        (rc = up(p->d_tw32, tw32)) || (rc = up(p->d_twr32, twr32))) {
+        (rc = up(p->d_tw32, tw32)) || (rc = up(p->d_twr32, twr32)) || (rc = up(p->d_lm2m, lm2m))) {
         mhx_plan_destroy(h);
         return rc;
     }
@@ -471,6 +471,7 @@ int mhx_plan_destroy(mhx_plan *h)
     free_dev(p->d_tw);
     free_dev(p->d_twr);
     free_dev(p->d_lm2l);
+    free_dev(p->d_lm2m);
     free_dev(p->d_tw32);
     free_dev(p->d_twr32);
     free_dev(p->d_syn16_P);
@@ -482,6 +483,8 @@ int mhx_plan_destroy(mhx_plan *h)
     free_dev(p->d_ana16_soff);
     free_dev(p->d_syn16_rs);
     free_dev(p->d_rec_seed);
+    free_dev(p->d_td_rad);
+    free_dev(p->d_td_lcut);
     free_dev(p->d_rec_aseed);
     free_dev(p->d_rec_aexp);
     free_dev(p->d_rec_segoff);
@@ -1137,6 +1140,84 @@ int mhx_nl_step(mhx_plan *h, int nlev, const double *const *in, size_t ld_lm, do
     int rc = nl_step_impl(h, nlev, in, ld_lm, out, or2, or4, orho1, LFfac, lcut, nullptr, stream);
     if (rc) return rc;
     return finish_step(h, stream);
+}
+
+int mhx_get_td(mhx_plan *h, int nlev, const double *const *in, size_t ld_lm, double *const *out,
+               const double *or2, const double *or4, const double *epsc, double CorFac,
+               const int *lcut, void *stream)
+{
+    if (!h) return fail(MHX_E_INVALID, "null plan");
+    Plan *P = &h->p;
+    if (nlev <= 0) return MHX_OK;
+    if (!in || !out || !or2 || !or4 || !epsc || !lcut) return fail(MHX_E_INVALID, "null pointer");
+    for (int i = 0; i < 10; ++i)
+        if (!in[i]) return fail(MHX_E_INVALID, "null input array " + std::to_string(i));
+    for (int i = 0; i < 7; ++i)
+        if (!out[i]) return fail(MHX_E_INVALID, "null output array " + std::to_string(i));
+    if (ld_lm < (size_t)P->lm_max) return fail(MHX_E_INVALID, "ld_lm below lm_max");
+    int rc;
+    for (int i = 0; i < nlev; ++i)
+        if ((rc = check_lcut(P, lcut[i]))) return rc;
+    if ((rc = order_after(P, stream))) return rc;
+    // host arrays staged through ws_H (inputs in, outputs back after the launch)
+    const size_t arr = (size_t)nlev * ld_lm * 2;
+    bool hin[10], hout[7];
+    int nh = 0;
+    for (int i = 0; i < 10; ++i) nh += (hin[i] = !is_device_ptr(in[i]));
+    for (int i = 0; i < 7; ++i) nh += (hout[i] = !is_device_ptr(out[i]));
+    if (nh && (rc = ensure_buf(P->ws_H, P->ws_H_bytes, (size_t)nh * arr * sizeof(double)))) return rc;
+    TdArgs a;
+    std::memset(&a, 0, sizeof(a));
+    int slot = 0;
+    for (int i = 0; i < 10; ++i) {
+        a.in[i] = in[i];
+        if (!hin[i]) continue;
+        double *d = P->ws_H + (size_t)slot++ * arr;
+        MHX_HIP_CHECK(hipMemcpyAsync(d, in[i], arr * sizeof(double), hipMemcpyHostToDevice, P->stream));
+        a.in[i] = d;
+    }
+    for (int i = 0; i < 7; ++i) a.out[i] = hout[i] ? P->ws_H + (size_t)slot++ * arr : out[i];
+    // per-level factors
+    if (P->td_nlev < nlev) {
+        free_dev(P->d_td_rad);
+        free_dev(P->d_td_lcut);
+        P->td_nlev = 0;
+        MHX_HIP_CHECK(hipMalloc(&P->d_td_rad, 3 * (size_t)nlev * sizeof(double)));
+        MHX_HIP_CHECK(hipMalloc(&P->d_td_lcut, (size_t)nlev * sizeof(int)));
+        P->td_nlev = nlev;
+    }
+    // the host copy stays valid until the copies are done (synchronised
+    // below, or by the next call's wait on the plan stream)
+    MHX_HIP_CHECK(hipStreamSynchronize(P->stream));
+    P->h_td.assign(3 * (size_t)nlev, 0.0);
+    for (int i = 0; i < nlev; ++i) {
+        P->h_td[i] = or2[i];
+        P->h_td[nlev + i] = or4[i];
+        P->h_td[2 * nlev + i] = epsc[i];
+    }
+    MHX_HIP_CHECK(hipMemcpyAsync(P->d_td_rad, P->h_td.data(), 3 * (size_t)nlev * sizeof(double),
+                                 hipMemcpyHostToDevice, P->stream));
+    P->h_td_lcut.assign(lcut, lcut + nlev);
+    MHX_HIP_CHECK(hipMemcpyAsync(P->d_td_lcut, P->h_td_lcut.data(), (size_t)nlev * sizeof(int),
+                                 hipMemcpyHostToDevice, P->stream));
+    a.ld_lm = ld_lm;
+    a.rad = P->d_td_rad;
+    a.lcut = P->d_td_lcut;
+    a.CorFac = CorFac;
+    a.nlev = nlev;
+    if ((rc = launch_get_td(P, a, P->stream))) return rc;
+    for (int i = 0; i < 7; ++i)
+        if (hout[i])
+            MHX_HIP_CHECK(hipMemcpyAsync(out[i], a.out[i], arr * sizeof(double), hipMemcpyDeviceToHost,
+                                         P->stream));
+    P->last_host_out = nh > 0;
+    if (stream && nh == 0) {
+        MHX_HIP_CHECK(hipEventRecord(P->ev_done, P->stream));
+        MHX_HIP_CHECK(hipStreamWaitEvent((hipStream_t)stream, P->ev_done, 0));
+        return MHX_OK;
+    }
+    MHX_HIP_CHECK(hipStreamSynchronize(P->stream));
+    return MHX_OK;
 }
 
 int mhx_nl_step_grids(mhx_plan *h, int nlev, const double *const *in, size_t ld_lm,

@@ -1,0 +1,123 @@
+// lm-space assembly of the explicit terms of the time step at interior
+// radial levels (src/get_td.f90): the step that follows the forward
+// transforms of the nonlinear terms, for a rotating Boussinesq MHD run
+// without the double-curl form:
+//   get_dwdt (:133-203)  dwdt = or2 AdvrLM + Coriolis (dPhi dw, dTheta2 z)
+//   get_dzdt (:379-444)  dzdt = dLh AdvpLM + Coriolis (dPhi z, dTheta3 dw,
+//                        dTheta4 w)
+//   get_dpdt (:314-377)  dpdt = -dLh AdvtLM + Coriolis (dPhi (dw + or1 dLh
+//                        w), dTheta3 z); l = 0 left at zero
+//   get_dsdt (:446-500)  dsdt = dLh VStLM, l = 0: epsc epscProf
+//   get_dbdt (:530-589)  dbdt = dLh VxBpLM, djdt = dLh or4 VxBrLM,
+//                        dVxBhLM = -dLh r^2 VxBtLM
+// with the operators of src/horizontal.f90:201-229 and the neighbours l -/+ 1
+// at the same m (src/blocking.f90:322-334); the Coriolis terms keep l < l_R,
+// the l = l_R terms drop the l + 1 neighbour, and vanish above l_R.
+// One thread per (lm, level), 256 consecutive lm per block: the l -/+ 1
+// neighbours of a coefficient are the adjacent entries of its row (m-major,
+// l fastest).  Memory bound: 10 complex in and 7 out per (lm, level).
+#include "mhx_internal.h"
+
+#include <cmath>
+
+namespace mhx {
+
+namespace {
+
+__device__ inline double2 cmulr(double a, double2 z) { return make_double2(a * z.x, a * z.y); }
+__device__ inline double2 cadd2(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub2(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// i m z
+__device__ inline double2 cim(double m, double2 z) { return make_double2(-m * z.y, m * z.x); }
+
+// sqrt((l+m)(l-m) / ((2l-1)(2l+1))), zero below l = m (src/horizontal.f90:201-205)
+__device__ inline double clm(int l, int m)
+{
+    if (l <= m) return 0.0;
+    const double dl = (double)l, dm = (double)m;
+    return sqrt((dl + dm) * (dl - dm) / ((2.0 * dl - 1.0) * (2.0 * dl + 1.0)));
+}
+
+__global__ __launch_bounds__(256) void k_get_td(TdArgs a)
+{
+    const int lm = blockIdx.x * blockDim.x + threadIdx.x, lev = blockIdx.y;
+    if (lm >= a.lm_max) return;
+    const int l = a.lm2l[lm], m = a.lm2m[lm], L = a.l_max;
+    const double dm = (double)m;
+    const size_t i = (size_t)lev * a.ld_lm + (size_t)lm;
+    const double or2 = a.rad[lev], or4 = a.rad[a.nlev + lev], epsc = a.rad[2 * a.nlev + lev];
+    const int lR = a.lcut[lev];
+    const double or1 = sqrt(or2), r2 = 1.0 / or2, c2 = 2.0 * a.CorFac;
+    const double2 *in[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) in[k] = reinterpret_cast<const double2 *>(a.in[k]);
+    const double2 *w = in[7], *dw = in[8], *z = in[9];
+    const double2 zero = make_double2(0.0, 0.0);
+    const double dl = (double)l, dLh = dl * (dl + 1.0);
+    const double cS = clm(l, m), cA = clm(l + 1, m);
+    const double t2S = (dl - 1.0) * cS, t2A = (dl + 2.0) * cA;
+    const double t3S = (dl - 1.0) * (dl + 1.0) * cS, t3A = dl * (dl + 2.0) * cA;
+    const double t4S = (dl + 1.0) * cS * ((dl - 1.0) * dl), t4A = dl * cA * ((dl + 1.0) * (dl + 2.0));
+    // neighbours: l - 1 exists for l > m (its factors vanish at l = m),
+    // l + 1 for l < l_max
+    const size_t iS = l > m ? i - 1 : i;
+    const bool hasA = l < L;
+    const double2 zA = hasA ? z[i + 1] : zero, dwA = hasA ? dw[i + 1] : zero;
+    const double2 wA = hasA ? w[i + 1] : zero;
+    const double2 zS = z[iS], dwS = dw[iS], wS = w[iS];
+    const double2 zl = z[i], dwl = dw[i], wl = w[i];
+    double2 dwdt, dzdt, dpdt, dsdt;
+    if (l == 0) {
+        // lm = 1 of the reference (m = 0 only)
+        dwdt = cadd2(cmulr(or2, in[0][i]), cmulr(c2 * or1 * t2A, zA));
+        dzdt = cmulr(c2 * or2, cadd2(cmulr(t3A, dwA), cmulr(or1 * t4A, wA)));
+        dpdt = zero;
+        dsdt = make_double2(epsc, 0.0);
+    } else {
+        double2 cw = zero, cz = zero, cp = zero;
+        if (l <= lR) {
+            const bool below = l < lR;  // the l + 1 neighbour enters below l_R only
+            cw = csub2(cim(dm, dwl), cmulr(t2S, zS));
+            if (below) cw = cadd2(cw, cmulr(t2A, zA));
+            cw = cmulr(c2 * or1, cw);
+            cz = csub2(cadd2(cim(dm, zl), cmulr(t3S, dwS)), cmulr(or1 * t4S, wS));
+            if (below) cz = cadd2(cz, cadd2(cmulr(t3A, dwA), cmulr(or1 * t4A, wA)));
+            cz = cmulr(c2 * or2, cz);
+            cp = cadd2(cmulr(-1.0, cim(dm, cadd2(dwl, cmulr(or1 * dLh, wl)))), cmulr(t3S, zS));
+            if (below) cp = cadd2(cp, cmulr(t3A, zA));
+            cp = cmulr(c2 * or2, cp);
+        }
+        dwdt = cadd2(cmulr(or2, in[0][i]), cw);
+        dzdt = cadd2(cmulr(dLh, in[2][i]), cz);
+        dpdt = cadd2(cmulr(-dLh, in[1][i]), cp);
+        dsdt = cmulr(dLh, in[3][i]);
+    }
+    double2 *o[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) o[k] = reinterpret_cast<double2 *>(a.out[k]);
+    o[0][i] = dwdt;
+    o[1][i] = dzdt;
+    o[2][i] = dpdt;
+    o[3][i] = dsdt;
+    o[4][i] = cmulr(dLh, in[6][i]);
+    o[5][i] = cmulr(dLh * or4, in[4][i]);
+    o[6][i] = cmulr(-dLh * r2, in[5][i]);
+}
+
+}  // namespace
+
+int launch_get_td(const Plan *p, const TdArgs &args, hipStream_t s)
+{
+    if (args.nlev <= 0) return MHX_OK;
+    TdArgs a = args;
+    a.lm2l = p->d_lm2l;
+    a.lm2m = p->d_lm2m;
+    a.l_max = p->l_max;
+    a.lm_max = p->lm_max;
+    dim3 grid((unsigned)((p->lm_max + 255) / 256), (unsigned)a.nlev);
+    k_get_td<<<grid, 256, 0, s>>>(a);
+    MHX_HIP_CHECK(hipGetLastError());
+    return MHX_OK;
+}
+
+}  // namespace mhx

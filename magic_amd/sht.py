@@ -52,6 +52,7 @@ _SIGS = {
     "mhx_spat_to_qst_batch": "Pipppzpppzpp",
     "mhx_nl_step": "Pipzppppdpp",
     "mhx_nl_step_grids": "Pipzppppdppp",
+    "mhx_get_td": "Pipzppppdpp",
     "mhx_set_timing": "Pi",
     "mhx_stage_times": "Pp",
     "mhx_launch_times": "Ppi",
@@ -331,6 +332,28 @@ def nl_step(plan, inputs, outputs, or2, or4, orho1, LFfac, lcut, grids=None, str
     else:
         _check(L.mhx_nl_step_grids(plan.h, nlev, ins, plan.lm_max, outs, d(or2), d(or4),
                                    d(orho1), float(LFfac), lc, _ptr(grids), st))
+
+
+TD_INPUTS = ("AdvrLM", "AdvtLM", "AdvpLM", "VStLM", "VxBrLM", "VxBtLM", "VxBpLM", "w", "dw", "z")
+TD_OUTPUTS = ("dwdt", "dzdt", "dpdt", "dsdt", "dbdt", "djdt", "dVxBhLM")
+
+
+def get_td(plan, inputs, outputs, or2, or4, epsc, CorFac, lcut, stream=None):
+    """lm-space assembly of the explicit terms at interior levels
+    (src/get_td.f90 get_dwdt, get_dzdt, get_dpdt, get_dsdt, get_dbdt; rotating
+    Boussinesq MHD, no double curl) for all levels at once.  ``inputs``: 10
+    complex128 tensors (nlev, lm_max) in TD_INPUTS order (host or device);
+    ``outputs``: 7 such tensors (TD_OUTPUTS order), written in place; per level
+    ``or2``, ``or4``, ``epsc`` (epsc times epscProf) and ``lcut`` (l_R)."""
+    L = plan._L
+    nlev = inputs[0].shape[0]
+    ins = (ctypes.c_void_p * 10)(*[t.data_ptr() for t in inputs])
+    outs = (ctypes.c_void_p * 7)(*[t.data_ptr() for t in outputs])
+    d = lambda v: (ctypes.c_double * nlev)(*[float(x) for x in v])  # noqa: E731
+    lc = (ctypes.c_int * nlev)(*[int(x) for x in lcut])
+    st = ctypes.c_void_p(stream.cuda_stream) if stream is not None else None
+    _check(L.mhx_get_td(plan.h, nlev, ins, plan.lm_max, outs, d(or2), d(or4), d(epsc),
+                        float(CorFac), lc, st))
 
 
 JOB_SCALAR, JOB_VECTOR = 0, 1

@@ -659,3 +659,70 @@ void orc_get_nl(const orc_plan *p, double or2, double or4, double orho1, double 
         out[8][i] = orho1 * or4 * (vr[i] * bt[i] - vt[i] * br[i]);
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* lm-space assembly of the explicit terms at an interior level (nBc = 0)
+   of a rotating Boussinesq MHD run without the double-curl form:
+   src/get_td.f90 get_dwdt (:133-203), get_dzdt (:379-444), get_dpdt
+   (:314-377), get_dsdt (:446-500, epscProf = 1), get_dbdt (:530-589);
+   operators of src/horizontal.f90:201-229; neighbours l -/+ 1 at the same m
+   (src/blocking.f90:322-334).  or1 = sqrt(or2), r = 1/sqrt(or2). */
+static double td_clm(int l, int m)
+{
+    if (l < m || l == 0) return 0.0;
+    return sqrt((double)(l + m) * (double)(l - m) / ((double)(2 * l - 1) * (double)(2 * l + 1)));
+}
+
+void orc_get_td(const orc_plan *p, int lR, double or2, double or4, double CorFac, double epsc,
+                const double *const *in, double *const *out)
+{
+    const cplx *Advr = (const cplx *)in[0], *Advt = (const cplx *)in[1], *Advp = (const cplx *)in[2];
+    const cplx *VSt = (const cplx *)in[3], *VxBr = (const cplx *)in[4], *VxBt = (const cplx *)in[5];
+    const cplx *VxBp = (const cplx *)in[6], *w = (const cplx *)in[7], *dw = (const cplx *)in[8];
+    const cplx *z = (const cplx *)in[9];
+    cplx *dwdt = (cplx *)out[0], *dzdt = (cplx *)out[1], *dpdt = (cplx *)out[2];
+    cplx *dsdt = (cplx *)out[3], *dbdt = (cplx *)out[4], *djdt = (cplx *)out[5];
+    cplx *dVxBh = (cplx *)out[6];
+    const double or1 = sqrt(or2), r = 1.0 / sqrt(or2), c2 = 2.0 * CorFac;
+    for (int lm = 0; lm < p->lm_max; ++lm) {
+        const int l = p->lm2l[lm], m = p->lm2m[lm];
+        const double dl = (double)l;
+        const double dLh = dl * (dl + 1.0);
+        const cplx dPhi = I * (double)m;
+        const double cS = td_clm(l, m), cA = td_clm(l + 1, m);
+        const double t2S = (dl - 1.0) * cS, t2A = (dl + 2.0) * cA;
+        const double t3S = (dl - 1.0) * (dl + 1.0) * cS, t3A = dl * (dl + 2.0) * cA;
+        const double t4S = (dl + 1.0) * cS * (dl - 1.0) * dl, t4A = dl * cA * (dl + 1.0) * (dl + 2.0);
+        /* neighbours; the l = m "S" neighbour is multiplied by zero */
+        const int lmS = (l > m) ? lm - 1 : lm;
+        const int lmA = (l < p->l_max) ? lm + 1 : -1;
+        const cplx zA = lmA >= 0 ? z[lmA] : 0.0, dwA = lmA >= 0 ? dw[lmA] : 0.0;
+        const cplx wA = lmA >= 0 ? w[lmA] : 0.0;
+        if (lm == 0) {
+            /* l = 0, m = 0 */
+            dwdt[0] = or2 * Advr[0] + c2 * or1 * t2A * zA;
+            dzdt[0] = c2 * or2 * (t3A * dwA + or1 * t4A * wA);
+            dpdt[0] = 0.0; /* get_dpdt starts at lm_min = 2 */
+            dsdt[0] = epsc;
+        } else {
+            cplx cw = 0.0, cz = 0.0, cp_ = 0.0;
+            if (l < lR) {
+                cw = c2 * or1 * (dPhi * dw[lm] + t2A * zA - t2S * z[lmS]);
+                cz = c2 * or2 * (dPhi * z[lm] + t3A * dwA + or1 * t4A * wA + t3S * dw[lmS] -
+                                 or1 * t4S * w[lmS]);
+                cp_ = c2 * or2 * (-dPhi * (dw[lm] + or1 * dLh * w[lm]) + t3A * zA + t3S * z[lmS]);
+            } else if (l == lR) {
+                cw = c2 * or1 * (dPhi * dw[lm] - t2S * z[lmS]);
+                cz = c2 * or2 * (dPhi * z[lm] + t3S * dw[lmS] - or1 * t4S * w[lmS]);
+                cp_ = c2 * or2 * (-dPhi * (dw[lm] + or1 * dLh * w[lm]) + t3S * z[lmS]);
+            }
+            dwdt[lm] = or2 * Advr[lm] + cw;
+            dzdt[lm] = dLh * Advp[lm] + cz;
+            dpdt[lm] = -dLh * Advt[lm] + cp_;
+            dsdt[lm] = dLh * VSt[lm];
+        }
+        dbdt[lm] = dLh * VxBp[lm];
+        dVxBh[lm] = -dLh * VxBt[lm] * r * r;
+        djdt[lm] = dLh * or4 * VxBr[lm];
+    }
+}

@@ -105,6 +105,12 @@ void orc_spat_to_sphertor(const orc_plan *p, const double *f, const double *g,
 void orc_get_nl(const orc_plan *p, double or2, double or4, double orho1,
                 double LFfac, const double *const *in, double *const *out);
 
+/* src/get_td.f90 (interior level, rotating Boussinesq MHD, no double curl):
+   in[10] = AdvrLM AdvtLM AdvpLM VStLM VxBrLM VxBtLM VxBpLM w dw z, out[7] =
+   dwdt dzdt dpdt dsdt dbdt djdt dVxBhLM; complex lm_max arrays. */
+void orc_get_td(const orc_plan *p, int lR, double or2, double or4, double CorFac,
+                double epsc, const double *const *in, double *const *out);
+
 #ifdef __cplusplus
 }
 #endif

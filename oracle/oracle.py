@@ -48,6 +48,7 @@ def lib():
         for name in ("orc_fft_forward", "orc_fft_inverse"):
             getattr(L, name).argtypes = [P, dp, dp]
         L.orc_get_nl.argtypes = [P, D, D, D, D, ctypes.POINTER(dp), ctypes.POINTER(dp)]
+        L.orc_get_td.argtypes = [P, I, D, D, D, D, ctypes.POINTER(dp), ctypes.POINTER(dp)]
         _lib = L
     return _lib
 
@@ -247,6 +248,19 @@ class Plan:
         arr_out = (ctypes.POINTER(ctypes.c_double) * 12)(*[_ptr(a) for a in outs])
         self._L.orc_get_nl(ctypes.c_void_p(self.h), float(or2), float(or4), float(orho1),
                            float(LFfac), arr_in, arr_out)
+        return outs
+
+
+    def get_td(self, lR, or2, or4, CorFac, epsc, arrays):
+        """arrays: 10 complex lm arrays (AdvrLM AdvtLM AdvpLM VStLM VxBrLM
+        VxBtLM VxBpLM w dw z).  Returns dwdt dzdt dpdt dsdt dbdt djdt
+        dVxBhLM (src/get_td.f90, interior level)."""
+        ins = [self._c(a) for a in arrays]
+        outs = [self.coef() for _ in range(7)]
+        arr_in = (ctypes.POINTER(ctypes.c_double) * 10)(*[_ptr(a) for a in ins])
+        arr_out = (ctypes.POINTER(ctypes.c_double) * 7)(*[_ptr(a) for a in outs])
+        self._L.orc_get_td(ctypes.c_void_p(self.h), int(lR), float(or2), float(or4),
+                           float(CorFac), float(epsc), arr_in, arr_out)
         return outs
 
 

@@ -7,8 +7,11 @@
 ! produces golden vectors and the CPU timing baseline.  None of the product
 ! code calls it.
 !
+! It also drives the reference's lm-space assembly of the explicit terms
+! (src/get_td.f90, module nonlinear_lm_mod) in mode td.
+!
 ! Usage:  ref_harness MODE l_max minc nalias infile outfile [nlev nrep]
-!   MODE = tables | api | nl | nlstep | bench
+!   MODE = tables | api | nl | nlstep | bench | td
 !
 ! File formats are little-endian streams: int32 header words followed by
 ! float64 arrays in Fortran (column-major) order.
@@ -24,8 +27,10 @@ program ref_harness
    use blocking, only: initialize_blocking, st_map
    use horizontal_data
    use radial_functions, only: or1, or2, or3, or4, orho1, beta, r, otemp1, &
-       &                       visc, lambda
-   use physical_parameters, only: LFfac, n_r_LCR
+       &                       visc, lambda, epscProf, l_R
+   use physical_parameters, only: LFfac, n_r_LCR, CorFac, epsc
+   use fields, only: w_Rloc, dw_Rloc, ddw_Rloc, z_Rloc, dz_Rloc
+   use nonlinear_lm_mod, only: nonlinear_lm_t
    use sht
    use shtransforms, only: Plm, dPlm, wPlm, wdPlm
    use grid_space_arrays_mod, only: grid_space_arrays_t
@@ -87,6 +92,8 @@ program ref_harness
       call do_bench()
    case ('nlstep')
       call do_nlstep()
+   case ('td')
+      call do_td()
    case default
       write(output_unit,*) 'unknown mode ', trim(mode)
       stop 2
@@ -287,6 +294,55 @@ contains
       close(u)
       close(v)
    end subroutine do_nlstep
+
+   subroutine do_td()
+      !-- reads nlev levels; per level: lcut, or2, or4, orho1, CorFac, epsc
+      !   and 10 coefficient arrays (AdvrLM AdvtLM AdvpLM VStLM VxBrLM VxBtLM
+      !   VxBpLM w dw z); runs get_dwdt, get_dzdt, get_dpdt, get_dsdt and
+      !   get_dbdt of an interior level (nBc = 0) of a rotating Boussinesq
+      !   MHD run (no double curl) and writes dwdt dzdt dpdt dsdt dbdt djdt
+      !   dVxBhLM
+      type(nonlinear_lm_t) :: td
+      complex(cp), allocatable :: O(:,:), dVSrLM(:)
+      integer :: u, v, lev, i
+      integer(int32) :: lcut32
+      real(cp) :: o2, o4, orh, cf, ep
+
+      l_conv=.true.; l_corr=.true.; l_heat=.true.; l_mag_kin=.false.
+      l_single_matrix=.false.; l_double_curl=.false.
+      call td%initialize(lm_max)
+      allocate( O(lm_max,7), dVSrLM(lm_max) )
+      allocate( w_Rloc(lm_max,1), dw_Rloc(lm_max,1), ddw_Rloc(lm_max,1) )
+      allocate( z_Rloc(lm_max,1), dz_Rloc(lm_max,1) )
+      if ( .not. allocated(l_R) ) allocate( l_R(1), epscProf(1) )
+      ddw_Rloc(:,1)=0.0_cp; dz_Rloc(:,1)=0.0_cp
+      open(newunit=u, file=trim(infile), access='stream', form='unformatted', &
+      &    status='old')
+      open(newunit=v, file=trim(outfile), access='stream', form='unformatted', &
+      &    status='replace')
+      call write_header(v)
+      do lev=1,nlev
+         read(u) lcut32, o2, o4, orh, cf, ep
+         call set_radial(o2, o4, orh)
+         l_R(1)=int(lcut32); CorFac=cf; epsc=ep; epscProf(1)=1.0_cp
+         read(u) td%AdvrLM(1:lm_max), td%AdvtLM(1:lm_max), td%AdvpLM(1:lm_max)
+         read(u) td%VStLM(1:lm_max), td%VxBrLM(1:lm_max), td%VxBtLM(1:lm_max)
+         read(u) td%VxBpLM(1:lm_max)
+         read(u) w_Rloc(:,1), dw_Rloc(:,1), z_Rloc(:,1)
+         !-- get_dpdt leaves l = 0 unset (its loop starts at lm_min = 2)
+         O(:,:)=0.0_cp
+         call td%get_dwdt(1, 0, O(:,1))
+         call td%get_dzdt(1, 0, O(:,2))
+         call td%get_dpdt(1, 0, O(:,3))
+         call td%get_dsdt(1, 0, O(:,4), dVSrLM)
+         call td%get_dbdt(1, 0, O(:,5), O(:,6), O(:,7))
+         do i=1,7
+            write(v) O(:,i)
+         end do
+      end do
+      close(u)
+      close(v)
+   end subroutine do_td
 
    subroutine do_bench()
       !-- Times the per-level transform sequence of a Boussinesq MHD bulk

@@ -147,6 +147,55 @@ def nlstep_case(tag, l_max, nlev, write=True):
     return data
 
 
+TD_IN = ("AdvrLM", "AdvtLM", "AdvpLM", "VStLM", "VxBrLM", "VxBtLM", "VxBpLM", "w", "dw", "z")
+TD_OUT = ("dwdt", "dzdt", "dpdt", "dsdt", "dbdt", "djdt", "dVxBhLM")
+
+
+def td_inputs(l_max, nlev, config=21):
+    """Seeded inputs of the lm-space assembly: the seven nonlinear outputs it
+    reads and the fields w, dw/dr, z at each level (complex, (l+1)^-2)."""
+    return np.stack([np.stack([workload.coefficients(l_max, config, k, lev, "l+1^-2", potential=False)
+                               for lev in range(nlev)]) for k in range(len(TD_IN))])
+
+
+def td_radial(nlev):
+    r = np.linspace(0.6, 1.4, nlev)
+    return 1.0 / r ** 2, 1.0 / r ** 4, 1.0 / (1.0 + 0.1 * r), 250.0, 0.7
+
+
+def td_case(tag, l_max, nlev, write=True):
+    """The reference's get_td (ref_harness td mode): dwdt, dzdt, dpdt, dsdt,
+    dbdt, djdt, dVxBhLM of interior levels, one with a lower l_R."""
+    X = td_inputs(l_max, nlev)
+    or2, or4, orho1, corfac, epsc = td_radial(nlev)
+    lcut = [l_max] * nlev
+    lcut[-1] = l_max - 4
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            for i in range(nlev):
+                np.array([lcut[i]], dtype="<i4").tofile(f)
+                np.array([or2[i], or4[i], orho1[i], corfac, epsc], dtype="<f8").tofile(f)
+                for k in range(len(TD_IN)):
+                    np.ascontiguousarray(X[k, i], dtype="<c16").tofile(f)
+        oracle.run_ref("td", l_max, 1, 20, fin, fout, nlev=nlev, threads=1)
+        with open(fout, "rb") as f:
+            hd = read_header(f)
+            lm = hd["lm_max"]
+            O = np.zeros((len(TD_OUT), nlev, lm), dtype=np.complex128)
+            for i in range(nlev):
+                for k in range(len(TD_OUT)):
+                    O[k, i] = np.fromfile(f, "<c16", lm)
+            assert f.read() == b""
+    data = dict(l_max=l_max, nlev=nlev, lcut=np.array(lcut), inputs=X, outs=O,
+                or2=or2, or4=or4, orho1=orho1, corfac=corfac, epsc=epsc)
+    if write:
+        path = os.path.join(OUT, "td_%s.npz" % tag)
+        np.savez_compressed(path, **data)
+        print("wrote", path, os.path.getsize(path))
+    return data
+
+
 def main():
     if not oracle.have_ref():
         sys.exit("oracle/_ref/ref_harness is not built (make -C oracle)")
@@ -159,6 +208,8 @@ def main():
     api_case("L64_m4", 64, 4, [64, 40], config=13)
     nl_case("L16", 16)
     nlstep_case("L16", 16, 3)
+    td_case("L16", 16, 3)
+    td_case("L64", 64, 2)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,77 @@
+"""lm-space assembly of the explicit terms (mhx_get_td, src/get_td.f90) on
+the GPU: against the reference's own nonlinear_lm_t (golden vectors written
+by ref_harness td) and against the oracle at l_max=255 x 97 levels, with
+host and device arrays and on a caller's stream."""
+import numpy as np
+import pytest
+
+from conftest import load_golden, rel_l2
+from magic_amd import sht, workload
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-13
+
+
+def _run(p, X, or2, or4, epsc, corfac, lcut, device=True, stream=None):
+    import torch
+    dev = torch.device("cuda") if device else torch.device("cpu")
+    nlev = X.shape[1]
+    tin = [torch.from_numpy(np.ascontiguousarray(X[k])).to(dev) for k in range(10)]
+    tout = [torch.zeros((nlev, p.lm_max), dtype=torch.complex128, device=dev) for _ in range(7)]
+    if stream is not None:
+        stream.wait_stream(torch.cuda.current_stream())  # the tensors were made there
+    sht.get_td(p, tin, tout, or2, or4, epsc, corfac, lcut, stream=stream)
+    if stream is not None:
+        torch.cuda.current_stream().wait_stream(stream)
+    torch.cuda.synchronize()
+    return np.stack([t.cpu().numpy() for t in tout])
+
+
+@pytest.mark.parametrize("fname", ["td_L16.npz", "td_L64.npz"])
+def test_get_td_matches_reference_golden(fname):
+    g = load_golden(fname)
+    L, nlev = int(g["l_max"]), int(g["nlev"])
+    print("get_td golden", fname, flush=True)
+    nt, nph = workload.grid_size(L)
+    p = sht.Plan(L, nt, nph, max_levels=nlev)
+    O = _run(p, g["inputs"], g["or2"], g["or4"], [float(g["epsc"])] * nlev, float(g["corfac"]),
+             list(g["lcut"]))
+    p.close()
+    for k in range(7):
+        for lev in range(nlev):
+            e = rel_l2(O[k, lev], g["outs"][k, lev])
+            assert e < TOL, (sht.TD_OUTPUTS[k], lev, e)
+
+
+def test_get_td_matches_oracle_full_size():
+    """l_max=255, 97 levels, l_R varying by level; host arrays give the same
+    numbers as device arrays, and so does a call on a caller's stream"""
+    import torch
+    from oracle import oracle
+    L, nlev = 255, 97
+    nt, nph = workload.grid_size(L)
+    rng = np.random.default_rng(5)
+    ls, _ = workload.lm_index(L)
+    X = (rng.standard_normal((10, nlev, ls.size)) + 1j * rng.standard_normal((10, nlev, ls.size)))
+    X = X / (ls + 1.0) ** 2
+    r = np.linspace(0.55, 1.45, nlev)
+    or2, or4 = 1.0 / r ** 2, 1.0 / r ** 4
+    epsc = 0.3 + 0.01 * np.arange(nlev)
+    corfac = 1.0e3
+    lcut = [L - (lev % 7) * 9 for lev in range(nlev)]
+    print("get_td full size", flush=True)
+    p = sht.Plan(L, nt, nph, max_levels=nlev)
+    O = _run(p, X, or2, or4, epsc, corfac, lcut)
+    o = oracle.Plan(L, nt, nph)
+    for lev in (0, 1, 5, 48, 96):
+        ref = o.get_td(lcut[lev], or2[lev], or4[lev], corfac, epsc[lev], [X[k, lev] for k in range(10)])
+        for k in range(7):
+            e = rel_l2(O[k, lev], ref[k])
+            assert e < TOL, (sht.TD_OUTPUTS[k], lev, e)
+    Oh = _run(p, X, or2, or4, epsc, corfac, lcut, device=False)
+    assert np.array_equal(Oh, O)
+    st = torch.cuda.Stream()
+    Os = _run(p, X, or2, or4, epsc, corfac, lcut, stream=st)
+    assert np.array_equal(Os, O)
+    p.close()

@@ -70,6 +70,53 @@ def test_oracle_nl_step_matches_reference():
 
 
 # ---------------------------------------------------------- independent
+@pytest.mark.parametrize("fname", ["td_L16.npz", "td_L64.npz"])
+def test_oracle_get_td_matches_reference(fname):
+    """lm-space assembly (src/get_td.f90) against the reference's own
+    nonlinear_lm_t run by ref_harness td, one level with l_R < l_max"""
+    g = load_golden(fname)
+    L = int(g["l_max"])
+    nt, nph = workload.grid_size(L)
+    o = oracle.Plan(L, nt, nph)
+    for lev in range(int(g["nlev"])):
+        outs = o.get_td(int(g["lcut"][lev]), g["or2"][lev], g["or4"][lev], float(g["corfac"]),
+                        float(g["epsc"]), [g["inputs"][k, lev] for k in range(10)])
+        for k in range(7):
+            assert rel_l2(outs[k], g["outs"][k, lev]) < 1e-14, (fname, lev, k)
+
+
+def test_get_td_coriolis_identities():
+    """closed forms of the Coriolis terms: a solid-body rotation z_10 gives
+    no Coriolis force on itself except through dTheta2A of l = 0; with all
+    nonlinear terms zero and w = dw = 0, dpdt and dwdt at (l, m) come from
+    z at l -/+ 1 only, with dTheta factors (l-1) c_lm and (l+2) c_{l+1,m}"""
+    L = 16
+    nt, nph = workload.grid_size(L)
+    o = oracle.Plan(L, nt, nph)
+    ls, ms = workload.lm_index(L)
+    zero = np.zeros(ls.size, dtype=np.complex128)
+    z = zero.copy()
+    i10 = np.flatnonzero((ls == 1) & (ms == 0))[0]
+    z[i10] = 1.0
+    or2 = 0.81
+    outs = o.get_td(L, or2, 1.0, 0.5, 0.0, [zero] * 7 + [zero, zero, z])
+    dwdt, dzdt, dpdt = outs[0], outs[1], outs[2]
+    c = lambda l, m: np.sqrt((l + m) * (l - m) / ((2 * l - 1.0) * (2 * l + 1.0)))  # noqa: E731
+    # dwdt(l=0) = 2 CorFac or1 dTheta2A(0) z_10, dTheta2A(0) = 2 c_10
+    assert abs(dwdt[0] - 2 * 0.5 * np.sqrt(or2) * 2 * c(1, 0)) < 1e-15
+    # dwdt(l=2, m=0) = -2 CorFac or1 dTheta2S z_10 with dTheta2S = (2-1) c_20
+    i20 = np.flatnonzero((ls == 2) & (ms == 0))[0]
+    assert abs(dwdt[i20] + 2 * 0.5 * np.sqrt(or2) * 1 * c(2, 0)) < 1e-15
+    # dpdt(l=2, m=0) = 2 CorFac or2 dTheta3S z_10, dTheta3S = (2-1)(2+1) c_20
+    assert abs(dpdt[i20] - 2 * 0.5 * or2 * 3 * c(2, 0)) < 1e-15
+    # z_10 itself: the toroidal Coriolis term is i m z = 0 at m = 0
+    assert abs(dzdt[i10]) < 1e-15
+    # every (l, m) other than l = 0, 2 at m = 0 is untouched
+    others = np.ones(ls.size, bool)
+    others[[0, i20]] = False
+    assert np.abs(dwdt[others]).max() == 0.0
+
+
 def test_gauleg_vs_numpy():
     for n in (24, 96, 384):
         th, w = oracle.gauleg(n)
