@@ -228,6 +228,24 @@ __device__ inline C rfft_post(const C *z, int k, int M, const C *twr, decltype(C
     return X;
 }
 
+// the same on a padded slot row (index i at i + i/32)
+template <typename C>
+__device__ inline C rfft_post_padded(const C *z, int k, int M, const C *twr, decltype(C::x) scale)
+{
+    typedef decltype(C::x) T;
+    const int k1 = k == M ? 0 : k, k2 = k == 0 ? 0 : M - k;
+    C zk = z[k1 + (k1 >> 5)];
+    C zc = cconj(z[k2 + (k2 >> 5)]);
+    C E = mk<C>(T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y + zc.y));
+    C D = csub(zk, zc);
+    C O = mk<C>(T(0.5) * D.y, T(-0.5) * D.x);
+    C X = cadd(E, cmul(twr[k], O));
+    X.x *= scale;
+    X.y *= scale;
+    if (k == 0) X.y = T(0);
+    return X;
+}
+
 template <typename C> __device__ void load_twiddles(C *s_tw, C *s_twr, const FFTArgs &f)
 {
     const C *tw = reinterpret_cast<const C *>(f.tw);
@@ -569,6 +587,12 @@ __device__ __forceinline__ void wave_order()
     asm volatile("" ::: "memory");
 }
 
+// slot rows are padded by one complex per 32 (index i at i + i/32): the
+// radix-12 pass writes 12 consecutive points per lane, a 12-point lane
+// stride that otherwise maps every 8th lane to the same LDS banks
+__device__ __forceinline__ int pidx(int i) { return i + (i >> 5); }
+template <int M> constexpr int padded_row() { return M + (M >> 5) + 1; }
+
 template <int R, int L, int M, int P, typename C>
 __device__ __forceinline__ void slot_pass(C *buf, int t, const C *tw, int sign)
 {
@@ -588,7 +612,7 @@ __device__ __forceinline__ void slot_pass(C *buf, int t, const C *tw, int sign)
         const int k = kof(b);
 #pragma unroll
         for (int s = 0; s < R; ++s) {
-            C v = buf[j + s * T];
+            C v = buf[pidx(j + s * T)];
             if constexpr (L > 1) {
                 if (s > 0) {
                     const int ti = s * k * tstride;
@@ -606,9 +630,9 @@ __device__ __forceinline__ void slot_pass(C *buf, int t, const C *tw, int sign)
         const int j = t + Q * b;
         const int k = kof(b);
         Dft<R, C>::run(x[b], sign);
-        C *op = buf + (j - k) * R + k;
+        const int o = (j - k) * R + k;
 #pragma unroll
-        for (int s = 0; s < R; ++s) op[s * L] = x[b][s];
+        for (int s = 0; s < R; ++s) buf[pidx(o + s * L)] = x[b][s];
     }
     wave_order();
 }
@@ -653,7 +677,8 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
     constexpr int NPT = FFT::P / 3;  // grid points per thread: N = NPT * 6 Q
     C *s_tw = reinterpret_cast<C *>(smem_raw);  // M + M / 8 (padded)
     C *s_twr = s_tw + M + M / 8;
-    C *rows = s_twr + M + 1;  // 6 slot rows of M
+    C *rows = s_twr + M + 1;  // 6 slot rows of MP (padded)
+    constexpr int MP = padded_row<M>();
     int lb;
     {
         const int nx = 8, nb = gridDim.x, b = blockIdx.x;
@@ -685,7 +710,7 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
     // round 1 = B, curl B (9-14), round 2 = du/dr (6-8), s, grad s (15-17)
     auto inverse6 = [&](int round) {
         if (slot < 6) {
-            C *z = rows + slot * M;
+            C *z = rows + slot * MP;
             const int comp = round == 0 ? slot : (round == 1 ? 9 + slot : (slot < 3 ? 6 + slot : 12 + slot));
             const C *X = Fi + (size_t)comp * n_m * n_theta;
             // Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/N} (X_k - conj X_{M-k})
@@ -714,9 +739,9 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
                         a.y = T(0);
                         c.y = T(0);
                     }
-                    z[k] = cadd(cadd(a, cconj(c)), cmuli(cmul(cconj(s_twr[k]), csub(a, cconj(c))), 1));
+                    z[pidx(k)] = cadd(cadd(a, cconj(c)), cmuli(cmul(cconj(s_twr[k]), csub(a, cconj(c))), 1));
                     if (k2 < M && k2 != k)
-                        z[k2] = cadd(cadd(c, cconj(a)), cmuli(cmul(cconj(s_twr[k2]), csub(c, cconj(a))), 1));
+                        z[pidx(k2)] = cadd(cadd(c, cconj(a)), cmuli(cmul(cconj(s_twr[k2]), csub(c, cconj(a))), 1));
                 }
             } else {
                 // pointers stepped by Q rows (no index products per step)
@@ -730,9 +755,9 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
                         a.y = T(0);
                         c.y = T(0);
                     }
-                    z[k] = cadd(cadd(a, cconj(c)), cmuli(cmul(cconj(s_twr[k]), csub(a, cconj(c))), 1));
+                    z[pidx(k)] = cadd(cadd(a, cconj(c)), cmuli(cmul(cconj(s_twr[k]), csub(a, cconj(c))), 1));
                     if (k2 < M && k2 != k)
-                        z[k2] = cadd(cadd(c, cconj(a)), cmuli(cmul(cconj(s_twr[k2]), csub(c, cconj(a))), 1));
+                        z[pidx(k2)] = cadd(cadd(c, cconj(a)), cmuli(cmul(cconj(s_twr[k2]), csub(c, cconj(a))), 1));
                 }
             }
             wave_order();
@@ -747,7 +772,7 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
             for (int r = 0; r < 6; ++r) {
                 C zz = mk<C>(T(0), T(0));
                 if (q < M) {
-                    zz = rows[r * M + q];
+                    zz = rows[r * MP + pidx(q)];
                     const int comp = round == 0 ? r : (round == 1 ? 9 + r : (r < 3 ? 6 + r : 12 + r));
                     if (full_grids) {
                         T *fg = full_grids + (((size_t)lev * 18 + comp) * N + 2 * q) * n_theta + th;
@@ -778,13 +803,13 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
                     if (r >= nc) break;
                     const bool horiz = (r % 3 != 0) && !(round == 1 && r == 3);
                     const T sc = horiz ? o2 : T(1);
-                    rows[r * M + q] = mk<C>(val[r][2 * p2] * sc, val[r][2 * p2 + 1] * sc);
+                    rows[r * MP + pidx(q)] = mk<C>(val[r][2 * p2] * sc, val[r][2 * p2 + 1] * sc);
                 }
             }
         }
         __syncthreads();
         if (slot < nc) {
-            C *z = rows + slot * M;
+            C *z = rows + slot * MP;
             FFT::run(z, t, s_tw, -1);
             const int comp = round == 0 ? (slot < 3 ? slot : slot + 3) : (slot < 3 ? slot + 3 : 9);
             const T invN = T(1.0 / (double)N);
@@ -792,7 +817,7 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
             constexpr int FS = BLK ? 16 : 1;
             C *fo = BLK ? Fo + (((size_t)comp * ntb + (th >> 4)) * n_m + t) * 16 + (th & 15)
                         : Fo + ((size_t)comp * n_theta + th) * n_m + t;
-            for (int k = t; k < n_m; k += Q, fo += Q * FS) *fo = rfft_post(z, k, M, s_twr, invN);
+            for (int k = t; k < n_m; k += Q, fo += Q * FS) *fo = rfft_post_padded(z, k, M, s_twr, invN);
         }
         __syncthreads();
     };
@@ -966,7 +991,7 @@ int launch_slots(const Plan *p, const FFTArgs &f, int nlev, const void *Fin, voi
 {
     constexpr int M = SlotFFT<A, B>::M, NPT = SlotFFT<A, B>::P / 3, nthr = SlotFFT<A, B>::NTHR;
     if (NPT * nthr < p->n_phi) return fail(MHX_E_UNSUPPORTED, "n_phi too large for the fused stage");
-    const size_t shm = ((size_t)2 * M + M / 8 + 1 + 6 * (size_t)M) * 2 * sizeof(T);
+    const size_t shm = ((size_t)2 * M + M / 8 + 1 + 6 * (size_t)padded_row<M>()) * 2 * sizeof(T);
     // set once per kernel (the attribute persists), so that a call under
     // graph capture makes no attribute call
     static int shm_set = 0;
