@@ -228,14 +228,14 @@ __device__ inline C rfft_post(const C *z, int k, int M, const C *twr, decltype(C
     return X;
 }
 
-// the same on a padded slot row (index i at i + i/32)
-template <typename C>
+// the same on a slot row (padded at 12 points per lane, see pidx)
+template <int P, typename C>
 __device__ inline C rfft_post_padded(const C *z, int k, int M, const C *twr, decltype(C::x) scale)
 {
     typedef decltype(C::x) T;
     const int k1 = k == M ? 0 : k, k2 = k == 0 ? 0 : M - k;
-    C zk = z[k1 + (k1 >> 5)];
-    C zc = cconj(z[k2 + (k2 >> 5)]);
+    C zk = z[P == 12 ? k1 + (k1 >> 5) : k1];
+    C zc = cconj(z[P == 12 ? k2 + (k2 >> 5) : k2]);
     C E = mk<C>(T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y + zc.y));
     C D = csub(zk, zc);
     C O = mk<C>(T(0.5) * D.y, T(-0.5) * D.x);
@@ -587,11 +587,13 @@ __device__ __forceinline__ void wave_order()
     asm volatile("" ::: "memory");
 }
 
-// slot rows are padded by one complex per 32 (index i at i + i/32): the
-// radix-12 pass writes 12 consecutive points per lane, a 12-point lane
-// stride that otherwise maps every 8th lane to the same LDS banks
-__device__ __forceinline__ int pidx(int i) { return i + (i >> 5); }
-template <int M> constexpr int padded_row() { return M + (M >> 5) + 1; }
+// slots of 12 points per lane pad their rows by one complex per 32 (index
+// i at i + i/32): the radix-12 pass writes 12 consecutive points per lane, a
+// 12-point lane stride that otherwise maps every 8th lane to the same LDS
+// banks (measured: 1.364 -> 1.327 ms at M = 384, 11.1 -> 10.5 ms at M = 768;
+// at 24 points per lane the padding cost 4 %, so M = 1536 is not padded)
+template <int P> __device__ __forceinline__ int pidx(int i) { return P == 12 ? i + (i >> 5) : i; }
+template <int M, int P> constexpr int padded_row() { return P == 12 ? M + (M >> 5) + 1 : M; }
 
 template <int R, int L, int M, int P, typename C>
 __device__ __forceinline__ void slot_pass(C *buf, int t, const C *tw, int sign)
@@ -612,7 +614,7 @@ __device__ __forceinline__ void slot_pass(C *buf, int t, const C *tw, int sign)
         const int k = kof(b);
 #pragma unroll
         for (int s = 0; s < R; ++s) {
-            C v = buf[pidx(j + s * T)];
+            C v = buf[pidx<P>(j + s * T)];
             if constexpr (L > 1) {
                 if (s > 0) {
                     const int ti = s * k * tstride;
@@ -632,7 +634,7 @@ __device__ __forceinline__ void slot_pass(C *buf, int t, const C *tw, int sign)
         Dft<R, C>::run(x[b], sign);
         const int o = (j - k) * R + k;
 #pragma unroll
-        for (int s = 0; s < R; ++s) buf[pidx(o + s * L)] = x[b][s];
+        for (int s = 0; s < R; ++s) buf[pidx<P>(o + s * L)] = x[b][s];
     }
     wave_order();
 }
@@ -678,7 +680,7 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
     C *s_tw = reinterpret_cast<C *>(smem_raw);  // M + M / 8 (padded)
     C *s_twr = s_tw + M + M / 8;
     C *rows = s_twr + M + 1;  // 6 slot rows of MP (padded)
-    constexpr int MP = padded_row<M>();
+    constexpr int MP = padded_row<M, FFT::P>();
     int lb;
     {
         const int nx = 8, nb = gridDim.x, b = blockIdx.x;
@@ -739,9 +741,9 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
                         a.y = T(0);
                         c.y = T(0);
                     }
-                    z[pidx(k)] = cadd(cadd(a, cconj(c)), cmuli(cmul(cconj(s_twr[k]), csub(a, cconj(c))), 1));
+                    z[pidx<FFT::P>(k)] = cadd(cadd(a, cconj(c)), cmuli(cmul(cconj(s_twr[k]), csub(a, cconj(c))), 1));
                     if (k2 < M && k2 != k)
-                        z[pidx(k2)] = cadd(cadd(c, cconj(a)), cmuli(cmul(cconj(s_twr[k2]), csub(c, cconj(a))), 1));
+                        z[pidx<FFT::P>(k2)] = cadd(cadd(c, cconj(a)), cmuli(cmul(cconj(s_twr[k2]), csub(c, cconj(a))), 1));
                 }
             } else {
                 // pointers stepped by Q rows (no index products per step)
@@ -755,9 +757,9 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
                         a.y = T(0);
                         c.y = T(0);
                     }
-                    z[pidx(k)] = cadd(cadd(a, cconj(c)), cmuli(cmul(cconj(s_twr[k]), csub(a, cconj(c))), 1));
+                    z[pidx<FFT::P>(k)] = cadd(cadd(a, cconj(c)), cmuli(cmul(cconj(s_twr[k]), csub(a, cconj(c))), 1));
                     if (k2 < M && k2 != k)
-                        z[pidx(k2)] = cadd(cadd(c, cconj(a)), cmuli(cmul(cconj(s_twr[k2]), csub(c, cconj(a))), 1));
+                        z[pidx<FFT::P>(k2)] = cadd(cadd(c, cconj(a)), cmuli(cmul(cconj(s_twr[k2]), csub(c, cconj(a))), 1));
                 }
             }
             wave_order();
@@ -772,7 +774,7 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
             for (int r = 0; r < 6; ++r) {
                 C zz = mk<C>(T(0), T(0));
                 if (q < M) {
-                    zz = rows[r * MP + pidx(q)];
+                    zz = rows[r * MP + pidx<FFT::P>(q)];
                     const int comp = round == 0 ? r : (round == 1 ? 9 + r : (r < 3 ? 6 + r : 12 + r));
                     if (full_grids) {
                         T *fg = full_grids + (((size_t)lev * 18 + comp) * N + 2 * q) * n_theta + th;
@@ -803,7 +805,7 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
                     if (r >= nc) break;
                     const bool horiz = (r % 3 != 0) && !(round == 1 && r == 3);
                     const T sc = horiz ? o2 : T(1);
-                    rows[r * MP + pidx(q)] = mk<C>(val[r][2 * p2] * sc, val[r][2 * p2 + 1] * sc);
+                    rows[r * MP + pidx<FFT::P>(q)] = mk<C>(val[r][2 * p2] * sc, val[r][2 * p2 + 1] * sc);
                 }
             }
         }
@@ -817,7 +819,7 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
             constexpr int FS = BLK ? 16 : 1;
             C *fo = BLK ? Fo + (((size_t)comp * ntb + (th >> 4)) * n_m + t) * 16 + (th & 15)
                         : Fo + ((size_t)comp * n_theta + th) * n_m + t;
-            for (int k = t; k < n_m; k += Q, fo += Q * FS) *fo = rfft_post_padded(z, k, M, s_twr, invN);
+            for (int k = t; k < n_m; k += Q, fo += Q * FS) *fo = rfft_post_padded<FFT::P>(z, k, M, s_twr, invN);
         }
         __syncthreads();
     };
@@ -991,7 +993,7 @@ int launch_slots(const Plan *p, const FFTArgs &f, int nlev, const void *Fin, voi
 {
     constexpr int M = SlotFFT<A, B>::M, NPT = SlotFFT<A, B>::P / 3, nthr = SlotFFT<A, B>::NTHR;
     if (NPT * nthr < p->n_phi) return fail(MHX_E_UNSUPPORTED, "n_phi too large for the fused stage");
-    const size_t shm = ((size_t)2 * M + M / 8 + 1 + 6 * (size_t)padded_row<M>()) * 2 * sizeof(T);
+    const size_t shm = ((size_t)2 * M + M / 8 + 1 + 6 * (size_t)padded_row<M, SlotFFT<A, B>::P>()) * 2 * sizeof(T);
     // set once per kernel (the attribute persists), so that a call under
     // graph capture makes no attribute call
     static int shm_set = 0;
