@@ -150,7 +150,7 @@ def run_td(L, nlev, tout, rad, lcut, reps=20):
             "frac_of_hbm_peak": byts / (ms * 1e-3) / 8.0e12}
 
 
-def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut):
+def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut, courant=False):
     import numpy as np
     import torch
 
@@ -159,17 +159,29 @@ def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut):
     p = sht.Plan(L, nt, nph, precision=precision, max_levels=nlev)
     dev = tin[0].device
     tout = [torch.zeros((nlev, p.lm_max), dtype=torch.complex128, device=dev) for _ in range(10)]
+    # with courant: the step with the fused Courant maxima of every level
+    # (courfac 2.5, alffac 1, Alfven damping of opm = 0.2 over delxr2 = 1e-3,
+    # delxh2 = 2e-2), which returns when the maxima are on the host
+    cvr = np.full(nlev, 0.36 / 1e-3)
+    cvh = np.full(nlev, 0.36 / 2e-2)
+
+    def step(stream=None):
+        if courant:
+            sht.nl_step_courant(p, tin, tout, rad[0], rad[1], rad[2], rad[3], lcut, 2.5, 1.0,
+                                cvr, cvh)
+        else:
+            sht.nl_step(p, tin, tout, rad[0], rad[1], rad[2], rad[3], lcut, stream=stream)
     # steps issued on a side stream: each call returns without waiting (the
     # device outputs are ordered on the stream), so the host prepares the
     # next step while the current one runs
     st = torch.cuda.Stream()
     st.wait_stream(torch.cuda.current_stream())  # inputs and outputs were made there
     for _ in range(max(warmup, 2)):  # the second call captures the step's launch graph
-        sht.nl_step(p, tin, tout, rad[0], rad[1], rad[2], rad[3], lcut, stream=st)
+        step(stream=st)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        sht.nl_step(p, tin, tout, rad[0], rad[1], rad[2], rad[3], lcut, stream=st)
+        step(stream=st)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     # per-launch durations (HIP events between the launches) in a separate pass
@@ -177,7 +189,7 @@ def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut):
     nt_steps = max(3, min(steps, 10))
     lt = np.zeros(len(LAUNCHES))
     for _ in range(nt_steps):
-        sht.nl_step(p, tin, tout, rad[0], rad[1], rad[2], rad[3], lcut)
+        step()
         lt += np.array(p.launch_times())
     p.set_timing(False)
     lt /= nt_steps
@@ -347,6 +359,9 @@ def main():
                          "l_max=511, 3 at l_max=1023, else 24)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["both", "split16", "fp64"], default="both")
+    ap.add_argument("--courant", action="store_true",
+                    help="time the step with the Courant maxima fused into its grid stage "
+                         "(mhx_nl_step_courant; synchronous per step)")
     ap.add_argument("--tables", choices=["recurrence", "stored"], default="recurrence",
                     help="--config 5: split-form Legendre tables generated in-kernel by the FP64 "
                          "recurrence (default) or read from stored FP16 tables")
@@ -399,7 +414,8 @@ def main():
 
     forms = {}
     if args.form in ("both", "fp64"):
-        forms["fp64"] = run_form(sht.PREC_FP64, L, nlev, args.steps, args.warmup, tin, rad, lcut)
+        forms["fp64"] = run_form(sht.PREC_FP64, L, nlev, args.steps, args.warmup, tin, rad, lcut,
+                                 courant=args.courant)
         log("fp64: %.3f ms/step, %.1f pairs/s" % (forms["fp64"]["ms_per_step"],
                                                   forms["fp64"]["pairs"]))
     if args.form in ("both", "split16"):
@@ -408,7 +424,7 @@ def main():
         rec = args.config == 5 and args.tables == "recurrence"
         forms["split16"] = run_form(sht.PREC_SPLIT16_REC if rec else sht.PREC_SPLIT16, L, nlev,
                                     args.steps, args.warmup, tin,
-                                    rad, lcut)
+                                    rad, lcut, courant=args.courant)
         log("split16: %.3f ms/step, %.1f pairs/s" % (forms["split16"]["ms_per_step"],
                                                      forms["split16"]["pairs"]))
     main_key = "split16" if "split16" in forms else "fp64"
@@ -459,7 +475,8 @@ def main():
                    "l_max": L, "n_theta": nt, "n_phi": nph, "n_r": nlev,
                    "dipole_l1_factor": dipole, "spectrum": kind, "precision": main_key,
                    "tables": ("recurrence" if args.config == 5 and args.tables == "recurrence"
-                              else "stored")},
+                              else "stored"),
+                   "courant_maxima": bool(args.courant)},
         "legendre_tflops_fp64_equiv": main["legendre_tflops"],
         "kernels": main["kernels"],
         "roofline": roofline(main, kern_key, (L, nlev)),

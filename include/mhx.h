@@ -177,6 +177,27 @@ int mhx_nl_step_grids(mhx_plan *p, int nlev, const double *const *in,
                       const double *or4, const double *orho1, double LFfac,
                       const int *lcut, double *grids, void *stream);
 
+/* the same step, also returning the Courant maxima of each level, reduced
+ * inside the grid stage from its velocity and magnetic grids: the
+ * reference's courant (src/courant.f90:54-277, XSH_COURANT = 0, magnetic
+ * branch l_mag, l_mag_LF, not l_mag_kin), called per level from
+ * src/rIter.f90:292-299 with the same grids,
+ *   vr2max = max or4 (courfac^2 orho2 vr^2 + alffac^2 valr^2/(valr+valri2)),
+ *   vh2max = max or2 (courfac^2 orho2 (vt^2+vp^2)/sin^2 +
+ *                     alffac^2 valh^2/(valh+valhi2)),
+ * valr = LFfac orho1 br^2, valh = LFfac orho1 (bt^2+bp^2)/sin^2, orho2 =
+ * orho1^2; valri2/valhi2 per level as the reference forms them
+ * ((1+opm)^2/4 / delxr2, / delxh2 with l_cour_alf_damp, else 0).  The
+ * caller then takes dtrkc = sqrt(delxr2/vr2max), dthkc = sqrt(delxh2/vh2max)
+ * (vr2max, vh2max: host arrays of nlev).  In the split precision the maxima
+ * come from the FP32 grids. */
+int mhx_nl_step_courant(mhx_plan *p, int nlev, const double *const *in, size_t ld_lm,
+                        double *const *out, const double *or2, const double *or4,
+                        const double *orho1, double LFfac, const int *lcut,
+                        double courfac, double alffac, const double *valri2,
+                        const double *valhi2, double *vr2max, double *vh2max,
+                        void *stream);
+
 /* lm-space assembly of the explicit terms at interior radial levels, the
  * step after the forward transforms of the nonlinear step: the reference's
  * nonlinear_lm_t get_dwdt, get_dzdt, get_dpdt, get_dsdt and get_dbdt

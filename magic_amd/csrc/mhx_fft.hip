@@ -365,11 +365,48 @@ __global__ void k_fft_rows(FFTArgs f, const T *__restrict__ g, size_t g_ld_lev, 
 // also written in the reference layout [lev][comp][phi][theta].
 constexpr int PMAX = 3;
 
+// Courant maxima of one level (src/courant.f90:209-275, magnetic branch with
+// the Alfven damping terms valri2/valhi2): over this thread's points the
+// maxima of or4 (cf^2 orho2 vr^2 + af^2 valr^2 / (valr + valri2)) and or2
+// (cf^2 orho2 (vt^2 + vp^2)/sin^2 + af^2 valh^2 / (valh + valhi2)), valr =
+// LFfac orho1 br^2, valh = LFfac orho1 (bt^2 + bp^2)/sin^2, orho2 = orho1^2;
+// reduced over the wave, one atomic maximum per wave and quantity
+template <typename T, int NP>
+__device__ inline void courant_max(const CourArgs &c, int lev, int nlev, T o2, T or2, T or4, T orho1,
+                                   T LFfac, const T (*u)[NP], const T (*b)[NP])
+{
+    const T cf2 = T(c.par[0]), af2 = T(c.par[1]);
+    const T vri2 = T(c.par[2 + lev]), vhi2 = T(c.par[2 + nlev + lev]);
+    const T orho2 = orho1 * orho1;
+    T mr = T(0), mh = T(0);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const T vr = u[0][p], vt = u[1][p], vp = u[2][p];
+        const T br = b[0][p], bt = b[1][p], bp = b[2][p];
+        const T valr = br * br * LFfac * orho1;
+        const T valr2 = valr * valr / (valr + vri2);
+        mr = fmax(mr, or4 * (cf2 * (orho2 * vr * vr) + af2 * valr2));
+        const T valh2 = (bt * bt + bp * bp) * LFfac * o2 * orho1;
+        const T valh2m = valh2 * valh2 / (valh2 + vhi2);
+        mh = fmax(mh, or2 * (cf2 * ((vt * vt + vp * vp) * o2 * orho2) + af2 * valh2m));
+    }
+    double dr = (double)mr, dh = (double)mh;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        dr = fmax(dr, __shfl_xor(dr, off, 64));
+        dh = fmax(dh, __shfl_xor(dh, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(c.max + 2 * lev, (unsigned long long)__double_as_longlong(dr));
+        atomicMax(c.max + 2 * lev + 1, (unsigned long long)__double_as_longlong(dh));
+    }
+}
+
 template <typename T, int G>
 __global__ void k_nl_fused(FFTArgs f, const T *__restrict__ Fin, T *__restrict__ Fout, int n_m,
                            int n_theta, int nlev, const double *__restrict__ osin2,
                            const double *__restrict__ rad, double LFfac_d,
-                           T *__restrict__ dgrid, T *__restrict__ full_grids)
+                           T *__restrict__ dgrid, T *__restrict__ full_grids, CourArgs cour)
 {
     // split form (T = float): theta-blocked forward side
     constexpr bool BLK = sizeof(T) == 4;
@@ -497,6 +534,7 @@ __global__ void k_nl_fused(FFTArgs f, const T *__restrict__ Fin, T *__restrict__
         vxb[1][p] = orho1 * or4 * (u[2][p] * bb[0][p] - u[0][p] * bb[2][p]);
         vxb[2][p] = orho1 * or4 * (u[0][p] * bb[1][p] - u[1][p] * bb[0][p]);
     }
+    if (cour.max) courant_max<T, PMAX>(cour, lev, nlev, o2, or2, or4, orho1, LFfac, u, bb);
     // curl B: Lorentz force added to Adv (src/get_nl.f90:242-258,
     // src/rIter.f90:650-656)
     inverse3(12);
@@ -669,7 +707,8 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
                                                   T *__restrict__ Fout, int n_m, int n_theta,
                                                   int nlev, const double *__restrict__ osin2,
                                                   const double *__restrict__ rad, double LFfac_d,
-                                                  T *__restrict__ dgrid, T *__restrict__ full_grids)
+                                                  T *__restrict__ dgrid, T *__restrict__ full_grids,
+                                                  CourArgs cour)
 {
     typedef cpx<T> C;
     constexpr bool BLK = sizeof(T) == 4;  // split form: theta-blocked F2
@@ -838,6 +877,7 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
     // B: V x B (src/get_nl.f90:376-388); curl B: Lorentz force added to Adv
     // (src/get_nl.f90:242-258, src/rIter.f90:650-656)
     inverse6(1);
+    if (cour.max) courant_max<T, NPT>(cour, lev, nlev, o2, or2, or4, orho1, LFfac, u, val);
 #pragma unroll
     for (int p = 0; p < NPT; ++p) {
         const T br = val[0][p], bt = val[1][p], bp = val[2][p];
@@ -1005,7 +1045,8 @@ int launch_slots(const Plan *p, const FFTArgs &f, int nlev, const void *Fin, voi
     dim3 grid((unsigned)(p->n_theta * nlev));
     k_nl_slots<T, A, B><<<grid, nthr, shm, s>>>(f, static_cast<const T *>(Fin), static_cast<T *>(Fout),
                                                 p->n_m, p->n_theta, nlev, p->d_osin2, rad, LFfac,
-                                                static_cast<T *>(dgrid), static_cast<T *>(full_grids));
+                                                static_cast<T *>(dgrid), static_cast<T *>(full_grids),
+                                                p->cour);
     MHX_HIP_CHECK(hipGetLastError());
     return MHX_OK;
 }
@@ -1052,7 +1093,7 @@ int launch_fused_t(const Plan *p, int nlev, const void *Fin, void *Fout, const d
     T *fo = static_cast<T *>(Fout), *dg = static_cast<T *>(dgrid), *fg = static_cast<T *>(full_grids);
     if (G == 3) {
         k_nl_fused<T, 3><<<grid, nthr, shm, s>>>(f, fi, fo, p->n_m, p->n_theta, nlev, p->d_osin2,
-                                                  rad, LFfac, dg, fg);
+                                                  rad, LFfac, dg, fg, p->cour);
     } else {
         // set once per instance and raised only (the attribute persists), so
         // that a call under graph capture makes no attribute call
@@ -1063,7 +1104,7 @@ int launch_fused_t(const Plan *p, int nlev, const void *Fin, void *Fout, const d
             shm_set = (int)shm;
         }
         k_nl_fused<T, 1><<<grid, nthr, shm, s>>>(f, fi, fo, p->n_m, p->n_theta, nlev, p->d_osin2,
-                                                  rad, LFfac, dg, fg);
+                                                  rad, LFfac, dg, fg, p->cour);
     }
     MHX_HIP_CHECK(hipGetLastError());
     return MHX_OK;

@@ -130,6 +130,15 @@ struct NAnaArgs {
     int l_max, m_max, minc, n_m, nh, n_theta, ntb, nq32, ngz;
 };
 
+// Courant maxima fused into the grid stage (src/courant.f90): par = [courfac^2,
+// alffac^2, valri2[nlev], valhi2[nlev]] (device), max = [vr2max, vh2max] per
+// level as the bit patterns of non-negative doubles (device, zeroed first);
+// max == nullptr: not computed
+struct CourArgs {
+    const double *par = nullptr;
+    unsigned long long *max = nullptr;
+};
+
 // lm-space assembly of the explicit terms (mhx_td.hip)
 struct TdArgs {
     const double *in[10];   // AdvrLM AdvtLM AdvpLM VStLM VxBrLM VxBtLM VxBpLM w dw z
@@ -172,6 +181,12 @@ struct Plan {
     // 16x16x32 MFMA operand layout, with power-of-2 row factors
     int nq32 = 0;                   // theta chunks of 32 (northern half)
     size_t syn16_size = 0, ana16_size = 0;   // halves
+    // Courant maxima of mhx_nl_step_courant (set for the call's launches)
+    CourArgs cour;
+    double *d_cour_par = nullptr;
+    unsigned long long *d_cour_max = nullptr;
+    int cour_nlev = 0;
+    std::vector<double> h_cour;
     // per-level factors of mhx_get_td (or2, or4, epsc; l_R)
     double *d_td_rad = nullptr;
     int *d_td_lcut = nullptr;

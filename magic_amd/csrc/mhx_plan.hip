@@ -483,6 +483,8 @@ int mhx_plan_destroy(mhx_plan *h)
     free_dev(p->d_ana16_soff);
     free_dev(p->d_syn16_rs);
     free_dev(p->d_rec_seed);
+    free_dev(p->d_cour_par);
+    free_dev(p->d_cour_max);
     free_dev(p->d_td_rad);
     free_dev(p->d_td_lcut);
     free_dev(p->d_rec_aseed);
@@ -841,10 +843,24 @@ int mhx_spat_to_qst_batch(mhx_plan *h, int nlev, const double *f, const double *
 }
 
 // ------------------------------------------------------------ nonlinear step
+// zeroes n 64-bit words (a kernel, not a memset node, inside captured graphs)
+__global__ void k_zero_u64(unsigned long long *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// the Courant inputs of mhx_nl_step_courant (host)
+struct CourIn {
+    double courfac, alffac;
+    const double *valri2, *valhi2;
+    double *vr2max, *vh2max;
+};
+
 static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t ld_lm,
                         double *const *out, const double *or2, const double *or4,
                         const double *orho1, double LFfac, const int *lcut, double *grids,
-                        void *stream)
+                        void *stream, const CourIn *ci = nullptr)
 {
     if (!h) return fail(MHX_E_INVALID, "null plan");
     Plan *P = &h->p;
@@ -915,6 +931,33 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
                                      P->stream));
         MHX_HIP_CHECK(hipEventRecord(P->ev_pin, P->stream));
         P->pin_pending = true;
+    }
+    // Courant maxima: device parameters and per-level maxima (zeroed by a
+    // kernel at the start of the step's launches)
+    P->cour = CourArgs();
+    if (ci) {
+        if (!ci->valri2 || !ci->valhi2 || !ci->vr2max || !ci->vh2max)
+            return fail(MHX_E_INVALID, "null Courant argument");
+        if (P->cour_nlev < nlev) {
+            free_dev(P->d_cour_par);
+            free_dev(P->d_cour_max);
+            P->cour_nlev = 0;
+            MHX_HIP_CHECK(hipMalloc(&P->d_cour_par, (2 + 2 * (size_t)nlev) * sizeof(double)));
+            MHX_HIP_CHECK(hipMalloc(&P->d_cour_max, 2 * (size_t)nlev * sizeof(unsigned long long)));
+            P->cour_nlev = nlev;
+        }
+        MHX_HIP_CHECK(hipStreamSynchronize(P->stream));  // the previous upload has been read
+        P->h_cour.assign(2 + 2 * (size_t)nlev, 0.0);
+        P->h_cour[0] = ci->courfac * ci->courfac;
+        P->h_cour[1] = ci->alffac * ci->alffac;
+        for (int i = 0; i < nlev; ++i) {
+            P->h_cour[2 + i] = ci->valri2[i];
+            P->h_cour[2 + nlev + i] = ci->valhi2[i];
+        }
+        MHX_HIP_CHECK(hipMemcpyAsync(P->d_cour_par, P->h_cour.data(), P->h_cour.size() * sizeof(double),
+                                     hipMemcpyHostToDevice, P->stream));
+        P->cour.par = P->d_cour_par;
+        P->cour.max = P->d_cour_max;
     }
     hipStream_t s = P->stream;
     // a split-precision plan runs the Legendre stage in scaled FP16 x3 and
@@ -1053,7 +1096,13 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
             }
             return join();
         };
-        auto launch_all = [&]() -> int { return split ? launch_split() : launch_fp64(); };
+        auto launch_all = [&]() -> int {
+            if (P->cour.max) {
+                k_zero_u64<<<(2 * nlev + 255) / 256, 256, 0, s>>>(P->cour.max, 2 * nlev);
+                MHX_HIP_CHECK(hipGetLastError());
+            }
+            return split ? launch_split() : launch_fp64();
+        };
         if (P->timing) {
             if ((rc = launch_all())) return rc;
         } else {
@@ -1066,7 +1115,8 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
             for (int i = 0; i < 10; ++i) key.push_back((unsigned long long)(uintptr_t)dout[i]);
             // workspaces the launches address (reallocated when nlev grows)
             for (const void *w : {(const void *)P->ws_F, (const void *)P->ws_F2, (const void *)P->ws_G,
-                                  (const void *)P->d_lcut, (const void *)P->d_rad})
+                                  (const void *)P->d_lcut, (const void *)P->d_rad,
+                                  (const void *)P->cour.max, (const void *)P->cour.par})
                 key.push_back((unsigned long long)(uintptr_t)w);
             if (P->gexec && key == P->gkey) {
                 MHX_HIP_CHECK(hipGraphLaunch(P->gexec, s));
@@ -1218,6 +1268,35 @@ int mhx_get_td(mhx_plan *h, int nlev, const double *const *in, size_t ld_lm, dou
     }
     MHX_HIP_CHECK(hipStreamSynchronize(P->stream));
     return MHX_OK;
+}
+
+int mhx_nl_step_courant(mhx_plan *h, int nlev, const double *const *in, size_t ld_lm,
+                        double *const *out, const double *or2, const double *or4,
+                        const double *orho1, double LFfac, const int *lcut, double courfac,
+                        double alffac, const double *valri2, const double *valhi2,
+                        double *vr2max, double *vh2max, void *stream)
+{
+    CourIn ci{courfac, alffac, valri2, valhi2, vr2max, vh2max};
+    int rc = nl_step_impl(h, nlev, in, ld_lm, out, or2, or4, orho1, LFfac, lcut, nullptr, stream, &ci);
+    Plan *P = h ? &h->p : nullptr;
+    if (rc) {
+        if (P) P->cour = CourArgs();
+        return rc;
+    }
+    // the maxima back to the host (the call then returns when the step is done)
+    std::vector<unsigned long long> mx(2 * (size_t)nlev);
+    MHX_HIP_CHECK(hipMemcpyAsync(mx.data(), P->d_cour_max, mx.size() * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost, P->stream));
+    MHX_HIP_CHECK(hipStreamSynchronize(P->stream));
+    P->cour = CourArgs();
+    for (int i = 0; i < nlev; ++i) {
+        double a, b;
+        std::memcpy(&a, &mx[2 * i], sizeof(a));
+        std::memcpy(&b, &mx[2 * i + 1], sizeof(b));
+        vr2max[i] = a;
+        vh2max[i] = b;
+    }
+    return finish_step(h, stream);
 }
 
 int mhx_nl_step_grids(mhx_plan *h, int nlev, const double *const *in, size_t ld_lm,

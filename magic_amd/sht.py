@@ -53,6 +53,7 @@ _SIGS = {
     "mhx_nl_step": "Pipzppppdpp",
     "mhx_nl_step_grids": "Pipzppppdppp",
     "mhx_get_td": "Pipzppppdpp",
+    "mhx_nl_step_courant": "Pipzppppdpddppppp",
     "mhx_set_timing": "Pi",
     "mhx_stage_times": "Pp",
     "mhx_launch_times": "Ppi",
@@ -332,6 +333,26 @@ def nl_step(plan, inputs, outputs, or2, or4, orho1, LFfac, lcut, grids=None, str
     else:
         _check(L.mhx_nl_step_grids(plan.h, nlev, ins, plan.lm_max, outs, d(or2), d(or4),
                                    d(orho1), float(LFfac), lc, _ptr(grids), st))
+
+
+def nl_step_courant(plan, inputs, outputs, or2, or4, orho1, LFfac, lcut, courfac, alffac,
+                    valri2, valhi2):
+    """nl_step that also returns the per-level Courant maxima (vr2max,
+    vh2max) of src/courant.f90's magnetic branch, reduced inside the grid
+    stage; dtrkc = sqrt(delxr2 / vr2max), dthkc = sqrt(delxh2 / vh2max)."""
+    import numpy as np
+    L = plan._L
+    nlev = inputs[0].shape[0]
+    ins = (ctypes.c_void_p * 11)(*[t.data_ptr() for t in inputs])
+    outs = (ctypes.c_void_p * 10)(*[t.data_ptr() for t in outputs])
+    d = lambda v: (ctypes.c_double * nlev)(*[float(x) for x in v])  # noqa: E731
+    lc = (ctypes.c_int * nlev)(*[int(x) for x in lcut])
+    vr2 = (ctypes.c_double * nlev)()
+    vh2 = (ctypes.c_double * nlev)()
+    _check(L.mhx_nl_step_courant(plan.h, nlev, ins, plan.lm_max, outs, d(or2), d(or4), d(orho1),
+                                 float(LFfac), lc, float(courfac), float(alffac), d(valri2),
+                                 d(valhi2), vr2, vh2, None))
+    return np.array(vr2[:]), np.array(vh2[:])
 
 
 TD_INPUTS = ("AdvrLM", "AdvtLM", "AdvpLM", "VStLM", "VxBrLM", "VxBtLM", "VxBpLM", "w", "dw", "z")

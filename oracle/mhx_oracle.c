@@ -726,3 +726,38 @@ void orc_get_td(const orc_plan *p, int lR, double or2, double or4, double CorFac
         djdt[lm] = dLh * or4 * VxBr[lm];
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* Courant maxima of one level (src/courant.f90:209-275, XSH_COURANT = 0,
+   magnetic branch l_mag, l_mag_LF, not l_mag_kin): over the grid,
+   vr2max = max or4 (cf^2 orho2 vr^2 + af^2 valr^2 / (valr + valri2)),
+   valr = br^2 LFfac orho1, and vh2max = max or2 (cf^2 (vt^2 + vp^2)
+   osin2 orho2 + af^2 valh^2 / (valh + valhi2)), valh = (bt^2 + bp^2) LFfac
+   osin2 orho1; orho2 = orho1^2.  in[6] = vr vt vp br bt bp grids. */
+void orc_courant(const orc_plan *p, double or2, double or4, double orho1, double LFfac,
+                 double courfac, double alffac, double valri2, double valhi2,
+                 const double *const *in, double *vr2max, double *vh2max)
+{
+    const int nt = p->n_theta, np_ = p->n_phi;
+    const double cf2 = courfac * courfac, af2 = alffac * alffac, orho2 = orho1 * orho1;
+    double mr = 0.0, mh = 0.0;
+    for (int ip = 0; ip < np_; ++ip)
+        for (int it = 0; it < nt; ++it) {
+            const size_t g = (size_t)ip * nt + it;
+            const double vr = in[0][g], vt = in[1][g], vp = in[2][g];
+            const double br = in[3][g], bt = in[4][g], bp = in[5][g];
+            const double o2 = p->osin2[it];
+            const double vflr2 = orho2 * vr * vr;
+            const double valr = br * br * LFfac * orho1;
+            const double valr2 = valr * valr / (valr + valri2);
+            const double a = or4 * (cf2 * vflr2 + af2 * valr2);
+            if (a > mr) mr = a;
+            const double vflh2 = (vt * vt + vp * vp) * o2 * orho2;
+            const double valh2 = (bt * bt + bp * bp) * LFfac * o2 * orho1;
+            const double valh2m = valh2 * valh2 / (valh2 + valhi2);
+            const double b = or2 * (cf2 * vflh2 + af2 * valh2m);
+            if (b > mh) mh = b;
+        }
+    *vr2max = mr;
+    *vh2max = mh;
+}

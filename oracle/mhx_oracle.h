@@ -111,6 +111,13 @@ void orc_get_nl(const orc_plan *p, double or2, double or4, double orho1,
 void orc_get_td(const orc_plan *p, int lR, double or2, double or4, double CorFac,
                 double epsc, const double *const *in, double *const *out);
 
+/* src/courant.f90 (magnetic branch): the level's radial and horizontal
+   maxima; dtrkc = sqrt(delxr2 / vr2max), dthkc = sqrt(delxh2 / vh2max).
+   in[6] = vr vt vp br bt bp grids (n_phi x n_theta, theta fastest). */
+void orc_courant(const orc_plan *p, double or2, double or4, double orho1, double LFfac,
+                 double courfac, double alffac, double valri2, double valhi2,
+                 const double *const *in, double *vr2max, double *vh2max);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,7 @@ def lib():
             getattr(L, name).argtypes = [P, dp, dp]
         L.orc_get_nl.argtypes = [P, D, D, D, D, ctypes.POINTER(dp), ctypes.POINTER(dp)]
         L.orc_get_td.argtypes = [P, I, D, D, D, D, ctypes.POINTER(dp), ctypes.POINTER(dp)]
+        L.orc_courant.argtypes = [P, D, D, D, D, D, D, D, D, ctypes.POINTER(dp), dp, dp]
         _lib = L
     return _lib
 
@@ -262,6 +263,18 @@ class Plan:
         self._L.orc_get_td(ctypes.c_void_p(self.h), int(lR), float(or2), float(or4),
                            float(CorFac), float(epsc), arr_in, arr_out)
         return outs
+
+
+    def courant(self, or2, or4, orho1, LFfac, courfac, alffac, valri2, valhi2, grids):
+        """grids: vr vt vp br bt bp (n_phi, n_theta).  Returns (vr2max, vh2max)
+        of src/courant.f90's magnetic branch."""
+        ins = [self._g(g) for g in grids]
+        arr = (ctypes.POINTER(ctypes.c_double) * 6)(*[_ptr(a) for a in ins])
+        a, b = np.zeros(1), np.zeros(1)
+        self._L.orc_courant(ctypes.c_void_p(self.h), float(or2), float(or4), float(orho1),
+                            float(LFfac), float(courfac), float(alffac), float(valri2),
+                            float(valhi2), arr, _ptr(a), _ptr(b))
+        return float(a[0]), float(b[0])
 
 
 def have_ref():

@@ -8,10 +8,11 @@
 ! code calls it.
 !
 ! It also drives the reference's lm-space assembly of the explicit terms
-! (src/get_td.f90, module nonlinear_lm_mod) in mode td.
+! (src/get_td.f90, module nonlinear_lm_mod) in mode td and its Courant
+! step limits (src/courant.f90) on the step's grids in mode courant.
 !
 ! Usage:  ref_harness MODE l_max minc nalias infile outfile [nlev nrep]
-!   MODE = tables | api | nl | nlstep | bench | td
+!   MODE = tables | api | nl | nlstep | bench | td | courant
 !
 ! File formats are little-endian streams: int32 header words followed by
 ! float64 arrays in Fortran (column-major) order.
@@ -26,9 +27,11 @@ program ref_harness
    use truncation
    use blocking, only: initialize_blocking, st_map
    use horizontal_data
-   use radial_functions, only: or1, or2, or3, or4, orho1, beta, r, otemp1, &
+   use radial_functions, only: or1, or2, or3, or4, orho1, orho2, beta, r, otemp1, &
        &                       visc, lambda, epscProf, l_R
-   use physical_parameters, only: LFfac, n_r_LCR, CorFac, epsc
+   use physical_parameters, only: LFfac, n_r_LCR, CorFac, epsc, opm
+   use num_param, only: delxr2, delxh2
+   use courant_mod, only: courant
    use fields, only: w_Rloc, dw_Rloc, ddw_Rloc, z_Rloc, dz_Rloc
    use nonlinear_lm_mod, only: nonlinear_lm_t
    use sht
@@ -94,6 +97,8 @@ program ref_harness
       call do_nlstep()
    case ('td')
       call do_td()
+   case ('courant')
+      call do_courant()
    case default
       write(output_unit,*) 'unknown mode ', trim(mode)
       stop 2
@@ -343,6 +348,46 @@ contains
       close(u)
       close(v)
    end subroutine do_td
+
+   subroutine do_courant()
+      !-- reads nlev levels; per level: lcut, or2, or4, orho1, LFfac,
+      !   courfac, alffac, opm, delxr2, delxh2 and the 11 coefficient arrays
+      !   of nlstep; builds the level's grids as nl_level does and writes the
+      !   radial and horizontal Courant steps of src/courant.f90 (magnetic
+      !   branch, l_cour_alf_damp) starting from 1e30
+      type(grid_space_arrays_t) :: gsa
+      complex(cp), allocatable :: C(:,:), O(:,:)
+      real(cp), allocatable :: uGs(:,:)
+      integer :: u, v, lev, i
+      integer(int32) :: lcut32
+      real(cp) :: o2, o4, orh, lff, cf, af, pm, dr2, dh2, dtr, dth
+
+      l_mag_kin=.false.; l_cour_alf_damp=.true.
+      allocate( C(lm_max,11), O(lm_max,10), uGs(n_theta_max,n_phi_max) )
+      if ( .not. allocated(orho2) ) allocate( orho2(1) )
+      if ( .not. allocated(delxr2) ) allocate( delxr2(1), delxh2(1) )
+      open(newunit=u, file=trim(infile), access='stream', form='unformatted', &
+      &    status='old')
+      open(newunit=v, file=trim(outfile), access='stream', form='unformatted', &
+      &    status='replace')
+      call write_header(v)
+      do lev=1,nlev
+         read(u) lcut32, o2, o4, orh, lff, cf, af, pm, dr2, dh2
+         call set_radial(o2, o4, orh)
+         orho2(1)=orh*orh; LFfac=lff; opm=pm; delxr2(1)=dr2; delxh2(1)=dh2
+         if ( lev == 1 ) call gsa%initialize()
+         do i=1,11
+            read(u) C(:,i)
+         end do
+         call nl_level(gsa, C, int(lcut32), O, uGs)
+         dtr=1.0e30_cp; dth=1.0e30_cp
+         call courant(1, dtr, dth, gsa%vrc, gsa%vtc, gsa%vpc, gsa%brc, gsa%btc, &
+         &            gsa%bpc, cf, af)
+         write(v) dtr, dth
+      end do
+      close(u)
+      close(v)
+   end subroutine do_courant
 
    subroutine do_bench()
       !-- Times the per-level transform sequence of a Boussinesq MHD bulk

@@ -196,6 +196,42 @@ def td_case(tag, l_max, nlev, write=True):
     return data
 
 
+def courant_case(tag, l_max, nlev, write=True):
+    """The reference's Courant limits (src/courant.f90, magnetic branch with
+    Alfven damping) on the grids of the nonlinear step (ref_harness courant,
+    nl_reference's inputs)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from nl_reference import make_inputs, radial
+    inputs = make_inputs(l_max, nlev)
+    or2, or4, orho1, lff = radial(nlev)
+    lcut = [l_max] * nlev
+    lcut[-1] = l_max - 3
+    courfac, alffac, opm = 2.5, 1.0, 0.2
+    delxr2 = 1.0e-3 * (1.0 + np.arange(nlev))
+    delxh2 = 2.0e-2 / (1.0 + np.arange(nlev))
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            for i in range(nlev):
+                np.array([lcut[i]], dtype="<i4").tofile(f)
+                np.array([or2[i], or4[i], orho1[i], lff, courfac, alffac, opm, delxr2[i], delxh2[i]],
+                         dtype="<f8").tofile(f)
+                for k in range(11):
+                    np.ascontiguousarray(inputs[k][i], dtype="<c16").tofile(f)
+        oracle.run_ref("courant", l_max, 1, 20, fin, fout, nlev=nlev, threads=1)
+        with open(fout, "rb") as f:
+            read_header(f)
+            dt = np.fromfile(f, "<f8", 2 * nlev).reshape(nlev, 2)
+            assert f.read() == b""
+    data = dict(l_max=l_max, nlev=nlev, lcut=np.array(lcut), courfac=courfac, alffac=alffac,
+                opm=opm, delxr2=delxr2, delxh2=delxh2, dtrkc=dt[:, 0], dthkc=dt[:, 1])
+    if write:
+        path = os.path.join(OUT, "courant_%s.npz" % tag)
+        np.savez_compressed(path, **data)
+        print("wrote", path, os.path.getsize(path))
+    return data
+
+
 def main():
     if not oracle.have_ref():
         sys.exit("oracle/_ref/ref_harness is not built (make -C oracle)")
@@ -210,6 +246,7 @@ def main():
     nlstep_case("L16", 16, 3)
     td_case("L16", 16, 3)
     td_case("L64", 64, 2)
+    courant_case("L16", 16, 3)
 
 
 if __name__ == "__main__":

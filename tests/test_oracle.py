@@ -117,6 +117,27 @@ def test_get_td_coriolis_identities():
     assert np.abs(dwdt[others]).max() == 0.0
 
 
+def test_oracle_courant_matches_reference():
+    """Courant maxima (src/courant.f90, magnetic branch, Alfven damping) on
+    the oracle's step grids against the reference's courant run by
+    ref_harness courant on the same inputs: dtrkc, dthkc to 1e-14"""
+    from nl_reference import make_inputs, oracle_step, radial
+    g = load_golden("courant_L16.npz")
+    L, nlev = int(g["l_max"]), int(g["nlev"])
+    nt, nph = workload.grid_size(L)
+    o = oracle.Plan(L, nt, nph)
+    rad = radial(nlev)
+    G, _ = oracle_step(o, make_inputs(L, nlev), *rad, list(g["lcut"]))
+    opm = float(g["opm"])
+    for lev in range(nlev):
+        dr2, dh2 = g["delxr2"][lev], g["delxh2"][lev]
+        vri2, vhi2 = (0.5 * (1 + opm)) ** 2 / dr2, (0.5 * (1 + opm)) ** 2 / dh2
+        a, b = o.courant(rad[0][lev], rad[1][lev], rad[2][lev], rad[3], float(g["courfac"]),
+                         float(g["alffac"]), vri2, vhi2, [G[lev, c] for c in (0, 1, 2, 9, 10, 11)])
+        assert abs(np.sqrt(dr2 / a) / g["dtrkc"][lev] - 1) < 1e-14
+        assert abs(np.sqrt(dh2 / b) / g["dthkc"][lev] - 1) < 1e-14
+
+
 def test_gauleg_vs_numpy():
     for n in (24, 96, 384):
         th, w = oracle.gauleg(n)
