@@ -129,6 +129,16 @@ int mhx_axi_to_spat(mhx_plan *p, const double *fl_ax, double *f);
 int mhx_toraxi_to_spat(mhx_plan *p, const double *fl_ax, double *ft,
                        double *fp, int lcut);
 
+/* src/nonlinear_bcs.f90:24-68 (get_br_v_bcs, called at src/rIter.f90:270
+ * and :275 for the outer and inner boundary): the induction products of a
+ * conducting mantle or inner core, br_vt = fac br vt and br_vp = br (fac vp
+ * - omega sin^2 theta) with fac = O_r_E_2 O_rho, analysed by
+ * spat_to_sphertor at l_max into br_vt_lm, br_vp_lm (complex lm_max).
+ * Grids n_theta x n_phi as the other per-level routines; host or device. */
+int mhx_get_br_v_bcs(mhx_plan *p, const double *br, const double *vt, const double *vp,
+                     double omega, double O_r_E_2, double O_rho, double *br_vt_lm,
+                     double *br_vp_lm);
+
 /* ---- batched over radial levels (device pointers, one launch per stage) ----
  * Level i of an input/output coefficient array starts at ptr + 2*i*ld_lm
  * doubles; level i of a grid at ptr + i*ld_grid doubles.  lcut[nlev] is the

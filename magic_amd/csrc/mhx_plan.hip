@@ -764,6 +764,49 @@ int mhx_spat_to_sphertor(mhx_plan *h, const double *f, const double *g, double *
     return analysis_call(P, ANA_SPHTOR, 2, gr, os, lcut, 2, out);
 }
 
+// src/nonlinear_bcs.f90:58-64: br*vt*fac and br*(fac*vp - omega sin^2)
+// with fac = O_r_E_2*O_rho.  a, b may alias br, vt (the staged grids): every
+// thread reads its point's three inputs before writing the two outputs.
+__global__ void k_br_v(size_t n, int nt, const double *br, const double *vt, const double *vp,
+                       const double *__restrict__ osin2, double fac, double omega, double *a,
+                       double *b)
+{
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double s2 = 1.0 / osin2[i % nt];
+    const double r = br[i], t = vt[i], f = vp[i];
+    a[i] = fac * r * t;
+    b[i] = r * (fac * f - omega * s2);
+}
+
+int mhx_get_br_v_bcs(mhx_plan *h, const double *br, const double *vt, const double *vp,
+                     double omega, double O_r_E_2, double O_rho, double *br_vt_lm,
+                     double *br_vp_lm)
+{
+    PLAN_OR_FAIL(h);
+    if (!br || !vt || !vp || !br_vt_lm || !br_vp_lm) return fail(MHX_E_INVALID, "null array");
+    const size_t gsz = (size_t)P->n_phi * P->n_theta;
+    const double *in[3] = {br, vt, vp};
+    const double *src[3];
+    for (int c = 0; c < 3; ++c) {
+        if (is_device_ptr(in[c])) {
+            src[c] = in[c];
+        } else {
+            MHX_HIP_CHECK(hipMemcpyAsync(P->ws_G + c * gsz, in[c], gsz * sizeof(double),
+                                         hipMemcpyHostToDevice, P->stream));
+            src[c] = P->ws_G + c * gsz;
+        }
+    }
+    double *a = P->ws_G, *b = P->ws_G + gsz;
+    k_br_v<<<(unsigned)((gsz + 255) / 256), 256, 0, P->stream>>>(
+        gsz, P->n_theta, src[0], src[1], src[2], P->d_osin2, O_r_E_2 * O_rho, omega, a, b);
+    MHX_HIP_CHECK(hipGetLastError());
+    const double *gr[2] = {a, b};
+    bool os[2] = {true, true};
+    double *out[2] = {br_vt_lm, br_vp_lm};
+    return analysis_call(P, ANA_SPHTOR, 2, gr, os, P->l_max, 2, out);
+}
+
 int mhx_axi_to_spat(mhx_plan *h, const double *fl_ax, double *f)
 {
     PLAN_OR_FAIL(h);

@@ -46,6 +46,7 @@ _SIGS = {
     "mhx_scal_to_SH": "Pppi",
     "mhx_spat_to_qst": "Pppppppi",
     "mhx_spat_to_sphertor": "Pppppi",
+    "mhx_get_br_v_bcs": "Ppppdddpp",
     "mhx_axi_to_spat": "Ppp",
     "mhx_toraxi_to_spat": "Ppppi",
     "mhx_torpol_to_spat_batch": "Pipppzpppzpp",
@@ -260,6 +261,15 @@ class Plan:
         a, b = self.coef(), self.coef()
         _check(self._L.mhx_spat_to_sphertor(self.h, _ptr(self._g(f)), _ptr(self._g(g)),
                                             _ptr(a), _ptr(b), lcut))
+        return a, b
+
+    def get_br_v_bcs(self, br, vt, vp, omega, O_r_E_2, O_rho):
+        """br_vt_lm, br_vp_lm: the induction products at a conducting
+        boundary analysed at l_max (src/nonlinear_bcs.f90:24-68)."""
+        a, b = self.coef(), self.coef()
+        _check(self._L.mhx_get_br_v_bcs(self.h, _ptr(self._g(br)), _ptr(self._g(vt)),
+                                        _ptr(self._g(vp)), float(omega), float(O_r_E_2),
+                                        float(O_rho), _ptr(a), _ptr(b)))
         return a, b
 
     def axi_to_spat(self, fl_ax):

@@ -761,3 +761,28 @@ void orc_courant(const orc_plan *p, double or2, double or4, double orho1, double
     *vr2max = mr;
     *vh2max = mh;
 }
+
+/* ------------------------------------------------------------------ */
+/* src/nonlinear_bcs.f90:24-68 (get_br_v_bcs): the induction products at a
+   conducting boundary, br*vt/(r^2 rho) and br*(vp/(r^2 rho) - omega sin^2),
+   analysed by spat_to_sphertor at l_max.  sinTheta_E2 = sin(colat)**2 in
+   the scrambled order (src/horizontal.f90:151-152). */
+void orc_get_br_v_bcs(const orc_plan *p, const double *br, const double *vt, const double *vp,
+                      double omega, double O_r_E_2, double O_rho, double *br_vt_lm,
+                      double *br_vp_lm)
+{
+    int nt = p->n_theta;
+    size_t npts = (size_t)nt * p->n_phi;
+    double *a = (double *)malloc(sizeof(double) * npts);
+    double *b = (double *)malloc(sizeof(double) * npts);
+    const double fac = O_r_E_2 * O_rho;
+    for (size_t i = 0; i < npts; ++i) {
+        double s = sin(p->theta_ord[(i % nt) / 2]);
+        double s2 = s * s;
+        a[i] = fac * br[i] * vt[i];
+        b[i] = br[i] * (fac * vp[i] - omega * s2);
+    }
+    orc_spat_to_sph_tor(p, a, b, br_vt_lm, br_vp_lm, p->l_max);
+    free(a);
+    free(b);
+}

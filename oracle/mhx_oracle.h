@@ -118,6 +118,12 @@ void orc_courant(const orc_plan *p, double or2, double or4, double orho1, double
                  double courfac, double alffac, double valri2, double valhi2,
                  const double *const *in, double *vr2max, double *vh2max);
 
+/* src/nonlinear_bcs.f90:24-68 (get_br_v_bcs): br_vt_lm, br_vp_lm (complex
+   lm_max) of the boundary products on grids br, vt, vp. */
+void orc_get_br_v_bcs(const orc_plan *p, const double *br, const double *vt, const double *vp,
+                      double omega, double O_r_E_2, double O_rho, double *br_vt_lm,
+                      double *br_vp_lm);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,7 @@ def lib():
         L.orc_get_nl.argtypes = [P, D, D, D, D, ctypes.POINTER(dp), ctypes.POINTER(dp)]
         L.orc_get_td.argtypes = [P, I, D, D, D, D, ctypes.POINTER(dp), ctypes.POINTER(dp)]
         L.orc_courant.argtypes = [P, D, D, D, D, D, D, D, D, ctypes.POINTER(dp), dp, dp]
+        L.orc_get_br_v_bcs.argtypes = [P, dp, dp, dp, D, D, D, dp, dp]
         _lib = L
     return _lib
 
@@ -275,6 +276,14 @@ class Plan:
                             float(LFfac), float(courfac), float(alffac), float(valri2),
                             float(valhi2), arr, _ptr(a), _ptr(b))
         return float(a[0]), float(b[0])
+
+    def get_br_v_bcs(self, br, vt, vp, omega, O_r_E_2, O_rho):
+        """br_vt_lm, br_vp_lm of src/nonlinear_bcs.f90 get_br_v_bcs."""
+        a, b = self.coef(), self.coef()
+        self._L.orc_get_br_v_bcs(ctypes.c_void_p(self.h), _ptr(self._g(br)), _ptr(self._g(vt)),
+                                 _ptr(self._g(vp)), float(omega), float(O_r_E_2), float(O_rho),
+                                 _ptr(a), _ptr(b))
+        return a, b
 
 
 def have_ref():

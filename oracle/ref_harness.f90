@@ -9,10 +9,12 @@
 !
 ! It also drives the reference's lm-space assembly of the explicit terms
 ! (src/get_td.f90, module nonlinear_lm_mod) in mode td and its Courant
-! step limits (src/courant.f90) on the step's grids in mode courant.
+! step limits (src/courant.f90) on the step's grids in mode courant, and
+! the boundary products of a conducting mantle or inner core
+! (src/nonlinear_bcs.f90, get_br_v_bcs) in mode brv.
 !
 ! Usage:  ref_harness MODE l_max minc nalias infile outfile [nlev nrep]
-!   MODE = tables | api | nl | nlstep | bench | td | courant
+!   MODE = tables | api | nl | nlstep | bench | td | courant | brv
 !
 ! File formats are little-endian streams: int32 header words followed by
 ! float64 arrays in Fortran (column-major) order.
@@ -32,6 +34,7 @@ program ref_harness
    use physical_parameters, only: LFfac, n_r_LCR, CorFac, epsc, opm
    use num_param, only: delxr2, delxh2
    use courant_mod, only: courant
+   use nonlinear_bcs, only: get_br_v_bcs
    use fields, only: w_Rloc, dw_Rloc, ddw_Rloc, z_Rloc, dz_Rloc
    use nonlinear_lm_mod, only: nonlinear_lm_t
    use sht
@@ -99,6 +102,8 @@ program ref_harness
       call do_td()
    case ('courant')
       call do_courant()
+   case ('brv')
+      call do_brv()
    case default
       write(output_unit,*) 'unknown mode ', trim(mode)
       stop 2
@@ -388,6 +393,32 @@ contains
       close(u)
       close(v)
    end subroutine do_courant
+
+   subroutine do_brv()
+      !-- reads nlev levels; per level: omega, O_r_E_2, O_rho and the grids
+      !   br, vt, vp (n_theta, n_phi); writes br_vt_lm, br_vp_lm of
+      !   src/nonlinear_bcs.f90 get_br_v_bcs
+      real(cp), allocatable :: G(:,:,:)
+      complex(cp), allocatable :: a(:), b(:)
+      real(cp) :: om, o2, orh
+      integer :: u, v, lev
+
+      allocate( G(n_theta_max,n_phi_max,3), a(lm_max), b(lm_max) )
+      open(newunit=u, file=trim(infile), access='stream', form='unformatted', &
+      &    status='old')
+      open(newunit=v, file=trim(outfile), access='stream', form='unformatted', &
+      &    status='replace')
+      call write_header(v)
+      do lev=1,nlev
+         read(u) om, o2, orh
+         read(u) G(:,:,:)
+         a(:)=(0.0_cp,0.0_cp); b(:)=(0.0_cp,0.0_cp)
+         call get_br_v_bcs(G(:,:,1), G(:,:,2), G(:,:,3), om, o2, orh, a, b)
+         write(v) a, b
+      end do
+      close(u)
+      close(v)
+   end subroutine do_brv
 
    subroutine do_bench()
       !-- Times the per-level transform sequence of a Boussinesq MHD bulk

@@ -232,6 +232,42 @@ def courant_case(tag, l_max, nlev, write=True):
     return data
 
 
+def brv_case(tag, l_max, minc, config, write=True):
+    """The reference's boundary products of a conducting mantle and inner
+    core (src/nonlinear_bcs.f90 get_br_v_bcs, ref_harness brv): two levels,
+    the outer boundary with omega_ma and the inner one with omega_ic, on
+    seeded grids uniform in [-1, 1]."""
+    n_theta, n_phi = workload.grid_size(l_max, 20, minc)
+    nlev = 2
+    omega = np.array([0.7, -1.3])
+    or2 = np.array([1.0 / 1.538 ** 2, 1.0 / 0.538 ** 2])
+    orho = np.array([1.0, 0.85])
+    grids = []
+    for lev in range(nlev):
+        u = workload.uniform01(workload.stream_seed(config, 77, lev), 3 * n_phi * n_theta)
+        grids.append((2.0 * u - 1.0).reshape(3, n_phi, n_theta))
+    grids = np.stack(grids)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            for i in range(nlev):
+                np.array([omega[i], or2[i], orho[i]], dtype="<f8").tofile(f)
+                np.ascontiguousarray(grids[i], dtype="<f8").tofile(f)
+        oracle.run_ref("brv", l_max, minc, 20, fin, fout, nlev=nlev, threads=1)
+        with open(fout, "rb") as f:
+            hdr = read_header(f)
+            lm_max = hdr["lm_max"]
+            out = np.fromfile(f, "<c16", 2 * nlev * lm_max).reshape(nlev, 2, lm_max)
+            assert f.read() == b""
+    data = dict(l_max=l_max, minc=minc, n_theta=n_theta, n_phi=n_phi, omega=omega, or2=or2,
+                orho=orho, grids=grids, br_vt_lm=out[:, 0], br_vp_lm=out[:, 1])
+    if write:
+        path = os.path.join(OUT, "brv_%s.npz" % tag)
+        np.savez_compressed(path, **data)
+        print("wrote", path, os.path.getsize(path))
+    return data
+
+
 def main():
     if not oracle.have_ref():
         sys.exit("oracle/_ref/ref_harness is not built (make -C oracle)")
@@ -247,6 +283,8 @@ def main():
     td_case("L16", 16, 3)
     td_case("L64", 64, 2)
     courant_case("L16", 16, 3)
+    brv_case("L16_m1", 16, 1, config=21)
+    brv_case("L64_m4", 64, 4, config=22)
 
 
 if __name__ == "__main__":

@@ -138,6 +138,39 @@ def test_oracle_courant_matches_reference():
         assert abs(np.sqrt(dh2 / b) / g["dthkc"][lev] - 1) < 1e-14
 
 
+@pytest.mark.parametrize("fname", ["brv_L16_m1.npz", "brv_L64_m4.npz"])
+def test_oracle_get_br_v_bcs_matches_reference(fname):
+    """Boundary products br*vt, br*(vp - omega sin^2) analysed to
+    (0, br_vt_lm, br_vp_lm) (src/nonlinear_bcs.f90 get_br_v_bcs) against the
+    reference's own routine run by ref_harness brv: 1e-12 relative"""
+    g = load_golden(fname)
+    p = oracle.Plan(int(g["l_max"]), int(g["n_theta"]), int(g["n_phi"]), minc=int(g["minc"]))
+    for lev in range(g["grids"].shape[0]):
+        a, b = p.get_br_v_bcs(*g["grids"][lev], g["omega"][lev], g["or2"][lev], g["orho"][lev])
+        assert rel_l2(a, g["br_vt_lm"][lev]) < 1e-12, (fname, lev)
+        assert rel_l2(b, g["br_vp_lm"][lev]) < 1e-12, (fname, lev)
+
+
+def test_br_v_bcs_solid_body_rotation():
+    """A uniform br = 1 over a fluid at rest on a boundary rotating at
+    omega: br_vp = -omega sin^2(theta) is solid-body rotation, so the
+    analysis holds only the toroidal l=1, m=0 coefficient and resynthesises
+    the grid; br_vt_lm vanishes."""
+    L = 16
+    nt, nph = workload.grid_size(L)
+    p = oracle.Plan(L, nt, nph)
+    one, zero = np.ones((nph, nt)), np.zeros((nph, nt))
+    a, b = p.get_br_v_bcs(one, zero, zero, 0.9, 2.0, 1.0)
+    assert np.abs(a).max() < 1e-15
+    assert abs(b[1]) > 0.1 and np.abs(np.delete(b, 1)).max() < 1e-14
+    t = p.tables()
+    s2 = np.empty(nt)
+    s2[0::2] = s2[1::2] = np.sin(t["theta"]) ** 2
+    vt, vp = p.sphtor_to_spat(a, b, L)
+    np.testing.assert_allclose(vt, 0.0, atol=1e-14)
+    np.testing.assert_allclose(vp, np.broadcast_to(-0.9 * s2, vp.shape), atol=1e-14)
+
+
 def test_gauleg_vs_numpy():
     for n in (24, 96, 384):
         th, w = oracle.gauleg(n)
