@@ -227,6 +227,17 @@ int mhx_get_td(mhx_plan *p, int nlev, const double *const *in, size_t ld_lm,
                double *const *out, const double *or2, const double *or4,
                const double *epsc, double CorFac, const int *lcut, void *stream);
 
+/* Per-level sums of the kinetic-energy diagnostic (src/kinetic_energy.f90
+ * get_e_kin, :119-160, the loop over radial levels and lm before the radial
+ * integrals): for nlev levels of w, dw, z (complex, level stride ld_lm,
+ * host or device) and per level or2, orho1 (host), writes sums[k*nlev + i]
+ * (host or device) for k = e_p_r, e_t_r, e_p_as_r, e_t_as_r, e_p_es_r,
+ * e_t_es_r, e_p_eas_r, e_t_eas_r.  The caller integrates them over radius
+ * (rInt_R, src/integration.f90:56-103) and multiplies by eScale/2.
+ * Synchronous; FP64. */
+int mhx_get_e_kin(mhx_plan *p, int nlev, const double *w, const double *dw, const double *z,
+                  size_t ld_lm, const double *or2, const double *orho1, double *sums);
+
 /* ---- uneven batches of independent transform jobs (split form) ----
  * A job is the round trip of nfields fields of one kind at one truncation
  * over nlev radial levels: synthesis to grids and analysis back, i.e. per

@@ -153,6 +153,17 @@ struct TdArgs {
     int l_max, lm_max;
 };
 
+struct EkArgs {
+    const double *w, *dw, *z;  // complex, level stride ld_lm
+    size_t ld_lm;
+    const double *rad;         // device: or2[nlev], orho1[nlev]
+    double *out;               // device: 8 sums x nlev
+    int nlev;
+    // filled by the launcher
+    const int *lm2l, *lm2m;
+    int lm_max;
+};
+
 // ------------------------------------------------------------------ plan
 struct Plan {
     int l_max, m_max, minc, n_theta, n_phi, nh, n_m, lm_max, precision;
@@ -188,6 +199,7 @@ struct Plan {
     int cour_nlev = 0;
     std::vector<double> h_cour;
     // per-level factors of mhx_get_td (or2, or4, epsc; l_R)
+    double *d_ek = nullptr;  int ek_nlev = 0;  // energy sums: radial factors + outputs
     double *d_td_rad = nullptr;
     int *d_td_lcut = nullptr;
     int td_nlev = 0;
@@ -264,6 +276,7 @@ int build_tables(Plan *p);
 int build_split_tables(Plan *p);
 int launch_nsyn16(const Plan *p, int kind, const NSynArgs &args, hipStream_t s);
 int launch_get_td(const Plan *p, const TdArgs &args, hipStream_t s);
+int launch_e_kin(const Plan *p, const EkArgs &args, hipStream_t s);
 int launch_nana16(const Plan *p, int kind, const NAnaArgs &args, hipStream_t s);
 int launch_syn(const Plan *p, const SynJob &job, hipStream_t s);
 int launch_ana(const Plan *p, const AnaJob &job, hipStream_t s);

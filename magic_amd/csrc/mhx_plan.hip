@@ -486,6 +486,7 @@ int mhx_plan_destroy(mhx_plan *h)
     free_dev(p->d_cour_par);
     free_dev(p->d_cour_max);
     free_dev(p->d_td_rad);
+    free_dev(p->d_ek);
     free_dev(p->d_td_lcut);
     free_dev(p->d_rec_aseed);
     free_dev(p->d_rec_aexp);
@@ -1233,6 +1234,61 @@ int mhx_nl_step(mhx_plan *h, int nlev, const double *const *in, size_t ld_lm, do
     int rc = nl_step_impl(h, nlev, in, ld_lm, out, or2, or4, orho1, LFfac, lcut, nullptr, stream);
     if (rc) return rc;
     return finish_step(h, stream);
+}
+
+int mhx_get_e_kin(mhx_plan *h, int nlev, const double *w, const double *dw, const double *z,
+                  size_t ld_lm, const double *or2, const double *orho1, double *sums)
+{
+    if (!h) return fail(MHX_E_INVALID, "null plan");
+    Plan *P = &h->p;
+    if (nlev <= 0) return MHX_OK;
+    if (!w || !dw || !z || !or2 || !orho1 || !sums) return fail(MHX_E_INVALID, "null pointer");
+    if (ld_lm < (size_t)P->lm_max) return fail(MHX_E_INVALID, "ld_lm below lm_max");
+    int rc;
+    const size_t arr = (size_t)nlev * ld_lm * 2;
+    const double *in[3] = {w, dw, z};
+    bool hin[3];
+    int nh = 0;
+    for (int i = 0; i < 3; ++i) nh += (hin[i] = !is_device_ptr(in[i]));
+    if (nh && (rc = ensure_buf(P->ws_H, P->ws_H_bytes, (size_t)nh * arr * sizeof(double)))) return rc;
+    if (P->ek_nlev < nlev) {
+        free_dev(P->d_ek);
+        P->ek_nlev = 0;
+        MHX_HIP_CHECK(hipMalloc(&P->d_ek, 10 * (size_t)nlev * sizeof(double)));
+        P->ek_nlev = nlev;
+    }
+    // the plan stream may still read the previous call's host staging
+    MHX_HIP_CHECK(hipStreamSynchronize(P->stream));
+    EkArgs a;
+    std::memset(&a, 0, sizeof(a));
+    const double *src[3];
+    int slot = 0;
+    for (int i = 0; i < 3; ++i) {
+        src[i] = in[i];
+        if (!hin[i]) continue;
+        double *d = P->ws_H + (size_t)slot++ * arr;
+        MHX_HIP_CHECK(hipMemcpyAsync(d, in[i], arr * sizeof(double), hipMemcpyHostToDevice, P->stream));
+        src[i] = d;
+    }
+    std::vector<double> rad(2 * (size_t)nlev);
+    for (int i = 0; i < nlev; ++i) {
+        rad[i] = or2[i];
+        rad[nlev + i] = orho1[i];
+    }
+    MHX_HIP_CHECK(hipMemcpyAsync(P->d_ek, rad.data(), 2 * (size_t)nlev * sizeof(double),
+                                 hipMemcpyHostToDevice, P->stream));
+    a.w = src[0];
+    a.dw = src[1];
+    a.z = src[2];
+    a.ld_lm = ld_lm;
+    a.rad = P->d_ek;
+    a.out = P->d_ek + 2 * (size_t)nlev;
+    a.nlev = nlev;
+    if ((rc = launch_e_kin(P, a, P->stream))) return rc;
+    MHX_HIP_CHECK(hipMemcpyAsync(sums, a.out, 8 * (size_t)nlev * sizeof(double), hipMemcpyDefault,
+                                 P->stream));
+    MHX_HIP_CHECK(hipStreamSynchronize(P->stream));
+    return MHX_OK;
 }
 
 int mhx_get_td(mhx_plan *h, int nlev, const double *const *in, size_t ld_lm, double *const *out,

@@ -16,6 +16,8 @@
 // One thread per (lm, level), 256 consecutive lm per block: the l -/+ 1
 // neighbours of a coefficient are the adjacent entries of its row (m-major,
 // l fastest).  Memory bound: 10 complex in and 7 out per (lm, level).
+// The same file holds the per-level kinetic-energy sums of the energy
+// diagnostics (k_e_kin), another lm-space pass over the levels' fields.
 #include "mhx_internal.h"
 
 #include <cmath>
@@ -104,7 +106,72 @@ __global__ __launch_bounds__(256) void k_get_td(TdArgs a)
     o[6][i] = cmulr(-dLh * r2, in[5][i]);
 }
 
+// Per-level sums of the kinetic energy (src/kinetic_energy.f90:119-160):
+// e_p = orho1 dLh (dLh or2 |w|^2 + |dw|^2), e_t = orho1 dLh |z|^2 with
+// |c|^2 = Re(c)^2 at m = 0 and 2 |c|^2 otherwise, split into all, m = 0
+// (axisymmetric), l + m even / odd (equatorially symmetric poloidal /
+// toroidal) and m = 0 with l even / odd.  One block per level; each thread
+// accumulates a strided subset of lm, then a fixed tree over the block, so
+// the result does not depend on timing.
+__global__ __launch_bounds__(256) void k_e_kin(EkArgs a)
+{
+    const int lev = blockIdx.x;
+    const double2 *w = reinterpret_cast<const double2 *>(a.w) + (size_t)lev * a.ld_lm;
+    const double2 *dw = reinterpret_cast<const double2 *>(a.dw) + (size_t)lev * a.ld_lm;
+    const double2 *z = reinterpret_cast<const double2 *>(a.z) + (size_t)lev * a.ld_lm;
+    const double or2 = a.rad[lev], orho = a.rad[a.nlev + lev];
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int lm = threadIdx.x; lm < a.lm_max; lm += blockDim.x) {
+        const int l = a.lm2l[lm], m = a.lm2m[lm];
+        const double dLh = (double)(l * (l + 1));
+        const double2 cw = w[lm], cd = dw[lm], cz = z[lm];
+        double nw, nd, nz;
+        if (m == 0) {
+            nw = cw.x * cw.x;
+            nd = cd.x * cd.x;
+            nz = cz.x * cz.x;
+        } else {
+            nw = 2.0 * (cw.x * cw.x + cw.y * cw.y);
+            nd = 2.0 * (cd.x * cd.x + cd.y * cd.y);
+            nz = 2.0 * (cz.x * cz.x + cz.y * cz.y);
+        }
+        const double ep = orho * dLh * (dLh * or2 * nw + nd);
+        const double et = orho * dLh * nz;
+        acc[0] += ep;
+        acc[1] += et;
+        if (m == 0) {
+            acc[2] += ep;
+            acc[3] += et;
+            if ((l & 1) == 0) acc[6] += ep;
+            else acc[7] += et;
+        }
+        if (((l + m) & 1) == 0) acc[4] += ep;
+        else acc[5] += et;
+    }
+    __shared__ double red[8][256];
+    for (int k = 0; k < 8; ++k) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int k = 0; k < 8; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x < 8) a.out[(size_t)threadIdx.x * a.nlev + lev] = red[threadIdx.x][0];
+}
+
 }  // namespace
+
+int launch_e_kin(const Plan *p, const EkArgs &args, hipStream_t s)
+{
+    if (args.nlev <= 0) return MHX_OK;
+    EkArgs a = args;
+    a.lm2l = p->d_lm2l;
+    a.lm2m = p->d_lm2m;
+    a.lm_max = p->lm_max;
+    k_e_kin<<<(unsigned)a.nlev, 256, 0, s>>>(a);
+    MHX_HIP_CHECK(hipGetLastError());
+    return MHX_OK;
+}
 
 int launch_get_td(const Plan *p, const TdArgs &args, hipStream_t s)
 {

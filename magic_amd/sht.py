@@ -54,6 +54,7 @@ _SIGS = {
     "mhx_nl_step": "Pipzppppdpp",
     "mhx_nl_step_grids": "Pipzppppdppp",
     "mhx_get_td": "Pipzppppdpp",
+    "mhx_get_e_kin": "Pipppzppp",
     "mhx_nl_step_courant": "Pipzppppdpddppppp",
     "mhx_set_timing": "Pi",
     "mhx_stage_times": "Pp",
@@ -367,6 +368,27 @@ def nl_step_courant(plan, inputs, outputs, or2, or4, orho1, LFfac, lcut, courfac
 
 TD_INPUTS = ("AdvrLM", "AdvtLM", "AdvpLM", "VStLM", "VxBrLM", "VxBtLM", "VxBpLM", "w", "dw", "z")
 TD_OUTPUTS = ("dwdt", "dzdt", "dpdt", "dsdt", "dbdt", "djdt", "dVxBhLM")
+
+
+def get_e_kin_levels(plan, w, dw, z, or2, orho1):
+    """Per-level kinetic-energy sums (src/kinetic_energy.f90:119-160) of the
+    (nlev, lm_max) complex arrays w, dw, z (numpy or device tensors): an
+    (8, nlev) array e_p, e_t, e_p_as, e_t_as, e_p_es, e_t_es, e_p_eas,
+    e_t_eas per level."""
+    nlev = int(w.shape[0])
+    arrs = []
+    for a in (w, dw, z):
+        if hasattr(a, "data_ptr"):
+            arrs.append(a)
+        else:
+            arrs.append(np.ascontiguousarray(a, dtype=np.complex128).reshape(nlev, plan.lm_max))
+    ptr = lambda a: ctypes.c_void_p(a.data_ptr() if hasattr(a, "data_ptr")  # noqa: E731
+                                    else a.ctypes.data)
+    d = lambda v: (ctypes.c_double * nlev)(*[float(x) for x in v])  # noqa: E731
+    out = np.zeros((8, nlev))
+    _check(plan._L.mhx_get_e_kin(plan.h, nlev, ptr(arrs[0]), ptr(arrs[1]), ptr(arrs[2]),
+                                 plan.lm_max, d(or2), d(orho1), ptr(out)))
+    return out
 
 
 def get_td(plan, inputs, outputs, or2, or4, epsc, CorFac, lcut, stream=None):

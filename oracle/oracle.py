@@ -301,3 +301,82 @@ def run_ref(mode, l_max, minc, nalias, infile, outfile, nlev=1, nrep=1, threads=
         REF_HARNESS, mode, l_max, minc, nalias, infile, outfile, nlev, nrep)
     return subprocess.run(["bash", "-c", cmd], env=env, cwd=cwd, check=True,
                           capture_output=True, text=True)
+
+
+# ---------------------------------------------------------------- radial / energies
+def cheb_nodes(N):
+    return np.cos(np.pi * np.arange(N) / (N - 1))
+
+
+def _cheb_interp(N):
+    """T[k, n] = T_n(x_k) at the Gauss-Lobatto nodes and its inverse (the
+    interpolant's coefficients of nodal values)."""
+    th = np.pi * np.arange(N) / (N - 1)
+    T = np.cos(np.outer(th, np.arange(N)))
+    return T, np.linalg.inv(T)
+
+
+def cheb_dr(f, r):
+    """d f / d r of nodal values f[..., k] on a linearly mapped Gauss-Lobatto
+    grid r (outer boundary first), by differentiating the interpolating
+    Chebyshev series: the reference's get_dr for the Chebyshev scheme
+    (src/radial_derivatives.f90, via costf1 and get_dcheb) in exact
+    arithmetic.  T_n'(x) = n sin(n th)/sin(th), n^2 at x = 1, (-1)^(n+1) n^2
+    at x = -1."""
+    N = r.size
+    T, Ti = _cheb_interp(N)
+    n = np.arange(N, dtype=np.float64)
+    th = np.pi * np.arange(N) / (N - 1)
+    Tp = np.empty((N, N))
+    Tp[0] = n ** 2
+    Tp[-1] = (-1.0) ** (n + 1) * n ** 2
+    Tp[1:-1] = n[None, :] * np.sin(np.outer(th[1:-1], n)) / np.sin(th[1:-1])[:, None]
+    drx = 2.0 / (r[0] - r[-1])
+    D = drx * (Tp @ Ti)
+    return np.tensordot(D, f, axes=([1], 0))
+
+
+def rint_interp(f, r):
+    """Integral over [r[-1], r[0]] of the polynomial interpolating f at the
+    radii (src/integration.f90:56-96, rInt_R, Chebyshev scheme), from the
+    interpolant's coefficients and int_{-1}^{1} T_n = 2/(1-n^2) (n even)."""
+    N = r.size
+    _, Ti = _cheb_interp(N)
+    n = np.arange(N)
+    wt = np.zeros(N)
+    wt[n % 2 == 0] = 2.0 / (1.0 - n[n % 2 == 0].astype(np.float64) ** 2)
+    return (wt @ Ti) @ np.asarray(f).T * (0.5 * (r[0] - r[-1]))
+
+
+def e_kin_levels(w, dw, z, or2, orho1, l_max, minc=1):
+    """Per-level sums of src/kinetic_energy.f90:119-160 (get_e_kin):
+    rows e_p, e_t, e_p_as, e_t_as, e_p_es, e_t_es, e_p_eas, e_t_eas."""
+    out = np.zeros((8, w.shape[0]))
+    lm = 0
+    for m in range(0, l_max + 1, minc):
+        for l in range(m, l_max + 1):
+            dLh = float(l * (l + 1))
+            cw, cd, cz = w[:, lm], dw[:, lm], z[:, lm]
+            if m == 0:
+                nw, nd, nz = cw.real ** 2, cd.real ** 2, cz.real ** 2
+            else:
+                nw = 2.0 * (cw.real ** 2 + cw.imag ** 2)
+                nd = 2.0 * (cd.real ** 2 + cd.imag ** 2)
+                nz = 2.0 * (cz.real ** 2 + cz.imag ** 2)
+            ep = orho1 * dLh * (dLh * or2 * nw + nd)
+            et = orho1 * dLh * nz
+            out[0] += ep
+            out[1] += et
+            if m == 0:
+                out[2] += ep
+                out[3] += et
+                if l % 2 == 0:
+                    out[6] += ep
+                else:
+                    out[7] += et
+            if (l + m) % 2 == 0:
+                out[4] += ep
+            else:
+                out[5] += et
+            lm += 1
+    return out

@@ -1,0 +1,87 @@
+"""Restart-file reader (magic_amd.checkpoint, src/readCheckPoints.f90
+readStartFields) and the kinetic-energy diagnostic (src/kinetic_energy.f90
+get_e_kin) pinned by the reference's own sample: the restart file of
+samples/boussBenchSat and its e_kin reference output, whose first row is the
+energy of the restart state itself (written before the first step).
+
+The oracle's energies of that state match the reference's printed values
+(ES16.8, 9 significant digits) to 2e-8 relative; the product's radial
+integral is exact on polynomials and equals the oracle's."""
+import os
+
+import numpy as np
+import pytest
+
+from magic_amd import checkpoint, energy, workload
+from oracle import oracle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CKPT = os.path.join(GOLDEN, "boussBenchSat_checkpoint_end.start")
+EKIN = os.path.join(GOLDEN, "boussBenchSat_e_kin_reference.out")
+
+
+def _state():
+    h = checkpoint.read_checkpoint(CKPT)
+    return h, np.loadtxt(EKIN)
+
+
+def test_checkpoint_header():
+    h, ref = _state()
+    assert h["version"] == 4 and h["family"] == "DIRK"
+    assert (h["l_max"], h["m_max"], h["minc"], h["n_r_max"], h["n_r_ic_max"]) == (64, 64, 4, 33, 17)
+    assert (h["n_theta_max"], h["n_phi_tot"], h["nalias"]) == (96, 192, 20)
+    assert h["lm_max"] == 561 == workload.lm_index(64, None, 4)[0].size
+    assert abs(h["time"] - ref[0, 0]) < 1e-9
+    assert (h["ra"], h["ek"], h["prmag"], h["radratio"]) == (1.1e5, 1e-3, 5.0, 0.35)
+    assert h["l_heat"] and h["l_mag"] and h["l_cond_ic"] and not h["l_chemical_conv"]
+    for k in ("w", "z", "p", "s", "b", "aj"):
+        assert h[k].shape == (33, 561)
+    assert h["b_ic"].shape == (17, 561)
+    # m = 0 coefficients are real, l = 0 of the potentials vanishes
+    assert np.abs(h["w"][:, :65].imag).max() < 1e-15 * np.abs(h["w"]).max()
+    assert np.abs(h["w"][:, 0]).max() == 0.0 and np.abs(h["z"][:, 0]).max() == 0.0
+
+
+def test_checkpoint_rejects_truncated(tmp_path):
+    data = open(CKPT, "rb").read()
+    p = tmp_path / "cut.start"
+    p.write_bytes(data[:-16])
+    with pytest.raises(ValueError):
+        checkpoint.read_checkpoint(str(p))
+
+
+def test_oracle_e_kin_matches_reference_output():
+    h, ref = _state()
+    r = h["r"]
+    w, z = h["w"], h["z"]
+    dw = oracle.cheb_dr(w, r)
+    sums = oracle.e_kin_levels(w, dw, z, 1.0 / r ** 2, np.ones_like(r), 64, 4)
+    e = 0.5 * oracle.rint_interp(sums, r)
+    np.testing.assert_allclose(e, ref[0, 1:9], rtol=2e-8, atol=0)
+
+
+def test_rint_cheb_exact_and_matches_oracle():
+    h, _ = _state()
+    r = h["r"]
+    ri, ro = r[-1], r[0]
+    for p in range(0, 33):
+        exact = (ro ** (p + 1) - ri ** (p + 1)) / (p + 1)
+        assert abs(energy.rint_cheb(r ** p, r) / exact - 1) < 1e-13, p
+    f = np.stack([np.exp(r), np.sin(3 * r), 1.0 / r])
+    np.testing.assert_allclose(energy.rint_cheb(f, r), oracle.rint_interp(f, r), rtol=1e-13)
+    with pytest.raises(ValueError):
+        energy.rint_cheb(r, np.linspace(ri, ro, r.size))
+
+
+def test_cheb_dr_exact_on_polynomials():
+    h, _ = _state()
+    r = h["r"]
+    # Chebyshev polynomials of the mapped coordinate up to the grid's degree
+    x = (2.0 * r - r[0] - r[-1]) / (r[0] - r[-1])
+    drx = 2.0 / (r[0] - r[-1])
+    for n in range(1, r.size):
+        c = np.zeros(n + 1)
+        c[n] = 1.0
+        exact = drx * np.polynomial.chebyshev.chebval(x, np.polynomial.chebyshev.chebder(c))
+        err = np.abs(oracle.cheb_dr(np.polynomial.chebyshev.chebval(x, c), r) - exact).max()
+        assert err < 1e-12 * drx * n * n, n
