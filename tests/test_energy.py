@@ -1,8 +1,9 @@
 """Restart-file reader (magic_amd.checkpoint, src/readCheckPoints.f90
 readStartFields) and the kinetic-energy diagnostic (src/kinetic_energy.f90
 get_e_kin) pinned by the reference's own sample: the restart file of
-samples/boussBenchSat and its e_kin reference output, whose first row is the
-energy of the restart state itself (written before the first step).
+samples/boussBenchSat and samples/doubleDiffusion and their e_kin reference
+outputs, whose first row is the energy of the restart state itself (written
+before the first step).
 
 The oracle's energies of that state match the reference's printed values
 (ES16.8, 9 significant digits) to 2e-8 relative; the product's radial
@@ -50,12 +51,15 @@ def test_checkpoint_rejects_truncated(tmp_path):
         checkpoint.read_checkpoint(str(p))
 
 
-def test_oracle_e_kin_matches_reference_output():
-    h, ref = _state()
+@pytest.mark.parametrize("sample", ["boussBenchSat", "doubleDiffusion"])
+def test_oracle_e_kin_matches_reference_output(sample):
+    h = checkpoint.read_checkpoint(os.path.join(GOLDEN, sample + "_checkpoint_end.start"))
+    ref = np.loadtxt(os.path.join(GOLDEN, sample + "_e_kin_reference.out"))
+    assert abs(h["time"] - ref[0, 0]) < 1e-9
     r = h["r"]
     w, z = h["w"], h["z"]
     dw = oracle.cheb_dr(w, r)
-    sums = oracle.e_kin_levels(w, dw, z, 1.0 / r ** 2, np.ones_like(r), 64, 4)
+    sums = oracle.e_kin_levels(w, dw, z, 1.0 / r ** 2, np.ones_like(r), h["l_max"], h["minc"])
     e = 0.5 * oracle.rint_interp(sums, r)
     np.testing.assert_allclose(e, ref[0, 1:9], rtol=2e-8, atol=0)
 

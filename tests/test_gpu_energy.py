@@ -1,6 +1,6 @@
 """Kinetic-energy diagnostic on the GPU (mhx_get_e_kin + magic_amd.energy,
 src/kinetic_energy.f90 get_e_kin): the energies of the reference's
-boussBenchSat restart state against the first row of its e_kin reference
+boussBenchSat and doubleDiffusion restart states against the first row of its e_kin reference
 output (2e-8 relative, the printed precision), the per-level sums against
 the oracle (1e-13) with host and device arrays, and at l_max=255 x 97."""
 import os
@@ -16,11 +16,12 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def test_e_kin_matches_reference_output():
+@pytest.mark.parametrize("sample", ["boussBenchSat", "doubleDiffusion"])
+def test_e_kin_matches_reference_output(sample):
     import torch
-    print("e_kin boussBenchSat", flush=True)
-    h = checkpoint.read_checkpoint(os.path.join(GOLDEN, "boussBenchSat_checkpoint_end.start"))
-    ref = np.loadtxt(os.path.join(GOLDEN, "boussBenchSat_e_kin_reference.out"))
+    print("e_kin", sample, flush=True)
+    h = checkpoint.read_checkpoint(os.path.join(GOLDEN, sample + "_checkpoint_end.start"))
+    ref = np.loadtxt(os.path.join(GOLDEN, sample + "_e_kin_reference.out"))
     r, w, z = h["r"], h["w"], h["z"]
     dw = oracle.cheb_dr(w, r)
     nt, nph = workload.grid_size(h["l_max"], h["nalias"], h["minc"])
