@@ -11,10 +11,11 @@
 ! (src/get_td.f90, module nonlinear_lm_mod) in mode td and its Courant
 ! step limits (src/courant.f90) on the step's grids in mode courant, and
 ! the boundary products of a conducting mantle or inner core
-! (src/nonlinear_bcs.f90, get_br_v_bcs) in mode brv.
+! (src/nonlinear_bcs.f90, get_br_v_bcs) in mode brv; mode rloop chains the
+! three for the bulk levels of the radial loop (src/rIter.f90:188-470).
 !
 ! Usage:  ref_harness MODE l_max minc nalias infile outfile [nlev nrep]
-!   MODE = tables | api | nl | nlstep | bench | td | courant | brv
+!   MODE = tables | api | nl | nlstep | bench | td | courant | brv | rloop
 !
 ! File formats are little-endian streams: int32 header words followed by
 ! float64 arrays in Fortran (column-major) order.
@@ -104,6 +105,8 @@ program ref_harness
       call do_courant()
    case ('brv')
       call do_brv()
+   case ('rloop')
+      call do_rloop()
    case default
       write(output_unit,*) 'unknown mode ', trim(mode)
       stop 2
@@ -393,6 +396,73 @@ contains
       close(u)
       close(v)
    end subroutine do_courant
+
+   subroutine do_rloop()
+      !-- the bulk levels of the radial loop (src/rIter.f90:188-470, nBc = 0)
+      !   in its order: nl_level (transforms and get_nl), courant on the
+      !   level's grids (dtrkc, dthkc from 1e10), then get_dwdt, get_dzdt,
+      !   get_dpdt, get_dsdt, get_dbdt on the forward outputs.  Reads nlev
+      !   levels; per level: lcut, or2, or4, orho1, LFfac, CorFac, epsc,
+      !   courfac, alffac, opm, delxr2, delxh2 and the 11 coefficient arrays
+      !   of nlstep; writes dwdt dzdt dpdt dsdt dbdt djdt dVxBhLM, dtrkc,
+      !   dthkc.
+      type(grid_space_arrays_t) :: gsa
+      type(nonlinear_lm_t) :: td
+      complex(cp), allocatable :: C(:,:), O(:,:), T(:,:), dVSrLM(:)
+      real(cp), allocatable :: uGs(:,:)
+      integer :: u, v, lev, i
+      integer(int32) :: lcut32
+      real(cp) :: o2, o4, orh, lff, cof, ep, cf, af, pm, dr2, dh2, dtr, dth
+
+      l_mag_kin=.false.; l_cour_alf_damp=.true.
+      l_conv=.true.; l_corr=.true.; l_heat=.true.
+      l_single_matrix=.false.; l_double_curl=.false.
+      call td%initialize(lm_max)
+      allocate( C(lm_max,11), O(lm_max,10), T(lm_max,7), dVSrLM(lm_max) )
+      allocate( uGs(n_theta_max,n_phi_max) )
+      allocate( w_Rloc(lm_max,1), dw_Rloc(lm_max,1), ddw_Rloc(lm_max,1) )
+      allocate( z_Rloc(lm_max,1), dz_Rloc(lm_max,1) )
+      if ( .not. allocated(l_R) ) allocate( l_R(1), epscProf(1) )
+      if ( .not. allocated(orho2) ) allocate( orho2(1) )
+      if ( .not. allocated(delxr2) ) allocate( delxr2(1), delxh2(1) )
+      open(newunit=u, file=trim(infile), access='stream', form='unformatted', &
+      &    status='old')
+      open(newunit=v, file=trim(outfile), access='stream', form='unformatted', &
+      &    status='replace')
+      call write_header(v)
+      do lev=1,nlev
+         read(u) lcut32, o2, o4, orh, lff, cof, ep, cf, af, pm, dr2, dh2
+         call set_radial(o2, o4, orh)
+         orho2(1)=orh*orh; LFfac=lff; CorFac=cof; epsc=ep; epscProf(1)=1.0_cp
+         opm=pm; delxr2(1)=dr2; delxh2(1)=dh2; l_R(1)=int(lcut32)
+         if ( lev == 1 ) call gsa%initialize()
+         do i=1,11
+            read(u) C(:,i)
+         end do
+         call nl_level(gsa, C, int(lcut32), O, uGs)
+         dtr=1.0e10_cp; dth=1.0e10_cp
+         call courant(1, dtr, dth, gsa%vrc, gsa%vtc, gsa%vpc, gsa%brc, gsa%btc, &
+         &            gsa%bpc, cf, af)
+         td%AdvrLM(1:lm_max)=O(:,1); td%AdvtLM(1:lm_max)=O(:,2)
+         td%AdvpLM(1:lm_max)=O(:,3); td%VStLM(1:lm_max)=O(:,5)
+         td%VxBrLM(1:lm_max)=O(:,7); td%VxBtLM(1:lm_max)=O(:,8)
+         td%VxBpLM(1:lm_max)=O(:,9)
+         w_Rloc(:,1)=C(:,1); dw_Rloc(:,1)=C(:,2); ddw_Rloc(:,1)=C(:,3)
+         z_Rloc(:,1)=C(:,4); dz_Rloc(:,1)=C(:,5)
+         T(:,:)=0.0_cp
+         call td%get_dwdt(1, 0, T(:,1))
+         call td%get_dzdt(1, 0, T(:,2))
+         call td%get_dpdt(1, 0, T(:,3))
+         call td%get_dsdt(1, 0, T(:,4), dVSrLM)
+         call td%get_dbdt(1, 0, T(:,5), T(:,6), T(:,7))
+         do i=1,7
+            write(v) T(:,i)
+         end do
+         write(v) dtr, dth
+      end do
+      close(u)
+      close(v)
+   end subroutine do_rloop
 
    subroutine do_brv()
       !-- reads nlev levels; per level: omega, O_r_E_2, O_rho and the grids

@@ -268,6 +268,43 @@ def brv_case(tag, l_max, minc, config, write=True):
     return data
 
 
+def rloop_case(tag, write=True):
+    """The bulk levels of the reference's radial loop (ref_harness rloop:
+    nl_level, courant, get_td per level) on the interior levels of the
+    boussBenchSat restart state (tests/restart_state.py)."""
+    import restart_state
+    h, ins, P = restart_state.load(np.arange(1, 32))
+    nlev = ins[0].shape[0]
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            for i in range(nlev):
+                np.array([P["lcut"][i]], dtype="<i4").tofile(f)
+                np.array([P["or2"][i], P["or4"][i], P["orho1"][i], P["LFfac"], P["CorFac"],
+                          P["epsc"][i], P["courfac"], P["alffac"], P["opm"], P["delxr2"][i],
+                          P["delxh2"][i]], dtype="<f8").tofile(f)
+                for k in range(11):
+                    np.ascontiguousarray(ins[k][i], dtype="<c16").tofile(f)
+        oracle.run_ref("rloop", h["l_max"], h["minc"], h["nalias"], fin, fout, nlev=nlev,
+                       threads=1)
+        lm = h["lm_max"]
+        td_out, dt = [], []
+        with open(fout, "rb") as f:
+            read_header(f)
+            for i in range(nlev):
+                td_out.append(np.fromfile(f, "<c16", 7 * lm).reshape(7, lm))
+                dt.append(np.fromfile(f, "<f8", 2))
+            assert f.read() == b""
+    td_out = np.stack(td_out, axis=1)  # (7, nlev, lm)
+    dt = np.array(dt)
+    data = dict(levels=np.arange(1, 32), td=td_out, dtrkc=dt[:, 0], dthkc=dt[:, 1])
+    if write:
+        path = os.path.join(OUT, "rloop_%s.npz" % tag)
+        np.savez_compressed(path, **data)
+        print("wrote", path, os.path.getsize(path))
+    return data
+
+
 def main():
     if not oracle.have_ref():
         sys.exit("oracle/_ref/ref_harness is not built (make -C oracle)")
@@ -285,6 +322,7 @@ def main():
     courant_case("L16", 16, 3)
     brv_case("L16_m1", 16, 1, config=21)
     brv_case("L64_m4", 64, 4, config=22)
+    rloop_case("boussBenchSat")
 
 
 if __name__ == "__main__":

@@ -171,6 +171,31 @@ def test_br_v_bcs_solid_body_rotation():
     np.testing.assert_allclose(vp, np.broadcast_to(-0.9 * s2, vp.shape), atol=1e-14)
 
 
+def test_oracle_radial_loop_matches_reference():
+    """The oracle's nonlinear step, Courant maxima and get_td chained over
+    the interior levels of the boussBenchSat restart state against the
+    reference's own chain (ref_harness rloop): 1e-12"""
+    import restart_state
+    from nl_reference import oracle_step
+    g = load_golden("rloop_boussBenchSat.npz")
+    h, ins, P = restart_state.load(g["levels"])
+    L = h["l_max"]
+    nt, nph = workload.grid_size(L, h["nalias"], h["minc"])
+    o = oracle.Plan(L, nt, nph, minc=h["minc"])
+    G, outs = oracle_step(o, ins, P["or2"], P["or4"], P["orho1"], P["LFfac"], P["lcut"])
+    for i in range(ins[0].shape[0]):
+        arr = [outs[k][i] for k in (0, 1, 2, 4, 6, 7, 8)] + [ins[0][i], ins[1][i], ins[3][i]]
+        td = o.get_td(L, P["or2"][i], P["or4"][i], P["CorFac"], P["epsc"][i], arr)
+        for k in range(7):
+            assert rel_l2(td[k], g["td"][k][i]) < 1e-12, (i, k)
+        vri2 = (0.5 * (1 + P["opm"])) ** 2 / P["delxr2"][i]
+        vhi2 = (0.5 * (1 + P["opm"])) ** 2 / P["delxh2"][i]
+        a, b = o.courant(P["or2"][i], P["or4"][i], P["orho1"][i], P["LFfac"], P["courfac"],
+                         P["alffac"], vri2, vhi2, [G[i, c] for c in (0, 1, 2, 9, 10, 11)])
+        assert abs(np.sqrt(P["delxr2"][i] / a) / g["dtrkc"][i] - 1) < 1e-13
+        assert abs(np.sqrt(P["delxh2"][i] / b) / g["dthkc"][i] - 1) < 1e-13
+
+
 def test_gauleg_vs_numpy():
     for n in (24, 96, 384):
         th, w = oracle.gauleg(n)
