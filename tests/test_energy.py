@@ -89,3 +89,60 @@ def test_cheb_dr_exact_on_polynomials():
         exact = drx * np.polynomial.chebyshev.chebval(x, np.polynomial.chebyshev.chebder(c))
         err = np.abs(oracle.cheb_dr(np.polynomial.chebyshev.chebval(x, c), r) - exact).max()
         assert err < 1e-12 * drx * n * n, n
+
+
+def _write_multistep(path, rng, l_max=8, minc=1, n_r=5, nexp=3, nimp=2, nold=2):
+    """A version-5 multistep restart file in the record order of the
+    reference's writer (src/storeCheckPoints.f90:45-277): MHD with a
+    conducting inner core, no composition or phase field."""
+    lm = checkpoint.lm_count(l_max, 0, l_max, minc)
+    n_ic = 3
+    f = {k: rng.standard_normal((n_r, lm)) + 1j * rng.standard_normal((n_r, lm))
+         for k in ("w", "z", "p", "s", "b", "aj")}
+    f["b_ic"] = rng.standard_normal((n_ic, lm)) + 0j
+    f["aj_ic"] = rng.standard_normal((n_ic, lm)) + 0j
+    nh = (nexp - 1) + (nimp - 1) + (nold - 1)
+    hist = {k: [rng.standard_normal((v.shape[0], lm)) + 0j for _ in range(nh)] for k, v in f.items()}
+    with open(path, "wb") as o:
+        w = lambda a, dt: np.asarray(a, dtype=dt).tofile(o)  # noqa: E731
+        w([5], "<i4")
+        w([12.5], "<f8")
+        o.write(b"MULTISTEP ")
+        w([nexp, nimp, nold], "<i4")
+        w(np.linspace(1e-4, 3e-4, nexp), "<f8")
+        w([77], "<i4")
+        w([1e5, 1.0, 0.0, 10.0, 5.0, 1e-3, 0.0, 0.35, 1.0], "<f8")
+        w([n_r, 12, 24, minc, 20, n_ic], "<i4")
+        w([l_max, 0, l_max], "<i4")
+        o.write(b"cheb".ljust(72))
+        w([n_r, 0], "<i4")
+        w([0.8, 0.0], "<f8")
+        w(np.linspace(1.5, 0.5, n_r), "<f8")
+        w(np.arange(2 * nh) + 0.5, "<f8")  # d omega_ic / dt, d omega_ma / dt histories
+        w(np.arange(12) * 0.1, "<f8")
+        w([1, 0, 0, 1, 1, 1], "<i4")  # heat, chem, phase, mag, press, cond_ic
+        for k in ("w", "z", "p", "s", "b", "aj", "b_ic", "aj_ic"):
+            w(f[k], "<c16")  # (n_r, lm) C order = (lm, n_r) Fortran order
+            for a in hist[k]:
+                w(a, "<c16")
+    return f, hist
+
+
+def test_checkpoint_multistep_layout(tmp_path):
+    rng = np.random.default_rng(5)
+    p = str(tmp_path / "ms.start")
+    f, hist = _write_multistep(p, rng)
+    h = checkpoint.read_checkpoint(p, with_history=True)
+    assert h["version"] == 5 and h["family"] == "MULTISTEP" and h["n_time_step"] == 77
+    assert (h["nexp"], h["nimp"], h["nold"]) == (3, 2, 2) and h["dt"].shape == (3,)
+    assert h["l_cond_ic"] and not h["l_chemical_conv"] and "xi" not in h
+    np.testing.assert_array_equal(h["omega"], np.arange(12) * 0.1)
+    for k, v in f.items():
+        np.testing.assert_array_equal(h[k], v)
+        assert len(h[k + "_hist"]) == len(hist[k])
+        for a, b in zip(h[k + "_hist"], hist[k]):
+            np.testing.assert_array_equal(a, b)
+    data = open(p, "rb").read()
+    (tmp_path / "long.start").write_bytes(data + b"\0" * 8)
+    with pytest.raises(ValueError):
+        checkpoint.read_checkpoint(str(tmp_path / "long.start"))
