@@ -150,6 +150,56 @@ def run_td(L, nlev, tout, rad, lcut, reps=20):
             "frac_of_hbm_peak": byts / (ms * 1e-3) / 8.0e12}
 
 
+def run_diag(L, nlev, tin, reps=20):
+    """Two FP64 calls next to the step, host-synchronous as the reference
+    makes them, timed per call with the host clock (launches, the copy of
+    the results and the wait included): the kinetic-energy sums over all
+    levels (mhx_get_e_kin; algorithmic bytes 3 complex per level and lm) and
+    the boundary products of one boundary (mhx_get_br_v_bcs on device grids:
+    3 grids read, 2 written, then the sphertor analysis)."""
+    import ctypes
+    import time
+
+    import numpy as np
+    import torch
+
+    from magic_amd import sht, workload
+    nt, nph = workload.grid_size(L)
+    p = sht.Plan(L, nt, nph, max_levels=1)
+    r = np.linspace(0.54, 1.54, nlev)
+    or2, orho = 1.0 / r ** 2, np.ones(nlev)
+    w, dw, z = tin[0], tin[1], tin[3]
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    ms_ek = timed(lambda: sht.get_e_kin_levels(p, w, dw, z, or2, orho))
+    by_ek = 48.0 * p.lm_max * nlev
+    g = torch.rand((3, nph, nt), dtype=torch.float64, device=tin[0].device) - 0.5
+    oa = torch.zeros(p.lm_max, dtype=torch.complex128, device=g.device)
+    ob = torch.zeros_like(oa)
+    vp = ctypes.c_void_p
+
+    def brv():
+        sht._check(p._L.mhx_get_br_v_bcs(p.h, vp(g[0].data_ptr()), vp(g[1].data_ptr()),
+                                         vp(g[2].data_ptr()), ctypes.c_double(0.7),
+                                         ctypes.c_double(0.42), ctypes.c_double(1.0),
+                                         vp(oa.data_ptr()), vp(ob.data_ptr())))
+    ms_bv = timed(brv)
+    p.close()
+    return ({"entry": "mhx_get_e_kin", "levels": nlev, "avg_ms_per_call": ms_ek,
+             "algorithmic_bytes": by_ek, "GB/s": by_ek / (ms_ek * 1e-3) / 1e9,
+             "timing": "host clock, synchronous call"},
+            {"entry": "mhx_get_br_v_bcs", "avg_ms_per_call": ms_bv,
+             "grid": [nt, nph], "timing": "host clock, synchronous call"})
+
+
 def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut, courant=False):
     import numpy as np
     import torch
@@ -218,8 +268,9 @@ def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut, courant=False):
                          "GB/s": v["bytes"] / v["n"] / (ms * 1e-3) / 1e9}
     outs = [t.cpu().numpy() for t in tout]
     td = run_td(L, nlev, tout, rad, lcut) if precision == sht.PREC_FP64 else None
+    diag = run_diag(L, nlev, tin) if precision == sht.PREC_FP64 else None
     p.close()
-    return {"td": td, "pairs": nlev * steps / dt, "ms_per_step": 1e3 * dt / steps,
+    return {"td": td, "diag": diag, "pairs": nlev * steps / dt, "ms_per_step": 1e3 * dt / steps,
             "per_kernel": per_kernel, "kernels": kernels,
             "legendre_tflops": total_flop / (flop_ms * 1e-3) / 1e12, "outs": outs}
 
@@ -485,6 +536,8 @@ def main():
     if "fp64" in forms and forms["fp64"].get("td"):
         # the next stage of the time step (src/get_td.f90), FP64, not in `value`
         res["td_assembly"] = forms["fp64"]["td"]
+        if forms["fp64"].get("diag"):
+            res["e_kin"], res["boundary_products"] = forms["fp64"]["diag"]
         log("get_td: %.3f ms per call (%.0f GB/s)" % (res["td_assembly"]["avg_ms_per_call"],
                                                        res["td_assembly"]["GB/s"]))
     if "fp64" in forms and main_key != "fp64":

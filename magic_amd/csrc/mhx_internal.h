@@ -153,10 +153,12 @@ struct TdArgs {
     int l_max, lm_max;
 };
 
+constexpr int EK_CHUNKS = 16;  // lm chunks (blocks) per level of the energy sums
 struct EkArgs {
     const double *w, *dw, *z;  // complex, level stride ld_lm
     size_t ld_lm;
     const double *rad;         // device: or2[nlev], orho1[nlev]
+    double *part;              // device: 8 x nlev x EK_CHUNKS partial sums
     double *out;               // device: 8 sums x nlev
     int nlev;
     // filled by the launcher

@@ -1254,7 +1254,7 @@ int mhx_get_e_kin(mhx_plan *h, int nlev, const double *w, const double *dw, cons
     if (P->ek_nlev < nlev) {
         free_dev(P->d_ek);
         P->ek_nlev = 0;
-        MHX_HIP_CHECK(hipMalloc(&P->d_ek, 10 * (size_t)nlev * sizeof(double)));
+        MHX_HIP_CHECK(hipMalloc(&P->d_ek, (10 + 8 * (size_t)EK_CHUNKS) * nlev * sizeof(double)));
         P->ek_nlev = nlev;
     }
     // the plan stream may still read the previous call's host staging
@@ -1283,6 +1283,7 @@ int mhx_get_e_kin(mhx_plan *h, int nlev, const double *w, const double *dw, cons
     a.ld_lm = ld_lm;
     a.rad = P->d_ek;
     a.out = P->d_ek + 2 * (size_t)nlev;
+    a.part = P->d_ek + 10 * (size_t)nlev;
     a.nlev = nlev;
     if ((rc = launch_e_kin(P, a, P->stream))) return rc;
     MHX_HIP_CHECK(hipMemcpyAsync(sums, a.out, 8 * (size_t)nlev * sizeof(double), hipMemcpyDefault,

@@ -20,6 +20,7 @@
 // diagnostics (k_e_kin), another lm-space pass over the levels' fields.
 #include "mhx_internal.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace mhx {
@@ -110,18 +111,20 @@ __global__ __launch_bounds__(256) void k_get_td(TdArgs a)
 // e_p = orho1 dLh (dLh or2 |w|^2 + |dw|^2), e_t = orho1 dLh |z|^2 with
 // |c|^2 = Re(c)^2 at m = 0 and 2 |c|^2 otherwise, split into all, m = 0
 // (axisymmetric), l + m even / odd (equatorially symmetric poloidal /
-// toroidal) and m = 0 with l even / odd.  One block per level; each thread
-// accumulates a strided subset of lm, then a fixed tree over the block, so
-// the result does not depend on timing.
+// toroidal) and m = 0 with l even / odd.  Blocks (chunk, level): each
+// thread accumulates a strided subset of the chunk's lm, a fixed tree over
+// the block gives the chunk's partial sums, and k_e_kin_sum adds the
+// partials of a level in chunk order, so the result does not depend on
+// timing.
 __global__ __launch_bounds__(256) void k_e_kin(EkArgs a)
 {
-    const int lev = blockIdx.x;
+    const int lev = blockIdx.y, nch = gridDim.x;
     const double2 *w = reinterpret_cast<const double2 *>(a.w) + (size_t)lev * a.ld_lm;
     const double2 *dw = reinterpret_cast<const double2 *>(a.dw) + (size_t)lev * a.ld_lm;
     const double2 *z = reinterpret_cast<const double2 *>(a.z) + (size_t)lev * a.ld_lm;
     const double or2 = a.rad[lev], orho = a.rad[a.nlev + lev];
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int lm = threadIdx.x; lm < a.lm_max; lm += blockDim.x) {
+    for (int lm = blockIdx.x * blockDim.x + threadIdx.x; lm < a.lm_max; lm += nch * blockDim.x) {
         const int l = a.lm2l[lm], m = a.lm2m[lm];
         const double dLh = (double)(l * (l + 1));
         const double2 cw = w[lm], cd = dw[lm], cz = z[lm];
@@ -156,7 +159,16 @@ __global__ __launch_bounds__(256) void k_e_kin(EkArgs a)
             for (int k = 0; k < 8; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + st];
         __syncthreads();
     }
-    if (threadIdx.x < 8) a.out[(size_t)threadIdx.x * a.nlev + lev] = red[threadIdx.x][0];
+    if (threadIdx.x < 8) a.part[((size_t)threadIdx.x * a.nlev + lev) * nch + blockIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_e_kin_sum(const double *__restrict__ part, int n, int nch, double *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // k * nlev + level
+    if (i >= n) return;
+    double s = 0.0;
+    for (int c = 0; c < nch; ++c) s += part[(size_t)i * nch + c];
+    out[i] = s;
 }
 
 }  // namespace
@@ -168,7 +180,11 @@ int launch_e_kin(const Plan *p, const EkArgs &args, hipStream_t s)
     a.lm2l = p->d_lm2l;
     a.lm2m = p->d_lm2m;
     a.lm_max = p->lm_max;
-    k_e_kin<<<(unsigned)a.nlev, 256, 0, s>>>(a);
+    const int nch = std::min(EK_CHUNKS, (a.lm_max + 255) / 256);
+    k_e_kin<<<dim3((unsigned)nch, (unsigned)a.nlev), 256, 0, s>>>(a);
+    MHX_HIP_CHECK(hipGetLastError());
+    const int n = 8 * a.nlev;
+    k_e_kin_sum<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a.part, n, nch, a.out);
     MHX_HIP_CHECK(hipGetLastError());
     return MHX_OK;
 }
