@@ -234,7 +234,7 @@ int mhx_get_td(mhx_plan *p, int nlev, const double *const *in, size_t ld_lm,
  * (host or device) for k = e_p_r, e_t_r, e_p_as_r, e_t_as_r, e_p_es_r,
  * e_t_es_r, e_p_eas_r, e_t_eas_r.  The caller integrates them over radius
  * (rInt_R, src/integration.f90:56-103) and multiplies by eScale/2.
- * Synchronous; FP64. */
+ * At most 65535 levels per call.  Synchronous; FP64. */
 int mhx_get_e_kin(mhx_plan *p, int nlev, const double *w, const double *dw, const double *z,
                   size_t ld_lm, const double *or2, const double *orho1, double *sums);
 

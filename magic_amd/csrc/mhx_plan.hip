@@ -1243,6 +1243,7 @@ int mhx_get_e_kin(mhx_plan *h, int nlev, const double *w, const double *dw, cons
     Plan *P = &h->p;
     if (nlev <= 0) return MHX_OK;
     if (!w || !dw || !z || !or2 || !orho1 || !sums) return fail(MHX_E_INVALID, "null pointer");
+    if (nlev > 65535) return fail(MHX_E_INVALID, "more than 65535 levels in one call");
     if (ld_lm < (size_t)P->lm_max) return fail(MHX_E_INVALID, "ld_lm below lm_max");
     int rc;
     const size_t arr = (size_t)nlev * ld_lm * 2;
