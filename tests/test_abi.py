@@ -75,3 +75,24 @@ def test_plan_without_gpu_fails_loudly():
         pytest.skip("libmhx.so not built")
     with pytest.raises(sht.MhxError):
         sht.Plan(16)
+
+
+def test_level_entries_reject_null_plan():
+    """The lm-space and boundary entries return an error (no GPU touched)
+    for a null plan, with a message."""
+    import ctypes
+    L = sht.lib()
+    vp = ctypes.c_void_p
+    z = vp(0)
+    calls = {
+        "mhx_get_br_v_bcs": lambda: L.mhx_get_br_v_bcs(None, z, z, z, 0.0, 1.0, 1.0, z, z),
+        "mhx_get_e_kin": lambda: L.mhx_get_e_kin(None, 1, z, z, z, 1, z, z, z),
+        "mhx_get_td": lambda: L.mhx_get_td(None, 1, z, 1, z, z, z, z, 1.0, z, None),
+        "mhx_nl_step_courant": lambda: L.mhx_nl_step_courant(None, 1, z, 1, z, z, z, z, 1.0, z,
+                                                             1.0, 1.0, z, z, z, z, None),
+    }
+    for name, call in calls.items():
+        rc = call()
+        assert rc < 0, name
+        msg = L.mhx_last_error()
+        assert msg and b"null" in (msg if isinstance(msg, bytes) else msg.encode()), name
