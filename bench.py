@@ -432,7 +432,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         log("bench.py measures one GPU only: the transform path is independent per radial "
-            "level and this build runs on a single MI355X (see DESIGN.md for sharding)")
+            "level, so several GPUs would each run their own share of the levels; only "
+            "--gpus 1 is measured")
         sys.exit(2)
 
     import numpy as np
