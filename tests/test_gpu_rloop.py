@@ -40,3 +40,31 @@ def test_radial_loop_bulk_matches_reference(precision, tol, tol_dt):
         np.testing.assert_allclose(dtr, g["dtrkc"], rtol=tol_dt, atol=0)
         np.testing.assert_allclose(dth, g["dthkc"], rtol=tol_dt, atol=0)
     p.close()
+
+
+def test_radial_loop_device_fields_and_shape_check():
+    """Device-tensor fields give the same result as host arrays; a field of
+    the wrong shape is refused before any launch."""
+    import torch
+    g = load_golden("rloop_boussBenchSat.npz")
+    h, ins, P = restart_state.load(g["levels"][:6])
+    nlev = ins[0].shape[0]
+    nt, nph = workload.grid_size(h["l_max"], h["nalias"], h["minc"])
+    print("radial loop device fields", nlev, flush=True)
+    p = sht.Plan(h["l_max"], nt, nph, minc=h["minc"], max_levels=nlev)
+    rl = rloop.RadialLoop(p, nlev)
+    args = (P["or2"], P["or4"], P["orho1"], P["LFfac"], P["CorFac"], P["epsc"], P["lcut"],
+            P["courfac"], P["alffac"], P["opm"], P["delxr2"], P["delxh2"])
+    td_h, _, dtr_h, dth_h = rl.bulk(ins, *args)
+    td_h = [t.cpu().numpy() for t in td_h]
+    dev = [torch.from_numpy(a).cuda() for a in ins]
+    td_d, _, dtr_d, dth_d = rl.bulk(dev, *args)
+    for k in range(7):
+        np.testing.assert_array_equal(td_d[k].cpu().numpy(), td_h[k])
+    np.testing.assert_array_equal(dtr_d, dtr_h)
+    np.testing.assert_array_equal(dth_d, dth_h)
+    for k in range(7):
+        assert rel_l2(td_h[k], g["td"][k][:6]) < 1e-12
+    with pytest.raises(ValueError):
+        rl.bulk([a[:, :-1] for a in ins], *args)
+    p.close()
