@@ -11,6 +11,10 @@ Grid sizing follows the reference's truncation rules
 30 l_max / nalias, n_phi = 2 n_theta rounded up to a 2^i 3^j 5^k multiple of
 minc, then n_theta = n_phi_tot / 2.
 """
+import concurrent.futures
+import functools
+import os
+
 import numpy as np
 
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
@@ -38,12 +42,20 @@ def stream_seed(config, field, level):
 
 
 def lm_index(l_max, m_max=None, minc=1):
+    """(l, m) of every lm in the standard m-major order (src/blocking.f90:309-317)."""
     m_max = l_max if m_max is None else min(m_max, l_max)
-    ls, ms = [], []
-    for m in range(0, m_max + 1, minc):
-        ls.extend(range(m, l_max + 1))
-        ms.extend([m] * (l_max - m + 1))
-    return np.array(ls, dtype=np.int64), np.array(ms, dtype=np.int64)
+    ls, ms = _lm_index(l_max, m_max, minc)
+    return ls.copy(), ms.copy()
+
+
+@functools.lru_cache(maxsize=16)
+def _lm_index(l_max, m_max, minc):
+    m = np.arange(0, m_max + 1, minc, dtype=np.int64)
+    cnt = l_max - m + 1
+    ms = np.repeat(m, cnt)
+    start = np.repeat(np.cumsum(cnt) - cnt, cnt)
+    ls = ms + np.arange(ms.size, dtype=np.int64) - start
+    return ls, ms
 
 
 def spectrum(l, kind):
@@ -99,8 +111,20 @@ def nl_inputs(l_max, nlev, config=2, kind="l^-5/3", minc=1, dipole=1.0):
             fac = 1.0 / lsafe
         if dipole != 1.0 and name in ("b", "db", "ddb"):
             fac = fac * np.where(ls == 1, dipole, 1.0)
-        out.append(np.stack([coefficients(l_max, config, f, lev, kf, potential=(name != "s"),
-                                          minc=minc) * fac for lev in range(nlev)]))
+        arr = np.empty((nlev, ls.size), dtype=np.complex128)
+
+        def fill(lev, f=f, kf=kf, fac=fac, name=name, arr=arr):
+            arr[lev] = coefficients(l_max, config, f, lev, kf, potential=(name != "s"),
+                                    minc=minc) * fac
+        # levels are independent streams: filled by a thread pool (numpy's
+        # array operations release the interpreter lock)
+        if nlev * ls.size >= 1 << 20:
+            with concurrent.futures.ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
+                list(ex.map(fill, range(nlev)))
+        else:
+            for lev in range(nlev):
+                fill(lev)
+        out.append(arr)
     return out
 
 
