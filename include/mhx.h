@@ -227,6 +227,70 @@ int mhx_get_td(mhx_plan *p, int nlev, const double *const *in, size_t ld_lm,
                double *const *out, const double *or2, const double *or4,
                const double *epsc, double CorFac, const int *lcut, void *stream);
 
+/* ---- the radial loop for any physics switches, interior levels ----------
+ * One call replaces the loop body of rIter_single_t%radialLoop
+ * (src/rIter.f90:188-444) for nlev interior levels (nBc = 0) of a time step
+ * stage, on the GPU in FP64 whatever the plan's precision: the inverse
+ * transforms of transform_to_grid_space (:466-622), the grid products of
+ * get_nl (src/get_nl.f90:213-441) merged as transform_to_lm_space
+ * (:644-680), the forward transforms (:682-707), the Courant steps
+ * (src/courant.f90:54-277, XSH_COURANT = 0) and the lm-space assembly of
+ * nonlinear_lm_t (src/get_td.f90: get_dwdt or get_dwdt_double_curl,
+ * get_dzdt, get_dpdt, get_dsdt, get_dxidt, get_dbdt).  Not covered (the
+ * caller runs the reference's own loop for such stages): boundary levels,
+ * the diagnostics computed inside the loop (graph, movie, TO, RMS, power,
+ * helicity, ...), l_full_sphere's centre, finite-difference l_parallel_solve
+ * and the temperature-diffusion single-matrix form.
+ *
+ * The switches and constants the loop reads from the reference's logic,
+ * physical_parameters and time-scheme data; booleans as 0/1. */
+typedef struct mhx_rloop_phys {
+    int l_conv, l_conv_nl, l_heat, l_heat_nl, l_chemical_conv, l_phase_field;
+    int l_mag, l_mag_LF, l_mag_nl, l_mag_kin, l_anel, l_anelastic_liquid;
+    int l_adv_curl, l_double_curl, l_corr, l_single_matrix, l_precession;
+    int l_centrifuge, l_cour_alf_damp, lPressNext;
+    int n_r_LCR;  /* levels with nR <= n_r_LCR lie outside the conducting region */
+    double LFfac, CorFac, epsc, epscXi, ViscHeatFac, OhmLossFac, opm;
+    double courfac, alffac;  /* tscheme%courfac, tscheme%alffac */
+    double epsPhase, penaltyFac, phaseDiffFac, tmelt;
+    double po, prec_angle, oek, timeStage, dilution_fac, ra, opr;
+} mhx_rloop_phys;
+
+/* per-level values (host arrays of nlev entries): the reference's
+ * radial_functions at each level, its radial index nR (1 = outer boundary)
+ * and cut-off l_R(nR) */
+typedef struct mhx_rloop_radial {
+    const double *r, *or1, *or2, *or4, *orho1, *beta, *otemp1, *visc, *lambda;
+    const double *temp0, *epscProf, *delxr2, *delxh2;
+    const int *nR, *l_R;
+} mhx_rloop_radial;
+
+/* coefficient inputs (complex, level stride ld_lm; the fields module's
+ * *_Rloc): w dw ddw z dz always, s with l_heat, xi with l_chemical_conv, phi
+ * with l_phase_field, b db ddb aj dj with l_mag; others may be NULL */
+typedef struct mhx_rloop_fields {
+    const double *w, *dw, *ddw, *z, *dz, *s, *xi, *phi, *b, *db, *ddb, *aj, *dj;
+} mhx_rloop_fields;
+
+/* outputs (complex, level stride ld_lm; the radialLoop_if arguments):
+ * dwdt dzdt dpdt always (dpdt skipped, may be NULL, with l_double_curl and
+ * not lPressNext), dsdt dVSrLM with l_heat, dxidt dVXirLM with
+ * l_chemical_conv, dphidt with l_phase_field, dbdt djdt dVxBhLM with l_mag,
+ * dVxVhLM with l_double_curl; dtrkc dthkc (double, nlev) the Courant steps
+ * (1e10 where the level imposes none) */
+typedef struct mhx_rloop_out {
+    double *dwdt, *dzdt, *dpdt, *dsdt, *dVSrLM, *dxidt, *dVXirLM, *dphidt;
+    double *dbdt, *djdt, *dVxBhLM, *dVxVhLM;
+    double *dtrkc, *dthkc;
+} mhx_rloop_out;
+
+/* Arrays host or device memory (the library detects which; host arrays are
+ * staged); the stream as in mhx_nl_step.  MHX_E_UNSUPPORTED for switches
+ * outside the list above (l_mag_kin, l_single_matrix with l_corr). */
+int mhx_rloop(mhx_plan *p, const mhx_rloop_phys *phys, int nlev, const mhx_rloop_radial *rad,
+              const mhx_rloop_fields *in, size_t ld_lm, const mhx_rloop_out *out,
+              void *stream);
+
 /* Per-level sums of the kinetic-energy diagnostic (src/kinetic_energy.f90
  * get_e_kin, :119-160, the loop over radial levels and lm before the radial
  * integrals): for nlev levels of w, dw, z (complex, level stride ld_lm,
@@ -237,6 +301,17 @@ int mhx_get_td(mhx_plan *p, int nlev, const double *const *in, size_t ld_lm,
  * At most 65535 levels per call.  Synchronous; FP64. */
 int mhx_get_e_kin(mhx_plan *p, int nlev, const double *w, const double *dw, const double *z,
                   size_t ld_lm, const double *or2, const double *orho1, double *sums);
+
+/* Per-level sums of the magnetic-energy diagnostic of the outer core
+ * (src/magnetic_energy.f90 get_e_mag, :264-340): for nlev levels of b, db,
+ * aj (complex, level stride ld_lm, host or device) and per level or2
+ * (host), writes sums[k*nlev + i] (host or device) for k = e_p_r, e_t_r,
+ * e_p_as_r, e_t_as_r, e_p_es_r, e_t_es_r, e_p_eas_r, e_t_eas_r,
+ * e_dipole_r, e_dipole_ax_r.  The caller integrates over radius and
+ * multiplies by LFfac eScale / 2 (els_r = (e_p_r + e_t_r) orho1 sigma).
+ * At most 65535 levels per call.  Synchronous; FP64. */
+int mhx_get_e_mag(mhx_plan *p, int nlev, const double *b, const double *db, const double *aj,
+                  size_t ld_lm, const double *or2, double *sums);
 
 /* ---- uneven batches of independent transform jobs (split form) ----
  * A job is the round trip of nfields fields of one kind at one truncation

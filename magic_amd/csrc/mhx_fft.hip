@@ -911,6 +911,259 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
     forward6(1);
 }
 
+// ---------------------------------------------------------------------------
+// Grid stage of the radial loop for any physics switches (mhx_rloop), FP64:
+// for one radial level and one theta row, the inverse FFTs of the level's
+// synthesised components into LDS rows, the reference's grid products
+// (src/get_nl.f90:240-436 for a bulk level, merged as src/rIter.f90:644-680)
+// and the Courant maxima (src/courant.f90:209-275) per grid point, and the
+// forward FFTs of the products.  Each LDS row holds one component's N = n_phi
+// grid values (M complex); the products of a point overwrite its inputs in
+// place (one thread owns a point), so the rows hold max(nin, nout) components.
+// FFTs run G rows at a time through the block-wide Stockham passes.
+
+// the grid products of one point: v[] the grid components (zero where
+// absent), o[] the products
+__device__ inline void rl_products(const RlPhys &ph, const double *v, double *o, double o2,
+                                   double cosn2, double sinT, double cosT, double osinT,
+                                   double phi, double r, double or1, double or2, double or4,
+                                   double orho1, double beta, double otemp1, double visc,
+                                   double lambda, bool lcr)
+{
+    const double vr = v[RI_VR], vt = v[RI_VT], vp = v[RI_VP];
+#pragma unroll
+    for (int k = 0; k < RO_N; ++k) o[k] = 0.0;
+    // Lorentz force (:242-258), not below the conducting region
+    double LFr = 0.0, LFt = 0.0, LFp = 0.0;
+    const bool lf = ph.mag_LF && !lcr;
+    if (lf) {
+        const double br = v[RI_BR], bt = v[RI_BT], bp = v[RI_BP];
+        const double cbr = v[RI_CBR], cbt = v[RI_CBT], cbp = v[RI_CBP];
+        LFr = ph.LFfac * o2 * (cbt * bp - cbp * bt);
+        LFt = ph.LFfac * or4 * (cbp * br - cbr * bp);
+        LFp = ph.LFfac * or4 * (cbr * bt - cbt * br);
+    }
+    double Ar = 0.0, At = 0.0, Ap = 0.0;
+    if (ph.conv_nl) {
+        if (ph.adv_curl) {
+            // curl(u) x u (:262-273)
+            const double cvr = v[RI_CVR], cvt = v[RI_CVT], cvp = v[RI_CVP];
+            Ar = -o2 * (cvt * vp - cvp * vt);
+            At = -or4 * (cvp * vr - cvr * vp);
+            Ap = -or4 * (cvr * vt - cvt * vr);
+        } else {
+            // u . grad u (:274-308)
+            const double dvrdr = v[RI_DVRDR], dvtdr = v[RI_DVTDR], dvpdr = v[RI_DVPDR];
+            const double dvrdt = v[RI_DVRDT], dvrdp = v[RI_DVRDP];
+            const double dvtdp = v[RI_DVTDP], dvpdp = v[RI_DVPDP], cvr = v[RI_CVR];
+            Ar = -or2 * orho1 *
+                 (vr * (dvrdr - (2.0 * or1 + beta) * vr) +
+                  o2 * (vt * (dvrdt - r * vt) + vp * (dvrdp - r * vp)));
+            At = or4 * orho1 *
+                 (-vr * (dvtdr - beta * vt) + vt * (cosn2 * vt + dvpdp + dvrdr) +
+                  vp * (cosn2 * vp - dvtdp));
+            Ap = or4 * orho1 * (-vr * (dvpdr - beta * vp) - vt * (dvtdp + cvr) - vp * dvpdp);
+        }
+    }
+    const double s = v[RI_S];
+    if (ph.phase) {
+        // phase field (:333-344)
+        const double ph2 = 1.0 / (ph.epsPhase * ph.epsPhase) / (ph.penaltyFac * ph.penaltyFac);
+        const double pc = v[RI_PHI];
+        Ar = Ar - pc * vr * ph2;
+        At = At - or2 * pc * vt * ph2;
+        Ap = Ap - or2 * pc * vp * ph2;
+        o[RO_PHI] = -1.0 / (ph.epsPhase * ph.epsPhase) * pc * (1.0 - pc) *
+                    (ph.phaseDiffFac * (1.0 - 2.0 * pc) + s - ph.tmelt);
+    }
+    // Adv + LF (src/rIter.f90:648-660): LF alone without nonlinear advection
+    if (ph.conv_nl) {
+        if (lf) {
+            Ar += LFr;
+            At += LFt;
+            Ap += LFp;
+        }
+    } else if (ph.mag_LF) {
+        Ar = LFr;
+        At = LFt;
+        Ap = LFp;
+    }
+    if (ph.prec) {
+        // precession (:346-356), added as src/rIter.f90:662-666
+        const double c = cos(ph.oek * ph.time + phi), sn = sin(ph.oek * ph.time + phi);
+        Ar += ph.posnalp * osinT * r * (c * vp * cosT + sn * vt);
+        At += -ph.posnalp * sinT * or2 * (c * vp + sn * or1 * vr);
+        Ap += ph.posnalp * sinT * c * or2 * (vt - or1 * vr * cosT);
+    }
+    if (ph.centr) {
+        // centrifugal acceleration (:358-374), added as src/rIter.f90:668-671
+        const double s2 = sinT * sinT;
+        Ar += -ph.dilution_fac * r * (s2 * s2) * ph.ra * ph.opr * s;
+        At += -ph.dilution_fac * r * (s2 * sinT) * cosT * ph.ra * ph.opr * s;
+    }
+    o[RO_ADVR] = Ar;
+    o[RO_ADVT] = At;
+    o[RO_ADVP] = Ap;
+    if (ph.heat_nl) {
+        // V s (:320-325)
+        o[RO_VSR] = vr * s;
+        o[RO_VST] = or2 * vt * s;
+        o[RO_VSP] = or2 * vp * s;
+    }
+    if (ph.chem) {
+        // V xi (:327-331)
+        const double xi = v[RI_XI];
+        o[RO_VXIR] = vr * xi;
+        o[RO_VXIT] = or2 * vt * xi;
+        o[RO_VXIP] = or2 * vp * xi;
+    }
+    if (ph.mag_nl) {
+        const double br = v[RI_BR], bt = v[RI_BT], bp = v[RI_BP];
+        if (!lcr) {
+            // V x B (:376-388)
+            o[RO_VXBR] = orho1 * o2 * (vt * bp - vp * bt);
+            o[RO_VXBT] = orho1 * or4 * (vp * br - vr * bp);
+            o[RO_VXBP] = orho1 * or4 * (vr * bt - vt * br);
+        } else {
+            // below the conducting region (:389-391): the horizontal part only
+            o[RO_VXBT] = or4 * orho1 * vp * br;
+            o[RO_VXBP] = -or4 * orho1 * vt * br;
+        }
+    }
+    if (ph.anel) {
+        // viscous heating and Ohmic losses (:402-436)
+        const double dvrdr = v[RI_DVRDR], dvtdr = v[RI_DVTDR], dvpdr = v[RI_DVPDR];
+        const double dvrdt = v[RI_DVRDT], dvrdp = v[RI_DVRDP];
+        const double dvtdp = v[RI_DVTDP], dvpdp = v[RI_DVPDP], cvr = v[RI_CVR];
+        const double t1 = dvrdr - (2.0 * or1 + beta) * vr;
+        const double t2 = cosn2 * vt + dvpdp + dvrdr - or1 * vr;
+        const double t3 = dvpdp + cosn2 * vt + or1 * vr;
+        const double t6 = 2.0 * dvtdp + cvr - 2.0 * cosn2 * vp;
+        const double t4 = r * dvtdr - (2.0 + beta * r) * vt + or1 * dvrdt;
+        const double t5 = r * dvpdr - (2.0 + beta * r) * vp + or1 * dvrdp;
+        const double bv = beta * vr;
+        double h = ph.ViscHeatFac * or4 * orho1 * otemp1 * visc *
+                   (2.0 * (t1 * t1) + 2.0 * (t2 * t2) + 2.0 * (t3 * t3) + t6 * t6 +
+                    o2 * (t4 * t4 + t5 * t5) - 2.0 * (1.0 / 3.0) * (bv * bv));
+        if (ph.mag_nl && !lcr) {
+            const double cbr = v[RI_CBR], cbt = v[RI_CBT], cbp = v[RI_CBP];
+            h = h + ph.OhmLossFac * or2 * otemp1 * lambda *
+                        (or2 * (cbr * cbr) + o2 * (cbt * cbt) + o2 * (cbp * cbp));
+        }
+        o[RO_HEAT] = h;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rl_grid(FFTArgs f, RlGridArgs a)
+{
+    typedef double2 C;
+    extern __shared__ char smem_raw[];
+    const int M = f.M, N = f.N;
+    C *s_tw = reinterpret_cast<C *>(smem_raw);
+    C *s_twr = s_tw + M;
+    C *B = s_twr + M + 1;             // G scratch rows
+    C *rows = B + (size_t)a.G * M;    // max(nin, nout) component rows
+    int lb;
+    {
+        const int nx = 8, nb = gridDim.x, b = blockIdx.x;
+        int per = nb / nx, rem = nb % nx, x = b % nx, i = b / nx;
+        lb = (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + i;
+    }
+    const int n_theta = a.n_theta, n_m = a.n_m, nlev = a.nlev;
+    const int th = lb % n_theta, lev = lb / n_theta;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    load_twiddles(s_tw, s_twr, f);
+    __syncthreads();
+
+    // inverse FFTs, G rows at a time, into rows[0, nin)
+    const C *Fi = reinterpret_cast<const C *>(a.Fin) + (size_t)lev * a.nin * n_theta * n_m + th;
+    for (int g0 = 0; g0 < a.nin; g0 += a.G) {
+        const int gn = min(a.G, a.nin - g0);
+        const int per = nthr / gn;
+        const int r = tid / per, t = tid - r * per;
+        if (r < gn)
+            irfft_pre(Fi + (size_t)(g0 + r) * n_m * n_theta, (size_t)n_theta, n_m, rows + (size_t)(g0 + r) * M,
+                      M, s_twr, t, per);
+        __syncthreads();
+        C *res = fft_pp(rows + (size_t)g0 * M, B, gn, f, s_tw, +1);
+        if (res != rows + (size_t)g0 * M) {
+            for (int e = tid; e < gn * M; e += nthr) rows[(size_t)g0 * M + e] = res[e];
+            __syncthreads();
+        }
+    }
+
+    // the products, point by point, in place
+    const double *rad = a.rad;
+    const double r = rad[RR_R * nlev + lev], or1 = rad[RR_OR1 * nlev + lev];
+    const double or2 = rad[RR_OR2 * nlev + lev], or4 = rad[RR_OR4 * nlev + lev];
+    const double orho1 = rad[RR_ORHO1 * nlev + lev], beta = rad[RR_BETA * nlev + lev];
+    const double otemp1 = rad[RR_OTEMP1 * nlev + lev], visc = rad[RR_VISC * nlev + lev];
+    const double lambda = rad[RR_LAMBDA * nlev + lev];
+    const bool lcr = rad[RR_LCR * nlev + lev] != 0.0;
+    const double o2 = a.th[RT_OSIN2 * n_theta + th], cosn2 = a.th[RT_COSN2 * n_theta + th];
+    const double sinT = a.th[RT_SIN * n_theta + th], cosT = a.th[RT_COS * n_theta + th];
+    const double osinT = a.th[RT_OSIN * n_theta + th];
+    double *R = reinterpret_cast<double *>(rows);  // row c: R[c * N + j]
+    double mr = 0.0, mh = 0.0;
+    const double vri2 = rad[RR_VALRI2 * nlev + lev], vhi2 = rad[RR_VALHI2 * nlev + lev];
+    const double orho2 = orho1 * orho1;
+    for (int j = tid; j < N; j += nthr) {
+        double v[RI_N], o[RO_N];
+#pragma unroll
+        for (int k = 0; k < RI_N; ++k) v[k] = a.in_of[k] >= 0 ? R[(size_t)a.in_of[k] * N + j] : 0.0;
+        rl_products(a.ph, v, o, o2, cosn2, sinT, cosT, osinT, j * a.dphi, r, or1, or2, or4, orho1,
+                    beta, otemp1, visc, lambda, lcr);
+        if (a.cmax) {
+            // src/courant.f90:209-275 (XSH_COURANT = 0)
+            const double vr = v[RI_VR], vt = v[RI_VT], vp = v[RI_VP];
+            const double vflr2 = orho2 * vr * vr;
+            const double vflh2 = (vt * vt + vp * vp) * o2 * orho2;
+            if (a.ph.cour_mag) {
+                const double br = v[RI_BR], bt = v[RI_BT], bp = v[RI_BP];
+                const double valr = br * br * a.ph.LFfac * orho1;
+                const double valr2 = valr * valr / (valr + vri2);
+                mr = fmax(mr, or4 * (a.ph.cf2 * vflr2 + a.ph.af2 * valr2));
+                const double valh2 = (bt * bt + bp * bp) * a.ph.LFfac * o2 * orho1;
+                const double valh2m = valh2 * valh2 / (valh2 + vhi2);
+                mh = fmax(mh, or2 * (a.ph.cf2 * vflh2 + a.ph.af2 * valh2m));
+            } else {
+                mr = fmax(mr, a.ph.cf2 * or4 * vflr2);
+                mh = fmax(mh, a.ph.cf2 * or2 * vflh2);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RO_N; ++k) {
+            const int row = a.out_of[k];
+            if (row >= 0) R[(size_t)row * N + j] = ((a.out_osin2 >> row) & 1u) ? o[k] * o2 : o[k];
+        }
+    }
+    if (a.cmax) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            mr = fmax(mr, __shfl_xor(mr, off, 64));
+            mh = fmax(mh, __shfl_xor(mh, off, 64));
+        }
+        if ((tid & 63) == 0) {
+            atomicMax(a.cmax + 2 * lev, (unsigned long long)__double_as_longlong(mr));
+            atomicMax(a.cmax + 2 * lev + 1, (unsigned long long)__double_as_longlong(mh));
+        }
+    }
+    __syncthreads();
+
+    // forward FFTs of rows[0, nout) into F2[lev][comp][theta][m]
+    C *Fo = reinterpret_cast<C *>(a.Fout) + (size_t)lev * a.nout * n_theta * n_m;
+    const double invN = 1.0 / (double)N;
+    for (int g0 = 0; g0 < a.nout; g0 += a.G) {
+        const int gn = min(a.G, a.nout - g0);
+        C *res = fft_pp(rows + (size_t)g0 * M, B, gn, f, s_tw, -1);
+        for (int idx = tid; idx < gn * n_m; idx += nthr) {
+            const int r2 = idx / n_m, k = idx - r2 * n_m;
+            Fo[((size_t)(g0 + r2) * n_theta + th) * n_m + k] = rfft_post(res + (size_t)r2 * M, k, M, s_twr, invN);
+        }
+        __syncthreads();
+    }
+}
+
 FFTArgs make_args(const Plan *p)
 {
     FFTArgs f;
@@ -1117,6 +1370,36 @@ int launch_nl_fused(const Plan *p, int nlev, const void *Fin, void *Fout, const 
     if (p->precision == MHX_PREC_SPLIT16)
         return launch_fused_t<float>(p, nlev, Fin, Fout, rad, LFfac, dgrid, full_grids, s);
     return launch_fused_t<double>(p, nlev, Fin, Fout, rad, LFfac, dgrid, full_grids, s);
+}
+
+int launch_rl_grid(const Plan *p, const RlGridArgs &args, hipStream_t s)
+{
+    if (args.nlev <= 0) return MHX_OK;
+    RlGridArgs a = args;
+    const FFTArgs f = make_args(p);
+    const size_t cs = sizeof(double2);
+    const int nrow = std::max(a.nin, a.nout);
+    if (nrow <= 0 || nrow > 32) return fail(MHX_E_INVALID, "radial loop: bad component count");
+    const size_t lds_max = 160 * 1024;
+    const size_t fixed = ((size_t)2 * p->M + 1 + (size_t)nrow * p->M) * cs;
+    if (fixed + (size_t)p->M * cs > lds_max)
+        return fail(MHX_E_UNSUPPORTED, "radial loop: the grid rows of this n_phi and these physics "
+                                       "switches exceed the 160 KB of LDS of a workgroup");
+    // as many FFT rows per pass as the LDS holds, at most 8 (32 threads per row)
+    int G = (int)std::min<size_t>(8, (lds_max - fixed) / ((size_t)p->M * cs));
+    G = std::max(1, std::min(G, nrow));
+    a.G = G;
+    const size_t shm = fixed + (size_t)G * p->M * cs;
+    static int shm_set = 0;
+    if (shm > 64 * 1024 && (int)shm > shm_set) {
+        MHX_HIP_CHECK(hipFuncSetAttribute((const void *)k_rl_grid,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        shm_set = (int)shm;
+    }
+    dim3 grid((unsigned)(p->n_theta * a.nlev));
+    k_rl_grid<<<grid, 256, shm, s>>>(f, a);
+    MHX_HIP_CHECK(hipGetLastError());
+    return MHX_OK;
 }
 
 int launch_axi(const Plan *p, const double *fl, double *f, int lcut, int tor, double *f2,

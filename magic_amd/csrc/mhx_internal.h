@@ -17,6 +17,8 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // ---------------------------------------------------------------- errors
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
+// true for device (or managed) memory, false for host memory
+bool is_device_ptr(const void *ptr);
 #define MHX_HIP_CHECK(expr)                                                   \
     do {                                                                      \
         hipError_t e_ = (expr);                                               \
@@ -153,14 +155,59 @@ struct TdArgs {
     int l_max, lm_max;
 };
 
+// ---- the radial loop of any physics switches (mhx_rloop.hip) -------------
+// grid components of its inverse side (the reference's grid_space_arrays_t
+// fields, src/get_nl.f90:39-67) and products of its forward side
+enum RlIn {
+    RI_VR = 0, RI_VT, RI_VP, RI_CVR, RI_CVT, RI_CVP, RI_DVRDR, RI_DVTDR, RI_DVPDR,
+    RI_DVRDT, RI_DVRDP, RI_DVTDP, RI_DVPDP, RI_S, RI_XI, RI_PHI,
+    RI_BR, RI_BT, RI_BP, RI_CBR, RI_CBT, RI_CBP, RI_N
+};
+enum RlOut {
+    RO_ADVR = 0, RO_ADVT, RO_ADVP, RO_VSR, RO_VST, RO_VSP, RO_VXIR, RO_VXIT, RO_VXIP,
+    RO_VXBR, RO_VXBT, RO_VXBP, RO_HEAT, RO_PHI, RO_N
+};
+// per-level radial factors of the grid stage, RL_NRAD arrays of nlev
+enum RlRad {
+    RR_R = 0, RR_OR1, RR_OR2, RR_OR4, RR_ORHO1, RR_BETA, RR_OTEMP1, RR_VISC, RR_LAMBDA,
+    RR_LCR, RR_VALRI2, RR_VALHI2, RL_NRAD
+};
+// per-theta-row factors (reference's scrambled rows), RL_NTH arrays of n_theta
+enum RlTheta { RT_OSIN2 = 0, RT_COSN2, RT_SIN, RT_COS, RT_OSIN, RL_NTH };
+
+struct RlPhys {
+    int conv_nl, adv_curl, heat_nl, chem, phase, prec, centr, mag_LF, mag_nl, anel;
+    int cour_mag;   // magnetic branch of the Courant maxima
+    double LFfac, ViscHeatFac, OhmLossFac, epsPhase, penaltyFac, phaseDiffFac, tmelt;
+    double posnalp, oek, time, dilution_fac, ra, opr, cf2, af2;
+};
+
+struct RlGridArgs {
+    const double *Fin;   // F[lev][nin][m][theta] complex
+    double *Fout;        // F2[lev][nout][theta][m] complex
+    int nin, nout;
+    int in_of[RI_N];     // row (compact comp index) of each grid component, -1 absent
+    int out_of[RO_N];    // row of each product, -1 absent
+    unsigned out_osin2;  // rows multiplied by 1/sin^2(theta) before the forward FFT
+    int n_m, n_theta, nlev, G;
+    double dphi;         // 2 pi / (n_phi minc)
+    const double *rad;   // RL_NRAD x nlev
+    const double *th;    // RL_NTH x n_theta
+    RlPhys ph;
+    unsigned long long *cmax;  // [lev][2] vr2max, vh2max (bit patterns), zeroed first
+};
+struct RlBuffers;  // the radial loop's device buffers (mhx_rloop.hip)
+
 constexpr int EK_CHUNKS = 16;  // lm chunks (blocks) per level of the energy sums
 struct EkArgs {
-    const double *w, *dw, *z;  // complex, level stride ld_lm
+    const double *w, *dw, *z;  // complex, level stride ld_lm (b, db, aj for mag)
     size_t ld_lm;
-    const double *rad;         // device: or2[nlev], orho1[nlev]
-    double *part;              // device: 8 x nlev x EK_CHUNKS partial sums
-    double *out;               // device: 8 sums x nlev
+    const double *rad;         // device: or2[nlev], orho1[nlev] (mag: orho1 unused)
+    double *part;              // device: nsum x nlev x EK_CHUNKS partial sums
+    double *out;               // device: nsum sums x nlev
     int nlev;
+    int mag = 0;               // magnetic energies (src/magnetic_energy.f90:264-340)
+    int nsum = 8;              // 8 (kinetic) or 10 (magnetic: + dipole, axial dipole)
     // filled by the launcher
     const int *lm2l, *lm2m;
     int lm_max;
@@ -169,6 +216,7 @@ struct EkArgs {
 // ------------------------------------------------------------------ plan
 struct Plan {
     int l_max, m_max, minc, n_theta, n_phi, nh, n_m, lm_max, precision;
+    int device = 0;     // the device current at plan creation
     int nt16;   // theta tiles of 16 (northern half)
     int nq4;    // theta quads (northern half)
     int M;      // complex FFT length n_phi/2
@@ -265,6 +313,8 @@ struct Plan {
     bool pin_pending = false;      // ev_pin recorded and not yet waited for
     bool last_host_out = false;    // the last nonlinear step copied outputs to host
 
+    RlBuffers *rl = nullptr;       // buffers of mhx_rloop, made on its first call
+
     bool timing = false;
     static constexpr int NEV = 16;
     int n_launch = 0;                 // launches timed in the last step
@@ -308,6 +358,8 @@ int launch_nl_fused(const Plan *p, int nlev, const void *Fin, void *Fout,
                     hipStream_t s);
 int launch_axi(const Plan *p, const double *fl, double *f, int lcut, int tor,
                double *f2, hipStream_t s);
+void rl_free(Plan *p);
+int launch_rl_grid(const Plan *p, const RlGridArgs &a, hipStream_t s);
 
 }  // namespace mhx
 

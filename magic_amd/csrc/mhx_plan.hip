@@ -21,6 +21,16 @@ int fail(int code, const std::string &msg)
     return code;
 }
 
+bool is_device_ptr(const void *ptr)
+{
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
 namespace {
 
 // Gauss-Legendre colatitudes and weights, src/horizontal.f90:279-340
@@ -59,16 +69,6 @@ bool factorize(int M, std::vector<int> &fac)
     while (M % 3 == 0) { fac.push_back(3); M /= 3; }
     while (M % 5 == 0) { fac.push_back(5); M /= 5; }
     return M == 1 && fac.size() <= 16;
-}
-
-bool is_device_ptr(const void *ptr)
-{
-    hipPointerAttribute_t attr;
-    if (hipPointerGetAttributes(&attr, ptr) != hipSuccess) {
-        (void)hipGetLastError();
-        return false;
-    }
-    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
 }
 
 template <typename T> void free_dev(T *&p)
@@ -426,6 +426,10 @@ int mhx_plan_create(int l_max, int m_max, int minc, int n_theta, int n_phi, int 
         return rc;
     }
     p->d_lm2l = d_lm2l;
+    if (hipGetDevice(&p->device) != hipSuccess) {
+        mhx_plan_destroy(h);
+        return fail(MHX_E_HIP, "hipGetDevice failed");
+    }
     if (hipStreamCreate(&p->stream) != hipSuccess) {
         mhx_plan_destroy(h);
         return fail(MHX_E_HIP, "hipStreamCreate failed");
@@ -458,6 +462,7 @@ int mhx_plan_destroy(mhx_plan *h)
     if (!h) return MHX_OK;
     Plan *p = &h->p;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
+    rl_free(p);
     free_dev(p->d_lstart);
     free_dev(p->d_syn_P);
     free_dev(p->d_syn_dP);
@@ -1236,15 +1241,18 @@ int mhx_nl_step(mhx_plan *h, int nlev, const double *const *in, size_t ld_lm, do
     return finish_step(h, stream);
 }
 
-int mhx_get_e_kin(mhx_plan *h, int nlev, const double *w, const double *dw, const double *z,
-                  size_t ld_lm, const double *or2, const double *orho1, double *sums)
+// the per-level energy sums of src/kinetic_energy.f90:119-160 (mag = 0:
+// w, dw, z, orho1) or src/magnetic_energy.f90:264-340 (mag = 1: b, db, aj)
+static int energy_sums(mhx_plan *h, int nlev, const double *w, const double *dw, const double *z,
+                       size_t ld_lm, const double *or2, const double *orho1, double *sums, int mag)
 {
     if (!h) return fail(MHX_E_INVALID, "null plan");
     Plan *P = &h->p;
     if (nlev <= 0) return MHX_OK;
-    if (!w || !dw || !z || !or2 || !orho1 || !sums) return fail(MHX_E_INVALID, "null pointer");
+    if (!w || !dw || !z || !or2 || (!mag && !orho1) || !sums) return fail(MHX_E_INVALID, "null pointer");
     if (nlev > 65535) return fail(MHX_E_INVALID, "more than 65535 levels in one call");
     if (ld_lm < (size_t)P->lm_max) return fail(MHX_E_INVALID, "ld_lm below lm_max");
+    const int nsum = mag ? 10 : 8;
     int rc;
     const size_t arr = (size_t)nlev * ld_lm * 2;
     const double *in[3] = {w, dw, z};
@@ -1255,7 +1263,7 @@ int mhx_get_e_kin(mhx_plan *h, int nlev, const double *w, const double *dw, cons
     if (P->ek_nlev < nlev) {
         free_dev(P->d_ek);
         P->ek_nlev = 0;
-        MHX_HIP_CHECK(hipMalloc(&P->d_ek, (10 + 8 * (size_t)EK_CHUNKS) * nlev * sizeof(double)));
+        MHX_HIP_CHECK(hipMalloc(&P->d_ek, (12 + 10 * (size_t)EK_CHUNKS) * nlev * sizeof(double)));
         P->ek_nlev = nlev;
     }
     // the plan stream may still read the previous call's host staging
@@ -1274,7 +1282,7 @@ int mhx_get_e_kin(mhx_plan *h, int nlev, const double *w, const double *dw, cons
     std::vector<double> rad(2 * (size_t)nlev);
     for (int i = 0; i < nlev; ++i) {
         rad[i] = or2[i];
-        rad[nlev + i] = orho1[i];
+        rad[nlev + i] = mag ? 1.0 : orho1[i];
     }
     MHX_HIP_CHECK(hipMemcpyAsync(P->d_ek, rad.data(), 2 * (size_t)nlev * sizeof(double),
                                  hipMemcpyHostToDevice, P->stream));
@@ -1284,13 +1292,27 @@ int mhx_get_e_kin(mhx_plan *h, int nlev, const double *w, const double *dw, cons
     a.ld_lm = ld_lm;
     a.rad = P->d_ek;
     a.out = P->d_ek + 2 * (size_t)nlev;
-    a.part = P->d_ek + 10 * (size_t)nlev;
+    a.part = P->d_ek + 12 * (size_t)nlev;
     a.nlev = nlev;
+    a.mag = mag;
+    a.nsum = nsum;
     if ((rc = launch_e_kin(P, a, P->stream))) return rc;
-    MHX_HIP_CHECK(hipMemcpyAsync(sums, a.out, 8 * (size_t)nlev * sizeof(double), hipMemcpyDefault,
+    MHX_HIP_CHECK(hipMemcpyAsync(sums, a.out, (size_t)nsum * nlev * sizeof(double), hipMemcpyDefault,
                                  P->stream));
     MHX_HIP_CHECK(hipStreamSynchronize(P->stream));
     return MHX_OK;
+}
+
+int mhx_get_e_kin(mhx_plan *h, int nlev, const double *w, const double *dw, const double *z,
+                  size_t ld_lm, const double *or2, const double *orho1, double *sums)
+{
+    return energy_sums(h, nlev, w, dw, z, ld_lm, or2, orho1, sums, 0);
+}
+
+int mhx_get_e_mag(mhx_plan *h, int nlev, const double *b, const double *db, const double *aj,
+                  size_t ld_lm, const double *or2, double *sums)
+{
+    return energy_sums(h, nlev, b, db, aj, ld_lm, or2, nullptr, sums, 1);
 }
 
 int mhx_get_td(mhx_plan *h, int nlev, const double *const *in, size_t ld_lm, double *const *out,

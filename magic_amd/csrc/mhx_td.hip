@@ -122,8 +122,12 @@ __global__ __launch_bounds__(256) void k_e_kin(EkArgs a)
     const double2 *w = reinterpret_cast<const double2 *>(a.w) + (size_t)lev * a.ld_lm;
     const double2 *dw = reinterpret_cast<const double2 *>(a.dw) + (size_t)lev * a.ld_lm;
     const double2 *z = reinterpret_cast<const double2 *>(a.z) + (size_t)lev * a.ld_lm;
-    const double or2 = a.rad[lev], orho = a.rad[a.nlev + lev];
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const double or2 = a.rad[lev], orho = a.mag ? 1.0 : a.rad[a.nlev + lev];
+    // the magnetic sums take the opposite l parity for their equatorially
+    // symmetric parts (src/magnetic_energy.f90:281-300 against
+    // src/kinetic_energy.f90:143-157)
+    const int par = a.mag ? 1 : 0;
+    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int lm = blockIdx.x * blockDim.x + threadIdx.x; lm < a.lm_max; lm += nch * blockDim.x) {
         const int l = a.lm2l[lm], m = a.lm2m[lm];
         const double dLh = (double)(l * (l + 1));
@@ -145,21 +149,26 @@ __global__ __launch_bounds__(256) void k_e_kin(EkArgs a)
         if (m == 0) {
             acc[2] += ep;
             acc[3] += et;
-            if ((l & 1) == 0) acc[6] += ep;
+            if ((l & 1) == par) acc[6] += ep;
             else acc[7] += et;
         }
-        if (((l + m) & 1) == 0) acc[4] += ep;
+        if (((l + m) & 1) == par) acc[4] += ep;
         else acc[5] += et;
+        if (l == 1) {
+            acc[8] += ep;               // dipole
+            if (m == 0) acc[9] += ep;   // axial dipole
+        }
     }
-    __shared__ double red[8][256];
-    for (int k = 0; k < 8; ++k) red[k][threadIdx.x] = acc[k];
+    __shared__ double red[10][256];
+    for (int k = 0; k < 10; ++k) red[k][threadIdx.x] = acc[k];
     __syncthreads();
     for (int st = 128; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st)
-            for (int k = 0; k < 8; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + st];
+            for (int k = 0; k < 10; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + st];
         __syncthreads();
     }
-    if (threadIdx.x < 8) a.part[((size_t)threadIdx.x * a.nlev + lev) * nch + blockIdx.x] = red[threadIdx.x][0];
+    if ((int)threadIdx.x < a.nsum)
+        a.part[((size_t)threadIdx.x * a.nlev + lev) * nch + blockIdx.x] = red[threadIdx.x][0];
 }
 
 __global__ void k_e_kin_sum(const double *__restrict__ part, int n, int nch, double *__restrict__ out)
@@ -183,7 +192,7 @@ int launch_e_kin(const Plan *p, const EkArgs &args, hipStream_t s)
     const int nch = std::min(EK_CHUNKS, (a.lm_max + 255) / 256);
     k_e_kin<<<dim3((unsigned)nch, (unsigned)a.nlev), 256, 0, s>>>(a);
     MHX_HIP_CHECK(hipGetLastError());
-    const int n = 8 * a.nlev;
+    const int n = a.nsum * a.nlev;
     k_e_kin_sum<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a.part, n, nch, a.out);
     MHX_HIP_CHECK(hipGetLastError());
     return MHX_OK;

@@ -55,6 +55,8 @@ _SIGS = {
     "mhx_nl_step_grids": "Pipzppppdppp",
     "mhx_get_td": "Pipzppppdpp",
     "mhx_get_e_kin": "Pipppzppp",
+    "mhx_get_e_mag": "Pipppzpp",
+    "mhx_rloop": "Ppippzpp",
     "mhx_nl_step_courant": "Pipzppppdpddppppp",
     "mhx_set_timing": "Pi",
     "mhx_stage_times": "Pp",
@@ -391,6 +393,27 @@ def get_e_kin_levels(plan, w, dw, z, or2, orho1):
     return out
 
 
+def get_e_mag_levels(plan, b, db, aj, or2):
+    """Per-level magnetic-energy sums of the outer core
+    (src/magnetic_energy.f90:264-340) of the (nlev, lm_max) complex arrays b,
+    db, aj (numpy or device tensors): a (10, nlev) array e_p, e_t, e_p_as,
+    e_t_as, e_p_es, e_t_es, e_p_eas, e_t_eas, e_dipole, e_dipole_ax."""
+    nlev = int(b.shape[0])
+    arrs = []
+    for a in (b, db, aj):
+        if hasattr(a, "data_ptr"):
+            arrs.append(a)
+        else:
+            arrs.append(np.ascontiguousarray(a, dtype=np.complex128).reshape(nlev, plan.lm_max))
+    ptr = lambda a: ctypes.c_void_p(a.data_ptr() if hasattr(a, "data_ptr")  # noqa: E731
+                                    else a.ctypes.data)
+    out = np.zeros((10, nlev))
+    _check(plan._L.mhx_get_e_mag(plan.h, nlev, ptr(arrs[0]), ptr(arrs[1]), ptr(arrs[2]),
+                                 plan.lm_max, (ctypes.c_double * nlev)(*[float(x) for x in or2]),
+                                 ptr(out)))
+    return out
+
+
 def get_td(plan, inputs, outputs, or2, or4, epsc, CorFac, lcut, stream=None):
     """lm-space assembly of the explicit terms at interior levels
     (src/get_td.f90 get_dwdt, get_dzdt, get_dpdt, get_dsdt, get_dbdt; rotating
@@ -407,6 +430,92 @@ def get_td(plan, inputs, outputs, or2, or4, epsc, CorFac, lcut, stream=None):
     st = ctypes.c_void_p(stream.cuda_stream) if stream is not None else None
     _check(L.mhx_get_td(plan.h, nlev, ins, plan.lm_max, outs, d(or2), d(or4), d(epsc),
                         float(CorFac), lc, st))
+
+
+# ---------------------------------------------------------------- radial loop
+RLOOP_INT = ("l_conv", "l_conv_nl", "l_heat", "l_heat_nl", "l_chemical_conv", "l_phase_field",
+             "l_mag", "l_mag_LF", "l_mag_nl", "l_mag_kin", "l_anel", "l_anelastic_liquid",
+             "l_adv_curl", "l_double_curl", "l_corr", "l_single_matrix", "l_precession",
+             "l_centrifuge", "l_cour_alf_damp", "lPressNext", "n_r_LCR")
+RLOOP_DBL = ("LFfac", "CorFac", "epsc", "epscXi", "ViscHeatFac", "OhmLossFac", "opm", "courfac",
+             "alffac", "epsPhase", "penaltyFac", "phaseDiffFac", "tmelt", "po", "prec_angle", "oek",
+             "timeStage", "dilution_fac", "ra", "opr")
+RLOOP_RAD = ("r", "or1", "or2", "or4", "orho1", "beta", "otemp1", "visc", "lambda", "temp0",
+             "epscProf", "delxr2", "delxh2")
+RLOOP_RAD_INT = ("nR", "l_R")
+RLOOP_IN = ("w", "dw", "ddw", "z", "dz", "s", "xi", "phi", "b", "db", "ddb", "aj", "dj")
+RLOOP_OUT = ("dwdt", "dzdt", "dpdt", "dsdt", "dVSrLM", "dxidt", "dVXirLM", "dphidt", "dbdt", "djdt",
+             "dVxBhLM", "dVxVhLM")
+
+
+class _RloopPhys(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in RLOOP_INT] + [(n, ctypes.c_double) for n in RLOOP_DBL]
+
+
+class _RloopRadial(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in RLOOP_RAD + RLOOP_RAD_INT]
+
+
+class _RloopFields(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in RLOOP_IN]
+
+
+class _RloopOut(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in RLOOP_OUT + ("dtrkc", "dthkc")]
+
+
+def rloop(plan, phys, nlev, radial, fields, outputs, dtrkc, dthkc, ld_lm=None, stream=None):
+    """The radial loop of interior levels (mhx_rloop) for any physics
+    switches.  ``phys``: dict of the switches (RLOOP_INT, 0/1) and constants
+    (RLOOP_DBL), missing entries 0; ``radial``: dict of per-level host arrays
+    (RLOOP_RAD float64, RLOOP_RAD_INT int32) of nlev entries; ``fields`` and
+    ``outputs``: dicts of (nlev, ld_lm) complex128 arrays (numpy or device
+    tensors) by the names of RLOOP_IN / RLOOP_OUT (absent ones None);
+    ``dtrkc``, ``dthkc``: float64 arrays of nlev written with the Courant
+    steps."""
+    p = _RloopPhys()
+    for n in RLOOP_INT:
+        setattr(p, n, int(phys.get(n, 0)))
+    for n in RLOOP_DBL:
+        setattr(p, n, float(phys.get(n, 0.0)))
+    keep = []
+    r = _RloopRadial()
+    for n in RLOOP_RAD:
+        a = np.ascontiguousarray(radial[n], dtype=np.float64)
+        assert a.shape == (nlev,), (n, a.shape)
+        keep.append(a)
+        setattr(r, n, a.ctypes.data)
+    for n in RLOOP_RAD_INT:
+        a = np.ascontiguousarray(radial[n], dtype=np.int32)
+        assert a.shape == (nlev,), (n, a.shape)
+        keep.append(a)
+        setattr(r, n, a.ctypes.data)
+    ld = plan.lm_max if ld_lm is None else int(ld_lm)
+
+    def ptr(a):
+        if a is None:
+            return None
+        if _is_torch(a):
+            assert a.is_contiguous() and a.shape[0] == nlev and a.shape[-1] >= ld
+            return a.data_ptr()
+        assert a.flags["C_CONTIGUOUS"] and a.dtype == np.complex128 and a.shape[0] == nlev
+        return a.ctypes.data
+
+    f = _RloopFields()
+    for n in RLOOP_IN:
+        setattr(f, n, ptr(fields.get(n)))
+    o = _RloopOut()
+    for n in RLOOP_OUT:
+        setattr(o, n, ptr(outputs.get(n)))
+    for n, a in (("dtrkc", dtrkc), ("dthkc", dthkc)):
+        if _is_torch(a):
+            setattr(o, n, a.data_ptr())
+        else:
+            assert a.flags["C_CONTIGUOUS"] and a.dtype == np.float64 and a.shape == (nlev,)
+            setattr(o, n, a.ctypes.data)
+    st = ctypes.c_void_p(stream.cuda_stream) if stream is not None else None
+    _check(plan._L.mhx_rloop(plan.h, ctypes.byref(p), nlev, ctypes.byref(r), ctypes.byref(f), ld,
+                             ctypes.byref(o), st))
 
 
 JOB_SCALAR, JOB_VECTOR = 0, 1

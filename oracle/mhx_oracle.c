@@ -221,16 +221,21 @@ static void cfft(int n, const cplx *in, cplx *out, int sign)
 void orc_fft_forward(const orc_plan *p, const double *grid, double *Fout)
 {
     int n = p->n_phi, nt = p->n_theta, nk = n / 2 + 1;
-    cplx *x = (cplx *)malloc(sizeof(cplx) * n), *X = (cplx *)malloc(sizeof(cplx) * n);
     cplx *F = (cplx *)Fout;
-    for (int t = 0; t < nt; ++t) {
-        for (int j = 0; j < n; ++j) x[j] = grid[(size_t)j * nt + t];
-        cfft(n, x, X, -1);
-        for (int k = 0; k < nk; ++k) F[(size_t)k * nt + t] = X[k] / (double)n;
-        F[t] = creal(F[t]);
-        F[(size_t)(nk - 1) * nt + t] = creal(F[(size_t)(nk - 1) * nt + t]);
+    /* rows are independent: one thread per row, same arithmetic as serial */
+#pragma omp parallel
+    {
+        cplx *x = (cplx *)malloc(sizeof(cplx) * n), *X = (cplx *)malloc(sizeof(cplx) * n);
+#pragma omp for schedule(static)
+        for (int t = 0; t < nt; ++t) {
+            for (int j = 0; j < n; ++j) x[j] = grid[(size_t)j * nt + t];
+            cfft(n, x, X, -1);
+            for (int k = 0; k < nk; ++k) F[(size_t)k * nt + t] = X[k] / (double)n;
+            F[t] = creal(F[t]);
+            F[(size_t)(nk - 1) * nt + t] = creal(F[(size_t)(nk - 1) * nt + t]);
+        }
+        free(x); free(X);
     }
-    free(x); free(X);
 }
 
 /* src/fft.f90:211-252 with fft991(isign=+1): x_j = sum_k c_k e^{2 pi i jk/n},
@@ -239,18 +244,22 @@ void orc_fft_inverse(const orc_plan *p, const double *Fin, double *grid)
 {
     int n = p->n_phi, nt = p->n_theta, nk = n / 2 + 1;
     const cplx *F = (const cplx *)Fin;
-    cplx *X = (cplx *)malloc(sizeof(cplx) * n), *x = (cplx *)malloc(sizeof(cplx) * n);
-    for (int t = 0; t < nt; ++t) {
-        X[0] = creal(F[t]);
-        X[n / 2] = creal(F[(size_t)(nk - 1) * nt + t]);
-        for (int k = 1; k < n / 2; ++k) {
-            X[k] = F[(size_t)k * nt + t];
-            X[n - k] = conj(X[k]);
+#pragma omp parallel
+    {
+        cplx *X = (cplx *)malloc(sizeof(cplx) * n), *x = (cplx *)malloc(sizeof(cplx) * n);
+#pragma omp for schedule(static)
+        for (int t = 0; t < nt; ++t) {
+            X[0] = creal(F[t]);
+            X[n / 2] = creal(F[(size_t)(nk - 1) * nt + t]);
+            for (int k = 1; k < n / 2; ++k) {
+                X[k] = F[(size_t)k * nt + t];
+                X[n - k] = conj(X[k]);
+            }
+            cfft(n, X, x, +1);
+            for (int j = 0; j < n; ++j) grid[(size_t)j * nt + t] = creal(x[j]);
         }
-        cfft(n, X, x, +1);
-        for (int j = 0; j < n; ++j) grid[(size_t)j * nt + t] = creal(x[j]);
+        free(X); free(x);
     }
-    free(X); free(x);
 }
 
 /* ------------------------------------------------------------------ */
@@ -270,6 +279,7 @@ void orc_sph_to_spat(const orc_plan *p, const double *Slm_, double *sc, int lcut
     const cplx *S = (const cplx *)Slm_;
     cplx *F = fourier_alloc(p);
     int nt = p->n_theta;
+#pragma omp parallel for schedule(dynamic)
     for (int mc = 0; mc < p->n_m_max; ++mc) {
         int m = mc * p->minc;
         if (m > lcut) continue;
@@ -294,6 +304,7 @@ void orc_sph_to_grad_spat(const orc_plan *p, const double *Slm_, double *gt, dou
     const cplx *S = (const cplx *)Slm_;
     cplx *Ft = fourier_alloc(p), *Fp = fourier_alloc(p);
     int nt = p->n_theta;
+#pragma omp parallel for schedule(dynamic)
     for (int mc = 0; mc < p->n_m_max; ++mc) {
         int m = mc * p->minc;
         if (m > lcut) continue;
@@ -327,6 +338,7 @@ static void sphtor_fourier(const orc_plan *p, const cplx *S, const cplx *T, cplx
     double *PC = (double *)malloc(sizeof(double) * p->lm_max);
     for (int lm = 0; lm < p->lm_max; ++lm) { bhG[lm] = S[lm] - I * T[lm]; bhC[lm] = S[lm] + I * T[lm]; }
     if (zero_l0) { bhG[0] = 0.0; bhC[0] = 0.0; }
+#pragma omp parallel for schedule(dynamic)
     for (int mc = 0; mc < p->n_m_max; ++mc) {
         int m = mc * p->minc;
         if (m > lcut) continue;
@@ -424,6 +436,7 @@ void orc_spat_to_sph(const orc_plan *p, const double *f, double *flm_, int lcut)
     orc_fft_forward(p, f, (double *)F);
     memset(flm, 0, sizeof(cplx) * p->lm_max);
     int mlim = p->m_max < lcut ? p->m_max : lcut;
+#pragma omp parallel for schedule(dynamic)
     for (int mc = 0; mc < p->n_m_max; ++mc) {
         int m = mc * p->minc;
         if (m > mlim) continue;
@@ -455,6 +468,7 @@ void orc_spat_to_sph_tor(const orc_plan *p, const double *vt, const double *vp,
     memset(f1, 0, sizeof(cplx) * p->lm_max);
     memset(f2, 0, sizeof(cplx) * p->lm_max);
     int mlim = p->m_max < lcut ? p->m_max : lcut;
+#pragma omp parallel for schedule(dynamic)
     for (int mc = 0; mc < p->n_m_max; ++mc) {
         int m = mc * p->minc;
         if (m > mlim) continue;
@@ -785,4 +799,305 @@ void orc_get_br_v_bcs(const orc_plan *p, const double *br, const double *vt, con
     orc_spat_to_sph_tor(p, a, b, br_vt_lm, br_vp_lm, p->l_max);
     free(a);
     free(b);
+}
+
+/* ------------------------------------------------------------------ */
+/* The radial loop of any physics switches, one interior level (nBc = 0).
+   Grid components in[ORC_RI_*] and products out[ORC_RO_*] are n_phi x
+   n_theta grids (theta fastest, scrambled rows); absent components are NULL.
+   flags[ORC_F_*], par[ORC_P_*], rad[ORC_R_*] as in mhx_oracle.h.
+   src/get_nl.f90:240-436 per point, then the merge of
+   src/rIter.f90:644-680 (Adv += LF, += precession, += centrifugal). */
+void orc_rl_products(const orc_plan *p, const int *flags, const double *par, const double *rad,
+                     const double *const *in, double *const *out)
+{
+    const int nt = p->n_theta, np_ = p->n_phi;
+    const double r = rad[ORC_R_R], or1 = rad[ORC_R_OR1], or2 = rad[ORC_R_OR2];
+    const double or4 = rad[ORC_R_OR4], orho1 = rad[ORC_R_ORHO1], beta = rad[ORC_R_BETA];
+    const double otemp1 = rad[ORC_R_OTEMP1], visc = rad[ORC_R_VISC], lambda = rad[ORC_R_LAMBDA];
+    const int lcr = flags[ORC_F_LCR];
+    const double LFfac = par[ORC_P_LFFAC];
+    /* posnalp = -2 oek po sin(prec_angle) (src/get_nl.f90:233) */
+    const double posnalp = -2.0 * par[ORC_P_OEK] * par[ORC_P_PO] * sin(par[ORC_P_PREC_ANGLE]);
+    const double dphi = 2.0 * ORC_PI / ((double)np_ * par[ORC_P_MINC]);
+#define G(k) (in[k] ? in[k][g] : 0.0)
+    for (int ip = 0; ip < np_; ++ip) {
+        const double phi = (double)ip * dphi;
+        for (int it = 0; it < nt; ++it) {
+            const size_t g = (size_t)ip * nt + it;
+            /* src/horizontal.f90:141-157 */
+            const double colat = p->theta_ord[it / 2];
+            const double sn = sin(colat), sgn = (it & 1) ? -1.0 : 1.0;
+            const double o2 = 1.0 / (sn * sn), cosn2 = sgn * cos(colat) / sn / sn;
+            const double sinT = sn, cosT = sgn * cos(colat), osinT = 1.0 / sn;
+            const double vr = G(ORC_RI_VR), vt = G(ORC_RI_VT), vp = G(ORC_RI_VP);
+            const double s = G(ORC_RI_S);
+            double LFr = 0.0, LFt = 0.0, LFp = 0.0, Ar = 0.0, At = 0.0, Ap = 0.0;
+            const int lf = flags[ORC_F_MAG_LF] && !lcr;
+            if (lf) { /* :242-258 */
+                const double br = G(ORC_RI_BR), bt = G(ORC_RI_BT), bp = G(ORC_RI_BP);
+                const double cbr = G(ORC_RI_CBR), cbt = G(ORC_RI_CBT), cbp = G(ORC_RI_CBP);
+                LFr = LFfac * o2 * (cbt * bp - cbp * bt);
+                LFt = LFfac * or4 * (cbp * br - cbr * bp);
+                LFp = LFfac * or4 * (cbr * bt - cbt * br);
+            }
+            if (flags[ORC_F_CONV_NL]) {
+                if (flags[ORC_F_ADV_CURL]) { /* :262-273 */
+                    const double cvr = G(ORC_RI_CVR), cvt = G(ORC_RI_CVT), cvp = G(ORC_RI_CVP);
+                    Ar = -o2 * (cvt * vp - cvp * vt);
+                    At = -or4 * (cvp * vr - cvr * vp);
+                    Ap = -or4 * (cvr * vt - cvt * vr);
+                } else { /* :274-308 */
+                    const double dvrdr = G(ORC_RI_DVRDR), dvtdr = G(ORC_RI_DVTDR);
+                    const double dvpdr = G(ORC_RI_DVPDR), dvrdt = G(ORC_RI_DVRDT);
+                    const double dvrdp = G(ORC_RI_DVRDP), dvtdp = G(ORC_RI_DVTDP);
+                    const double dvpdp = G(ORC_RI_DVPDP), cvr = G(ORC_RI_CVR);
+                    Ar = -or2 * orho1 * (vr * (dvrdr - (2.0 * or1 + beta) * vr) +
+                                         o2 * (vt * (dvrdt - r * vt) + vp * (dvrdp - r * vp)));
+                    At = or4 * orho1 * (-vr * (dvtdr - beta * vt) + vt * (cosn2 * vt + dvpdp + dvrdr) +
+                                        vp * (cosn2 * vp - dvtdp));
+                    Ap = or4 * orho1 * (-vr * (dvpdr - beta * vp) - vt * (dvtdp + cvr) - vp * dvpdp);
+                }
+            }
+            if (flags[ORC_F_PHASE]) { /* :333-344 */
+                const double eps = par[ORC_P_EPSPHASE], pen = par[ORC_P_PENALTY];
+                const double pc = G(ORC_RI_PHI);
+                Ar = Ar - pc * vr / (eps * eps) / (pen * pen);
+                At = At - or2 * pc * vt / (eps * eps) / (pen * pen);
+                Ap = Ap - or2 * pc * vp / (eps * eps) / (pen * pen);
+                out[ORC_RO_PHI][g] = -1.0 / (eps * eps) * pc * (1.0 - pc) *
+                                     (par[ORC_P_PHASEDIFF] * (1.0 - 2.0 * pc) + s - par[ORC_P_TMELT]);
+            }
+            if (flags[ORC_F_CONV_NL]) {
+                if (lf) { Ar += LFr; At += LFt; Ap += LFp; }
+            } else if (flags[ORC_F_MAG_LF]) {
+                Ar = LFr; At = LFt; Ap = LFp;
+            }
+            if (flags[ORC_F_PREC]) { /* :346-356 */
+                const double a = par[ORC_P_OEK] * par[ORC_P_TIME] + phi;
+                Ar += posnalp * osinT * r * (cos(a) * vp * cosT + sin(a) * vt);
+                At += -posnalp * sinT * or2 * (cos(a) * vp + sin(a) * or1 * vr);
+                Ap += posnalp * sinT * cos(a) * or2 * (vt - or1 * vr * cosT);
+            }
+            if (flags[ORC_F_CENTR]) { /* :358-374 */
+                const double f = par[ORC_P_DILUTION] * par[ORC_P_RA] * par[ORC_P_OPR];
+                Ar += -f * r * pow(sinT, 4) * s;
+                At += -f * r * pow(sinT, 3) * cosT * s;
+            }
+            if (out[ORC_RO_ADVR]) {
+                out[ORC_RO_ADVR][g] = Ar;
+                out[ORC_RO_ADVT][g] = At;
+                out[ORC_RO_ADVP][g] = Ap;
+            }
+            if (out[ORC_RO_VSR]) { /* :320-325 */
+                const int on = flags[ORC_F_HEAT_NL];
+                out[ORC_RO_VSR][g] = on ? vr * s : 0.0;
+                out[ORC_RO_VST][g] = on ? or2 * vt * s : 0.0;
+                out[ORC_RO_VSP][g] = on ? or2 * vp * s : 0.0;
+            }
+            if (flags[ORC_F_CHEM]) { /* :327-331 */
+                const double xi = G(ORC_RI_XI);
+                out[ORC_RO_VXIR][g] = vr * xi;
+                out[ORC_RO_VXIT][g] = or2 * vt * xi;
+                out[ORC_RO_VXIP][g] = or2 * vp * xi;
+            }
+            if (flags[ORC_F_MAG_NL]) { /* :376-391 */
+                const double br = G(ORC_RI_BR), bt = G(ORC_RI_BT), bp = G(ORC_RI_BP);
+                if (!lcr) {
+                    out[ORC_RO_VXBR][g] = orho1 * o2 * (vt * bp - vp * bt);
+                    out[ORC_RO_VXBT][g] = orho1 * or4 * (vp * br - vr * bp);
+                    out[ORC_RO_VXBP][g] = orho1 * or4 * (vr * bt - vt * br);
+                } else {
+                    out[ORC_RO_VXBR][g] = 0.0;
+                    out[ORC_RO_VXBT][g] = or4 * orho1 * vp * br;
+                    out[ORC_RO_VXBP][g] = -or4 * orho1 * vt * br;
+                }
+            }
+            if (flags[ORC_F_ANEL]) { /* :402-436 */
+                const double dvrdr = G(ORC_RI_DVRDR), dvtdr = G(ORC_RI_DVTDR);
+                const double dvpdr = G(ORC_RI_DVPDR), dvrdt = G(ORC_RI_DVRDT);
+                const double dvrdp = G(ORC_RI_DVRDP), dvtdp = G(ORC_RI_DVTDP);
+                const double dvpdp = G(ORC_RI_DVPDP), cvr = G(ORC_RI_CVR);
+                double h = par[ORC_P_VISCHEAT] * or4 * orho1 * otemp1 * visc *
+                           (2.0 * pow(dvrdr - (2.0 * or1 + beta) * vr, 2) +
+                            2.0 * pow(cosn2 * vt + dvpdp + dvrdr - or1 * vr, 2) +
+                            2.0 * pow(dvpdp + cosn2 * vt + or1 * vr, 2) +
+                            pow(2.0 * dvtdp + cvr - 2.0 * cosn2 * vp, 2) +
+                            o2 * (pow(r * dvtdr - (2.0 + beta * r) * vt + or1 * dvrdt, 2) +
+                                  pow(r * dvpdr - (2.0 + beta * r) * vp + or1 * dvrdp, 2)) -
+                            2.0 / 3.0 * pow(beta * vr, 2));
+                if (flags[ORC_F_MAG_NL] && !lcr) {
+                    const double cbr = G(ORC_RI_CBR), cbt = G(ORC_RI_CBT), cbp = G(ORC_RI_CBP);
+                    h = h + par[ORC_P_OHMLOSS] * or2 * otemp1 * lambda *
+                                (or2 * cbr * cbr + o2 * cbt * cbt + o2 * cbp * cbp);
+                }
+                out[ORC_RO_HEAT][g] = h;
+            }
+        }
+    }
+#undef G
+}
+
+/* src/courant.f90:161-276 (XSH_COURANT = 0) for one level, from 1e10
+   (src/rIter.f90:217-218): the magnetic branch when cour_mag (l_mag,
+   l_mag_LF, not l_mag_kin), else the velocity-only one.  par: courfac,
+   alffac, LFfac, opm, l_cour_alf_damp; rad: or2, or4, orho1, delxr2,
+   delxh2.  in: vr vt vp br bt bp (the last three unused without cour_mag). */
+void orc_rl_courant(const orc_plan *p, int cour_mag, const double *par, const double *rad,
+                    const double *const *in, double *dtrkc, double *dthkc)
+{
+    const int nt = p->n_theta, np_ = p->n_phi;
+    const double courfac = par[0], alffac = par[1], LFfac = par[2], opm = par[3];
+    const int damp = par[4] != 0.0;
+    const double or2 = rad[0], or4 = rad[1], orho1 = rad[2], delxr2 = rad[3], delxh2 = rad[4];
+    const double valri2 = damp ? (0.5 * (1.0 + opm)) * (0.5 * (1.0 + opm)) / delxr2 : 0.0;
+    const double valhi2 = damp ? (0.5 * (1.0 + opm)) * (0.5 * (1.0 + opm)) / delxh2 : 0.0;
+    const double cf2 = courfac * courfac, af2 = alffac * alffac, orho2 = orho1 * orho1;
+    double vr2max = 0.0, vh2max = 0.0;
+    for (int ip = 0; ip < np_; ++ip)
+        for (int it = 0; it < nt; ++it) {
+            const size_t g = (size_t)ip * nt + it;
+            const double vr = in[0][g], vt = in[1][g], vp = in[2][g];
+            const double o2 = p->osin2[it];
+            const double vflr2 = orho2 * vr * vr;
+            const double vflh2 = (vt * vt + vp * vp) * o2 * orho2;
+            if (cour_mag) {
+                const double br = in[3][g], bt = in[4][g], bp = in[5][g];
+                const double valr = br * br * LFfac * orho1;
+                const double valr2 = valr * valr / (valr + valri2);
+                vr2max = fmax(vr2max, or4 * (cf2 * vflr2 + af2 * valr2));
+                const double valh2 = (bt * bt + bp * bp) * LFfac * o2 * orho1;
+                const double valh2m = valh2 * valh2 / (valh2 + valhi2);
+                vh2max = fmax(vh2max, or2 * (cf2 * vflh2 + af2 * valh2m));
+            } else {
+                vr2max = fmax(vr2max, cf2 * or4 * vflr2);
+                vh2max = fmax(vh2max, cf2 * or2 * vflh2);
+            }
+        }
+    *dtrkc = 1e10;
+    *dthkc = 1e10;
+    if (vr2max != 0.0) *dtrkc = fmin(*dtrkc, sqrt(delxr2 / vr2max));
+    if (vh2max != 0.0) *dthkc = fmin(*dthkc, sqrt(delxh2 / vh2max));
+}
+
+/* nonlinear_lm_t at one interior level (nBc = 0), src/get_td.f90: get_dwdt
+   (:133-203) or get_dwdt_double_curl (:210-309), get_dzdt (:379-444),
+   get_dpdt (:311-377), get_dsdt (:459-521), get_dxidt (:523-556),
+   get_dbdt (:558-589); the dTheta operators of src/horizontal.f90:201-229
+   and the neighbours lm2lmS / lm2lmA of src/blocking.f90:322-334.
+   nl[ORC_NL_*] the forward outputs (NULL when switched off), fld = w dw ddw
+   z dz, out[ORC_TO_*] (NULL where not wanted). */
+void orc_rl_td(const orc_plan *p, const int *flags, const double *par, const double *rad, int lR,
+               const double *const *nl_, const double *const *fld, double *const *out_)
+{
+    const cplx *nl[ORC_NL_N];
+    for (int k = 0; k < ORC_NL_N; ++k) nl[k] = (const cplx *)nl_[k];
+    const cplx *w = (const cplx *)fld[0], *dw = (const cplx *)fld[1], *ddw = (const cplx *)fld[2];
+    const cplx *z = (const cplx *)fld[3], *dz = (const cplx *)fld[4];
+    cplx *out[ORC_TO_N];
+    for (int k = 0; k < ORC_TO_N; ++k) out[k] = (cplx *)out_[k];
+    const double or1 = rad[ORC_R_OR1], or2 = rad[ORC_R_OR2], or4 = rad[ORC_R_OR4];
+    const double orho1 = rad[ORC_R_ORHO1], beta = rad[ORC_R_BETA], r = rad[ORC_R_R];
+    const double CorFac = par[ORC_P_CORFAC];
+    const int conv_nl = flags[ORC_F_CONV_NL], corr = flags[ORC_F_CORR];
+#define NLV(k, i) (nl[k] ? nl[k][i] : 0.0)
+    for (int lm = 0; lm < p->lm_max; ++lm) {
+        const int l = p->lm2l[lm], m = p->lm2m[lm];
+        const double dl = (double)l, dLh = dl * (dl + 1.0);
+        const cplx dPhi = I * (double)m;
+        const double cS = td_clm(l, m), cA = td_clm(l + 1, m);
+        const double t2S = (dl - 1.0) * cS, t2A = (dl + 2.0) * cA;
+        const double t3S = (dl - 1.0) * (dl + 1.0) * cS, t3A = dl * (dl + 2.0) * cA;
+        const double t4S = (dl + 1.0) * cS * (dl - 1.0) * dl, t4A = dl * cA * (dl + 1.0) * (dl + 2.0);
+        const int lmS = (l > m) ? lm - 1 : lm;
+        const int lmA = (l < p->l_max) ? lm + 1 : -1;
+        const cplx zA = lmA >= 0 ? z[lmA] : 0.0, dwA = lmA >= 0 ? dw[lmA] : 0.0;
+        const cplx wA = lmA >= 0 ? w[lmA] : 0.0, dzA = lmA >= 0 ? dz[lmA] : 0.0;
+        /* dwdt */
+        cplx dwdt, dvxvh = 0.0;
+        if (lm == 0) {
+            dwdt = conv_nl ? or2 * NLV(ORC_NL_ADVR, 0) : 0.0;
+            if (corr && !flags[ORC_F_SINGLE]) dwdt += 2.0 * CorFac * or1 * t2A * zA;
+        } else if (!flags[ORC_F_DCURL]) {
+            dwdt = conv_nl ? or2 * NLV(ORC_NL_ADVR, lm) : 0.0;
+            if (corr) {
+                cplx c = 0.0;
+                if (l < lR) c = 2.0 * CorFac * or1 * (dPhi * dw[lm] + t2A * zA - t2S * z[lmS]);
+                else if (l == lR) c = 2.0 * CorFac * or1 * (dPhi * dw[lm] - t2S * z[lmS]);
+                dwdt += c;
+            }
+        } else {
+            dwdt = conv_nl ? dLh * or4 * orho1 * NLV(ORC_NL_ADVR, lm) : 0.0;
+            dvxvh = conv_nl ? -orho1 * r * r * dLh * NLV(ORC_NL_ADVT, lm) : 0.0;
+            if (corr) {
+                cplx c = 0.0;
+                const cplx ph = dPhi * (-ddw[lm] + beta * dw[lm] + (beta * or1 + or2) * dLh * w[lm]);
+                if (l < lR)
+                    c = 2.0 * CorFac * or2 * orho1 *
+                        (ph + t3A * (dzA - beta * zA) + t3S * (dz[lmS] - beta * z[lmS]) +
+                         or1 * (t4A * zA - t4S * z[lmS]));
+                else if (l == lR)
+                    c = 2.0 * CorFac * or2 * orho1 *
+                        (ph + t3S * (dz[lmS] - beta * z[lmS]) - or1 * t4S * z[lmS]);
+                dwdt += c;
+            }
+        }
+        if (out[ORC_TO_DWDT]) out[ORC_TO_DWDT][lm] = dwdt;
+        if (out[ORC_TO_DVXVH]) out[ORC_TO_DVXVH][lm] = dvxvh;
+        /* dzdt */
+        cplx dzdt;
+        if (lm == 0) {
+            dzdt = corr ? 2.0 * CorFac * or2 * (t3A * dwA + or1 * t4A * wA) : 0.0;
+        } else {
+            dzdt = conv_nl ? dLh * NLV(ORC_NL_ADVP, lm) : 0.0;
+            if (corr) {
+                if (l < lR)
+                    dzdt += 2.0 * CorFac * or2 * (dPhi * z[lm] + t3A * dwA + or1 * t4A * wA +
+                                                  t3S * dw[lmS] - or1 * t4S * w[lmS]);
+                else if (l == lR)
+                    dzdt += 2.0 * CorFac * or2 * (dPhi * z[lm] + t3S * dw[lmS] - or1 * t4S * w[lmS]);
+            }
+        }
+        if (out[ORC_TO_DZDT]) out[ORC_TO_DZDT][lm] = dzdt;
+        /* dpdt (l = 0 not set by the reference: zero) */
+        if (out[ORC_TO_DPDT]) {
+            cplx dpdt = 0.0;
+            if (lm > 0) {
+                dpdt = conv_nl ? -dLh * NLV(ORC_NL_ADVT, lm) : 0.0;
+                if (corr) {
+                    if (l < lR)
+                        dpdt += 2.0 * CorFac * or2 * (-dPhi * (dw[lm] + or1 * dLh * w[lm]) + t3A * zA +
+                                                      t3S * z[lmS]);
+                    else if (l == lR)
+                        dpdt += 2.0 * CorFac * or2 * (-dPhi * (dw[lm] + or1 * dLh * w[lm]) + t3S * z[lmS]);
+                }
+            }
+            out[ORC_TO_DPDT][lm] = dpdt;
+        }
+        /* dsdt */
+        if (out[ORC_TO_DSDT]) {
+            cplx dsdt;
+            const double tf = flags[ORC_F_ANEL_LIQ] ? rad[ORC_R_TEMP0] : 1.0;
+            if (lm == 0) dsdt = par[ORC_P_EPSC] * rad[ORC_R_EPSCPROF];
+            else dsdt = dLh * NLV(ORC_NL_VST, lm);
+            if (flags[ORC_F_ANEL]) dsdt += (flags[ORC_F_ANEL_LIQ] ? tf : 1.0) * NLV(ORC_NL_HEAT, lm);
+            out[ORC_TO_DSDT][lm] = dsdt;
+        }
+        /* dxidt */
+        if (out[ORC_TO_DXIDT])
+            out[ORC_TO_DXIDT][lm] = lm == 0 ? par[ORC_P_EPSCXI] : dLh * NLV(ORC_NL_VXIT, lm);
+        /* dbdt, djdt, dVxBhLM */
+        if (out[ORC_TO_DBDT]) {
+            if (flags[ORC_F_MAG_DBDT]) {
+                out[ORC_TO_DBDT][lm] = dLh * NLV(ORC_NL_VXBP, lm);
+                out[ORC_TO_DVXBH][lm] = -dLh * NLV(ORC_NL_VXBT, lm) * r * r;
+                out[ORC_TO_DJDT][lm] = dLh * or4 * NLV(ORC_NL_VXBR, lm);
+            } else {
+                out[ORC_TO_DBDT][lm] = 0.0;
+                out[ORC_TO_DVXBH][lm] = 0.0;
+                out[ORC_TO_DJDT][lm] = 0.0;
+            }
+        }
+    }
+#undef NLV
 }

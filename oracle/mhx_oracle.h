@@ -124,6 +124,48 @@ void orc_get_br_v_bcs(const orc_plan *p, const double *br, const double *vt, con
                       double omega, double O_r_E_2, double O_rho, double *br_vt_lm,
                       double *br_vp_lm);
 
+/* ---- the radial loop of any physics switches, one interior level ---- */
+/* grid components (the reference's grid_space_arrays_t fields) */
+enum { ORC_RI_VR = 0, ORC_RI_VT, ORC_RI_VP, ORC_RI_CVR, ORC_RI_CVT, ORC_RI_CVP, ORC_RI_DVRDR,
+       ORC_RI_DVTDR, ORC_RI_DVPDR, ORC_RI_DVRDT, ORC_RI_DVRDP, ORC_RI_DVTDP, ORC_RI_DVPDP,
+       ORC_RI_S, ORC_RI_XI, ORC_RI_PHI, ORC_RI_BR, ORC_RI_BT, ORC_RI_BP, ORC_RI_CBR, ORC_RI_CBT,
+       ORC_RI_CBP, ORC_RI_N };
+/* grid products */
+enum { ORC_RO_ADVR = 0, ORC_RO_ADVT, ORC_RO_ADVP, ORC_RO_VSR, ORC_RO_VST, ORC_RO_VSP, ORC_RO_VXIR,
+       ORC_RO_VXIT, ORC_RO_VXIP, ORC_RO_VXBR, ORC_RO_VXBT, ORC_RO_VXBP, ORC_RO_HEAT, ORC_RO_PHI,
+       ORC_RO_N };
+/* switches (0/1) */
+enum { ORC_F_CONV_NL = 0, ORC_F_ADV_CURL, ORC_F_HEAT_NL, ORC_F_CHEM, ORC_F_PHASE, ORC_F_PREC,
+       ORC_F_CENTR, ORC_F_MAG_LF, ORC_F_MAG_NL, ORC_F_ANEL, ORC_F_LCR, ORC_F_CORR, ORC_F_SINGLE,
+       ORC_F_DCURL, ORC_F_ANEL_LIQ, ORC_F_MAG_DBDT, ORC_F_N };
+/* constants */
+enum { ORC_P_LFFAC = 0, ORC_P_VISCHEAT, ORC_P_OHMLOSS, ORC_P_EPSPHASE, ORC_P_PENALTY,
+       ORC_P_PHASEDIFF, ORC_P_TMELT, ORC_P_PO, ORC_P_PREC_ANGLE, ORC_P_OEK, ORC_P_TIME,
+       ORC_P_DILUTION, ORC_P_RA, ORC_P_OPR, ORC_P_MINC, ORC_P_CORFAC, ORC_P_EPSC, ORC_P_EPSCXI,
+       ORC_P_N };
+/* per-level radial values */
+enum { ORC_R_R = 0, ORC_R_OR1, ORC_R_OR2, ORC_R_OR4, ORC_R_ORHO1, ORC_R_BETA, ORC_R_OTEMP1,
+       ORC_R_VISC, ORC_R_LAMBDA, ORC_R_TEMP0, ORC_R_EPSCPROF, ORC_R_N };
+/* forward outputs entering the lm-space assembly */
+enum { ORC_NL_ADVR = 0, ORC_NL_ADVT, ORC_NL_ADVP, ORC_NL_VST, ORC_NL_VXIT, ORC_NL_VXBR,
+       ORC_NL_VXBT, ORC_NL_VXBP, ORC_NL_HEAT, ORC_NL_N };
+/* lm-space assembly outputs */
+enum { ORC_TO_DWDT = 0, ORC_TO_DZDT, ORC_TO_DPDT, ORC_TO_DSDT, ORC_TO_DXIDT, ORC_TO_DBDT,
+       ORC_TO_DJDT, ORC_TO_DVXBH, ORC_TO_DVXVH, ORC_TO_N };
+
+/* src/get_nl.f90:240-436 and src/rIter.f90:644-680 on one level's grids
+   (in[ORC_RI_N], NULL where absent; out[ORC_RO_N], NULL where not wanted) */
+void orc_rl_products(const orc_plan *p, const int *flags, const double *par, const double *rad,
+                     const double *const *in, double *const *out);
+/* src/courant.f90:161-276 for one level: par = courfac, alffac, LFfac,
+   opm, l_cour_alf_damp; rad = or2, or4, orho1, delxr2, delxh2; in = vr vt
+   vp br bt bp grids */
+void orc_rl_courant(const orc_plan *p, int cour_mag, const double *par, const double *rad,
+                    const double *const *in, double *dtrkc, double *dthkc);
+/* src/get_td.f90 nonlinear_lm_t at one interior level */
+void orc_rl_td(const orc_plan *p, const int *flags, const double *par, const double *rad, int lR,
+               const double *const *nl, const double *const *fld, double *const *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,11 @@ def lib():
         L.orc_get_td.argtypes = [P, I, D, D, D, D, ctypes.POINTER(dp), ctypes.POINTER(dp)]
         L.orc_courant.argtypes = [P, D, D, D, D, D, D, D, D, ctypes.POINTER(dp), dp, dp]
         L.orc_get_br_v_bcs.argtypes = [P, dp, dp, dp, D, D, D, dp, dp]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_rl_products.argtypes = [P, ip, dp, dp, ctypes.POINTER(dp), ctypes.POINTER(dp)]
+        L.orc_rl_courant.argtypes = [P, I, dp, dp, ctypes.POINTER(dp), dp, dp]
+        L.orc_rl_td.argtypes = [P, ip, dp, dp, I, ctypes.POINTER(dp), ctypes.POINTER(dp),
+                                ctypes.POINTER(dp)]
         _lib = L
     return _lib
 
@@ -286,6 +291,64 @@ class Plan:
         return a, b
 
 
+    # ---- the radial loop of any physics switches, one interior level
+    def rl_products(self, flags, par, rad, grids, want):
+        """src/get_nl.f90:240-436 merged as src/rIter.f90:644-680.  ``grids``:
+        dict ORC_RI index -> grid; ``want``: ORC_RO indices to compute.
+        Returns dict ORC_RO index -> grid."""
+        P = ctypes.POINTER(ctypes.c_double)
+        ins = (P * RL_NI)(*[(_ptr(self._g(grids[k])) if k in grids else None) for k in range(RL_NI)])
+        outs = {k: self.grid() for k in want}
+        arr_out = (P * RL_NO)(*[(_ptr(outs[k]) if k in outs else None) for k in range(RL_NO)])
+        fl = (ctypes.c_int * len(flags))(*[int(x) for x in flags])
+        pa = (ctypes.c_double * len(par))(*[float(x) for x in par])
+        ra = (ctypes.c_double * len(rad))(*[float(x) for x in rad])
+        self._L.orc_rl_products(ctypes.c_void_p(self.h), fl, pa, ra, ins, arr_out)
+        return outs
+
+    def rl_courant(self, cour_mag, par, rad, grids):
+        """(dtrkc, dthkc) of src/courant.f90 on vr vt vp br bt bp"""
+        P = ctypes.POINTER(ctypes.c_double)
+        z = self.grid()
+        arr = (P * 6)(*[_ptr(self._g(g)) if g is not None else _ptr(z) for g in grids])
+        pa = (ctypes.c_double * 5)(*[float(x) for x in par])
+        ra = (ctypes.c_double * 5)(*[float(x) for x in rad])
+        a, b = np.zeros(1), np.zeros(1)
+        self._L.orc_rl_courant(ctypes.c_void_p(self.h), int(cour_mag), pa, ra, arr, _ptr(a), _ptr(b))
+        return float(a[0]), float(b[0])
+
+    def rl_td(self, flags, par, rad, lR, nl, fld, want):
+        """src/get_td.f90 nonlinear_lm_t at one interior level: ``nl`` dict
+        ORC_NL index -> coefficients, ``fld`` = w dw ddw z dz, ``want``
+        ORC_TO indices.  Returns dict ORC_TO index -> coefficients."""
+        P = ctypes.POINTER(ctypes.c_double)
+        nla = (P * RL_NNL)(*[(_ptr(self._c(nl[k])) if k in nl else None) for k in range(RL_NNL)])
+        fa = (P * 5)(*[_ptr(self._c(f)) for f in fld])
+        outs = {k: self.coef() for k in want}
+        oa = (P * RL_NTO)(*[(_ptr(outs[k]) if k in outs else None) for k in range(RL_NTO)])
+        fl = (ctypes.c_int * len(flags))(*[int(x) for x in flags])
+        pa = (ctypes.c_double * len(par))(*[float(x) for x in par])
+        ra = (ctypes.c_double * len(rad))(*[float(x) for x in rad])
+        self._L.orc_rl_td(ctypes.c_void_p(self.h), fl, pa, ra, int(lR), nla, fa, oa)
+        return outs
+
+
+# index sets of the radial-loop oracle (mhx_oracle.h)
+RL_NI, RL_NO, RL_NNL, RL_NTO = 22, 14, 9, 9
+(RI_VR, RI_VT, RI_VP, RI_CVR, RI_CVT, RI_CVP, RI_DVRDR, RI_DVTDR, RI_DVPDR, RI_DVRDT, RI_DVRDP,
+ RI_DVTDP, RI_DVPDP, RI_S, RI_XI, RI_PHI, RI_BR, RI_BT, RI_BP, RI_CBR, RI_CBT, RI_CBP) = range(22)
+(RO_ADVR, RO_ADVT, RO_ADVP, RO_VSR, RO_VST, RO_VSP, RO_VXIR, RO_VXIT, RO_VXIP, RO_VXBR, RO_VXBT,
+ RO_VXBP, RO_HEAT, RO_PHI) = range(14)
+(F_CONV_NL, F_ADV_CURL, F_HEAT_NL, F_CHEM, F_PHASE, F_PREC, F_CENTR, F_MAG_LF, F_MAG_NL, F_ANEL,
+ F_LCR, F_CORR, F_SINGLE, F_DCURL, F_ANEL_LIQ, F_MAG_DBDT) = range(16)
+(P_LFFAC, P_VISCHEAT, P_OHMLOSS, P_EPSPHASE, P_PENALTY, P_PHASEDIFF, P_TMELT, P_PO, P_PREC_ANGLE,
+ P_OEK, P_TIME, P_DILUTION, P_RA, P_OPR, P_MINC, P_CORFAC, P_EPSC, P_EPSCXI) = range(18)
+(R_R, R_OR1, R_OR2, R_OR4, R_ORHO1, R_BETA, R_OTEMP1, R_VISC, R_LAMBDA, R_TEMP0,
+ R_EPSCPROF) = range(11)
+(NL_ADVR, NL_ADVT, NL_ADVP, NL_VST, NL_VXIT, NL_VXBR, NL_VXBT, NL_VXBP, NL_HEAT) = range(9)
+(TO_DWDT, TO_DZDT, TO_DPDT, TO_DSDT, TO_DXIDT, TO_DBDT, TO_DJDT, TO_DVXBH, TO_DVXVH) = range(9)
+
+
 def have_ref():
     return os.path.exists(REF_HARNESS)
 
@@ -378,5 +441,49 @@ def e_kin_levels(w, dw, z, or2, orho1, l_max, minc=1):
                 out[4] += ep
             else:
                 out[5] += et
+            lm += 1
+    return out
+
+
+def e_mag_levels(b, db, aj, or2, l_max, minc=1):
+    """Per-level sums of src/magnetic_energy.f90:264-340 (get_e_mag, outer
+    core): rows e_p, e_t, e_p_as, e_t_as, e_p_es, e_t_es, e_p_eas, e_t_eas,
+    e_dipole, e_dipole_ax (the equatorial parities opposite to the kinetic
+    ones: the poloidal potential of an equatorially symmetric field has odd
+    l + m)."""
+    out = np.zeros((10, b.shape[0]))
+    lm = 0
+    for m in range(0, l_max + 1, minc):
+        for l in range(m, l_max + 1):
+            if l == 0:
+                lm += 1
+                continue
+            dLh = float(l * (l + 1))
+            cb, cd, cj = b[:, lm], db[:, lm], aj[:, lm]
+            if m == 0:
+                nb, nd, nj = cb.real ** 2, cd.real ** 2, cj.real ** 2
+            else:
+                nb = 2.0 * (cb.real ** 2 + cb.imag ** 2)
+                nd = 2.0 * (cd.real ** 2 + cd.imag ** 2)
+                nj = 2.0 * (cj.real ** 2 + cj.imag ** 2)
+            ep = dLh * (dLh * or2 * nb + nd)
+            et = dLh * nj
+            out[0] += ep
+            out[1] += et
+            if m == 0:
+                out[2] += ep
+                out[3] += et
+                if l % 2 == 1:
+                    out[6] += ep
+                else:
+                    out[7] += et
+            if (l + m) % 2 == 1:
+                out[4] += ep
+            else:
+                out[5] += et
+            if l == 1:
+                out[8] += ep
+                if m == 0:
+                    out[9] += ep
             lm += 1
     return out

@@ -146,3 +146,22 @@ def test_checkpoint_multistep_layout(tmp_path):
     (tmp_path / "long.start").write_bytes(data + b"\0" * 8)
     with pytest.raises(ValueError):
         checkpoint.read_checkpoint(str(tmp_path / "long.start"))
+
+
+def test_oracle_e_mag_matches_reference_output():
+    """outer-core magnetic energies (src/magnetic_energy.f90 get_e_mag) of the
+    boussBenchSat restart state against the first row of the reference
+    program's e_mag_oc output for that state (tests/golden/integ_bouss200.npz,
+    written before its first step): e_p, e_t, their axisymmetric,
+    equatorially symmetric and equatorially symmetric axisymmetric parts,
+    LFfac/2 = 100 (E = 1e-3, Pm = 5)"""
+    h, _ = _state()
+    with np.load(os.path.join(GOLDEN, "integ_bouss200.npz"), allow_pickle=False) as z:
+        ref = z["e_mag_oc"][0]
+    assert abs(h["time"] - ref[0]) < 1e-9
+    r = h["r"]
+    b, aj = h["b"], h["aj"]
+    sums = oracle.e_mag_levels(b, oracle.cheb_dr(b, r), aj, 1.0 / r ** 2, h["l_max"], h["minc"])
+    e = 0.5 * (1.0 / (h["ek"] * h["prmag"])) * oracle.rint_interp(sums, r)
+    np.testing.assert_allclose(e[:4], ref[1:5], rtol=2e-8, atol=0)
+    np.testing.assert_allclose(e[4:8], ref[7:11], rtol=2e-8, atol=0)
