@@ -15,6 +15,7 @@ import argparse
 import json
 import os
 import re
+import subprocess
 import sys
 import tempfile
 import time
@@ -85,25 +86,47 @@ def algorithmic(kind, w, nsrc, nh, lm_max, n_theta, n_phi, n_m, nlev, word, twor
     return flops, float(byts)
 
 
-def cpu_baseline(l_max, nlev_sample):
+def host_cpu_model():
+    """The host CPU's model name (lscpu), or the /proc/cpuinfo entry."""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        m = re.search(r"Model name:\s*(.+)", out)
+        if m:
+            return m.group(1).strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        with open("/proc/cpuinfo") as f:
+            m = re.search(r"model name\s*:\s*(.+)", f.read())
+            return m.group(1).strip() if m else None
+    except OSError:
+        return None
+
+
+def cpu_baseline(l_max, nlev_sample, nrep=5):
     """The reference's own OpenMP native path (oracle/_ref, the reference's
-    Fortran modules) on this host's cores over a bounded sample of levels."""
+    Fortran modules) on this host's cores over a bounded sample of levels:
+    one warm-up pass, then the median of nrep timed passes."""
+    import numpy as np
     from oracle import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     if oracle.have_ref():
         with tempfile.TemporaryDirectory() as td:
             out = os.path.join(td, "bench.txt")
             t0 = time.time()
-            oracle.run_ref("bench", l_max, 1, 20, "/dev/null", out, nlev=nlev_sample, nrep=1,
+            oracle.run_ref("bench", l_max, 1, 20, "/dev/null", out, nlev=nlev_sample, nrep=nrep,
                            threads=threads)
             wall = time.time() - t0
-            line = open(out).readline()
-        kv = dict(re.findall(r"(\w+)=\s*([^\s]+)", line))
-        mean_s = float(kv["mean_s"])
-        return {"value": nlev_sample / mean_s, "unit": "radial-level transform pairs/s",
-                "cores": int(kv["threads"]), "kind": "reference",
-                "sample": "%d radial levels of the l_max=%d nonlinear step, 1 warm-up + 1 timed "
-                          "pass (%.1f s wall)" % (nlev_sample, l_max, wall)}
+            lines = open(out).read().split("\n")
+        kv = dict(re.findall(r"(\w+)=\s*([^\s]+)", lines[0]))
+        times = [float(x) for x in lines[2:] if x.strip()]
+        med = float(np.median(times))
+        return {"value": nlev_sample / med, "unit": "radial-level transform pairs/s",
+                "cores": int(kv["threads"]), "kind": "reference", "host_cpu": host_cpu_model(),
+                "pass_times_s": times,
+                "sample": "%d radial levels of the l_max=%d nonlinear step (the reference's "
+                          "nl_level chain, its own inputs), 1 warm-up + median of %d timed "
+                          "passes (%.1f s wall)" % (nlev_sample, l_max, len(times), wall)}
     # without the reference build: the FP64 oracle, single thread, one level
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from magic_amd import workload

@@ -13,6 +13,9 @@
   against the oracle on one level and the split form against FP64 on all 64
   levels per 64-wide l band.
 
+FP64 against the oracle means relative L2 below 1e-12 per output field, or
+below 100 x the oracle's own sensitivity to one-ulp changes of its inputs
+where that is larger (an ill-conditioned output: see _fp64_vs_oracle).
 The oracle runs on the host (OpenMP over orders and FFT rows)."""
 import numpy as np
 import pytest
@@ -69,11 +72,25 @@ def _fp64_vs_oracle(L, inputs, rad, lcut, levels):
     for lev in levels:
         sub, subrad = _sub(inputs, rad, lev)
         _, ref = oracle_step(o, sub, *subrad, [lcut[lev]])
+        # the oracle's own sensitivity: how far its outputs move when every
+        # input coefficient is changed by about one unit in the last place.
+        # An output that is a small residue of cancelling terms (the toroidal
+        # part of a mostly poloidal advection: AdvpLM at l_max=1023 with white
+        # velocity spectra is 1/400 of AdvrLM and moves by 5.7e-13) differs
+        # between two FP64 summation orders by far more than 1e-12.
+        rng = np.random.default_rng(1000 + lev)
+        pert = [x + x * (1e-16 * rng.standard_normal(x.shape)) for x in sub]
+        _, ref_p = oracle_step(o, pert, *subrad, [lcut[lev]])
         errs = [rel_l2(outs[k][lev], ref[k][0]) for k in range(10)]
+        sens = [rel_l2(ref_p[k][0], ref[k][0]) for k in range(10)]
         print("l_max=%d level %d FP64 vs oracle: %s" % (L, lev, " ".join("%.1e" % e for e in errs)),
               flush=True)
+        print("  oracle sensitivity to 1-ulp input changes: %s" % " ".join("%.1e" % s for s in sens),
+              flush=True)
         for k, e in enumerate(errs):
-            assert e < 1e-12, (L, lev, sht.NL_OUTPUTS[k], e)
+            # tolerance: 1e-12, or 100 x the oracle's own 1-ulp sensitivity
+            # where that is larger
+            assert e < max(1e-12, 100.0 * sens[k]), (L, lev, sht.NL_OUTPUTS[k], e, sens[k])
     del o
     return outs
 
