@@ -313,6 +313,24 @@ int mhx_get_e_kin(mhx_plan *p, int nlev, const double *w, const double *dw, cons
 int mhx_get_e_mag(mhx_plan *p, int nlev, const double *b, const double *db, const double *aj,
                   size_t ld_lm, const double *or2, double *sums);
 
+/* The kinetic-energy sums of mhx_get_e_kin without any allocation.  This
+ * entry has no counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It calls no hipMalloc, hipFree or
+ * allocating library routine: the kernels' partial sums live in a fixed
+ * device array of the library (2.6 MB per device, held until the process
+ * ends).  Every array is device memory: w, dw, z (complex, level stride
+ * ld_lm), or2 and orho1 (nlev each) and sums (8 x nlev, the layout of
+ * mhx_get_e_kin); host arrays are refused (MHX_E_INVALID), as are more than
+ * 2048 levels per call.  The plan's device must be current.  Asynchronous:
+ * the kernels are queued on `stream` (null: the null stream) and the call
+ * returns; sums are valid once the stream reaches that point.  Calls that
+ * overlap on one device share its array and race: order them on one stream
+ * or wait between them. */
+int mhx_get_e_kin_noalloc(mhx_plan *p, int nlev, const double *w, const double *dw,
+                          const double *z, size_t ld_lm, const double *or2, const double *orho1,
+                          double *sums, void *stream);
+
 /* ---- uneven batches of independent transform jobs (split form) ----
  * A job is the round trip of nfields fields of one kind at one truncation
  * over nlev radial levels: synthesis to grids and analysis back, i.e. per

@@ -202,7 +202,8 @@ constexpr int EK_CHUNKS = 16;  // lm chunks (blocks) per level of the energy sum
 struct EkArgs {
     const double *w, *dw, *z;  // complex, level stride ld_lm (b, db, aj for mag)
     size_t ld_lm;
-    const double *rad;         // device: or2[nlev], orho1[nlev] (mag: orho1 unused)
+    const double *or2;         // device: or2[nlev]
+    const double *orho1;       // device: orho1[nlev], null for 1 (mag: unused)
     double *part;              // device: nsum x nlev x EK_CHUNKS partial sums
     double *out;               // device: nsum sums x nlev
     int nlev;

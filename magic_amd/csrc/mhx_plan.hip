@@ -1290,7 +1290,8 @@ static int energy_sums(mhx_plan *h, int nlev, const double *w, const double *dw,
     a.dw = src[1];
     a.z = src[2];
     a.ld_lm = ld_lm;
-    a.rad = P->d_ek;
+    a.or2 = P->d_ek;
+    a.orho1 = mag ? nullptr : P->d_ek + nlev;
     a.out = P->d_ek + 2 * (size_t)nlev;
     a.part = P->d_ek + 12 * (size_t)nlev;
     a.nlev = nlev;

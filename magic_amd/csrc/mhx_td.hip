@@ -22,6 +22,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <mutex>
 
 namespace mhx {
 
@@ -122,7 +124,7 @@ __global__ __launch_bounds__(256) void k_e_kin(EkArgs a)
     const double2 *w = reinterpret_cast<const double2 *>(a.w) + (size_t)lev * a.ld_lm;
     const double2 *dw = reinterpret_cast<const double2 *>(a.dw) + (size_t)lev * a.ld_lm;
     const double2 *z = reinterpret_cast<const double2 *>(a.z) + (size_t)lev * a.ld_lm;
-    const double or2 = a.rad[lev], orho = a.mag ? 1.0 : a.rad[a.nlev + lev];
+    const double or2 = a.or2[lev], orho = (a.mag || !a.orho1) ? 1.0 : a.orho1[lev];
     // the magnetic sums take the opposite l parity for their equatorially
     // symmetric parts (src/magnetic_energy.f90:281-300 against
     // src/kinetic_energy.f90:143-157)
@@ -213,3 +215,93 @@ int launch_get_td(const Plan *p, const TdArgs &args, hipStream_t s)
 }
 
 }  // namespace mhx
+
+// ---------------------------------------------------------------------------
+// Allocation-free kinetic-energy sums (mhx_get_e_kin_noalloc): the kernels of
+// mhx_get_e_kin on device arrays only, with their partial sums in a
+// file-scope device array instead of plan memory that grows with hipMalloc.
+// Each device holds its own copy of the array (until the process ends); its
+// address on a device comes from hipGetSymbolAddress, looked up on the entry's
+// first call on that device with the device current.  Every call on a device
+// shares that device's array: calls that overlap on one device race.  No entry
+// needs clearing: every partial sum the final reduction reads is written by
+// the same call.
+
+namespace mhx {
+namespace {
+
+constexpr int EK_NOALLOC_MAX_LEVELS = 2048;
+constexpr int EK_NOALLOC_MAX_DEVICES = 64;
+
+// the partial sums: 10 sums x levels x EK_CHUNKS chunks (2.6 MB)
+static __device__ __align__(256) double scratch[10 * (size_t)EK_CHUNKS * EK_NOALLOC_MAX_LEVELS];
+
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[EK_NOALLOC_MAX_DEVICES] = {};
+
+// the device address of `scratch` on device dev (current), looked up once
+int scratch_on_device(int dev, double **out)
+{
+    if (dev < 0 || dev >= EK_NOALLOC_MAX_DEVICES)
+        return fail(MHX_E_UNSUPPORTED, "device ordinal " + std::to_string(dev) + " beyond " +
+                                           std::to_string(EK_NOALLOC_MAX_DEVICES));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        double *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(reinterpret_cast<void **>(&p), HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return fail(MHX_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return MHX_OK;
+}
+
+}  // namespace
+
+}  // namespace mhx
+
+using namespace mhx;
+
+int mhx_get_e_kin_noalloc(mhx_plan *h, int nlev, const double *w, const double *dw, const double *z,
+                          size_t ld_lm, const double *or2, const double *orho1, double *sums,
+                          void *stream)
+{
+    if (!h) return fail(MHX_E_INVALID, "null plan");
+    Plan *P = &h->p;
+    if (nlev <= 0) return MHX_OK;
+    if (nlev > EK_NOALLOC_MAX_LEVELS)
+        return fail(MHX_E_INVALID, "mhx_get_e_kin_noalloc: " + std::to_string(nlev) +
+                                       " levels exceed its fixed scratch of " +
+                                       std::to_string(EK_NOALLOC_MAX_LEVELS) + " levels");
+    if (!w || !dw || !z || !or2 || !orho1 || !sums) return fail(MHX_E_INVALID, "null pointer");
+    if (ld_lm < (size_t)P->lm_max) return fail(MHX_E_INVALID, "ld_lm below lm_max");
+    const void *arrs[6] = {w, dw, z, or2, orho1, sums};
+    for (const void *a : arrs)
+        if (!is_device_ptr(a))
+            return fail(MHX_E_INVALID, "mhx_get_e_kin_noalloc takes device arrays only (a host array "
+                                       "would need staging memory)");
+    int dev = -1;
+    MHX_HIP_CHECK(hipGetDevice(&dev));
+    if (dev != P->device)
+        return fail(MHX_E_INVALID, "the current device is not the plan's device");
+    double *part = nullptr;
+    int rc = scratch_on_device(dev, &part);
+    if (rc) return rc;
+    EkArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.w = w;
+    a.dw = dw;
+    a.z = z;
+    a.ld_lm = ld_lm;
+    a.or2 = or2;
+    a.orho1 = orho1;
+    a.part = part;
+    a.out = sums;
+    a.nlev = nlev;
+    a.mag = 0;
+    a.nsum = 8;
+    return launch_e_kin(P, a, (hipStream_t)stream);
+}

@@ -56,6 +56,7 @@ _SIGS = {
     "mhx_get_td": "Pipzppppdpp",
     "mhx_get_e_kin": "Pipppzppp",
     "mhx_get_e_mag": "Pipppzpp",
+    "mhx_get_e_kin_noalloc": "Pipppzpppp",
     "mhx_rloop": "Ppippzpp",
     "mhx_nl_step_courant": "Pipzppppdpddppppp",
     "mhx_set_timing": "Pi",
@@ -391,6 +392,23 @@ def get_e_kin_levels(plan, w, dw, z, or2, orho1):
     _check(plan._L.mhx_get_e_kin(plan.h, nlev, ptr(arrs[0]), ptr(arrs[1]), ptr(arrs[2]),
                                  plan.lm_max, d(or2), d(orho1), ptr(out)))
     return out
+
+
+def get_e_kin_noalloc(plan, w, dw, z, or2, orho1, sums, stream=None):
+    """The kinetic-energy sums of get_e_kin_levels through the entry that
+    allocates nothing (mhx_get_e_kin_noalloc): w, dw, z (nlev, lm_max)
+    complex, or2 and orho1 (nlev) float64 and sums (8, nlev) float64, all
+    contiguous device tensors.  Queued on `stream` (a torch stream, or the
+    null stream); sums are valid once that stream has run."""
+    nlev = int(w.shape[0])
+    for a in (w, dw, z, or2, orho1, sums):
+        if not _is_torch(a) or not a.is_cuda:
+            raise ValueError("get_e_kin_noalloc takes device tensors only")
+    if tuple(sums.shape) != (8, nlev) or or2.numel() != nlev or orho1.numel() != nlev:
+        raise ValueError("sums must be (8, nlev) and or2, orho1 nlev long")
+    st = ctypes.c_void_p(stream.cuda_stream) if stream is not None else None
+    _check(plan._L.mhx_get_e_kin_noalloc(plan.h, nlev, _ptr(w), _ptr(dw), _ptr(z), w.shape[1],
+                                         _ptr(or2), _ptr(orho1), _ptr(sums), st))
 
 
 def get_e_mag_levels(plan, b, db, aj, or2):
