@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmhx.so")
 SOURCES = ["mhx_plan.hip", "mhx_tables.hip", "mhx_legendre.hip", "mhx_split.hip", "mhx_nlsplit.hip",
-           "mhx_fft.hip", "mhx_batch.hip", "mhx_td.hip", "mhx_rloop.hip"]
+           "mhx_fft.hip", "mhx_batch.hip", "mhx_td.hip", "mhx_rloop.hip", "mhx_grid.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -1,0 +1,208 @@
+// Building blocks shared by the FFT kernels (mhx_fft.hip, mhx_grid.hip):
+// complex helpers, in-register DFTs of sizes 2, 3, 4, 5, 12 and 16 with the
+// kernel exp(sign 2 pi i nk / R), the wave-local LDS ordering point and the
+// per-level Courant maxima of the grid stages.
+#pragma once
+
+#include "mhx_internal.h"
+
+namespace mhx {
+
+namespace {
+
+template <typename T> struct CT;
+template <> struct CT<double> { typedef double2 t; };
+template <> struct CT<float> { typedef float2 t; };
+template <typename T> using cpx = typename CT<T>::t;
+
+template <typename C> __device__ inline C mk(decltype(C::x) a, decltype(C::x) b)
+{
+    C r;
+    r.x = a;
+    r.y = b;
+    return r;
+}
+template <typename C> __device__ inline C cmul(C a, C b)
+{
+    return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename C> __device__ inline C cconj(C a) { return mk<C>(a.x, -a.y); }
+template <typename C> __device__ inline C cadd(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
+template <typename C> __device__ inline C csub(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y); }
+template <typename C> __device__ inline C cmuli(C a, int sign)  // * (sign i)
+{
+    return sign > 0 ? mk<C>(-a.y, a.x) : mk<C>(a.y, -a.x);
+}
+
+// in-register DFT of size R with kernel exp(sign 2 pi i / R)
+template <int R, typename C> struct Dft;
+template <typename C> struct Dft<2, C> {
+    __device__ static void run(C *x, int)
+    {
+        C a = x[0], b = x[1];
+        x[0] = cadd(a, b);
+        x[1] = csub(a, b);
+    }
+};
+template <typename C> struct Dft<4, C> {
+    __device__ static void run(C *x, int sign)
+    {
+        C a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
+        C b0 = cadd(x[1], x[3]), b1 = cmuli(csub(x[1], x[3]), sign);
+        x[0] = cadd(a0, b0);
+        x[2] = csub(a0, b0);
+        x[1] = cadd(a1, b1);
+        x[3] = csub(a1, b1);
+    }
+};
+template <typename C> struct Dft<3, C> {
+    __device__ static void run(C *x, int sign)
+    {
+        typedef decltype(C::x) T;
+        const T c = T(-0.5), s = T(0.86602540378443864676) * T(sign);
+        C t1 = cadd(x[1], x[2]), t2 = csub(x[1], x[2]);
+        C y0 = cadd(x[0], t1);
+        C m1 = mk<C>(x[0].x + c * t1.x, x[0].y + c * t1.y);
+        C m2 = mk<C>(-s * t2.y, s * t2.x);
+        x[0] = y0;
+        x[1] = cadd(m1, m2);
+        x[2] = csub(m1, m2);
+    }
+};
+template <typename C> struct Dft<5, C> {
+    __device__ static void run(C *x, int sign)
+    {
+        typedef decltype(C::x) T;
+        const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+        const T s1 = T(0.95105651629515357212) * T(sign), s2 = T(0.58778525229247312917) * T(sign);
+        C a1 = cadd(x[1], x[4]), b1 = csub(x[1], x[4]);
+        C a2 = cadd(x[2], x[3]), b2 = csub(x[2], x[3]);
+        C y0 = cadd(x[0], cadd(a1, a2));
+        C r1 = mk<C>(x[0].x + c1 * a1.x + c2 * a2.x, x[0].y + c1 * a1.y + c2 * a2.y);
+        C r2 = mk<C>(x[0].x + c2 * a1.x + c1 * a2.x, x[0].y + c2 * a1.y + c1 * a2.y);
+        C i1 = mk<C>(-(s1 * b1.y + s2 * b2.y), s1 * b1.x + s2 * b2.x);
+        C i2 = mk<C>(-(s2 * b1.y - s1 * b2.y), s2 * b1.x - s1 * b2.x);
+        x[0] = y0;
+        x[1] = cadd(r1, i1);
+        x[4] = csub(r1, i1);
+        x[2] = cadd(r2, i2);
+        x[3] = csub(r2, i2);
+    }
+};
+
+// 12-point DFT as 4 x 3 with the prime-factor (Good-Thomas) index maps
+// n = 3 n1 + 4 n2, k = 9 k1 + 4 k2 (mod 12): no internal twiddles
+template <typename C> struct Dft<12, C> {
+    __device__ __forceinline__ static void run(C *x, int sign)
+    {
+        C y[4][3];
+#pragma unroll
+        for (int n2 = 0; n2 < 3; ++n2) {
+            C a[4];
+#pragma unroll
+            for (int n1 = 0; n1 < 4; ++n1) a[n1] = x[(3 * n1 + 4 * n2) % 12];
+            Dft<4, C>::run(a, sign);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; ++k1) y[k1][n2] = a[k1];
+        }
+        C out[12];
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            C b[3] = {y[k1][0], y[k1][1], y[k1][2]};
+            Dft<3, C>::run(b, sign);
+#pragma unroll
+            for (int k2 = 0; k2 < 3; ++k2) out[(9 * k1 + 4 * k2) % 12] = b[k2];
+        }
+#pragma unroll
+        for (int i = 0; i < 12; ++i) x[i] = out[i];
+    }
+};
+
+// 16-point DFT as 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2; inner DFTs over n1,
+// twiddles exp(sign 2 pi i n2 k1 / 16), outer DFTs over n2
+template <typename C> struct Dft<16, C> {
+    __device__ __forceinline__ static void run(C *x, int sign)
+    {
+        typedef decltype(C::x) T;
+        const T c1 = T(0.92387953251128675613), s1 = T(0.38268343236508977173);
+        const T r2 = T(0.70710678118654752440), sg = T(sign);
+        C a[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+            C t[4] = {x[n2], x[4 + n2], x[8 + n2], x[12 + n2]};
+            Dft<4, C>::run(t, sign);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = t[k1];
+        }
+        // exp(sign 2 pi i e / 16) for e = n2 k1
+        auto tw = [&](C v, int e) -> C {
+            switch (e) {
+            case 1: return mk<C>(c1 * v.x - sg * s1 * v.y, c1 * v.y + sg * s1 * v.x);
+            case 2: return mk<C>(r2 * (v.x - sg * v.y), r2 * (v.y + sg * v.x));
+            case 3: return mk<C>(s1 * v.x - sg * c1 * v.y, s1 * v.y + sg * c1 * v.x);
+            case 4: return cmuli(v, sign);
+            case 6: return mk<C>(-r2 * (v.x + sg * v.y), r2 * (sg * v.x - v.y));
+            case 9: return mk<C>(-c1 * v.x + sg * s1 * v.y, -c1 * v.y - sg * s1 * v.x);
+            default: return v;
+            }
+        };
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            C t[4];
+#pragma unroll
+            for (int n2 = 0; n2 < 4; ++n2) t[n2] = tw(a[n2][k1], n2 * k1);
+            Dft<4, C>::run(t, sign);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) x[k1 + 4 * k2] = t[k2];
+        }
+    }
+};
+
+// compiler-level ordering of LDS accesses between the lanes of one wave
+__device__ __forceinline__ void wave_order()
+{
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+
+
+// Courant maxima of one level (src/courant.f90:209-275, magnetic branch with
+// the Alfven damping terms valri2/valhi2): over this thread's points the
+// maxima of or4 (cf^2 orho2 vr^2 + af^2 valr^2 / (valr + valri2)) and or2
+// (cf^2 orho2 (vt^2 + vp^2)/sin^2 + af^2 valh^2 / (valh + valhi2)), valr =
+// LFfac orho1 br^2, valh = LFfac orho1 (bt^2 + bp^2)/sin^2, orho2 = orho1^2;
+// reduced over the wave, one atomic maximum per wave and quantity
+template <typename T, int NP>
+__device__ inline void courant_max(const CourArgs &c, int lev, int nlev, T o2, T or2, T or4, T orho1,
+                                   T LFfac, const T (*u)[NP], const T (*b)[NP])
+{
+    const T cf2 = T(c.par[0]), af2 = T(c.par[1]);
+    const T vri2 = T(c.par[2 + lev]), vhi2 = T(c.par[2 + nlev + lev]);
+    const T orho2 = orho1 * orho1;
+    T mr = T(0), mh = T(0);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const T vr = u[0][p], vt = u[1][p], vp = u[2][p];
+        const T br = b[0][p], bt = b[1][p], bp = b[2][p];
+        const T valr = br * br * LFfac * orho1;
+        const T valr2 = valr * valr / (valr + vri2);
+        mr = fmax(mr, or4 * (cf2 * (orho2 * vr * vr) + af2 * valr2));
+        const T valh2 = (bt * bt + bp * bp) * LFfac * o2 * orho1;
+        const T valh2m = valh2 * valh2 / (valh2 + vhi2);
+        mh = fmax(mh, or2 * (cf2 * ((vt * vt + vp * vp) * o2 * orho2) + af2 * valh2m));
+    }
+    double dr = (double)mr, dh = (double)mh;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        dr = fmax(dr, __shfl_xor(dr, off, 64));
+        dh = fmax(dh, __shfl_xor(dh, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(c.max + 2 * lev, (unsigned long long)__double_as_longlong(dr));
+        atomicMax(c.max + 2 * lev + 1, (unsigned long long)__double_as_longlong(dh));
+    }
+}
+
+}  // namespace
+
+}  // namespace mhx

@@ -1,0 +1,457 @@
+// Grid stage of the split-form nonlinear step on theta-row tiles, for the
+// n_phi = 768 grid (l_max = 255): per radial level and 8 consecutive theta
+// rows, the 18 inverse FFTs of the synthesised components, the grid products
+// of src/get_nl.f90 (curl advection :262-273, Lorentz force :242-258 merged as
+// src/rIter.f90:650-656, V s :320-325, V x B :376-388, u_h . grad_h s), the
+// Courant maxima (src/courant.f90:209-275) and the 10 forward FFTs, in FP32.
+//
+// One workgroup = one level x TR = 8 theta rows, one wave per row.  The
+// Fourier coefficients are moved between HBM and the waves through LDS tiles
+// [comp][row][m], so that global accesses are 64-byte runs of 8 theta rows
+// (F[lev][comp][m][theta] on the inverse side, the theta-blocked
+// F2[lev][comp][theta/16][m][theta%16] on the forward side) instead of one
+// 8-byte access per lane.
+//
+// Two real components share one complex FFT of length N = 768 (y = x1 + i x2,
+// Y_k = X1_k + i X2_k with the Hermitian halves, and back X1_k = (Y_k + conj
+// Y_{N-k})/2, X2_k = (Y_k - conj Y_{N-k})/(2i)).  Each wave runs its FFTs
+// alone, Stockham passes in its own LDS buffer with no block barrier:
+//   inverse radices 16 (L = 1), 4 (L = 16), 12 (L = 64)
+//   forward radices 12 (L = 1), 4 (L = 12), 16 (L = 48)
+// so that after the inverse (and before the forward) FFT lane j holds the
+// grid points n = j + 64 q, q < 12, of both components: the products of a
+// point are formed in the registers of the lane that owns it.  The buffer
+// index paddings make every LDS address a per-lane base plus a constant.
+//
+// Normalisation as the reference: inverse unnormalised using the real parts
+// of mode 0 (src/fft.f90:386-398), forward scaled by 1/N (src/fft.f90:464).
+#include "mhx_fftcore.h"
+
+namespace mhx {
+
+namespace {
+
+constexpr int GT_N = 768;
+constexpr int GT_TR = 8;                 // theta rows (waves) per workgroup
+constexpr int GT_NTHR = 64 * GT_TR;
+constexpr int GT_BUF = 960;              // complex per wave FFT buffer (padded)
+constexpr int GT_MAXNM = 256;            // n_m <= N / 3
+constexpr int GT_ROWP = GT_MAXNM + 4;    // tile row pitch (complex)
+constexpr int GT_SLOT = 2 * GT_TR * GT_ROWP;  // complex per tile slot (2 comps)
+constexpr int GT_MT = GT_MAXNM * GT_TR / GT_NTHR;  // tile loads per thread and comp
+// LDS twiddle tables (complex, lane-contiguous): inverse pass 2 [3][16],
+// inverse pass 3 [11][64], forward pass 2 [3][12], forward pass 3 [15][48]
+constexpr int TW_I2 = 0, TW_I3 = TW_I2 + 3 * 16, TW_F2 = TW_I3 + 11 * 64, TW_F3 = TW_F2 + 3 * 12;
+constexpr int TW_N = TW_F3 + 15 * 48;
+constexpr size_t GT_LDS = (size_t)(TW_N + 2 * GT_SLOT + GT_TR * GT_BUF) * sizeof(float2);
+
+typedef float2 C;
+
+struct GtArgs {
+    const float *Fin;     // [lev][18][m][theta] complex
+    float *Fout;          // [lev][10][theta/16][m][theta%16] complex
+    float *dgrid;         // [lev][3][theta][phi] du/dr grids
+    float *full_grids;    // optional [lev][18][phi][theta]
+    const float *twn;     // exp(-2 pi i k / N), k < N
+    const double *osin2;  // per theta row
+    const double *rad;    // or2[nlev], or4[nlev], orho1[nlev]
+    double LFfac;
+    int n_m, n_theta, nlev;
+    CourArgs cour;
+};
+
+// inverse FFT of the components in tile rows r1, r2 (row pointers into the
+// tile, n_m coefficients each) into y[q] = x1[n] + i x2[n], n = j + 64 q
+__device__ __forceinline__ void gt_inverse(const C *__restrict__ r1, const C *__restrict__ r2, int n_m,
+                                           C *buf, const C *tw, int j, C (&y)[12])
+{
+    // pass 1 (radix 16, L = 1, 48 butterflies): Y_k, k = j + 48 s
+    if (j < 48) {
+        C x[16];
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            x[s] = mk<C>(0.f, 0.f);
+            if (s <= 5) {
+                const int k = j + 48 * s;
+                if (k < n_m) {
+                    C a = r1[k], b = r2[k];
+                    if (k == 0) a.y = b.y = 0.f;
+                    x[s] = mk<C>(a.x - b.y, a.y + b.x);          // a + i b
+                }
+            } else if (s >= 10) {
+                const int kk = GT_N - j - 48 * s;               // N - k
+                if (kk < n_m) {
+                    const C a = r1[kk], b = r2[kk];
+                    x[s] = mk<C>(a.x + b.y, b.x - a.y);          // conj a + i conj b
+                }
+            }
+        }
+        Dft<16, C>::run(x, +1);
+        // out index 16 j + s, padded i + (i >> 4) = 17 j + s
+        C *o = buf + 17 * j;
+#pragma unroll
+        for (int s = 0; s < 16; ++s) o[s] = x[s];
+    }
+    wave_order();
+    // pass 2 (radix 4, L = 16): butterflies b = j + 64 i, k = j mod 16
+    {
+        const int k = j & 15;
+        C x[3][4];
+        const C *ib = buf + (j + (j >> 4));                   // pad_ib(b) = pad_ib(j) + 68 i
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                C v = ib[68 * i + 204 * s];
+                if (s > 0) v = cmul(v, tw[TW_I2 + (s - 1) * 16 + k]);
+                x[i][s] = v;
+            }
+        wave_order();
+        // out (b - k) 4 + k + 16 s, padded i + ((i >> 5) << 3):
+        // 80 (j >> 4) + k + 320 i + 16 s + 8 (s >> 1)
+        C *ob = buf + (80 * (j >> 4) + k);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            Dft<4, C>::run(x[i], +1);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) ob[320 * i + 16 * s + 8 * (s >> 1)] = x[i][s];
+        }
+    }
+    wave_order();
+    // pass 3 (radix 12, L = 64): in j + 64 s, padded (j + 8 (j >> 5)) + 80 s
+    {
+        const C *ib = buf + (j + 8 * (j >> 5));
+#pragma unroll
+        for (int s = 0; s < 12; ++s) {
+            C v = ib[80 * s];
+            if (s > 0) v = cmul(v, tw[TW_I3 + (s - 1) * 64 + j]);
+            y[s] = v;
+        }
+        Dft<12, C>::run(y, +1);
+    }
+    wave_order();
+}
+
+// forward FFT of y[q] = p1[n] + i p2[n], n = j + 64 q, into tile rows o1, o2
+// (n_m coefficients each, scaled by 1/N)
+__device__ __forceinline__ void gt_forward(C (&y)[12], C *__restrict__ o1, C *__restrict__ o2, int n_m,
+                                           C *buf, const C *tw, int j)
+{
+    // pass 1 (radix 12, L = 1): out 12 j + s, padded i + (i >> 2) = 15 j + s + (s >> 2)
+    Dft<12, C>::run(y, -1);
+    {
+        C *o = buf + 15 * j;
+#pragma unroll
+        for (int s = 0; s < 12; ++s) o[s + (s >> 2)] = y[s];
+    }
+    wave_order();
+    // pass 2 (radix 4, L = 12): butterflies b = j + 64 i, k = b mod 12;
+    // in padded (j + (j >> 2)) + 80 i + 240 s; out 48 (b / 12) + k + 12 s
+    {
+        C x[3][4];
+        int kk[3];
+        const C *ib = buf + (j + (j >> 2));
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int b = j + 64 * i;
+            kk[i] = b % 12;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                C v = ib[80 * i + 240 * s];
+                if (s > 0) v = cmul(v, tw[TW_F2 + (s - 1) * 12 + kk[i]]);
+                x[i][s] = v;
+            }
+        }
+        wave_order();
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            Dft<4, C>::run(x[i], -1);
+            const int b = j + 64 * i;
+            C *ob = buf + (b - kk[i]) * 4 + kk[i];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) ob[12 * s] = x[i][s];
+        }
+    }
+    wave_order();
+    // pass 3 (radix 16, L = 48, 48 butterflies): in j + 48 s, out j + 48 s
+    if (j < 48) {
+        C x[16];
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            C v = buf[j + 48 * s];
+            if (s > 0) v = cmul(v, tw[TW_F3 + (s - 1) * 48 + j]);
+            x[s] = v;
+        }
+        wave_order();
+        Dft<16, C>::run(x, -1);
+#pragma unroll
+        for (int s = 0; s < 16; ++s) buf[j + 48 * s] = x[s];
+    } else {
+        wave_order();
+    }
+    wave_order();
+    // X1_k = (Y_k + conj Y_{N-k}) / 2N, X2_k = (Y_k - conj Y_{N-k}) / 2iN
+    const float h = 0.5f / (float)GT_N;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int k = j + 64 * q;
+        if (k < n_m) {
+            const C a = buf[k];
+            const C c = buf[k == 0 ? 0 : GT_N - k];
+            o1[k] = mk<C>(h * (a.x + c.x), h * (a.y - c.y));
+            o2[k] = mk<C>(h * (a.y + c.y), h * (c.x - a.x));
+        }
+    }
+    wave_order();
+}
+
+__global__ __launch_bounds__(GT_NTHR) void k_grid_tile(GtArgs a)
+{
+    extern __shared__ char smem_raw[];
+    C *s_tw = reinterpret_cast<C *>(smem_raw);
+    C *s_slot = s_tw + TW_N;                           // 2 slots of [2][TR][ROWP]
+    C *s_buf = s_slot + 2 * GT_SLOT;                   // per-wave FFT buffers
+    const int tid = threadIdx.x, wave = tid >> 6, j = tid & 63;
+    const int n_m = a.n_m, n_theta = a.n_theta, nlev = a.nlev;
+    const int ntile = n_theta / GT_TR;
+    // XCD-aware order: consecutive tiles (which share the 128-byte lines of
+    // the Fourier arrays) run on the same XCD at the same time
+    int lb;
+    {
+        const int nx = 8, nb = gridDim.x, b = blockIdx.x;
+        const int per = nb / nx, rem = nb % nx, x = b % nx, i = b / nx;
+        lb = (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + i;
+    }
+    const int tile = lb % ntile, lev = lb / ntile;
+    const int th0 = tile * GT_TR, th = th0 + wave;
+    C *buf = s_buf + wave * GT_BUF;
+
+    // twiddle tables from exp(-2 pi i k / N)
+    {
+        const C *tn = reinterpret_cast<const C *>(a.twn);
+        for (int e = tid; e < TW_N; e += GT_NTHR) {
+            int idx, conj;
+            if (e < TW_I3) {
+                const int s = e / 16 + 1, k = e % 16;
+                idx = 12 * s * k, conj = 1;
+            } else if (e < TW_F2) {
+                const int s = (e - TW_I3) / 64 + 1, k = (e - TW_I3) % 64;
+                idx = s * k, conj = 1;
+            } else if (e < TW_F3) {
+                const int s = (e - TW_F2) / 12 + 1, k = (e - TW_F2) % 12;
+                idx = 16 * s * k, conj = 0;
+            } else {
+                const int s = (e - TW_F3) / 48 + 1, k = (e - TW_F3) % 48;
+                idx = s * k, conj = 0;
+            }
+            C v = tn[idx];
+            if (conj) v.y = -v.y;
+            s_tw[e] = v;
+        }
+    }
+
+    const float o2 = (float)a.osin2[th];
+    const float or2 = (float)a.rad[lev], or4 = (float)a.rad[nlev + lev];
+    const float orho1 = (float)a.rad[2 * nlev + lev], LFfac = (float)a.LFfac;
+
+    // cooperative tile moves: thread -> (row r, m = mm + 64 t)
+    const int tr = tid & (GT_TR - 1), tmm = tid >> 3;
+    const C *Fi = reinterpret_cast<const C *>(a.Fin) + (size_t)lev * 18 * n_m * n_theta + th0 + tr;
+    C pre[2][GT_MT];
+    auto load_pair = [&](int c0, int c1) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const C *src = Fi + (size_t)(c ? c1 : c0) * n_m * n_theta;
+#pragma unroll
+            for (int t = 0; t < GT_MT; ++t) {
+                const int m = tmm + 64 * t;
+                pre[c][t] = m < n_m ? src[(size_t)m * n_theta] : mk<C>(0.f, 0.f);
+            }
+        }
+    };
+    auto store_pair = [&](C *slot) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int t = 0; t < GT_MT; ++t) {
+                const int m = tmm + 64 * t;
+                if (m < n_m) slot[(c * GT_TR + tr) * GT_ROWP + m] = pre[c][t];
+            }
+    };
+    const int ntb = n_theta / 16;
+    C *Fo = reinterpret_cast<C *>(a.Fout) + (size_t)lev * 10 * ntb * 16 * n_m;
+    auto flush_pair = [&](const C *slot, int c0, int c1) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            C *dst = Fo + (((size_t)(c ? c1 : c0) * ntb + (th0 >> 4)) * n_m) * 16 + (th0 & 15) + tr;
+#pragma unroll
+            for (int t = 0; t < GT_MT; ++t) {
+                const int m = tmm + 64 * t;
+                if (m < n_m) dst[(size_t)m * 16] = slot[(c * GT_TR + tr) * GT_ROWP + m];
+            }
+        }
+    };
+    auto row = [&](C *slot, int c) { return slot + (c * GT_TR + wave) * GT_ROWP; };
+    float *fg = a.full_grids;
+    auto grids_out = [&](const C (&y)[12], int c0, int c1) {
+        if (!fg) return;
+#pragma unroll
+        for (int q = 0; q < 12; ++q) {
+            const int n = j + 64 * q;
+            fg[(((size_t)lev * 18 + c0) * GT_N + n) * n_theta + th] = y[q].x;
+            fg[(((size_t)lev * 18 + c1) * GT_N + n) * n_theta + th] = y[q].y;
+        }
+    };
+    float *dg = a.dgrid + (size_t)lev * 3 * n_theta * GT_N;
+    auto dgrid_out = [&](int c, const C (&y)[12], bool imag) {
+        float *d = dg + ((size_t)c * n_theta + th) * GT_N + j;
+#pragma unroll
+        for (int q = 0; q < 12; ++q) d[64 * q] = imag ? y[q].y : y[q].x;
+    };
+
+    float u[3][12], acc[3][12], bb[3][12], pend[12];
+    C y[12];
+    int slot_i = 0;
+    auto slot = [&]() { return s_slot + (slot_i & 1) * GT_SLOT; };
+
+    // one inverse pair: comps (c0, c1) already in pre; the next pair's loads
+    // are issued before the FFT
+    auto inverse = [&](int c0, int c1, int n0, int n1) {
+        C *sl = slot();
+        store_pair(sl);
+        __syncthreads();
+        if (n0 >= 0) load_pair(n0, n1);
+        gt_inverse(row(sl, 0), row(sl, 1), n_m, buf, s_tw, j, y);
+        grids_out(y, c0, c1);
+        ++slot_i;
+    };
+    // one forward pair: y already holds the products
+    auto forward = [&](int c0, int c1) {
+        C *sl = slot();
+        gt_forward(y, row(sl, 0), row(sl, 1), n_m, buf, s_tw, j);
+        __syncthreads();
+        flush_pair(sl, c0, c1);
+        ++slot_i;
+    };
+
+    load_pair(0, 1);
+    __syncthreads();  // twiddle tables
+
+    // u: (vr, vt), (vp, s); V s, forward (VSr, VSt)
+    inverse(0, 1, 2, 15);
+#pragma unroll
+    for (int q = 0; q < 12; ++q) u[0][q] = y[q].x, u[1][q] = y[q].y;
+    inverse(2, 15, 16, 17);
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+        u[2][q] = y[q].x;
+        const float sv = y[q].y;
+        pend[q] = or2 * u[2][q] * sv * o2;                               // VSp / sin^2
+        y[q] = mk<C>(u[0][q] * sv, or2 * u[1][q] * sv * o2);             // VSr, VSt / sin^2
+    }
+    forward(3, 4);
+    // grad s: u_h . grad_h s; forward (VSp, uGs)
+    inverse(16, 17, 3, 4);
+#pragma unroll
+    for (int q = 0; q < 12; ++q)
+        y[q] = mk<C>(pend[q], or2 * o2 * (u[1][q] * y[q].x + u[2][q] * y[q].y));
+    forward(5, 9);
+    // curl u: Adv = -curl(u) x u (src/get_nl.f90:262-273)
+    inverse(3, 4, 5, 6);
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+        const float cvr = y[q].x, cvt = y[q].y;
+        acc[0][q] = -o2 * (cvt * u[2][q]);
+        acc[1][q] = or4 * (cvr * u[2][q]);
+        acc[2][q] = -or4 * (cvr * u[1][q] - cvt * u[0][q]);
+    }
+    inverse(5, 6, 9, 10);
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+        const float cvp = y[q].x;
+        acc[0][q] += o2 * (cvp * u[1][q]);
+        acc[1][q] -= or4 * (cvp * u[0][q]);
+    }
+    dgrid_out(0, y, true);
+    // B; V x B (src/get_nl.f90:376-388) and the Courant maxima
+    inverse(9, 10, 11, 7);
+#pragma unroll
+    for (int q = 0; q < 12; ++q) bb[0][q] = y[q].x, bb[1][q] = y[q].y;
+    inverse(11, 7, 12, 13);
+#pragma unroll
+    for (int q = 0; q < 12; ++q) bb[2][q] = y[q].x;
+    dgrid_out(1, y, true);
+    if (a.cour.max) courant_max<float, 12>(a.cour, lev, nlev, o2, or2, or4, orho1, LFfac, u, bb);
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+        const float vxr = orho1 * o2 * (u[1][q] * bb[2][q] - u[2][q] * bb[1][q]);
+        const float vxt = orho1 * or4 * (u[2][q] * bb[0][q] - u[0][q] * bb[2][q]);
+        pend[q] = orho1 * or4 * (u[0][q] * bb[1][q] - u[1][q] * bb[0][q]) * o2;   // VxBp / sin^2
+        y[q] = mk<C>(vxr, vxt * o2);
+    }
+    forward(6, 7);
+    // curl B: Lorentz force added to Adv (src/get_nl.f90:242-258)
+    inverse(12, 13, 14, 8);
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+        const float cbr = y[q].x, cbt = y[q].y;
+        acc[0][q] += LFfac * o2 * (cbt * bb[2][q]);
+        acc[1][q] -= LFfac * or4 * (cbr * bb[2][q]);
+        acc[2][q] += LFfac * or4 * (cbr * bb[1][q] - cbt * bb[0][q]);
+    }
+    inverse(14, 8, -1, -1);
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+        const float cbp = y[q].x;
+        acc[0][q] -= LFfac * o2 * (cbp * bb[1][q]);
+        acc[1][q] += LFfac * or4 * (cbp * bb[0][q]);
+    }
+    dgrid_out(2, y, true);
+    // forward (VxBp, Advr), (Advt, Advp), the horizontal comps / sin^2
+#pragma unroll
+    for (int q = 0; q < 12; ++q) y[q] = mk<C>(pend[q], acc[0][q]);
+    forward(8, 0);
+#pragma unroll
+    for (int q = 0; q < 12; ++q) y[q] = mk<C>(acc[1][q] * o2, acc[2][q] * o2);
+    forward(1, 2);
+}
+
+}  // namespace
+
+bool grid_tile_supported(const Plan *p)
+{
+    return p->n_phi == GT_N && p->n_m <= GT_MAXNM && 3 * p->n_m <= GT_N && p->n_theta % 16 == 0 &&
+           p->d_twn32 != nullptr;
+}
+
+int launch_grid_tile(const Plan *p, int nlev, const void *Fin, void *Fout, const double *rad,
+                     double LFfac, void *dgrid, void *full_grids, hipStream_t s)
+{
+    if (nlev <= 0) return MHX_OK;
+    if (!grid_tile_supported(p)) return fail(MHX_E_UNSUPPORTED, "grid tile kernel: unsupported grid");
+    GtArgs a;
+    a.Fin = static_cast<const float *>(Fin);
+    a.Fout = static_cast<float *>(Fout);
+    a.dgrid = static_cast<float *>(dgrid);
+    a.full_grids = static_cast<float *>(full_grids);
+    a.twn = p->d_twn32;
+    a.osin2 = p->d_osin2;
+    a.rad = rad;
+    a.LFfac = LFfac;
+    a.n_m = p->n_m;
+    a.n_theta = p->n_theta;
+    a.nlev = nlev;
+    a.cour = p->cour;
+    static bool shm_set = false;
+    if (!shm_set) {
+        MHX_HIP_CHECK(hipFuncSetAttribute((const void *)k_grid_tile, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)GT_LDS));
+        shm_set = true;
+    }
+    dim3 grid((unsigned)(p->n_theta / GT_TR * nlev));
+    k_grid_tile<<<grid, GT_NTHR, GT_LDS, s>>>(a);
+    MHX_HIP_CHECK(hipGetLastError());
+    return MHX_OK;
+}
+
+}  // namespace mhx

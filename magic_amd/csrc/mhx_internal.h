@@ -238,6 +238,8 @@ struct Plan {
     double *d_tw = nullptr;         // M complex: exp(-2 pi i k/M)
     double *d_twr = nullptr;        // (M+1) complex: exp(-2 pi i k/n_phi)
     float *d_tw32 = nullptr, *d_twr32 = nullptr;  // the same in FP32
+    float *d_twn32 = nullptr;       // n_phi complex: exp(-2 pi i k/n_phi), FP32
+    int grid_form = 0;              // grid stage of the split step: 0 tiles where supported, 1 slots
 
     // split-precision tables (MHX_PREC_SPLIT16): FP16 hi/lo pairs in the
     // 16x16x32 MFMA operand layout, with power-of-2 row factors
@@ -361,6 +363,10 @@ int launch_axi(const Plan *p, const double *fl, double *f, int lcut, int tor,
                double *f2, hipStream_t s);
 void rl_free(Plan *p);
 int launch_rl_grid(const Plan *p, const RlGridArgs &a, hipStream_t s);
+// the split step's grid stage on theta-row tiles (mhx_grid.hip): n_phi = 768
+bool grid_tile_supported(const Plan *p);
+int launch_grid_tile(const Plan *p, int nlev, const void *Fin, void *Fout, const double *rad,
+                     double LFfac, void *dgrid, void *full_grids, hipStream_t s);
 
 }  // namespace mhx
 

@@ -416,12 +416,22 @@ int mhx_plan_create(int l_max, int m_max, int minc, int n_theta, int n_phi, int 
         return MHX_OK;
     };
     std::vector<float> tw32(tw.begin(), tw.end()), twr32(twr.begin(), twr.end());
+    std::vector<float> twn32(2 * (size_t)n_phi);
+    for (int k = 0; k < n_phi; ++k) {
+        twn32[2 * k] = (float)std::cos(2.0 * pi * k / n_phi);
+        twn32[2 * k + 1] = (float)-std::sin(2.0 * pi * k / n_phi);
+    }
+    {
+        const char *gf = std::getenv("MHX_GRID_FORM");
+        p->grid_form = (gf && std::strcmp(gf, "slots") == 0) ? 1 : 0;
+    }
     int rc;
     int *d_lm2l = nullptr;
     if ((rc = up(p->d_lstart, p->lstart)) || (rc = up(p->d_theta, p->theta_ord)) ||
         (rc = up(p->d_gauss, p->gauss)) || (rc = up(p->d_osin2, osin2)) ||
         (rc = up(p->d_tw, tw)) || (rc = up(p->d_twr, twr)) || (rc = up(d_lm2l, lm2l)) ||
-        (rc = up(p->d_tw32, tw32)) || (rc = up(p->d_twr32, twr32)) || (rc = up(p->d_lm2m, lm2m))) {
+        (rc = up(p->d_tw32, tw32)) || (rc = up(p->d_twr32, twr32)) || (rc = up(p->d_lm2m, lm2m)) ||
+        (rc = up(p->d_twn32, twn32))) {
         mhx_plan_destroy(h);
         return rc;
     }
@@ -479,6 +489,7 @@ int mhx_plan_destroy(mhx_plan *h)
     free_dev(p->d_lm2m);
     free_dev(p->d_tw32);
     free_dev(p->d_twr32);
+    free_dev(p->d_twn32);
     free_dev(p->d_syn16_P);
     free_dev(p->d_syn16_dP);
     free_dev(p->d_ana16_P);

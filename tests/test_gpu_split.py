@@ -172,3 +172,54 @@ def test_split_graph_replay_matches_eager():
     for k in range(10):
         assert np.array_equal(got[k], want[k]), sht.NL_OUTPUTS[k]
         assert not np.array_equal(got[k], ref[k]), sht.NL_OUTPUTS[k]
+
+
+def test_grid_tile_form_matches_slot_form():
+    """The split step's two grid-stage kernels on the l_max=255 grid: the
+    theta-row tile kernel (the default for n_phi = 768) against the one-row
+    slot kernel (MHX_GRID_FORM=slots at plan creation), same inputs: the ten
+    outputs, the 18 synthesised grids and the Courant maxima agree to FP32
+    rounding (both forms are held to 1e-6 of FP64 at the full size by
+    test_split_full_config_vs_fp64, which runs the tile form)."""
+    import os
+    import torch
+    L, nlev = 255, 6
+    nt, nph = workload.grid_size(L)
+    inputs = make_inputs(L, nlev)
+    rad = radial(nlev)
+    lcut = [L] * nlev
+    lcut[2] = L - 5
+    dev = torch.device("cuda")
+    res = {}
+    for form in ("slots", "tile"):
+        if form == "slots":
+            os.environ["MHX_GRID_FORM"] = "slots"
+        try:
+            p = sht.Plan(L, nt, nph, precision=sht.PREC_SPLIT16, max_levels=nlev)
+        finally:
+            os.environ.pop("MHX_GRID_FORM", None)
+        print("grid form %s: l_max=%d x %d levels" % (form, L, nlev), flush=True)
+        tin = [torch.from_numpy(x).to(dev) for x in inputs]
+        tout = [torch.zeros((nlev, p.lm_max), dtype=torch.complex128, device=dev) for _ in range(10)]
+        grids = torch.zeros((nlev, 18, nph, nt), dtype=torch.float32, device=dev)
+        sht.nl_step(p, tin, tout, rad[0], rad[1], rad[2], rad[3], lcut, grids)
+        torch.cuda.synchronize()
+        outs = [t.cpu().numpy() for t in tout]
+        g = grids.cpu().numpy()
+        vr2, vh2 = sht.nl_step_courant(p, tin, tout, rad[0], rad[1], rad[2], rad[3], lcut, 2.5, 1.0,
+                                       [0.01] * nlev, [0.02] * nlev)
+        p.close()
+        res[form] = (outs, g, vr2, vh2)
+    (o1, g1, r1, h1), (o2, g2, r2, h2) = res["slots"], res["tile"]
+    errs = [rel_l2(o2[k], o1[k]) for k in range(10)]
+    gerr = [rel_l2(g2[:, c], g1[:, c]) for c in range(18)]
+    print("tile vs slots outputs:", " ".join("%.1e" % e for e in errs), flush=True)
+    print("tile vs slots grids:", " ".join("%.1e" % e for e in gerr), flush=True)
+    # two FP32 FFT orderings (radices 12.4.4.2 against 16.4.12, two real
+    # rows per complex FFT) each about 1.5e-7 from the FP64 result
+    for k, e in enumerate(errs):
+        assert e < 5e-7, (sht.NL_OUTPUTS[k], e)
+    for c, e in enumerate(gerr):
+        assert e < 5e-7, (c, e)
+    np.testing.assert_allclose(r2, r1, rtol=1e-5)
+    np.testing.assert_allclose(h2, h1, rtol=1e-5)
