@@ -45,8 +45,11 @@ LAUNCHES = [
 ]
 # kernels behind each role (prefix of the rocprofv3 kernel name)
 KERNELS = {
-    "split16": {"syn_sg": "k_nsyn16p<0>", "syn_qst": "k_nsyn16p<", "grid": "k_nl_slots<float",
+    "split16": {"syn_sg": "k_nsyn16p<0>", "syn_qst": "k_nsyn16p<", "grid": "k_grid_tile",
                 "ana_qst": "k_nana16p<0>", "ana_sh": "k_nana16p<1>"},
+    # l_max < 255 (n_phi != 768): the one-row slot grid stage
+    "split16_slots": {"syn_sg": "k_nsyn16p<0>", "syn_qst": "k_nsyn16p<", "grid": "k_nl_slots<float",
+                      "ana_qst": "k_nana16p<0>", "ana_sh": "k_nana16p<1>"},
     # l_max > 255: the chunked split Legendre kernels
     "split16_chunked": {"syn_sg": "k_nsyn16<0>", "syn_qst": "k_nsyn16<", "grid": "k_nl_slots<float",
                         "ana_qst": "k_nana16<0>", "ana_sh": "k_nana16<1>"},
@@ -503,7 +506,12 @@ def main():
         log("split16: %.3f ms/step, %.1f pairs/s" % (forms["split16"]["ms_per_step"],
                                                      forms["split16"]["pairs"]))
     main_key = "split16" if "split16" in forms else "fp64"
-    kern_key = "split16_chunked" if main_key == "split16" and L > 255 else main_key
+    kern_key = main_key
+    if main_key == "split16":
+        # the theta-row tile grid stage runs on the n_phi = 768 grid unless
+        # MHX_GRID_FORM=slots selects the one-row kernel
+        tile = nph == 768 and os.environ.get("MHX_GRID_FORM") != "slots"
+        kern_key = "split16_chunked" if L > 255 else ("split16" if tile else "split16_slots")
     main = forms[main_key]
     errs = None
     if "fp64" in forms and "split16" in forms:
