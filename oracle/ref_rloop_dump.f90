@@ -1,8 +1,8 @@
 !
-! Test infrastructure only: a stand-in for the reference's module radialLoop
-! (src/radialLoop.f90) with the same public interface, linked into the
-! reference's own program (oracle/_ref/magic_dump).  It runs the reference's
-! radial loop (rIter_single_t of src/rIter.f90) unchanged and, on the call
+! Test infrastructure only: an instrumented module radialLoop with the
+! public interface of src/radialLoop.f90, linked into the reference's own
+! program (oracle/_ref/magic_dump) in place of that file.  It runs the
+! reference's radial loop (rIter_single_t of src/rIter.f90) unchanged and, on the call
 ! number given by the environment variable MHX_DUMP_CALL, writes that call's
 ! inputs (the R-distributed fields, the radial profiles, the physics switches
 ! and parameters the loop reads) and outputs (the explicit terms, the Courant
