@@ -741,9 +741,10 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
 #pragma unroll
                 for (int tt = 0; tt < WTT; ++tt) {
                     const int tile = min(tile0 + tt, a.nt16 - 1);
-                    const size_t to = (PAR ? toff1 : toff0) + (((size_t)ks * a.nt16 + tile) * 64 + lane) * 16;
+                    // the fragment's 64 hi parts, then its 64 lo parts
+                    const size_t to = (PAR ? toff1 : toff0) + (((size_t)ks * a.nt16 + tile) * 128 + lane) * 8;
                     fr[S % 4].h[tt] = *reinterpret_cast<const h8 *>(a.tP + to);
-                    fr[S % 4].l[tt] = *reinterpret_cast<const h8 *>(a.tP + to + 8);
+                    fr[S % 4].l[tt] = *reinterpret_cast<const h8 *>(a.tP + to + 512);
                 }
             }
         };
@@ -1408,9 +1409,10 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
 #pragma unroll
             for (int tt = 0; tt < PTT; ++tt) {
                 const int tile = min(wave + 4 * tt, ntl - 1);
-                const size_t to = (PAR ? toff1 : toff0) + (((size_t)tile * a.nq32 + kc) * 64 + lane) * 16;
+                // the fragment's 64 hi parts, then its 64 lo parts
+                const size_t to = (PAR ? toff1 : toff0) + (((size_t)tile * a.nq32 + kc) * 128 + lane) * 8;
                 fr[H % 4].h[tt] = *reinterpret_cast<const h8 *>(a.tP + to);
-                fr[H % 4].l[tt] = *reinterpret_cast<const h8 *>(a.tP + to + 8);
+                fr[H % 4].l[tt] = *reinterpret_cast<const h8 *>(a.tP + to + 512);
             }
         }
     };

@@ -192,11 +192,12 @@ __global__ void k_split_synx(int l_max, int minc, int n_m, int nh, int nt16,
         int par = (l - m) & 1, kk = (l - m) >> 1;
         int kc = kk >> 5, kin = kk & 31;
         int lane = (th & 15) + 16 * (kin >> 3);
-        size_t o16 = offx[2 * mc + par] + (((size_t)kc * nt16 + (th >> 4)) * 64 + lane) * 16 + (kin & 7);
+        // per 64-lane fragment: the 64 lanes' hi halves, then their lo halves
+        size_t o16 = offx[2 * mc + par] + (((size_t)kc * nt16 + (th >> 4)) * 128 + lane) * 8 + (kin & 7);
         _Float16 h, lo;
         split_h((float)(syn_p(sP, off, mc, m, l, l_max, th, nt16, ct) * sc), h, lo);
         hP[o16] = h;
-        hP[o16 + 8] = lo;
+        hP[o16 + 512] = lo;
     }
 }
 
@@ -241,11 +242,12 @@ __global__ void k_split_anax(int l_max, int minc, int n_m, int nh, int nq4, int 
         double s = (double)rs[soffx[2 * mc + par] + kk];
         int kc = th >> 5, kin = th & 31;
         int lane = (kk & 15) + 16 * (kin >> 3);
-        size_t o16 = offx[2 * mc + par] + (((size_t)(kk >> 4) * nq32 + kc) * 64 + lane) * 16 + (kin & 7);
+        // per 64-lane fragment: the 64 lanes' hi halves, then their lo halves
+        size_t o16 = offx[2 * mc + par] + (((size_t)(kk >> 4) * nq32 + kc) * 128 + lane) * 8 + (kin & 7);
         _Float16 h, lo;
         split_h((float)(ana_p(aP, off, mc, m, l, l_max, th, nq4, ct) * s), h, lo);
         hP[o16] = h;
-        hP[o16 + 8] = lo;
+        hP[o16 + 512] = lo;
     }
 }
 
