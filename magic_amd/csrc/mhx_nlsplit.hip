@@ -509,7 +509,8 @@ template <int N, class F> __device__ __forceinline__ void static_for(F &&f)
 constexpr int PKP = 160;                 // k per parity held (l_max + 1 - m)/2 + 1 <= 160
 constexpr int PROW = 4 * PKP + 16;       // bytes per LDS row (padded)
 
-template <int V>
+// NRL: 128-wide rounds of l that can hold l <= l_max (1 for l_max < 128, else 2)
+template <int V, int NRL>
 __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
 {
     constexpr bool SCAL = V == NS_SG || V == NS_S;
@@ -553,46 +554,66 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
                 cv[it][q] = d <= a.l_max + 2 - m ? cr[d] : 0.f;
             }
     }
-    // ---- 1. Q, S, T (SG: s) in FP32 registers, l = m + 4 (g0 + 32 it) + j
+    // ---- 1. Q, S, T (SG: s) in FP32 registers, l = m + 4 (g0 + 32 it) + j.
+    // The sources are read lane-consecutive (thread g0 of a level reads
+    // l = m + 32 r + g0, r < 4 NRL: each four-lane group one 64-byte run),
+    // the pointwise FP64 column values rounded to FP32 and moved through the
+    // B-row area of LDS (free until step 3) to the layout of steps 2 and 3.
+    // Rounds it >= NRL hold only l > l_max (zero sources): they are not read.
     float2 cq[3][4], cs[3][4], ct[3][4];  // column 0 (Q or s), S (or s), T
     {
+        constexpr int NSC = SCAL ? 1 : 3;      // staged columns
+        constexpr int SRP = 128 * NRL + 4;     // staged row pitch (float2)
+        static_assert((size_t)WLG * NSC * SRP * sizeof(float2) <= (size_t)SB,
+                      "staged columns exceed the B-row area");
+        float2 *srow = reinterpret_cast<float2 *>(sb) + (size_t)levl * NSC * SRP;
         const double o2 = (V == NS_QSTC && lok) ? a.or2[lev] : 0.0;
-        // all loads of the three rounds first, unconditional at a clamped l
-        // (a load under a branch waits for itself before the next one);
-        // values past lhi are zeroed after
-        double2 xa[3][4][NSRC];
+        // all loads first, unconditional at a clamped l (a load under a branch
+        // waits for itself before the next one); values past lhi are zeroed
+        double2 xa[4 * NRL][NSRC];
 #pragma unroll
-        for (int it = 0; it < 3; ++it)
+        for (int r = 0; r < 4 * NRL; ++r) {
+            const int l = m + 32 * r + g0;
+            const int lc = min(l, max(lhi, m));
+            const size_t o = ((size_t)(lok ? lev : lev0) * a.src_ld + (size_t)(l0 + lc - m)) * 2;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int l = m + 4 * (g0 + 32 * it) + j;
-                const int lc = min(l, max(lhi, m));
-                const size_t o = ((size_t)(lok ? lev : lev0) * a.src_ld + (size_t)(l0 + lc - m)) * 2;
-#pragma unroll
-                for (int s2 = 0; s2 < NSRC; ++s2) xa[it][j][s2] = *reinterpret_cast<const double2 *>(a.src[s2] + o);
-            }
+            for (int s2 = 0; s2 < NSRC; ++s2) xa[r][s2] = *reinterpret_cast<const double2 *>(a.src[s2] + o);
+        }
         __builtin_amdgcn_sched_barrier(0);
+        const float2 z = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int r = 0; r < 4 * NRL; ++r) {
+            const int l = m + 32 * r + g0, li = 32 * r + g0;
+            const bool in = l <= lhi;
+            const double2 *x = xa[r];
+            const double ll = (double)l * (double)(l + 1);
+            if constexpr (SCAL) {
+                srow[li] = in ? make_float2((float)x[0].x, (float)x[0].y) : z;
+            } else {
+                srow[li] = in ? make_float2((float)(ll * x[0].x), (float)(ll * x[0].y)) : z;
+                srow[SRP + li] = in ? make_float2((float)x[1].x, (float)x[1].y) : z;
+                if constexpr (V == NS_QSTC)
+                    srow[2 * SRP + li] = in ? make_float2((float)(o2 * ll * x[2].x - x[3].x),
+                                                          (float)(o2 * ll * x[2].y - x[3].y))
+                                            : z;
+                else
+                    srow[2 * SRP + li] = in ? make_float2((float)x[2].x, (float)x[2].y) : z;
+            }
+        }
+        __syncthreads();
 #pragma unroll
         for (int it = 0; it < 3; ++it)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int l = m + 4 * (g0 + 32 * it) + j;
-                const bool in = l <= lhi;
-                const double2 *x = xa[it][j];
-                const double ll = (double)l * (double)(l + 1);
-                if constexpr (SCAL) {
-                    const float2 v = in ? make_float2((float)x[0].x, (float)x[0].y) : make_float2(0.f, 0.f);
-                    cq[it][j] = v;
-                    cs[it][j] = v;
+                const int li = 4 * (g0 + 32 * it) + j;
+                if (it < NRL) {
+                    cq[it][j] = srow[li];
+                    cs[it][j] = SCAL ? cq[it][j] : srow[SRP + li];
+                    if constexpr (!SCAL) ct[it][j] = srow[2 * SRP + li];
                 } else {
-                    cq[it][j] = in ? make_float2((float)(ll * x[0].x), (float)(ll * x[0].y)) : make_float2(0.f, 0.f);
-                    cs[it][j] = in ? make_float2((float)x[1].x, (float)x[1].y) : make_float2(0.f, 0.f);
-                    if constexpr (V == NS_QSTC)
-                        ct[it][j] = in ? make_float2((float)(o2 * ll * x[2].x - x[3].x),
-                                                     (float)(o2 * ll * x[2].y - x[3].y))
-                                       : make_float2(0.f, 0.f);
-                    else
-                        ct[it][j] = in ? make_float2((float)x[2].x, (float)x[2].y) : make_float2(0.f, 0.f);
+                    cq[it][j] = z;
+                    cs[it][j] = z;
+                    ct[it][j] = z;
                 }
             }
     }
@@ -1532,7 +1553,7 @@ int launch_nsyn16(const Plan *p, int kind, const NSynArgs &args, hipStream_t s)
     a.nh = p->nh;
     a.n_theta = p->n_theta;
     a.nt16 = p->nt16;
-    if (p->syn_form != 2 && (p->l_max + 1) / 2 + 1 <= PKP && p->n_theta <= WNT) {
+    if (p->syn_form != 2 && p->l_max <= 255 && p->n_theta <= WNT) {
         // whole-K form on P only: 12 theta tiles per block
         a.tP = p->d_syn16x;
         a.off = p->d_syn16x_off;
@@ -1541,11 +1562,24 @@ int launch_nsyn16(const Plan *p, int kind, const NSynArgs &args, hipStream_t s)
         a.ngz = (p->nt16 + a.tpb - 1) / a.tpb;
         const int ng = (a.nlev + WLG - 1) / WLG;
         dim3 grid((unsigned)(8 * ((p->n_m + 7) / 8) * ng * a.ngz));
+        const bool one = p->l_max + 1 <= 128;
         switch (kind) {
-        case NS_SG: k_nsyn16p<NS_SG><<<grid, 256, 0, s>>>(a); break;
-        case NS_QST: k_nsyn16p<NS_QST><<<grid, 256, 0, s>>>(a); break;
-        case NS_QSTC: k_nsyn16p<NS_QSTC><<<grid, 256, 0, s>>>(a); break;
-        case NS_S: k_nsyn16p<NS_S><<<grid, 256, 0, s>>>(a); break;
+        case NS_SG:
+            if (one) k_nsyn16p<NS_SG, 1><<<grid, 256, 0, s>>>(a);
+            else k_nsyn16p<NS_SG, 2><<<grid, 256, 0, s>>>(a);
+            break;
+        case NS_QST:
+            if (one) k_nsyn16p<NS_QST, 1><<<grid, 256, 0, s>>>(a);
+            else k_nsyn16p<NS_QST, 2><<<grid, 256, 0, s>>>(a);
+            break;
+        case NS_QSTC:
+            if (one) k_nsyn16p<NS_QSTC, 1><<<grid, 256, 0, s>>>(a);
+            else k_nsyn16p<NS_QSTC, 2><<<grid, 256, 0, s>>>(a);
+            break;
+        case NS_S:
+            if (one) k_nsyn16p<NS_S, 1><<<grid, 256, 0, s>>>(a);
+            else k_nsyn16p<NS_S, 2><<<grid, 256, 0, s>>>(a);
+            break;
         default: return fail(MHX_E_INVALID, "unknown nonlinear-step synthesis kind");
         }
         MHX_HIP_CHECK(hipGetLastError());
