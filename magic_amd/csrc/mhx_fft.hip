@@ -1256,6 +1256,7 @@ int launch_rl_grid(const Plan *p, const RlGridArgs &args, hipStream_t s)
 int launch_axi(const Plan *p, const double *fl, double *f, int lcut, int tor, double *f2,
                hipStream_t s)
 {
+    if (int rc = ensure_fp64_tables(const_cast<Plan *>(p))) return rc;
     k_axi<<<(p->nh + 127) / 128, 128, 0, s>>>(p->l_max, p->nh, p->d_syn_P, p->d_syn_dP,
                                                p->d_syn_off, p->nt16, fl, lcut, tor, f, f2);
     MHX_HIP_CHECK(hipGetLastError());

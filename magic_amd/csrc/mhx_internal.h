@@ -14,6 +14,18 @@ namespace mhx {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// row factor of the split tables: the power of two s with max * s in
+// [2^14, 2^15); 1 for max == 0; capped at 2^115 so that rows of tiny values
+// (P_lm near the poles) stay finite
+__device__ inline float pow2_scale(float mx)
+{
+    if (!(mx > 0.f)) return 1.f;
+    int e;
+    frexpf(mx, &e);          // mx = f 2^e, f in [0.5, 1)
+    if (e < -100) e = -100;
+    return ldexpf(1.f, 15 - e);
+}
+
 // ---------------------------------------------------------------- errors
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
@@ -363,6 +375,14 @@ int launch_axi(const Plan *p, const double *fl, double *f, int lcut, int tor,
                double *f2, hipStream_t s);
 void rl_free(Plan *p);
 int launch_rl_grid(const Plan *p, const RlGridArgs &a, hipStream_t s);
+// FP64 tables (mhx_tables.hip): table_layout fills the offsets, build_tables
+// also allocates and computes the tables; ensure_fp64_tables builds them on
+// first use where plan creation left them out (recurrence-table plans);
+// split_rowscales_rec computes the split tables' row factors from the
+// recurrence, without FP64 tables
+int table_layout(Plan *p);
+int ensure_fp64_tables(Plan *p);
+int split_rowscales_rec(Plan *p, float *syn_rs, float *ana_rs, const size_t *ana_soff);
 // the split step's grid stage on theta-row tiles (mhx_grid.hip): n_phi = 768
 bool grid_tile_supported(const Plan *p);
 int launch_grid_tile(const Plan *p, int nlev, const void *Fin, void *Fout, const double *rad,

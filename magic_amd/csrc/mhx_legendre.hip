@@ -477,6 +477,7 @@ __global__ __launch_bounds__(WG) void k_ana(AnaArgs a)
 
 int launch_syn(const Plan *p, const SynJob &job, hipStream_t s)
 {
+    if (int rc = ensure_fp64_tables(const_cast<Plan *>(p))) return rc;
     if (job.nlev <= 0) return MHX_OK;
     SynArgs a;
     a.job = job;
@@ -506,6 +507,7 @@ int launch_syn(const Plan *p, const SynJob &job, hipStream_t s)
 
 int launch_ana(const Plan *p, const AnaJob &job, hipStream_t s)
 {
+    if (int rc = ensure_fp64_tables(const_cast<Plan *>(p))) return rc;
     if (job.nlev <= 0) return MHX_OK;
     AnaArgs a;
     a.job = job;

@@ -1553,7 +1553,7 @@ int launch_nsyn16(const Plan *p, int kind, const NSynArgs &args, hipStream_t s)
     a.nh = p->nh;
     a.n_theta = p->n_theta;
     a.nt16 = p->nt16;
-    if (p->syn_form != 2 && p->l_max <= 255 && p->n_theta <= WNT) {
+    if (p->syn_form != 2 && p->l_max <= 255 && p->n_theta <= WNT && p->d_syn16x) {
         // whole-K form on P only: 12 theta tiles per block
         a.tP = p->d_syn16x;
         a.off = p->d_syn16x_off;
@@ -1640,7 +1640,8 @@ int launch_nana16(const Plan *p, int kind, const NAnaArgs &args, hipStream_t s)
     a.ngz = (ntl0 + 7) / 8;
     const int ntlx = ((p->l_max + 1) / 2 + 1 + 15) / 16;  // l tiles with l <= l_max + 1
     const bool gfit = 3 * WLG * (2 * (p->l_max + 2) + 2) * 4 <= 6 * 16 * AROW;
-    if (p->ana_form != 2 && (kind == NA_QST || kind == NA_SH) && p->nh <= AKP && ntlx <= 4 * PTT && gfit) {
+    if (p->ana_form != 2 && (kind == NA_QST || kind == NA_SH) && p->nh <= AKP && ntlx <= 4 * PTT && gfit &&
+        p->d_ana16x) {
         // whole-theta form on wP only
         a.tP = p->d_ana16x;
         a.off = p->d_ana16x_off;

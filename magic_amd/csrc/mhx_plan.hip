@@ -444,7 +444,10 @@ int mhx_plan_create(int l_max, int m_max, int minc, int n_theta, int n_phi, int 
         mhx_plan_destroy(h);
         return fail(MHX_E_HIP, "hipStreamCreate failed");
     }
-    if ((rc = build_tables(p)) ||
+    // recurrence-table plans above l_max = 255 keep no FP64 tables: their
+    // split kernels generate the tables in-kernel, and the per-call entries
+    // that read FP64 tables build them on first use (ensure_fp64_tables)
+    if ((rc = (rec && l_max > 255) ? table_layout(p) : build_tables(p)) ||
         (precision == MHX_PREC_SPLIT16 && (rc = build_split_tables(p))) ||
         (rc = ensure_ws(p, std::max(1, max_levels)))) {
         mhx_plan_destroy(h);
@@ -559,6 +562,8 @@ int mhx_plan_tables(const mhx_plan *h, double *theta_ord, double *gauss, double 
     if (theta_ord) std::memcpy(theta_ord, p->theta_ord.data(), sizeof(double) * p->nh);
     if (gauss) std::memcpy(gauss, p->gauss.data(), sizeof(double) * p->nh);
     if (plm || dplm) {
+        int rc;
+        if ((rc = ensure_fp64_tables(const_cast<Plan *>(p)))) return rc;
         std::vector<double> sP(p->syn_size), sdP(p->syn_size);
         MHX_HIP_CHECK(hipMemcpy(sP.data(), p->d_syn_P, p->syn_size * 8, hipMemcpyDeviceToHost));
         MHX_HIP_CHECK(hipMemcpy(sdP.data(), p->d_syn_dP, p->syn_size * 8, hipMemcpyDeviceToHost));

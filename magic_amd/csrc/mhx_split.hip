@@ -21,17 +21,6 @@ namespace mhx {
 
 namespace {
 
-// power-of-2 factor s with max*s in [2^14, 2^15); 1 for max == 0; capped
-// at 2^115 so that rows of tiny values (P_lm near the poles) stay finite
-__device__ inline float pow2_scale(float mx)
-{
-    if (!(mx > 0.f)) return 1.f;
-    int e;
-    frexpf(mx, &e);          // mx = f 2^e, f in [0.5, 1)
-    if (e < -100) e = -100;
-    return ldexpf(1.f, 15 - e);
-}
-
 __device__ inline void split_h(float v, _Float16 &hi, _Float16 &lo)
 {
     hi = (_Float16)v;
@@ -355,6 +344,18 @@ __global__ void k_rec_ana_seed(int l_max, int minc, int n_m, int nh, const doubl
     }
 }
 
+// c_l per (m, l <= l_max + 2) for the dP identity of the split kernels
+static int crec_table(Plan *p)
+{
+    const size_t ncr = (size_t)p->lstart[p->n_m - 1] + 2 * (p->n_m - 1) +
+                       (p->l_max + 3 - (p->n_m - 1) * p->minc);
+    MHX_HIP_CHECK(hipMalloc(&p->d_crec, ncr * sizeof(float)));
+    dim3 g((p->l_max + 3 + 127) / 128, p->n_m);
+    k_crec<<<g, 128>>>(p->l_max, p->minc, p->n_m, p->d_lstart, p->d_crec);
+    MHX_HIP_CHECK(hipGetLastError());
+    return MHX_OK;
+}
+
 int build_split_tables(Plan *p)
 {
     const int nh = p->nh;
@@ -398,8 +399,16 @@ int build_split_tables(Plan *p)
     MHX_HIP_CHECK(hipMalloc(&p->d_syn16_rs, (size_t)p->n_m * nh * sizeof(float)));
     MHX_HIP_CHECK(hipMalloc(&p->d_ana16_rs, sr * sizeof(float)));
     int n = p->n_m * nh;
-    k_rowscale_syn<<<(n + 127) / 128, 128>>>(p->l_max, p->minc, p->n_m, nh, p->nt16, p->d_syn_P,
-                                             p->d_syn_dP, p->d_syn_off, p->d_syn16_rs);
+    // plans without FP64 tables (recurrence tables, l_max > 255) take their
+    // row factors from the recurrence and have no P-only tables (those serve
+    // l_max <= 255 only)
+    const bool have64 = p->d_syn_P != nullptr;
+    int rc;
+    if (have64)
+        k_rowscale_syn<<<(n + 127) / 128, 128>>>(p->l_max, p->minc, p->n_m, nh, p->nt16, p->d_syn_P,
+                                                 p->d_syn_dP, p->d_syn_off, p->d_syn16_rs);
+    else if ((rc = split_rowscales_rec(p, p->d_syn16_rs, p->d_ana16_rs, p->d_ana16_soff)))
+        return rc;
     if (p->rec) {
         // no stored synthesis tables: FP64 seeds for the in-kernel recurrence
         MHX_HIP_CHECK(hipMalloc(&p->d_rec_seed, (size_t)n * sizeof(double2)));
@@ -411,9 +420,10 @@ int build_split_tables(Plan *p)
                                               p->d_syn_dP, p->d_syn_off, p->d_syn16_off, p->d_syn16_rs,
                                               p->d_syn16_P, p->d_syn16_dP);
     }
-    k_rowscale_ana<<<(int)((sr + 127) / 128), 128>>>(p->l_max, p->minc, p->n_m, nh, p->nq4,
-                                                     p->d_ana_wP, p->d_ana_wdP, p->d_ana_off,
-                                                     p->d_ana16_soff, p->d_ana16_rs, (int)sr);
+    if (have64)
+        k_rowscale_ana<<<(int)((sr + 127) / 128), 128>>>(p->l_max, p->minc, p->n_m, nh, p->nq4,
+                                                         p->d_ana_wP, p->d_ana_wdP, p->d_ana_off,
+                                                         p->d_ana16_soff, p->d_ana16_rs, (int)sr);
     if (p->rec) {
         std::vector<size_t> segoff(p->n_m + 1);
         size_t ns = 0;
@@ -437,6 +447,12 @@ int build_split_tables(Plan *p)
     }
     MHX_HIP_CHECK(hipGetLastError());
 
+    if (!have64) {
+        crec_table(p);
+        MHX_HIP_CHECK(hipGetLastError());
+        MHX_HIP_CHECK(hipDeviceSynchronize());
+        return MHX_OK;
+    }
     // P-only tables with l <= l_max + 1
     std::vector<size_t> offx(2 * p->n_m), aoffx(2 * p->n_m), soffx(2 * p->n_m + 1);
     size_t sox = 0, aox = 0, srx = 0;
@@ -475,13 +491,7 @@ int build_split_tables(Plan *p)
     k_split_anax<<<(n + 127) / 128, 128>>>(p->l_max, p->minc, p->n_m, nh, p->nq4, p->nq32,
                                            p->d_ana_wP, p->d_ana_off, p->d_ana16x_off,
                                            p->d_ana16x_soff, p->d_theta, p->d_ana16x_rs, p->d_ana16x);
-    {
-        const size_t ncr = (size_t)p->lstart[p->n_m - 1] + 2 * (p->n_m - 1) +
-                           (p->l_max + 3 - (p->n_m - 1) * p->minc);
-        MHX_HIP_CHECK(hipMalloc(&p->d_crec, ncr * sizeof(float)));
-        dim3 g((p->l_max + 3 + 127) / 128, p->n_m);
-        k_crec<<<g, 128>>>(p->l_max, p->minc, p->n_m, p->d_lstart, p->d_crec);
-    }
+    if ((rc = crec_table(p))) return rc;
     MHX_HIP_CHECK(hipGetLastError());
     MHX_HIP_CHECK(hipDeviceSynchronize());
     return MHX_OK;

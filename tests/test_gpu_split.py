@@ -223,3 +223,35 @@ def test_grid_tile_form_matches_slot_form():
         assert e < 5e-7, (c, e)
     np.testing.assert_allclose(r2, r1, rtol=1e-5)
     np.testing.assert_allclose(h2, h1, rtol=1e-5)
+
+
+def test_recurrence_plan_builds_fp64_tables_on_first_use():
+    """A recurrence-table plan (l_max = 1023) keeps no FP64 tables at
+    creation (they would take 12.9 GB at this size): its split kernels
+    generate theirs in-kernel, and its row factors come from the recurrence.
+    The first per-call FP64 entry builds the tables; its result equals a
+    plain FP64 plan's bit for bit."""
+    import torch
+    L = 1023
+    nt, nph = workload.grid_size(L)
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    p = sht.Plan(L, nt, nph, precision=sht.PREC_SPLIT16_REC, max_levels=2)
+    torch.cuda.synchronize()
+    free1 = torch.cuda.mem_get_info()[0]
+    print("recurrence plan l_max=1023: %.2f GB at creation" % ((free0 - free1) / 1e9), flush=True)
+    assert free0 - free1 < 3e9, free0 - free1
+    ls, ms = workload.lm_index(L)
+    rng = np.random.default_rng(21)
+    s = (rng.standard_normal(ls.size) + 1j * rng.standard_normal(ls.size)) / (ls + 1.0) ** 2
+    s[ms == 0] = s[ms == 0].real
+    got = p.scal_to_spat(s, L)
+    torch.cuda.synchronize()
+    free2 = torch.cuda.mem_get_info()[0]
+    print("after the first FP64 entry: %.2f GB more" % ((free1 - free2) / 1e9), flush=True)
+    assert free1 - free2 > 1e10, free1 - free2
+    p.close()
+    p64 = sht.Plan(L, nt, nph, max_levels=1)
+    want = p64.scal_to_spat(s, L)
+    p64.close()
+    assert np.array_equal(got, want)
