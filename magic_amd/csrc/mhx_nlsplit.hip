@@ -521,6 +521,9 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
     __shared__ __align__(16) unsigned char sb[SB];
     __shared__ unsigned lvmax[NCOL * WLG];
     __shared__ float lvsc[NCOL * WLG];
+    // the order's c_l (l - m <= l_max + 2 - m) and row factors, read once
+    __shared__ float scr[2 * PKP + 8];
+    __shared__ float srsv[WNT / 2];
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -536,6 +539,12 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
     const int K0 = (a.l_max + 1 - m) / 2 + 1, K1 = (a.l_max + 2 - m) / 2;
     const int nks0 = (K0 + KS - 1) / KS, nks1 = (K1 + KS - 1) / KS;
     if (tid < NCOL * WLG) lvmax[tid] = 0u;
+    {
+        const float *cr = a.crec + a.lstart[mc] + 2 * mc;
+        const int dmax = a.l_max + 2 - m;
+        for (int d = tid; d < 2 * PKP + 8; d += 256) scr[d] = d <= dmax ? cr[d] : 0.f;
+        for (int t = tid; t < a.nh; t += 256) srsv[t] = a.rowscale[(size_t)mc * a.nh + t];
+    }
     const int levl = tid >> 5, g0 = tid & 31;   // staging: level, group of 4 l
     const int lev = lev0 + levl;
     const bool lok = levl < nlh;
@@ -544,16 +553,13 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
 
     // c_l for the thread's l = lb .. lb + 4 of each round (0 past l_max + 2)
     float cv[3][5];
-    {
-        const float *cr = a.crec + a.lstart[mc] + 2 * mc;
 #pragma unroll
-        for (int it = 0; it < 3; ++it)
+    for (int it = 0; it < 3; ++it)
 #pragma unroll
-            for (int q = 0; q < 5; ++q) {
-                const int d = 4 * (g0 + 32 * it) + q;
-                cv[it][q] = d <= a.l_max + 2 - m ? cr[d] : 0.f;
-            }
-    }
+        for (int q = 0; q < 5; ++q) {
+            const int d = 4 * (g0 + 32 * it) + q;
+            cv[it][q] = d < 2 * PKP + 8 ? scr[d] : 0.f;
+        }
     // ---- 1. Q, S, T (SG: s) in FP32 registers, l = m + 4 (g0 + 32 it) + j.
     // The sources are read lane-consecutive (thread g0 of a level reads
     // l = m + 32 r + g0, r < 4 NRL: each four-lane group one 64-byte run),
@@ -746,7 +752,7 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int th = min((tile0 + tt) * 16 + 4 * (lane >> 4) + j, a.nh - 1);
-            rsv[tt][j] = a.rowscale[(size_t)mc * a.nh + th];
+            rsv[tt][j] = srsv[th];
         }
     struct Frag { h8 h[WTT], l[WTT]; };
     const size_t toff0 = a.off[2 * mc], toff1 = a.off[2 * mc + 1];
