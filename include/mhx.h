@@ -227,6 +227,19 @@ int mhx_get_td(mhx_plan *p, int nlev, const double *const *in, size_t ld_lm,
                double *const *out, const double *or2, const double *or4,
                const double *epsc, double CorFac, const int *lcut, void *stream);
 
+/* mhx_nl_step followed by mhx_get_td on its outputs in one call: the
+ * radial loop's explicit terms of the interior levels (src/rIter.f90:
+ * 412-429 after :483-707) with the lm-space assembly as the step's last
+ * launch, inside the step's captured graph.  out[10] as mhx_nl_step (still
+ * written: dVSrLM = VSrLM is the reference's own output); td_out[7] = dwdt
+ * dzdt dpdt dsdt dbdt djdt dVxBhLM as mhx_get_td's out; epsc, CorFac as
+ * mhx_get_td, the other arguments as mhx_nl_step.  The assembly is FP64 on
+ * the step's outputs in either precision. */
+int mhx_nl_step_td(mhx_plan *p, int nlev, const double *const *in, size_t ld_lm,
+                   double *const *out, double *const *td_out, const double *or2,
+                   const double *or4, const double *orho1, double LFfac, const double *epsc,
+                   double CorFac, const int *lcut, void *stream);
+
 /* ---- the radial loop for any physics switches, interior levels ----------
  * One call replaces the loop body of rIter_single_t%radialLoop
  * (src/rIter.f90:188-444) for nlev interior levels (nBc = 0) of a time step

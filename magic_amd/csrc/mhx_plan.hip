@@ -922,10 +922,17 @@ struct CourIn {
     double *vr2max, *vh2max;
 };
 
+// the lm-space assembly appended to the step (mhx_nl_step_td)
+struct TdIn {
+    double *const *out;  // dwdt dzdt dpdt dsdt dbdt djdt dVxBhLM
+    const double *epsc;  // per level
+    double CorFac;
+};
+
 static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t ld_lm,
                         double *const *out, const double *or2, const double *or4,
                         const double *orho1, double LFfac, const int *lcut, double *grids,
-                        void *stream, const CourIn *ci = nullptr)
+                        void *stream, const CourIn *ci = nullptr, const TdIn *ti = nullptr)
 {
     if (!h) return fail(MHX_E_INVALID, "null plan");
     Plan *P = &h->p;
@@ -935,21 +942,40 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
         if (!in[i]) return fail(MHX_E_INVALID, "null input array " + std::to_string(i));
     for (int i = 0; i < 10; ++i)
         if (!out[i]) return fail(MHX_E_INVALID, "null output array " + std::to_string(i));
+    if (ti) {
+        if (!ti->out || !ti->epsc) return fail(MHX_E_INVALID, "null pointer");
+        for (int i = 0; i < 7; ++i)
+            if (!ti->out[i]) return fail(MHX_E_INVALID, "null td output array " + std::to_string(i));
+    }
+    if (ld_lm < (size_t)P->lm_max) return fail(MHX_E_INVALID, "ld_lm below lm_max");
     int rc;
     if ((rc = order_after(P, stream)) || (rc = ensure_ws(P, nlev))) return rc;
     for (int i = 0; i < nlev; ++i)
         if ((rc = check_lcut(P, lcut[i]))) return rc;
+    if (ti && P->td_nlev < nlev) {
+        // per-level factors of the assembly (or2, or4, epsc), made before any
+        // capture
+        MHX_HIP_CHECK(hipStreamSynchronize(P->stream));
+        free_dev(P->d_td_rad);
+        free_dev(P->d_td_lcut);
+        P->td_nlev = 0;
+        MHX_HIP_CHECK(hipMalloc(&P->d_td_rad, 3 * (size_t)nlev * sizeof(double)));
+        MHX_HIP_CHECK(hipMalloc(&P->d_td_lcut, (size_t)nlev * sizeof(int)));
+        P->td_nlev = nlev;
+    }
     // host arrays (the caller's *_Rloc fields) are staged through ws_H:
     // inputs copied in here, outputs copied back after the last launch
     const double *din[11];
-    double *dout[10];
-    bool hout[10];
+    double *dout[10], *dtd[7] = {};
+    bool hout[10], htd[7] = {};
     {
         const size_t arr = (size_t)nlev * ld_lm * 2;  // doubles per array
         int nh_in = 0, nh_out = 0;
         bool hin[11];
         for (int i = 0; i < 11; ++i) nh_in += (hin[i] = !is_device_ptr(in[i]));
         for (int i = 0; i < 10; ++i) nh_out += (hout[i] = !is_device_ptr(out[i]));
+        if (ti)
+            for (int i = 0; i < 7; ++i) nh_out += (htd[i] = !is_device_ptr(ti->out[i]));
         if (nh_in + nh_out &&
             (rc = ensure_buf(P->ws_H, P->ws_H_bytes, (size_t)(nh_in + nh_out) * arr * sizeof(double))))
             return rc;
@@ -963,6 +989,8 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
             din[i] = d;
         }
         for (int i = 0; i < 10; ++i) dout[i] = hout[i] ? P->ws_H + (size_t)slot++ * arr : out[i];
+        if (ti)
+            for (int i = 0; i < 7; ++i) dtd[i] = htd[i] ? P->ws_H + (size_t)slot++ * arr : ti->out[i];
         P->last_host_out = nh_out > 0;
     }
     std::vector<double> &rad = P->h_rad;
@@ -976,7 +1004,8 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
         // per-level factors and lcut through a pinned buffer: the copies are
         // then asynchronous (the previous call's copies completed before its
         // end-of-call synchronisation, so the buffer is free to refill)
-        const size_t need = rad.size() * sizeof(double) + (size_t)nlev * sizeof(int);
+        const size_t ntd = ti ? 3 * (size_t)nlev : 0;
+        const size_t need = (rad.size() + ntd) * sizeof(double) + (size_t)nlev * sizeof(int);
         if (P->h_pin_bytes < need) {
             if (P->h_pin) (void)hipHostFree(P->h_pin);
             P->h_pin = nullptr;
@@ -987,11 +1016,21 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
         if (P->pin_pending) MHX_HIP_CHECK(hipEventSynchronize(P->ev_pin));  // previous copies read
         P->pin_pending = false;
         double *hr = static_cast<double *>(P->h_pin);
-        int *hl = reinterpret_cast<int *>(hr + rad.size());
+        int *hl = reinterpret_cast<int *>(hr + rad.size() + ntd);
         std::memcpy(hr, rad.data(), rad.size() * sizeof(double));
         std::memcpy(hl, lcut, (size_t)nlev * sizeof(int));
         MHX_HIP_CHECK(hipMemcpyAsync(P->d_rad, hr, rad.size() * sizeof(double), hipMemcpyHostToDevice,
                                      P->stream));
+        if (ti) {
+            double *ht = hr + rad.size();
+            for (int i = 0; i < nlev; ++i) {
+                ht[i] = or2[i];
+                ht[nlev + i] = or4[i];
+                ht[2 * nlev + i] = ti->epsc[i];
+            }
+            MHX_HIP_CHECK(hipMemcpyAsync(P->d_td_rad, ht, ntd * sizeof(double), hipMemcpyHostToDevice,
+                                         P->stream));
+        }
         MHX_HIP_CHECK(hipMemcpyAsync(P->d_lcut, hl, (size_t)nlev * sizeof(int), hipMemcpyHostToDevice,
                                      P->stream));
         MHX_HIP_CHECK(hipEventRecord(P->ev_pin, P->stream));
@@ -1166,7 +1205,22 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
                 k_zero_u64<<<(2 * nlev + 255) / 256, 256, 0, s>>>(P->cour.max, 2 * nlev);
                 MHX_HIP_CHECK(hipGetLastError());
             }
-            return split ? launch_split() : launch_fp64();
+            int rc2 = split ? launch_split() : launch_fp64();
+            if (rc2 || !ti) return rc2;
+            // the lm-space assembly on the step's outputs (src/get_td.f90)
+            TdArgs a;
+            std::memset(&a, 0, sizeof(a));
+            const double *tin[10] = {dout[0], dout[1], dout[2], dout[4], dout[6], dout[7], dout[8],
+                                     din[W], din[DW], din[Z]};
+            for (int i = 0; i < 10; ++i) a.in[i] = tin[i];
+            for (int i = 0; i < 7; ++i) a.out[i] = dtd[i];
+            a.ld_lm = ld_lm;
+            a.rad = P->d_td_rad;
+            a.lcut = P->d_lcut;
+            a.CorFac = ti->CorFac;
+            a.nlev = nlev;
+            if ((rc2 = launch_get_td(P, a, s)) || (rc2 = mark())) return rc2;
+            return MHX_OK;
         };
         if (P->timing) {
             if ((rc = launch_all())) return rc;
@@ -1178,6 +1232,14 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
             key.push_back(lf);
             for (int i = 0; i < 11; ++i) key.push_back((unsigned long long)(uintptr_t)din[i]);
             for (int i = 0; i < 10; ++i) key.push_back((unsigned long long)(uintptr_t)dout[i]);
+            key.push_back(ti ? 1ull : 0ull);
+            if (ti) {
+                for (int i = 0; i < 7; ++i) key.push_back((unsigned long long)(uintptr_t)dtd[i]);
+                unsigned long long cf;
+                std::memcpy(&cf, &ti->CorFac, sizeof(cf));
+                key.push_back(cf);
+                key.push_back((unsigned long long)(uintptr_t)P->d_td_rad);
+            }
             // workspaces the launches address (reallocated when nlev grows)
             for (const void *w : {(const void *)P->ws_F, (const void *)P->ws_F2, (const void *)P->ws_G,
                                   (const void *)P->d_lcut, (const void *)P->d_rad,
@@ -1214,6 +1276,11 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
         if (hout[i])
             MHX_HIP_CHECK(hipMemcpyAsync(out[i], dout[i], (size_t)nlev * ld_lm * 2 * sizeof(double),
                                          hipMemcpyDeviceToHost, P->stream));
+    if (ti)
+        for (int i = 0; i < 7; ++i)
+            if (htd[i])
+                MHX_HIP_CHECK(hipMemcpyAsync(ti->out[i], dtd[i], (size_t)nlev * ld_lm * 2 * sizeof(double),
+                                             hipMemcpyDeviceToHost, P->stream));
     if (P->timing) {
         // launches: 6 synthesis, fused grid stage, 4 analysis
         MHX_HIP_CHECK(hipEventSynchronize(P->ev[nev - 1]));
@@ -1253,6 +1320,17 @@ int mhx_nl_step(mhx_plan *h, int nlev, const double *const *in, size_t ld_lm, do
                 const int *lcut, void *stream)
 {
     int rc = nl_step_impl(h, nlev, in, ld_lm, out, or2, or4, orho1, LFfac, lcut, nullptr, stream);
+    if (rc) return rc;
+    return finish_step(h, stream);
+}
+
+int mhx_nl_step_td(mhx_plan *h, int nlev, const double *const *in, size_t ld_lm, double *const *out,
+                   double *const *td_out, const double *or2, const double *or4, const double *orho1,
+                   double LFfac, const double *epsc, double CorFac, const int *lcut, void *stream)
+{
+    TdIn ti{td_out, epsc, CorFac};
+    int rc = nl_step_impl(h, nlev, in, ld_lm, out, or2, or4, orho1, LFfac, lcut, nullptr, stream,
+                          nullptr, &ti);
     if (rc) return rc;
     return finish_step(h, stream);
 }

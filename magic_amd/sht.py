@@ -57,6 +57,7 @@ _SIGS = {
     "mhx_get_e_kin": "Pipppzppp",
     "mhx_get_e_mag": "Pipppzpp",
     "mhx_get_e_kin_noalloc": "Pipppzpppp",
+    "mhx_nl_step_td": "Pipzpppppdpdpp",
     "mhx_rloop": "Ppippzpp",
     "mhx_nl_step_courant": "Pipzppppdpddppppp",
     "mhx_set_timing": "Pi",
@@ -347,6 +348,25 @@ def nl_step(plan, inputs, outputs, or2, or4, orho1, LFfac, lcut, grids=None, str
     else:
         _check(L.mhx_nl_step_grids(plan.h, nlev, ins, plan.lm_max, outs, d(or2), d(or4),
                                    d(orho1), float(LFfac), lc, _ptr(grids), st))
+
+
+def nl_step_td(plan, inputs, outputs, td_outputs, or2, or4, orho1, LFfac, epsc, CorFac, lcut,
+               stream=None):
+    """nl_step followed by the lm-space assembly of get_td on its outputs in
+    one call (mhx_nl_step_td): ``td_outputs`` are 7 device tensors (nlev,
+    lm_max) complex128 in TD_OUTPUTS order, written besides ``outputs``;
+    ``epsc`` per level as get_td's (epsc * epscProf), ``CorFac`` as the
+    reference's."""
+    L = plan._L
+    nlev = inputs[0].shape[0]
+    ins = (ctypes.c_void_p * 11)(*[t.data_ptr() for t in inputs])
+    outs = (ctypes.c_void_p * 10)(*[t.data_ptr() for t in outputs])
+    tds = (ctypes.c_void_p * 7)(*[t.data_ptr() for t in td_outputs])
+    d = lambda v: (ctypes.c_double * nlev)(*[float(x) for x in v])  # noqa: E731
+    lc = (ctypes.c_int * nlev)(*[int(x) for x in lcut])
+    st = ctypes.c_void_p(stream.cuda_stream) if stream is not None else None
+    _check(L.mhx_nl_step_td(plan.h, nlev, ins, plan.lm_max, outs, tds, d(or2), d(or4), d(orho1),
+                            float(LFfac), d(epsc), float(CorFac), lc, st))
 
 
 def nl_step_courant(plan, inputs, outputs, or2, or4, orho1, LFfac, lcut, courfac, alffac,

@@ -75,3 +75,47 @@ def test_get_td_matches_oracle_full_size():
     Os = _run(p, X, or2, or4, epsc, corfac, lcut, stream=st)
     assert np.array_equal(Os, O)
     p.close()
+
+
+@pytest.mark.parametrize("precision", [sht.PREC_FP64, sht.PREC_SPLIT16])
+def test_nl_step_td_equals_separate_calls(precision):
+    """mhx_nl_step_td (the step with the lm-space assembly as its last launch,
+    inside the step's graph) against mhx_nl_step followed by mhx_get_td on
+    its outputs: both the ten step outputs and the seven explicit terms are
+    the same bit for bit, on the eager first call, the second call (graph
+    capture) and a replay, with the outputs zeroed before each call."""
+    import torch
+    from nl_reference import make_inputs, radial
+    L, nlev = 255, 6
+    nt, nph = workload.grid_size(L)
+    inputs = make_inputs(L, nlev)
+    or2, or4, orho1, lff = radial(nlev)
+    lcut = [L] * nlev
+    lcut[2] = L - 7
+    epsc = [0.3 + 0.1 * i for i in range(nlev)]
+    corfac = 2.0e3
+    dev = torch.device("cuda")
+    p = sht.Plan(L, nt, nph, precision=precision, max_levels=nlev)
+    print("nl_step_td precision %d, l_max=%d x %d" % (precision, L, nlev), flush=True)
+    tin = [torch.from_numpy(x).to(dev) for x in inputs]
+    new = lambda n: [torch.zeros((nlev, p.lm_max), dtype=torch.complex128, device=dev)  # noqa: E731
+                     for _ in range(n)]
+    ref_out, ref_td = new(10), new(7)
+    sht.nl_step(p, tin, ref_out, or2, or4, orho1, lff, lcut)
+    td_in = ref_out[0:3] + [ref_out[4]] + ref_out[6:9] + [tin[0], tin[1], tin[3]]
+    sht.get_td(p, td_in, ref_td, or2, or4, epsc, corfac, lcut)
+    torch.cuda.synchronize()
+    ro = [t.cpu().numpy() for t in ref_out]
+    rt = [t.cpu().numpy() for t in ref_td]
+    out, td = new(10), new(7)
+    for call in range(3):
+        for t in out + td:
+            t.zero_()
+        torch.cuda.synchronize()
+        sht.nl_step_td(p, tin, out, td, or2, or4, orho1, lff, epsc, corfac, lcut)
+        torch.cuda.synchronize()
+        for k in range(10):
+            assert np.array_equal(out[k].cpu().numpy(), ro[k]), (call, sht.NL_OUTPUTS[k])
+        for k in range(7):
+            assert np.array_equal(td[k].cpu().numpy(), rt[k]), (call, sht.TD_OUTPUTS[k])
+    p.close()
