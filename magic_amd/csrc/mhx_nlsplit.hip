@@ -506,6 +506,11 @@ template <int N, class F> __device__ __forceinline__ void static_for(F &&f)
 // 4. contraction over the k steps of both parities, the table fragments
 //    loaded PFD steps ahead (the step's MFMA work is shorter than an L2 hit);
 // 5. epilogue: outputs staged per level in LDS, written as theta runs.
+// table fragments in flight: a ring of SRING (synthesis) / ARING (analysis)
+// steps, loaded SRING - 1 / ARING - 1 steps ahead of their MFMAs (deeper
+// rings measured no faster: 5 / 6 gave 0.1351 / 0.1612 ms per QST launch
+// against 0.1362 / 0.1599, and the scalar analysis lost a wave per SIMD)
+constexpr int SRING = 4, ARING = 4;
 constexpr int PKP = 160;                 // k per parity held (l_max + 1 - m)/2 + 1 <= 160
 constexpr int PROW = 4 * PKP + 16;       // bytes per LDS row (padded)
 
@@ -759,7 +764,7 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
     // NC chunks per parity (nks0 >= nks1); one straight-line body per NC
     auto contract = [&](auto ncc) {
         constexpr int NC = decltype(ncc)::value, NS = 2 * NC;
-        Frag fr[4];
+        Frag fr[SRING];
         auto load_s = [&](auto sc) {
             constexpr int S = decltype(sc)::value;
             if constexpr (S < NS) {
@@ -770,15 +775,15 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
                     const int tile = min(tile0 + tt, a.nt16 - 1);
                     // the fragment's 64 hi parts, then its 64 lo parts
                     const size_t to = (PAR ? toff1 : toff0) + (((size_t)ks * a.nt16 + tile) * 128 + lane) * 8;
-                    fr[S % 4].h[tt] = *reinterpret_cast<const h8 *>(a.tP + to);
-                    fr[S % 4].l[tt] = *reinterpret_cast<const h8 *>(a.tP + to + 512);
+                    fr[S % SRING].h[tt] = *reinterpret_cast<const h8 *>(a.tP + to);
+                    fr[S % SRING].l[tt] = *reinterpret_cast<const h8 *>(a.tP + to + 512);
                 }
             }
         };
         auto comp_s = [&](auto sc) {
             constexpr int S = decltype(sc)::value;
             constexpr int PAR = S / NC, KC = S % NC;
-            const Frag &f = fr[S % 4];
+            const Frag &f = fr[S % SRING];
             const int kb = 2 * (KC * KS + 8 * (lane >> 4));
 #pragma unroll
             for (int c = 0; c < NCOL; ++c) {
@@ -795,13 +800,11 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
                 }
             }
         };
-        load_s(std::integral_constant<int, 0>());
-        load_s(std::integral_constant<int, 1>());
-        load_s(std::integral_constant<int, 2>());
+        static_for<SRING - 1>([&](auto sc) { load_s(sc); });
         __builtin_amdgcn_sched_barrier(0);
         static_for<NS>([&](auto sc) {
             constexpr int S = decltype(sc)::value;
-            load_s(std::integral_constant<int, S + 3>());
+            load_s(std::integral_constant<int, S + SRING - 1>());
             __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of this step
             comp_s(sc);
             __builtin_amdgcn_sched_barrier(0);
@@ -1426,7 +1429,7 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
     // LDS); tiles past ntl are dropped in the epilogue.
     constexpr int NHS = 2 * (AKP / KS);
     struct AF { h8 h[PTT], l[PTT]; };
-    AF fr[4];
+    AF fr[ARING];
     const size_t toff0 = a.off[2 * mc], toff1 = a.off[2 * mc + 1];
     auto load_h = [&](auto hc) {
         constexpr int H = decltype(hc)::value;
@@ -1438,15 +1441,15 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
                 const int tile = min(wave + 4 * tt, ntl - 1);
                 // the fragment's 64 hi parts, then its 64 lo parts
                 const size_t to = (PAR ? toff1 : toff0) + (((size_t)tile * a.nq32 + kc) * 128 + lane) * 8;
-                fr[H % 4].h[tt] = *reinterpret_cast<const h8 *>(a.tP + to);
-                fr[H % 4].l[tt] = *reinterpret_cast<const h8 *>(a.tP + to + 512);
+                fr[H % ARING].h[tt] = *reinterpret_cast<const h8 *>(a.tP + to);
+                fr[H % ARING].l[tt] = *reinterpret_cast<const h8 *>(a.tP + to + 512);
             }
         }
     };
     auto comp_h = [&](auto hc) {
         constexpr int H = decltype(hc)::value;
         constexpr int PAR = H % 2, KC = H / 2;
-        const AF &f = fr[H % 4];
+        const AF &f = fr[H % ARING];
         const int kb = 2 * (KC * KS + 8 * (lane >> 4));
 #pragma unroll
         for (int c = 0; c < NCOL; ++c) {
@@ -1464,13 +1467,11 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
             }
         }
     };
-    load_h(std::integral_constant<int, 0>());
-    load_h(std::integral_constant<int, 1>());
-    load_h(std::integral_constant<int, 2>());
+    static_for<ARING - 1>([&](auto hc) { load_h(hc); });
     __builtin_amdgcn_sched_barrier(0);
     static_for<NHS>([&](auto hc) {
         constexpr int H = decltype(hc)::value;
-        load_h(std::integral_constant<int, H + 3>());
+        load_h(std::integral_constant<int, H + ARING - 1>());
         __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of this step
         comp_h(hc);
         __builtin_amdgcn_sched_barrier(0);
