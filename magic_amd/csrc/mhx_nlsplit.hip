@@ -1299,6 +1299,32 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
     bool tok[PTT];
 #pragma unroll
     for (int i = 0; i < PTT; ++i) tok[i] = wave + 4 * i < ntl;
+    // ---- 1. Sym/Anti of the theta pairs in registers; exact maxima per
+    // (level, group); after the scales, hi/lo pairs (two theta_h) written once
+    // to the LDS rows [hi 192][lo 192] (theta_h >= nh zero).
+    // The block's Fourier loads go first, before the per-block staging and
+    // the barrier that waits for it: unconditional (a load under a branch
+    // waits for itself before the next one) at in-bounds addresses (entries
+    // past the group's levels read its first level; rows past nh lie in the
+    // padding of their 16-row block), zeroed after the barrier where the
+    // level, the row or the level's lcut leaves them out.
+    const int nent = WLG * a.ntb * 4;
+    constexpr int NE = (WLG * (AKP / 8) * 4 + 255) / 256;  // 3
+    f4 xv[NE][NCOL][2];
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+        const int e = tid + 256 * i;
+        const int q = e & 3, tb = (e >> 2) % a.ntb, levl = (e >> 2) / a.ntb;
+        const int lev = lev0 + ((e < nent && levl < nlh) ? levl : 0);
+#pragma unroll
+        for (int k = 0; k < NCOL; ++k) {
+            const size_t o = ((size_t)lev * a.fin_ld +
+                              (((size_t)a.in_comp[k] * a.ntb + tb) * a.n_m + mc) * 16 + 4 * q) * 2;
+            xv[i][k][0] = *reinterpret_cast<const f4 *>(a.fin + o);
+            xv[i][k][1] = *reinterpret_cast<const f4 *>(a.fin + o + 4);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
     if (tid < 2 * WLG) lvmax[tid] = 0u;
     if (tid < WLG) slc[tid] = tid < nlh ? a.lcut[lev0 + tid] : -1;
     {
@@ -1310,36 +1336,22 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
         }
     }
     __syncthreads();
-
-    // ---- 1. Sym/Anti of the theta pairs in registers; exact maxima per
-    // (level, group); after the scales, hi/lo pairs (two theta_h) written once
-    // to the LDS rows [hi 192][lo 192] (theta_h >= nh zero)
-    const int nent = WLG * a.ntb * 4;
-    constexpr int NE = (WLG * (AKP / 8) * 4 + 255) / 256;  // 3
-    f4 xv[NE][NCOL][2];
     {
 #pragma unroll
         for (int i = 0; i < NE; ++i) {
             const int e = tid + 256 * i;
             const int q = e & 3, tb = (e >> 2) % a.ntb, levl = (e >> 2) / a.ntb;
-            const int lev = lev0 + levl;
             const int th0 = 8 * tb + 2 * q;
             bool valid = e < nent && levl < nlh && th0 < a.nh;
             if (valid) {
                 const int lc = slc[levl];
                 valid = m <= (lc < a.m_max ? lc : a.m_max);
             }
-            const bool ok1 = th0 + 1 < a.nh;
+            const bool ok1 = valid && th0 + 1 < a.nh;
 #pragma unroll
             for (int k = 0; k < NCOL; ++k) {
-                xv[i][k][0] = f4{0.f, 0.f, 0.f, 0.f};
-                xv[i][k][1] = f4{0.f, 0.f, 0.f, 0.f};
-                if (valid) {
-                    const size_t o = ((size_t)lev * a.fin_ld +
-                                      (((size_t)a.in_comp[k] * a.ntb + tb) * a.n_m + mc) * 16 + 4 * q) * 2;
-                    xv[i][k][0] = *reinterpret_cast<const f4 *>(a.fin + o);
-                    if (ok1) xv[i][k][1] = *reinterpret_cast<const f4 *>(a.fin + o + 4);
-                }
+                if (!valid) xv[i][k][0] = f4{0.f, 0.f, 0.f, 0.f};
+                if (!ok1) xv[i][k][1] = f4{0.f, 0.f, 0.f, 0.f};
             }
         }
 #pragma unroll
