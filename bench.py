@@ -388,22 +388,29 @@ def bench_batch(args, dev):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     pairs = sum(nlev for _, _, _, nlev in specs)
-    # algorithmic bytes and FP64-equivalent Legendre flops of one execute
+    b.status()
+    form, shapes = b.shapes()
+    # algorithmic bytes and FP64-equivalent Legendre flops of one execute:
+    # coefficients in and out, the Fourier rows written and read on both
+    # sides, and (per-job launches only) the grids written and read
     byts = flops = 0.0
     for L, kind, nf, nlev in specs:
         nc = 3 if kind == sht.JOB_VECTOR else 1
         nt = 3 * (L + 1) // 2
         nph, nm, lm, R = 2 * nt, L + 1, (L + 1) * (L + 2) // 2, nf * nlev
         four, grid = nc * R * nm * nt * 8, nc * R * nph * nt * 4
-        byts += 2 * nc * R * lm * 16 + 2 * four + 2 * grid + 2 * four
+        byts += 2 * nc * R * lm * 16 + 2 * four + (0 if form == 1 else 2 * grid) + 2 * four
         flops += (40 if nc == 3 else 8) * (nt // 2) * lm * R
     tune = b.tuning()
     b.close()
     achieved = byts / dt / 1e9
     log("batch: %d jobs, %d (job, level) pairs, %.3f ms per execute, max round-trip err %.2e, "
-        "made in %.1f s; tuned %s" % (len(specs), pairs, dt * 1e3, err, t_make, tune))
+        "made in %.1f s; form %d, shapes %s, tuned %s" % (len(specs), pairs, dt * 1e3, err, t_make, form,
+                                                          shapes, tune))
+    one = form == 1
     return {
-        "metric": "SHT round-trip pairs/s, uneven batch of 64 jobs (split form, one graph)",
+        "metric": "SHT round-trip pairs/s, uneven batch of 64 jobs (split form, one %s)"
+                  % ("persistent launch" if one else "graph of per-job launches"),
         "value": pairs / dt, "unit": "(job, radial level) round-trip pairs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -415,11 +422,15 @@ def bench_batch(args, dev):
                    "jobs": len(specs), "pairs_per_execute": pairs},
         "max_round_trip_rel_l2_err": err,
         "legendre_tflops_fp64_equiv": flops / dt / 1e12,
-        "tuning": [{"l_max": t[0], "synthesis_form": t[1], "analysis_form": t[2],
-                    "fft_rows_per_block": t[3]} for t in tune],
+        "launch_form": "persistent" if one else "graph",
+        "tile_shapes": [{"l_max": t[0], "orders_per_item": t[1], "level_groups_per_item": t[2],
+                         "theta_rows_per_row_item": t[3]} for t in shapes] if one else None,
+        "tuning": None if one else [{"l_max": t[0], "synthesis_form": t[1], "analysis_form": t[2],
+                                     "fft_rows_per_block": t[3]} for t in tune],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": "whole batch graph (algorithmic bytes of all launches / time)"},
+                     "kernel": ("k_pbatch (algorithmic bytes / time of the one launch)" if one else
+                                "whole batch graph (algorithmic bytes of all launches / time)")},
         "cpu_baseline": None,
     }
 

@@ -354,11 +354,18 @@ int mhx_get_e_kin_noalloc(mhx_plan *p, int nlev, const double *w, const double *
  * n_phi = 2 n_theta, lcut = l_max.  Arrays are device memory laid out
  * [comp][field * nlev + lev][lm_max] complex: scalar comp s; vector in =
  * W, dW, Z and out = Q, S, T.  mhx_batch_create builds one plan per
- * distinct l_max and the jobs' workspaces, chooses the kernel forms per
- * l_max by timing them on its largest job, and captures all the jobs'
- * launches into one graph (jobs on parallel branches); mhx_batch_execute
- * replays it (with a stream: returns at once, ordered on the stream;
- * without: returns when done).  The jobs' arrays must stay allocated. */
+ * distinct l_max and the jobs' workspaces and captures the batch into a
+ * graph: by default one persistent launch that runs every job as a work
+ * list of synthesis items, row items (inverse and forward row FFTs in
+ * LDS, no grid in device memory) and analysis items, with the tile shapes
+ * of every l_max chosen by timing the whole launch (autotuner); with the
+ * environment variable MHX_BATCH_FORM=graph, or where a plan does not admit
+ * that kernel, the jobs' per-job launches on parallel graph branches, their
+ * kernel forms chosen per l_max by timing its largest job.
+ * mhx_batch_execute replays the graph (with a stream: returns at once,
+ * ordered on the stream; without: returns when done).  The jobs' arrays
+ * must stay allocated.  The reference runs these calls one level and field
+ * at a time; the batch entries have no counterpart there. */
 #define MHX_JOB_SCALAR 0
 #define MHX_JOB_VECTOR 1
 typedef struct mhx_job {
@@ -371,9 +378,22 @@ int mhx_batch_create(const mhx_job *jobs, int njobs, int precision, mhx_batch **
 int mhx_batch_execute(mhx_batch *batch, void *stream);
 /* the tuned forms per l_max group (n entries at most): synthesis and
  * analysis form (1 whole-K / whole-theta, 2 chunked) and row-FFT rows per
- * block; returns the number of groups */
+ * block (persistent form: theta rows per row item); returns the number of
+ * groups */
 int mhx_batch_tuning(const mhx_batch *batch, int *l_max, int *syn_form, int *ana_form,
                      int *fft_rows, int n);
+/* the launch form (*form: 1 one persistent launch, 0 per-job launches) and,
+ * per l_max group (n entries at most), the tile shapes the autotuner chose
+ * for the persistent launch: orders per synthesis/analysis item (1: one m;
+ * 2: m and m_max - m), level groups of 8 rows (field x level) per item,
+ * theta rows per row item (graph form: 1, 0, row-FFT rows per block);
+ * returns the number of groups */
+int mhx_batch_shapes(const mhx_batch *batch, int *form, int *l_max, int *orders, int *groups,
+                     int *theta_rows, int n);
+/* waits until the batch's executions issued so far are done; MHX_E_HIP if
+ * in one of them a work item of the persistent launch gave up waiting for
+ * its inputs (its outputs are then not valid), MHX_OK otherwise */
+int mhx_batch_status(mhx_batch *batch);
 int mhx_batch_destroy(mhx_batch *batch);
 
 /* durations (ms) of the stages of the last mhx_nl_step on this plan,

@@ -4,17 +4,27 @@
 // radial levels: synthesis to grids (scal_to_spat, or torpol_to_spat) and
 // analysis back (scal_to_SH, or spat_to_qst) -- the calls the reference makes
 // per level and field (src/rIter.f90:483-707, src/sht_native.f90).  The batch
-// keeps one plan (tables) per distinct l_max, one set of FP32 workspaces per
-// job, and runs every job's four launches (Legendre synthesis, inverse row
-// FFTs, forward row FFTs, Legendre analysis) from one captured graph whose
-// jobs lie on parallel branches, so the small jobs of low l_max fill the GPU
-// beside the large ones.  The kernel forms are chosen per l_max when the
-// batch is made, by timing the candidates on that l_max's largest job
-// (autotuner): rows per block of the row FFTs, and for vector jobs the
-// whole-K/whole-theta or the chunked Legendre kernels.
+// keeps one plan (tables) per distinct l_max and one set of FP32 workspaces
+// per job.
+//
+// Persistent form (the default): one captured launch of k_pbatch runs the
+// whole batch as a work list -- synthesis items, row items (inverse and
+// forward row FFTs in LDS, so no grid reaches HBM), analysis items -- with
+// the tile shapes of every l_max chosen when the batch is made by timing the
+// whole launch (autotuner, coordinate descent over the l_max groups):
+// orders per synthesis/analysis item (one m, or the pair m, m_max - m whose
+// l ranges add up to l_max + 1), level groups of 8 rows per item (1, 2, 4),
+// theta rows per row item (4, 8, 16 as the LDS allows).
+//
+// Graph form (MHX_BATCH_FORM=graph, or a plan the persistent kernel does not
+// admit): every job's four launches (Legendre synthesis, inverse row FFTs to
+// grids in HBM, forward row FFTs, Legendre analysis) in one captured graph
+// whose jobs lie on parallel branches, the kernel forms chosen per l_max by
+// timing the candidates on that l_max's largest job.
 #include "mhx_internal.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -23,6 +33,8 @@ namespace {
 
 constexpr int NBR = 4;  // parallel branches of the captured graph
 
+constexpr int PB_GROUP = 8;  // rows (field x level) per level group of the persistent form
+
 struct Job {
     mhx_job d;
     int group = 0;
@@ -30,12 +42,14 @@ struct Job {
     int rows = 0;   // nfields * nlev
     float *F = nullptr, *G = nullptr, *F2 = nullptr;
     int *lcut = nullptr;
+    int c_syn = 0;  // persistent form: first done counter of the job's level groups
 };
 
 struct Group {
     int l_max = 0;
     mhx_plan *plan = nullptr;
-    int fft_rows = 4, syn_form = 1, ana_form = 1;
+    int fft_rows = 4, syn_form = 1, ana_form = 1;  // graph form
+    int fold = 1, lpi = 1, tbr = 8;                // persistent form tile shapes
 };
 
 template <typename T> void dfree(T *&p)
@@ -67,6 +81,15 @@ struct mhx_batch {
     hipEvent_t ev_fork = nullptr, ev_join[mhx::NBR] = {}, ev_in = nullptr, ev_out = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     hipGraphExec_t exec = nullptr;
+    // persistent form
+    bool persistent = false;
+    mhx::PBJob *d_jobs = nullptr;
+    mhx::PBItem *d_items = nullptr;
+    size_t items_cap = 0;
+    mhx::PBQueues q = {};
+    int *d_ctr = nullptr;
+    int nctr = 0;
+    int pgrid = 0;
 };
 
 namespace mhx {
@@ -206,6 +229,283 @@ int capture(mhx_batch *b)
     return MHX_OK;
 }
 
+// ------------------------------------------------------- persistent form
+int ngroups(const Job &j) { return (j.rows + PB_GROUP - 1) / PB_GROUP; }
+
+// theta rows per row item that the launch's LDS holds at this l_max
+bool tbr_fits(const Plan *P, int tbr) { return pbatch_row_lds(P->M, tbr) <= pbatch_lds(); }
+
+// the jobs' descriptors with the groups' current row-item shapes
+void make_pjobs(const mhx_batch *b, std::vector<PBJob> &out)
+{
+    out.assign(b->jobs.size(), PBJob());
+    for (size_t ji = 0; ji < b->jobs.size(); ++ji) {
+        const Job &j = b->jobs[ji];
+        const Group &g = b->groups[j.group];
+        const Plan *P = &g.plan->p;
+        PBJob &pj = out[ji];
+        std::memset(&pj, 0, sizeof(pj));
+        const bool vec = j.d.kind == MHX_JOB_VECTOR;
+        const size_t lm = (size_t)P->lm_max, R = (size_t)j.rows;
+        const int nc = j.ncomp;
+        for (int k = 0; k < nc; ++k) pj.syn.src[k] = j.d.in + (size_t)k * R * lm * 2;
+        pj.syn.src_ld = lm;
+        pj.syn.fout = j.F;
+        pj.syn.fout_ld = (size_t)nc * P->n_m * P->n_theta;
+        pj.syn.lcut = j.lcut;
+        pj.syn.nlev = j.rows;
+        pj.ana.fin = j.F2;
+        pj.ana.fin_ld = (size_t)nc * P->n_m * ((P->n_theta + 15) / 16 * 16);
+        for (int k = 0; k < 3; ++k) {
+            pj.ana.in_comp[k] = k < nc ? k : -1;
+            pj.ana.out[k] = k < nc ? j.d.out + (size_t)k * R * lm * 2 : nullptr;
+        }
+        pj.ana.out_ld = lm;
+        pj.ana.lcut = j.lcut;
+        pj.ana.nlev = j.rows;
+        (void)fill_pbatch_job(P, pj);  // admitted when the form was chosen
+        pj.syn_kind = vec ? NS_QST : NS_S;
+        pj.ana_kind = vec ? NA_QST : NA_SH;
+        pj.osin2_mask = vec ? 6u : 0u;
+        pj.F = j.F;
+        pj.F2 = j.F2;
+        pj.f_ld = pj.syn.fout_ld;
+        pj.f2_ld = pj.ana.fin_ld;
+        pj.nc = nc;
+        pj.rows = j.rows;
+        pj.tbr = g.tbr;
+        pj.ntbr = (P->n_theta + g.tbr - 1) / g.tbr;
+        pj.c_syn = j.c_syn;
+        pj.c_row = j.c_syn + ngroups(j);
+    }
+}
+
+// The work list with the groups' current tile shapes.  A chain is the
+// synthesis, row and analysis items of one job's run of `lpi` level groups;
+// chains are dealt to the XCD queues largest first, each to the least loaded
+// queue (estimated work: l range x rows x components for the Legendre items,
+// rows x components x n_phi for the row items); within a queue the
+// synthesis items come first, then the row items, then the analysis items,
+// each phase largest first.
+void make_items(const mhx_batch *b, std::vector<PBItem> &items, PBQueues &qs)
+{
+    struct W {
+        double cost;
+        PBItem it;
+    };
+    struct Chain {
+        double cost = 0.0;
+        std::vector<W> syn, row, ana;
+    };
+    std::vector<Chain> chains;
+    for (size_t ji = 0; ji < b->jobs.size(); ++ji) {
+        const Job &j = b->jobs[ji];
+        const Group &g = b->groups[j.group];
+        const Plan *P = &g.plan->p;
+        const int ng = ngroups(j), n_m = P->n_m;
+        const double wc = j.ncomp == 3 ? 3.0 : 1.0;
+        std::vector<std::pair<int, int>> ords;
+        if (g.fold) {
+            for (int mc = 0; mc < n_m / 2; ++mc) ords.push_back({mc, n_m - 1 - mc});
+            if (n_m % 2) ords.push_back({n_m / 2, -1});
+        } else {
+            for (int mc = 0; mc < n_m; ++mc) ords.push_back({mc, -1});
+        }
+        const int ntbr = (P->n_theta + g.tbr - 1) / g.tbr;
+        for (int g0 = 0; g0 < ng; g0 += g.lpi) {
+            Chain ch;
+            const int gn = std::min(g.lpi, ng - g0);
+            int nr = 0;
+            for (int gg = g0; gg < g0 + gn; ++gg) nr += std::min(PB_GROUP, j.rows - gg * PB_GROUP);
+            for (const auto &o : ords) {
+                PBItem it;
+                std::memset(&it, 0, sizeof(it));
+                it.kind = PB_SYN;
+                it.job = (short)ji;
+                it.m0 = (short)o.first;
+                it.m1 = (short)o.second;
+                it.g0 = (short)g0;
+                it.ng = (short)gn;
+                double kl = (double)(P->l_max + 2 - o.first * P->minc);
+                if (o.second >= 0) kl += (double)(P->l_max + 2 - o.second * P->minc);
+                const double c = kl * nr * wc * P->n_theta;
+                ch.syn.push_back({c, it});
+                it.kind = PB_ANA;
+                ch.ana.push_back({c, it});
+                ch.cost += 2 * c;
+            }
+            for (int gg = g0; gg < g0 + gn; ++gg) {
+                const int nrg = std::min(PB_GROUP, j.rows - gg * PB_GROUP);
+                for (int tb = 0; tb < ntbr; ++tb) {
+                    PBItem it;
+                    std::memset(&it, 0, sizeof(it));
+                    it.kind = PB_ROW;
+                    it.job = (short)ji;
+                    it.m0 = it.m1 = -1;
+                    it.g0 = (short)gg;
+                    it.ng = 1;
+                    it.tb = (short)tb;
+                    const double c = (double)nrg * j.ncomp * P->n_phi * g.tbr;
+                    ch.row.push_back({c, it});
+                    ch.cost += c;
+                }
+            }
+            chains.push_back(std::move(ch));
+        }
+    }
+    std::vector<int> order(chains.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return chains[x].cost > chains[y].cost; });
+    const int nq = std::max(1, b->q.nq);
+    std::vector<double> load(nq, 0.0);
+    std::vector<std::vector<int>> mine(nq);
+    for (int ci : order) {
+        const int qi = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[qi] += chains[ci].cost;
+        mine[qi].push_back(ci);
+    }
+    auto by_cost = [](const W &x, const W &y) { return x.cost > y.cost; };
+    items.clear();
+    qs = b->q;
+    for (int qi = 0; qi < nq; ++qi) {
+        qs.off[qi] = (int)items.size();
+        std::vector<W> syn, row, ana;
+        for (int ci : mine[qi]) {
+            syn.insert(syn.end(), chains[ci].syn.begin(), chains[ci].syn.end());
+            row.insert(row.end(), chains[ci].row.begin(), chains[ci].row.end());
+            ana.insert(ana.end(), chains[ci].ana.begin(), chains[ci].ana.end());
+        }
+        std::stable_sort(syn.begin(), syn.end(), by_cost);
+        std::stable_sort(row.begin(), row.end(), by_cost);
+        std::stable_sort(ana.begin(), ana.end(), by_cost);
+        for (auto *v : {&syn, &row, &ana})
+            for (const W &w : *v) items.push_back(w.it);
+    }
+    qs.off[nq] = (int)items.size();
+}
+
+// uploads the descriptors and the work list of the current shapes
+int stage(mhx_batch *b)
+{
+    std::vector<PBJob> pj;
+    std::vector<PBItem> items;
+    make_pjobs(b, pj);
+    make_items(b, items, b->q);
+    if (items.size() > b->items_cap) {
+        dfree(b->d_items);
+        if (hipMalloc(&b->d_items, items.size() * sizeof(PBItem)) != hipSuccess)
+            return fail(MHX_E_NOMEM, "batch work-list allocation failed");
+        b->items_cap = items.size();
+    }
+    if (!b->d_jobs && hipMalloc(&b->d_jobs, pj.size() * sizeof(PBJob)) != hipSuccess)
+        return fail(MHX_E_NOMEM, "batch descriptor allocation failed");
+    if (hipMemcpy(b->d_jobs, pj.data(), pj.size() * sizeof(PBJob), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_items, items.data(), items.size() * sizeof(PBItem), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(MHX_E_HIP, "batch work-list upload failed");
+    return MHX_OK;
+}
+
+int launch_p(mhx_batch *b, hipStream_t s)
+{
+    return launch_pbatch(b->d_jobs, b->d_items, b->q, b->d_ctr, b->nctr, b->pgrid, s);
+}
+
+// MHX_OK, or an error if a wait of the persistent launch ran out (the flag is
+// cleared)
+int check_flag(mhx_batch *b)
+{
+    int flag = 0;
+    MHX_HIP_CHECK(hipStreamSynchronize(b->cs));
+    MHX_HIP_CHECK(hipMemcpy(&flag, b->d_ctr + PB_ERR, sizeof(int), hipMemcpyDeviceToHost));
+    if (!flag) return MHX_OK;
+    MHX_HIP_CHECK(hipMemset(b->d_ctr + PB_ERR, 0, sizeof(int)));
+    return fail(MHX_E_HIP, (flag & 1) ? "a work item of the persistent batch launch gave up waiting for its "
+                                        "inputs; the outputs of that execution are not valid"
+                                      : "a work queue of the persistent batch launch was left unclaimed (an "
+                                        "XCD ran none of its workgroups); the outputs are not valid");
+}
+
+// the least time of `reps` launches of the staged work list, in ms
+int time_p(mhx_batch *b, int reps, float &best)
+{
+    best = 1e30f;
+    for (int r = 0; r < reps; ++r) {
+        MHX_HIP_CHECK(hipEventRecord(b->t0, b->cs));
+        int rc = launch_p(b, b->cs);
+        if (rc) return rc;
+        MHX_HIP_CHECK(hipEventRecord(b->t1, b->cs));
+        MHX_HIP_CHECK(hipEventSynchronize(b->t1));
+        float ms = 0.f;
+        MHX_HIP_CHECK(hipEventElapsedTime(&ms, b->t0, b->t1));
+        best = std::min(best, ms);
+    }
+    return check_flag(b);
+}
+
+// tile shapes per l_max group by coordinate descent on the whole launch:
+// (orders per item, level groups per item) then theta rows per row item
+int autotune_p(mhx_batch *b)
+{
+    for (Group &g : b->groups) {
+        g.fold = 1;
+        g.lpi = 1;
+        g.tbr = tbr_fits(&g.plan->p, 8) ? 8 : 4;
+    }
+    int rc;
+    float best;
+    if ((rc = stage(b)) || (rc = time_p(b, 2, best))) return rc;
+    const int shapes[4][2] = {{0, 1}, {1, 1}, {1, 2}, {1, 4}};
+    for (Group &g : b->groups) {
+        const int f0 = g.fold, l0 = g.lpi;
+        int bf = f0, bl = l0;
+        for (const auto &sh : shapes) {
+            if (sh[0] == f0 && sh[1] == l0) continue;
+            g.fold = sh[0];
+            g.lpi = sh[1];
+            float t;
+            if ((rc = stage(b)) || (rc = time_p(b, 2, t))) return rc;
+            if (t < best) {
+                best = t;
+                bf = sh[0];
+                bl = sh[1];
+            }
+        }
+        g.fold = bf;
+        g.lpi = bl;
+        const int t0 = g.tbr;
+        int bt = t0;
+        for (int tbr : {4, 8, 16}) {
+            if (tbr == t0 || !tbr_fits(&g.plan->p, tbr)) continue;
+            g.tbr = tbr;
+            float t;
+            if ((rc = stage(b)) || (rc = time_p(b, 2, t))) return rc;
+            if (t < best) {
+                best = t;
+                bt = tbr;
+            }
+        }
+        g.tbr = bt;
+    }
+    return stage(b);
+}
+
+int capture_p(mhx_batch *b)
+{
+    hipGraph_t graph = nullptr;
+    MHX_HIP_CHECK(hipStreamBeginCapture(b->cs, hipStreamCaptureModeThreadLocal));
+    const int rc = launch_p(b, b->cs);
+    const hipError_t ec = hipStreamEndCapture(b->cs, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (ec != hipSuccess) return fail(MHX_E_HIP, "graph capture of the batch failed");
+    const hipError_t ei = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(MHX_E_HIP, "graph instantiation of the batch failed");
+    return MHX_OK;
+}
+
 }  // namespace
 }  // namespace mhx
 
@@ -259,6 +559,13 @@ int mhx_batch_create(const mhx_job *jobs, int njobs, int precision, mhx_batch **
         j.group = gi;
         b->jobs.push_back(j);
     }
+    // the persistent form unless asked for the graph form or a plan does not
+    // admit it
+    {
+        const char *e = std::getenv("MHX_BATCH_FORM");
+        b->persistent = !(e && std::strcmp(e, "graph") == 0);
+        for (const Group &g : b->groups) b->persistent = b->persistent && pbatch_supported(&g.plan->p);
+    }
     for (Job &j : b->jobs) {
         const Plan *P = &b->groups[j.group].plan->p;
         const size_t R = (size_t)j.rows, nc = (size_t)j.ncomp;
@@ -266,7 +573,7 @@ int mhx_batch_create(const mhx_job *jobs, int njobs, int precision, mhx_batch **
         const size_t fG = R * nc * P->n_phi * P->n_theta;
         const size_t fF2 = R * nc * P->n_m * ((P->n_theta + 15) / 16 * 16) * 2;
         if (hipMalloc(&j.F, fF * sizeof(float)) != hipSuccess ||
-            hipMalloc(&j.G, fG * sizeof(float)) != hipSuccess ||
+            (!b->persistent && hipMalloc(&j.G, fG * sizeof(float)) != hipSuccess) ||
             hipMalloc(&j.F2, fF2 * sizeof(float)) != hipSuccess ||
             hipMalloc(&j.lcut, R * sizeof(int)) != hipSuccess)
             return bail(fail(MHX_E_NOMEM, "batch workspace allocation failed"));
@@ -285,7 +592,40 @@ int mhx_batch_create(const mhx_job *jobs, int njobs, int precision, mhx_batch **
         hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) != hipSuccess ||
         hipEventCreate(&b->t0) != hipSuccess || hipEventCreate(&b->t1) != hipSuccess)
         return bail(fail(MHX_E_HIP, "event creation failed"));
-    if ((rc = autotune(b)) || (rc = capture(b))) return bail(rc);
+    if (b->persistent) {
+        b->nctr = PB_CTR0;
+        for (Job &j : b->jobs) {
+            j.c_syn = b->nctr;
+            b->nctr += 2 * ngroups(j);
+        }
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+            return bail(fail(MHX_E_HIP, "device query failed"));
+        b->pgrid = 2 * ncu;  // two resident workgroups per CU
+        if (hipMalloc(&b->d_ctr, (size_t)std::max(b->nctr, b->pgrid) * sizeof(int)) != hipSuccess)
+            return bail(fail(MHX_E_NOMEM, "batch counter allocation failed"));
+        // the XCDs the launch's workgroups run on: one queue per XCC_ID seen
+        {
+            std::vector<int> ids(b->pgrid);
+            if ((rc = probe_xcc_ids(b->d_ctr, b->pgrid, b->cs))) return bail(rc);
+            if (hipMemcpyAsync(ids.data(), b->d_ctr, ids.size() * sizeof(int), hipMemcpyDeviceToHost, b->cs) !=
+                    hipSuccess ||
+                hipStreamSynchronize(b->cs) != hipSuccess)
+                return bail(fail(MHX_E_HIP, "XCD probe failed"));
+            for (int &x : b->q.xmap) x = -1;
+            b->q.nq = 0;
+            for (int id : ids) {
+                const int x = id & (PB_MAXQ - 1);
+                if (b->q.xmap[x] < 0) b->q.xmap[x] = b->q.nq++;
+            }
+        }
+        if (hipMemset(b->d_ctr, 0, (size_t)b->nctr * sizeof(int)) != hipSuccess)
+            return bail(fail(MHX_E_HIP, "batch counter clear failed"));
+        if ((rc = autotune_p(b)) || (rc = capture_p(b))) return bail(rc);
+    } else if ((rc = autotune(b)) || (rc = capture(b))) {
+        return bail(rc);
+    }
     *batch = b;
     return MHX_OK;
 }
@@ -317,9 +657,35 @@ int mhx_batch_tuning(const mhx_batch *b, int *l_max, int *syn_form, int *ana_for
         if (l_max) l_max[i] = g.l_max;
         if (syn_form) syn_form[i] = g.syn_form;
         if (ana_form) ana_form[i] = g.ana_form;
-        if (fft_rows) fft_rows[i] = g.fft_rows;
+        if (fft_rows) fft_rows[i] = b->persistent ? g.tbr : g.fft_rows;
     }
     return (int)b->groups.size();
+}
+
+int mhx_batch_shapes(const mhx_batch *b, int *form, int *l_max, int *orders, int *groups, int *rows,
+                     int n)
+{
+    if (!b) return fail(MHX_E_INVALID, "null batch");
+    if (form) *form = b->persistent ? 1 : 0;
+    const int k = std::min(n, (int)b->groups.size());
+    for (int i = 0; i < k; ++i) {
+        const Group &g = b->groups[i];
+        if (l_max) l_max[i] = g.l_max;
+        if (orders) orders[i] = b->persistent ? (g.fold ? 2 : 1) : 1;
+        if (groups) groups[i] = b->persistent ? g.lpi : 0;
+        if (rows) rows[i] = b->persistent ? g.tbr : g.fft_rows;
+    }
+    return (int)b->groups.size();
+}
+
+int mhx_batch_status(mhx_batch *b)
+{
+    if (!b) return fail(MHX_E_INVALID, "null batch");
+    if (!b->persistent) {
+        MHX_HIP_CHECK(hipStreamSynchronize(b->cs));
+        return MHX_OK;
+    }
+    return check_flag(b);
 }
 
 int mhx_batch_destroy(mhx_batch *b)
@@ -333,6 +699,9 @@ int mhx_batch_destroy(mhx_batch *b)
         dfree(j.F2);
         dfree(j.lcut);
     }
+    dfree(b->d_jobs);
+    dfree(b->d_items);
+    dfree(b->d_ctr);
     for (Group &g : b->groups)
         if (g.plan) mhx_plan_destroy(g.plan);
     for (int i = 0; i < NBR; ++i) {

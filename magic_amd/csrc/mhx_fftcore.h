@@ -1,7 +1,9 @@
-// Building blocks shared by the FFT kernels (mhx_fft.hip, mhx_grid.hip):
-// complex helpers, in-register DFTs of sizes 2, 3, 4, 5, 12 and 16 with the
-// kernel exp(sign 2 pi i nk / R), the wave-local LDS ordering point and the
-// per-level Courant maxima of the grid stages.
+// Building blocks shared by the FFT kernels (mhx_fft.hip, mhx_grid.hip, the
+// batched jobs in mhx_nlsplit.hip): complex helpers, in-register DFTs of
+// sizes 2, 3, 4, 5, 12 and 16 with the kernel exp(sign 2 pi i nk / R), the
+// wave-local LDS ordering point, the per-level Courant maxima of the grid
+// stages, and generic Stockham row FFTs in LDS with the real-FFT pre/post
+// steps.
 #pragma once
 
 #include "mhx_internal.h"
@@ -201,6 +203,136 @@ __device__ inline void courant_max(const CourArgs &c, int lev, int nlev, T o2, T
         atomicMax(c.max + 2 * lev, (unsigned long long)__double_as_longlong(dr));
         atomicMax(c.max + 2 * lev + 1, (unsigned long long)__double_as_longlong(dh));
     }
+}
+
+// ---------------------------------------------------------------------------
+// Generic LDS row FFTs (any length M = 2^a 3^b 4^c 5^d), used by the per-call
+// API kernels and the batched transform jobs.
+struct FFTArgs {
+    int M, N, nfac, fac[16];
+    const void *tw;   // exp(-2 pi i k / M), k < M (precision of the kernel)
+    const void *twr;  // exp(-2 pi i k / N), k <= M
+};
+
+// one Stockham pass from `in` to `out` over G rows of M complex values
+template <int R, typename C>
+__device__ void pass_pp(const C *in, C *out, int G, int M, int L, const C *tw, int sign)
+{
+    const int T = M / R;
+    const int tstride = M / (L * R);
+    const int nthr = blockDim.x;
+    for (int idx = threadIdx.x; idx < G * T; idx += nthr) {
+        int g = idx / T, j = idx - g * T;
+        int k = j % L;
+        const C *ip = in + g * M;
+        C x[R];
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            C v = ip[j + s * T];
+            if (s > 0) {
+                C w = tw[s * k * tstride];
+                if (sign > 0) w = cconj(w);
+                v = cmul(v, w);
+            }
+            x[s] = v;
+        }
+        Dft<R, C>::run(x, sign);
+        C *op = out + g * M + (j - k) * R + k;
+#pragma unroll
+        for (int s = 0; s < R; ++s) op[s * L] = x[s];
+    }
+}
+
+// complex FFT of G rows held in buffer a (b is scratch); returns the buffer
+// that holds the result
+template <typename C>
+__device__ C *fft_pp(C *a, C *b, int G, const FFTArgs &f, const C *tw, int sign)
+{
+    int L = 1;
+    for (int s = 0; s < f.nfac; ++s) {
+        int R = f.fac[s];
+        switch (R) {
+        case 4: pass_pp<4>(a, b, G, f.M, L, tw, sign); break;
+        case 2: pass_pp<2>(a, b, G, f.M, L, tw, sign); break;
+        case 3: pass_pp<3>(a, b, G, f.M, L, tw, sign); break;
+        default: pass_pp<5>(a, b, G, f.M, L, tw, sign); break;
+        }
+        __syncthreads();
+        C *t = a;
+        a = b;
+        b = t;
+        L *= R;
+    }
+    return a;
+}
+
+// Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/N} (X_k - conj X_{M-k}), real
+// parts only for X_0 and X_M; X_k is read from global memory at X[k xs]
+// (zero for k >= n_m), Z written to LDS row z
+template <typename C>
+__device__ void irfft_pre(const C *__restrict__ X, size_t xs, int n_m, C *z, int M, const C *twr,
+                          int tid0, int nthr)
+{
+    typedef decltype(C::x) T;
+    for (int k = tid0; k <= M / 2; k += nthr) {
+        int k2 = M - k;
+        C a = (k < n_m) ? X[(size_t)k * xs] : mk<C>(T(0), T(0));
+        C c = (k2 < n_m) ? X[(size_t)k2 * xs] : mk<C>(T(0), T(0));
+        if (k == 0) {
+            a.y = T(0);
+            c.y = T(0);
+        }
+        C za = cadd(cadd(a, cconj(c)), cmuli(cmul(cconj(twr[k]), csub(a, cconj(c))), 1));
+        z[k] = za;
+        if (k2 < M && k2 != k) {
+            C zb = cadd(cadd(c, cconj(a)), cmuli(cmul(cconj(twr[k2]), csub(c, cconj(a))), 1));
+            z[k2] = zb;
+        }
+    }
+}
+
+// X_k = (E_k + e^{-2 pi i k/N} O_k) * scale for k < n_m from the FFT of
+// z_n = x_{2n} + i x_{2n+1}
+template <typename C>
+__device__ inline C rfft_post(const C *z, int k, int M, const C *twr, decltype(C::x) scale)
+{
+    typedef decltype(C::x) T;
+    C zk = z[k == M ? 0 : k];
+    C zc = cconj(z[k == 0 ? 0 : M - k]);
+    C E = mk<C>(T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y + zc.y));
+    C D = csub(zk, zc);
+    C O = mk<C>(T(0.5) * D.y, T(-0.5) * D.x);
+    C X = cadd(E, cmul(twr[k], O));
+    X.x *= scale;
+    X.y *= scale;
+    if (k == 0) X.y = T(0);
+    return X;
+}
+
+// the same on a slot row (padded at 12 points per lane, see pidx)
+template <int P, typename C>
+__device__ inline C rfft_post_padded(const C *z, int k, int M, const C *twr, decltype(C::x) scale)
+{
+    typedef decltype(C::x) T;
+    const int k1 = k == M ? 0 : k, k2 = k == 0 ? 0 : M - k;
+    C zk = z[P == 12 ? k1 + (k1 >> 5) : k1];
+    C zc = cconj(z[P == 12 ? k2 + (k2 >> 5) : k2]);
+    C E = mk<C>(T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y + zc.y));
+    C D = csub(zk, zc);
+    C O = mk<C>(T(0.5) * D.y, T(-0.5) * D.x);
+    C X = cadd(E, cmul(twr[k], O));
+    X.x *= scale;
+    X.y *= scale;
+    if (k == 0) X.y = T(0);
+    return X;
+}
+
+template <typename C> __device__ void load_twiddles(C *s_tw, C *s_twr, const FFTArgs &f)
+{
+    const C *tw = reinterpret_cast<const C *>(f.tw);
+    const C *twr = reinterpret_cast<const C *>(f.twr);
+    for (int k = threadIdx.x; k < f.M; k += blockDim.x) s_tw[k] = tw[k];
+    for (int k = threadIdx.x; k <= f.M; k += blockDim.x) s_twr[k] = twr[k];
 }
 
 }  // namespace

@@ -371,6 +371,62 @@ int launch_fft_f32(const Plan *p, const float *grid, size_t g_ld_row, size_t g_l
 int launch_nl_fused(const Plan *p, int nlev, const void *Fin, void *Fout,
                     const double *rad, double LFfac, void *dgrid, void *full_grids,
                     hipStream_t s);
+
+// ------------------------------------------ persistent launch of a batch
+// One launch runs every job of an uneven batch of round-trip jobs as a work
+// list whose items the workgroups claim in order from a ticket counter:
+// synthesis items (orders x level groups of 8 rows), then row items (a
+// level group's theta rows through the inverse and the forward FFT in LDS,
+// no grid in HBM), then analysis items.  A row item waits until every
+// synthesis unit of its level group is done, an analysis item until every
+// row item of its level group is done; waits are only on items of lower
+// tickets, claimed by running workgroups, so they end.
+enum PBKind { PB_SYN = 0, PB_ROW = 1, PB_ANA = 2 };
+constexpr int PB_MAXFAC = 12;
+struct PBJob {
+    NSynArgs syn;              // whole-K synthesis, filled by fill_pbatch_job
+    NAnaArgs ana;              // whole-theta analysis, filled by fill_pbatch_job
+    int syn_kind, ana_kind;    // NS_S / NS_QST, NA_SH / NA_QST
+    int nrl;                   // 128-wide rounds of l holding l <= l_max
+    int M, N, nfac, fac[PB_MAXFAC];
+    const float *tw, *twr;     // FP32 exp(-2 pi i k / M), exp(-2 pi i k / N)
+    const double *osin2;       // 1/sin^2(theta) of the plan's rows
+    unsigned osin2_mask;       // comps scaled by it in the forward FFT
+    const float *F;            // F[row][comp][m][theta] (synthesis output)
+    float *F2;                 // F2[row][comp][theta/16][m][theta%16]
+    size_t f_ld, f2_ld;        // complex elements per row
+    int nc, n_m, n_theta, nt16, rows, tbr, ntbr;
+    int c_syn, c_row;          // first done counter of the job's level groups
+};
+// item: kind, job; PB_SYN/PB_ANA: orders m0 (and m1 >= 0), level groups
+// g0 .. g0 + ng - 1; PB_ROW: level group g0, theta block tb
+struct PBItem {
+    short kind, job, m0, m1, g0, ng, tb, pad;
+};
+// The work list is split into one queue per XCD (accelerator complex die,
+// with its own L2): a level group's synthesis, row and analysis items all lie
+// in one queue, claimed only by workgroups running on that XCD, so a
+// producer's stores reach the consumer through the shared L2 once they are
+// complete.  The workgroup's XCD is read from its hardware register (XCC_ID)
+// and mapped to a queue by `xmap` (measured when the batch is made).
+constexpr int PB_MAXQ = 16;
+struct PBQueues {
+    int nq;                    // queues
+    int off[PB_MAXQ + 1];      // items of queue q: off[q] .. off[q + 1] - 1
+    int xmap[PB_MAXQ];         // XCC_ID -> queue (-1: none)
+};
+// counters: [q] ticket of queue q, [PB_MAXQ] workgroups done, [PB_MAXQ + 1]
+// error (a wait ran out, or a queue was left unclaimed; left set), then the
+// jobs' done counters; the last workgroup out resets all but the error flag
+constexpr int PB_DONE = PB_MAXQ, PB_ERR = PB_MAXQ + 1, PB_CTR0 = PB_MAXQ + 2;
+bool pbatch_supported(const Plan *p);
+size_t pbatch_row_lds(int M, int tbr);
+size_t pbatch_lds();
+int fill_pbatch_job(const Plan *p, PBJob &j);
+// XCC_ID of each of n workgroups (device array ids), for the queue map
+int probe_xcc_ids(int *ids, int n, hipStream_t s);
+int launch_pbatch(const PBJob *jobs, const PBItem *items, const PBQueues &q, int *ctr, int nctr,
+                  int grid, hipStream_t s);
 int launch_axi(const Plan *p, const double *fl, double *f, int lcut, int tor,
                double *f2, hipStream_t s);
 void rl_free(Plan *p);

@@ -23,7 +23,7 @@
 // accumulators thus carry a single scale and need no per-chunk rescaling.
 // Blocks are dealt to the 8 XCDs by m modulo 8 (balanced work per XCD; the
 // level groups of one m run back to back on one XCD and share its L2).
-#include "mhx_internal.h"
+#include "mhx_fftcore.h"
 #include <utility>
 
 #include <cmath>
@@ -514,29 +514,38 @@ constexpr int SRING = 4, ARING = 4;
 constexpr int PKP = 160;                 // k per parity held (l_max + 1 - m)/2 + 1 <= 160
 constexpr int PROW = 4 * PKP + 16;       // bytes per LDS row (padded)
 
+// LDS of one whole-K synthesis item: B rows [column][parity][16 columns][hi |
+// lo], per-(column, level) maxima and scales, the order's c_l (l - m <= l_max
+// + 2 - m) and row factors
+template <int V> struct NSynPL {
+    static constexpr int NCOL = V == NS_S ? 1 : (V == NS_SG ? 2 : 3);
+    static constexpr size_t SB = (size_t)NCOL * 2 * 16 * PROW;
+    static constexpr size_t O_LVMAX = SB, O_LVSC = O_LVMAX + 4 * NCOL * WLG;
+    static constexpr size_t O_SCR = O_LVSC + 4 * NCOL * WLG, O_SRSV = O_SCR + 4 * (2 * PKP + 8);
+    static constexpr size_t BYTES = O_SRSV + 4 * (WNT / 2);
+};
+
+// One whole-K synthesis item: order mc, level group gy, theta group gz, on
+// the LDS area `lds` (NSynPL<V>::BYTES), 256 threads; tid = threadIdx.x
+// (a parameter, so that a caller looping over items can keep the values
+// derived from it inside the loop).
 // NRL: 128-wide rounds of l that can hold l <= l_max (1 for l_max < 128, else 2)
 template <int V, int NRL>
-__global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
+__device__ __forceinline__ void nsyn16p_item(const NSynArgs &a, int mc, int gy, int gz, unsigned char *lds,
+                                             const int tid)
 {
     constexpr bool SCAL = V == NS_SG || V == NS_S;
     constexpr int NSRC = SCAL ? 1 : (V == NS_QSTC ? 4 : 3);
-    constexpr int NCOL = V == NS_S ? 1 : (V == NS_SG ? 2 : 3), NROW = NCOL * 2 * 16;
+    constexpr int NCOL = NSynPL<V>::NCOL, NROW = NCOL * 2 * 16;
     constexpr int NOUT = V == NS_S ? 1 : 3;
-    constexpr int SB = NROW * PROW;  // B rows [column][parity][16 columns][hi | lo]
-    __shared__ __align__(16) unsigned char sb[SB];
-    __shared__ unsigned lvmax[NCOL * WLG];
-    __shared__ float lvsc[NCOL * WLG];
-    // the order's c_l (l - m <= l_max + 2 - m) and row factors, read once
-    __shared__ float scr[2 * PKP + 8];
-    __shared__ float srsv[WNT / 2];
+    constexpr int SB = NROW * PROW;
+    unsigned char *sb = lds;
+    unsigned *lvmax = reinterpret_cast<unsigned *>(lds + NSynPL<V>::O_LVMAX);
+    float *lvsc = reinterpret_cast<float *>(lds + NSynPL<V>::O_LVSC);
+    float *scr = reinterpret_cast<float *>(lds + NSynPL<V>::O_SCR);
+    float *srsv = reinterpret_cast<float *>(lds + NSynPL<V>::O_SRSV);
 
-    const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const int ng = (a.nlev + WLG - 1) / WLG;
-    int mc, rest;
-    xcd_deal(blockIdx.x, ng * a.ngz, mc, rest);
-    if (mc >= a.n_m) return;
-    const int gy = rest % ng, gz = rest / ng;
     const int m = mc * a.minc;
     const float fm = (float)m;
     const int lev0 = gy * WLG, nlh = min(WLG, a.nlev - lev0);
@@ -864,6 +873,17 @@ __global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
             }
         }
     }
+}
+
+template <int V, int NRL>
+__global__ __launch_bounds__(256, 2) void k_nsyn16p(NSynArgs a)
+{
+    __shared__ __align__(16) unsigned char lds[NSynPL<V>::BYTES];
+    const int ng = (a.nlev + WLG - 1) / WLG;
+    int mc, rest;
+    xcd_deal(blockIdx.x, ng * a.ngz, mc, rest);
+    if (mc >= a.n_m) return;
+    nsyn16p_item<V, NRL>(a, mc, rest % ng, rest / ng, lds, threadIdx.x);
 }
 
 // -------------------------------------------------------------- analysis
@@ -1270,27 +1290,34 @@ constexpr int AROW = 4 * AKP + 16;       // bytes per LDS row (padded)
 // removed) in LDS as [group][level][l - m] and forms the outputs per (level, l).
 constexpr int PTT = 3;  // l tiles per wave
 
-template <int V>
-__global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
-{
-    constexpr int NCOL = V == NA_SH ? 1 : 3, NROW = NCOL * 2 * 16;
-    constexpr int SB = NROW * AROW;
-    __shared__ __align__(16) unsigned char sb[SB];
-    __shared__ unsigned lvmax[2 * WLG];
-    __shared__ float lvsc[2 * WLG];
-    constexpr int cgrp[3] = {0, 1, 1};
-    // per block: lcut of the levels, c_l (l - m = 0 .. l_max + 2 - m), row factors
-    __shared__ int slc[WLG];
-    __shared__ float scr[4 * 16 * PTT * 2];
-    __shared__ float srs[2][16 * 4 * PTT];
+// LDS of one whole-theta analysis item: B rows, per-(group, level) maxima and
+// scales, and per item the lcut of the levels, c_l (l - m = 0 .. l_max + 2 -
+// m) and row factors
+template <int V> struct NAnaPL {
+    static constexpr int NCOL = V == NA_SH ? 1 : 3;
+    static constexpr size_t SB = (size_t)NCOL * 2 * 16 * AROW;
+    static constexpr size_t O_LVMAX = SB, O_LVSC = O_LVMAX + 4 * 2 * WLG, O_SLC = O_LVSC + 4 * 2 * WLG;
+    static constexpr size_t O_SCR = O_SLC + 4 * WLG, O_SRS = O_SCR + 4 * (4 * 16 * PTT * 2);
+    static constexpr size_t BYTES = O_SRS + 4 * (2 * 16 * 4 * PTT);
+};
 
-    const int tid = threadIdx.x;
+// One whole-theta analysis item: order mc, level group gy, on the LDS area
+// `lds` (NAnaPL<V>::BYTES), 256 threads; tid = threadIdx.x (as above)
+template <int V>
+__device__ __forceinline__ void nana16p_item(const NAnaArgs &a, int mc, int gy, unsigned char *lds,
+                                             const int tid)
+{
+    constexpr int NCOL = NAnaPL<V>::NCOL, NROW = NCOL * 2 * 16;
+    constexpr int SB = NROW * AROW;
+    unsigned char *sb = lds;
+    unsigned *lvmax = reinterpret_cast<unsigned *>(lds + NAnaPL<V>::O_LVMAX);
+    float *lvsc = reinterpret_cast<float *>(lds + NAnaPL<V>::O_LVSC);
+    constexpr int cgrp[3] = {0, 1, 1};
+    int *slc = reinterpret_cast<int *>(lds + NAnaPL<V>::O_SLC);
+    float *scr = reinterpret_cast<float *>(lds + NAnaPL<V>::O_SCR);
+    float (*srs)[16 * 4 * PTT] = reinterpret_cast<float (*)[16 * 4 * PTT]>(lds + NAnaPL<V>::O_SRS);
+
     const int wave = tid >> 6, lane = tid & 63;
-    const int ng = (a.nlev + WLG - 1) / WLG;
-    int mc, rest;
-    xcd_deal(blockIdx.x, ng, mc, rest);
-    if (mc >= a.n_m) return;
-    const int gy = rest;
     const int m = mc * a.minc;
     const int lev0 = gy * WLG, nlh = min(WLG, a.nlev - lev0);
     const int l0 = a.lstart[mc];
@@ -1554,6 +1581,233 @@ __global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
     }
 }
 
+template <int V>
+__global__ __launch_bounds__(256, 2) void k_nana16p(NAnaArgs a)
+{
+    __shared__ __align__(16) unsigned char lds[NAnaPL<V>::BYTES];
+    const int ng = (a.nlev + WLG - 1) / WLG;
+    int mc, rest;
+    xcd_deal(blockIdx.x, ng, mc, rest);
+    if (mc >= a.n_m) return;
+    nana16p_item<V>(a, mc, rest, lds, threadIdx.x);
+}
+
+// ------------------------------------------------ persistent batch launch
+// LDS of a row item: FP32 twiddles (M, then M + 1 entries, padded to an even
+// count) and two ping-pong buffers of tbr rows of M points.  The tile of the
+// synthesis output (tbr theta rows x n_m orders) is read into the second
+// buffer, which the pre-step has consumed before the first FFT pass writes it.
+__host__ __device__ constexpr size_t pb_row_bytes(int M, int tbr)
+{
+    return ((size_t)2 * M + 2 + (size_t)2 * tbr * M) * sizeof(float2);
+}
+constexpr size_t cmax(size_t x, size_t y) { return x > y ? x : y; }
+constexpr size_t PB_LDS = cmax(cmax(NSynPL<NS_QST>::BYTES, NSynPL<NS_S>::BYTES),
+                               cmax(NAnaPL<NA_QST>::BYTES, NAnaPL<NA_SH>::BYTES));
+// a wait gives up after this many polls of its counter (each a device-scope
+// load and a short sleep: about a second in all), far beyond any real wait
+constexpr int PB_SPIN = 1 << 20;
+
+// one row item: the theta rows th0 .. th0 + tbr of the rows (field x level)
+// of level group g, every comp: the synthesis output F[row][comp][m][theta]
+// through the inverse real FFT (src/fft.f90:386-398) into a grid row held in
+// LDS, and that row through the forward real FFT (scale 1/n_phi, and
+// 1/sin^2(theta) for the comps of osin2_mask) into the theta-blocked F2
+// that the analysis items read.  Row for row the arithmetic of the per-call
+// row FFT kernels (the same passes and twiddles).
+__device__ __forceinline__ void pb_row_item(const PBJob &J, int g, int tb, unsigned char *lds)
+{
+    typedef float2 C;
+    const int M = J.M, TBR = J.tbr, n_m = J.n_m, nth = J.n_theta;
+    C *s_tw = reinterpret_cast<C *>(lds);
+    C *s_twr = s_tw + M;
+    C *A = s_tw + 2 * M + 2;
+    C *B = A + (size_t)TBR * M;
+    const int tid = threadIdx.x;
+    for (int k = tid; k < M; k += 256) s_tw[k] = reinterpret_cast<const C *>(J.tw)[k];
+    for (int k = tid; k <= M; k += 256) s_twr[k] = reinterpret_cast<const C *>(J.twr)[k];
+    // complex FFT of the TBR rows in buffer a (b scratch), radices read per
+    // pass from the job; returns the buffer that holds the result
+    auto fft = [&](C *a, C *b, int sign) {
+        int L = 1;
+        for (int s = 0; s < J.nfac; ++s) {
+            const int R = J.fac[s];
+            switch (R) {
+            case 4: pass_pp<4>(a, b, TBR, M, L, s_tw, sign); break;
+            case 2: pass_pp<2>(a, b, TBR, M, L, s_tw, sign); break;
+            case 3: pass_pp<3>(a, b, TBR, M, L, s_tw, sign); break;
+            default: pass_pp<5>(a, b, TBR, M, L, s_tw, sign); break;
+            }
+            __syncthreads();
+            C *t = a;
+            a = b;
+            b = t;
+            L *= R;
+        }
+        return a;
+    };
+    const int th0 = tb * TBR, nrows = min(TBR, nth - th0);
+    const int r0 = g * WLG, r1 = min(r0 + WLG, J.rows);
+    const float invN = (float)(1.0 / (double)J.N);
+    const int per = 256 / TBR, pr = tid / per, pq = tid - pr * per;
+    for (int r = r0; r < r1; ++r)
+        for (int c = 0; c < J.nc; ++c) {
+            __syncthreads();  // twiddles in; the previous row's buffers read
+            const C *Fp = reinterpret_cast<const C *>(J.F) + (size_t)r * J.f_ld + (size_t)c * n_m * nth + th0;
+            for (int idx = tid; idx < n_m * TBR; idx += 256) {
+                const int m = idx / TBR, t = idx - m * TBR;
+                B[t * n_m + m] = t < nrows ? Fp[(size_t)m * nth + t] : make_float2(0.f, 0.f);
+            }
+            __syncthreads();
+            irfft_pre(B + pr * n_m, (size_t)1, n_m, A + pr * M, M, s_twr, pq, per);
+            __syncthreads();
+            C *res = fft(A, B, +1);
+            res = fft(res, res == A ? B : A, -1);
+            const bool os = (J.osin2_mask >> c) & 1u;
+            C *Fo = reinterpret_cast<C *>(J.F2) + (size_t)r * J.f2_ld;
+            for (int idx = tid; idx < n_m * TBR; idx += 256) {
+                const int k = idx / TBR, t = idx - k * TBR;
+                if (t >= nrows) continue;
+                const int th = th0 + t;
+                float sc = invN;
+                if (os) sc *= (float)J.osin2[th];
+                Fo[(((size_t)c * J.nt16 + (th >> 4)) * n_m + k) * 16 + (th & 15)] =
+                    rfft_post(res + t * M, k, M, s_twr, sc);
+            }
+        }
+}
+
+__device__ __forceinline__ void pb_syn_unit(const PBJob &J, int mc, int g, unsigned char *lds, int tid)
+{
+    if (J.syn_kind == NS_S) {
+        if (J.nrl == 1) nsyn16p_item<NS_S, 1>(J.syn, mc, g, 0, lds, tid);
+        else nsyn16p_item<NS_S, 2>(J.syn, mc, g, 0, lds, tid);
+    } else {
+        if (J.nrl == 1) nsyn16p_item<NS_QST, 1>(J.syn, mc, g, 0, lds, tid);
+        else nsyn16p_item<NS_QST, 2>(J.syn, mc, g, 0, lds, tid);
+    }
+}
+
+__device__ __forceinline__ void pb_ana_unit(const PBJob &J, int mc, int g, unsigned char *lds, int tid)
+{
+    if (J.ana_kind == NA_SH) nana16p_item<NA_SH>(J.ana, mc, g, lds, tid);
+    else nana16p_item<NA_QST>(J.ana, mc, g, lds, tid);
+}
+
+// polls counter c until it reaches need (then an acquire at device scope) or
+// until the wait limit or another workgroup's failure: false
+__device__ bool pb_wait(int *c, int need, int *fail_flag)
+{
+    for (int i = 0; i < PB_SPIN; ++i) {
+        if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            return true;
+        }
+        if (__hip_atomic_load(fail_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+        __builtin_amdgcn_s_sleep(8);
+    }
+    return false;
+}
+
+// XCC_ID hardware register (bits 3:0): the XCD the wave runs on
+__device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)); }
+
+__global__ void k_xcc_ids(int *ids)
+{
+    if (threadIdx.x == 0) ids[blockIdx.x] = xcc_id();
+}
+
+// The work list of a batch, claimed by `gridDim.x` workgroups of 256
+// threads, each from the queue of its XCD in ticket order (a workgroup claims
+// its next ticket while it works on the current item).  A producer item's
+// stores are complete (every thread waits for its own) before the barrier
+// after which it counts; its consumers, on the same XCD, read them through
+// the shared L2.
+__global__ __launch_bounds__(256, 2) void k_pbatch(const PBJob *__restrict__ jobs,
+                                                   const PBItem *__restrict__ items, PBQueues qs,
+                                                   int *ctr, int nctr)
+{
+    __shared__ __align__(16) unsigned char lds[PB_LDS];
+    __shared__ int s_tk[2];
+    __shared__ int s_flag;
+    const int tid = threadIdx.x;
+    const int q = qs.xmap[xcc_id() & (PB_MAXQ - 1)];
+    const int qb = q >= 0 ? qs.off[q] : 0, qe = q >= 0 ? qs.off[q + 1] : 0;
+    if (tid == 0) s_tk[0] = qe > qb ? qb + atomicAdd(&ctr[q], 1) : qe;
+    __syncthreads();
+    int t = __builtin_amdgcn_readfirstlane(s_tk[0]);
+    for (int it = 0; t < qe; ++it) {
+        int nxt = 0;
+        if (tid == 0) nxt = qb + atomicAdd(&ctr[q], 1);
+        const PBItem w = items[t];
+        const PBJob &J = jobs[w.job];
+        if (w.kind != PB_SYN) {
+            if (tid == 0) {
+                // a row item: every synthesis unit of its level group; an
+                // analysis item: every row item of each of its level groups
+                bool ok = true;
+                if (w.kind == PB_ROW) ok = pb_wait(&ctr[J.c_syn + w.g0], J.n_m, &ctr[PB_ERR]);
+                else
+                    for (int g = w.g0; ok && g < w.g0 + w.ng; ++g) ok = pb_wait(&ctr[J.c_row + g], J.ntbr, &ctr[PB_ERR]);
+                if (!ok) atomicOr(&ctr[PB_ERR], 1);
+                s_flag = ok ? 0 : 1;
+            }
+            __syncthreads();
+            if (__builtin_amdgcn_readfirstlane(s_flag)) break;
+        }
+        if (w.kind == PB_ROW) {
+            const PBJob *jp = &J;
+            asm volatile("" : "+s"(jp));
+            pb_row_item(*jp, w.g0, w.tb, lds);
+        } else {
+            for (int u = 0; u < 2; ++u) {
+                const int mc = u ? w.m1 : w.m0;
+                if (mc < 0) continue;
+                for (int g = w.g0; g < w.g0 + w.ng; ++g) {
+                    // the job's fields and everything derived from the
+                    // thread index are computed inside each unit, not
+                    // hoisted out of these loops and held across them
+                    const PBJob *jp = &J;
+                    int ltid = tid;
+                    asm volatile("" : "+s"(jp), "+v"(ltid));
+                    if (w.kind == PB_SYN) pb_syn_unit(*jp, mc, g, lds, ltid);
+                    else pb_ana_unit(*jp, mc, g, lds, ltid);
+                    __syncthreads();  // the LDS is the next unit's
+                }
+            }
+        }
+        // every thread's stores complete (in the XCD's L2) before the item counts
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) {
+            if (w.kind == PB_SYN) {
+                const int nord = w.m1 >= 0 ? 2 : 1;
+                for (int g = w.g0; g < w.g0 + w.ng; ++g) atomicAdd(&ctr[J.c_syn + g], nord);
+            } else if (w.kind == PB_ROW) {
+                atomicAdd(&ctr[J.c_row + w.g0], 1);
+            }
+            s_tk[(it + 1) & 1] = nxt;
+        }
+        __syncthreads();
+        t = __builtin_amdgcn_readfirstlane(s_tk[(it + 1) & 1]);
+    }
+    // the last workgroup out checks that every queue was claimed to its end
+    // (else its items never ran: error) and clears the counters for the next
+    // launch (the error flag stays for the host)
+    __syncthreads();
+    if (tid == 0) s_flag = atomicAdd(&ctr[PB_DONE], 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (s_flag) {
+        __threadfence();
+        // thread i checks ticket i (i < nq) before it clears counters i, i + 256, ...
+        if (tid < qs.nq && __hip_atomic_load(&ctr[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                               qs.off[tid + 1] - qs.off[tid])
+            atomicOr(&ctr[PB_ERR], 2);
+        for (int i = tid; i < nctr; i += 256)
+            if (i != PB_ERR) ctr[i] = 0;
+    }
+}
+
 }  // namespace
 
 int launch_nsyn16(const Plan *p, int kind, const NSynArgs &args, hipStream_t s)
@@ -1695,6 +1949,87 @@ int launch_nana16(const Plan *p, int kind, const NAnaArgs &args, hipStream_t s)
         default: return fail(MHX_E_INVALID, "unknown nonlinear-step analysis kind");
         }
     }
+    MHX_HIP_CHECK(hipGetLastError());
+    return MHX_OK;
+}
+
+// the plan's tables and sizes admit the whole-K synthesis, the whole-theta
+// analysis and row items of 4 theta rows within the persistent launch's LDS
+bool pbatch_supported(const Plan *p)
+{
+    const int ntlx = ((p->l_max + 1) / 2 + 1 + 15) / 16;
+    const bool gfit = 3 * WLG * (2 * (p->l_max + 2) + 2) * 4 <= 6 * 16 * AROW;
+    return p->l_max <= 255 && p->n_theta <= WNT && p->d_syn16x && p->d_ana16x && p->nh <= AKP &&
+           ntlx <= 4 * PTT && gfit && (int)p->fac.size() <= PB_MAXFAC && pb_row_bytes(p->M, 4) <= PB_LDS;
+}
+
+size_t pbatch_row_lds(int M, int tbr) { return pb_row_bytes(M, tbr); }
+size_t pbatch_lds() { return PB_LDS; }
+
+// the plan's part of a job: tables and sizes of the whole-K synthesis and
+// the whole-theta analysis (as their launchers set them), the FP32 FFT
+// factors and twiddles; the job's arrays, counts and kinds are the caller's
+int fill_pbatch_job(const Plan *p, PBJob &j)
+{
+    if (!pbatch_supported(p)) return fail(MHX_E_UNSUPPORTED, "plan does not admit the persistent batch form");
+    NSynArgs &a = j.syn;
+    a.crec = p->d_crec;
+    a.tP = p->d_syn16x;
+    a.tdP = nullptr;
+    a.off = p->d_syn16x_off;
+    a.rowscale = p->d_syn16x_rs;
+    a.lstart = p->d_lstart;
+    a.l_max = p->l_max;
+    a.minc = p->minc;
+    a.n_m = p->n_m;
+    a.nh = p->nh;
+    a.n_theta = p->n_theta;
+    a.nt16 = p->nt16;
+    a.tpb = 4 * WTT;
+    a.ngz = 1;
+    NAnaArgs &c = j.ana;
+    c.crec = p->d_crec;
+    c.tP = p->d_ana16x;
+    c.tdP = nullptr;
+    c.off = p->d_ana16x_off;
+    c.rowscale = p->d_ana16x_rs;
+    c.soff = p->d_ana16x_soff;
+    c.lstart = p->d_lstart;
+    c.l_max = p->l_max;
+    c.m_max = p->m_max;
+    c.minc = p->minc;
+    c.n_m = p->n_m;
+    c.nh = p->nh;
+    c.n_theta = p->n_theta;
+    c.ntb = (p->n_theta + 15) / 16;
+    c.nq32 = p->nq32;
+    c.ngz = 1;
+    j.nrl = p->l_max + 1 <= 128 ? 1 : 2;
+    j.M = p->M;
+    j.N = p->n_phi;
+    j.nfac = (int)p->fac.size();
+    for (int i = 0; i < PB_MAXFAC; ++i) j.fac[i] = i < j.nfac ? p->fac[i] : 1;
+    j.tw = p->d_tw32;
+    j.twr = p->d_twr32;
+    j.osin2 = p->d_osin2;
+    j.n_m = p->n_m;
+    j.n_theta = p->n_theta;
+    j.nt16 = (p->n_theta + 15) / 16;
+    return MHX_OK;
+}
+
+int probe_xcc_ids(int *ids, int n, hipStream_t s)
+{
+    k_xcc_ids<<<dim3((unsigned)n), 64, 0, s>>>(ids);
+    MHX_HIP_CHECK(hipGetLastError());
+    return MHX_OK;
+}
+
+int launch_pbatch(const PBJob *jobs, const PBItem *items, const PBQueues &q, int *ctr, int nctr,
+                  int grid, hipStream_t s)
+{
+    if (q.nq <= 0 || q.off[q.nq] <= 0) return MHX_OK;
+    k_pbatch<<<dim3((unsigned)grid), 256, 0, s>>>(jobs, items, q, ctr, nctr);
     MHX_HIP_CHECK(hipGetLastError());
     return MHX_OK;
 }

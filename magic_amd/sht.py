@@ -66,6 +66,8 @@ _SIGS = {
     "mhx_batch_create": "piip",
     "mhx_batch_execute": "pp",
     "mhx_batch_tuning": "pppppi",
+    "mhx_batch_shapes": "ppppppi",
+    "mhx_batch_status": "p",
     "mhx_batch_destroy": "p",
 }
 _CT = {"P": ctypes.c_void_p, "p": ctypes.c_void_p, "i": ctypes.c_int, "d": ctypes.c_double,
@@ -566,7 +568,8 @@ class _Job(ctypes.Structure):
 
 class Batch:
     """An uneven batch of independent transform jobs run from one captured
-    graph (mhx_batch_*).  ``jobs``: (l_max, kind, inp, out) with ``inp`` and
+    graph (mhx_batch_*): one persistent launch over a work list of all the
+    jobs, or (MHX_BATCH_FORM=graph) the jobs' launches on graph branches.  ``jobs``: (l_max, kind, inp, out) with ``inp`` and
     ``out`` complex128 device tensors of shape (ncomp, nfields, nlev, lm_max),
     ncomp 1 (JOB_SCALAR: s) or 3 (JOB_VECTOR: W, dW, Z in; Q, S, T out).
     Each job is the round trip synthesis -> grids -> analysis of its fields
@@ -603,6 +606,23 @@ class Batch:
         if k < 0:
             _check(k)
         return [tuple(x[i] for x in a) for i in range(min(k, n))]
+
+    def shapes(self):
+        """(form, [(l_max, orders per item, level groups per item, theta rows
+        per row item)]): form 1 is the one persistent launch (the default),
+        0 the per-job launches (MHX_BATCH_FORM=graph)."""
+        n = 16
+        form = ctypes.c_int()
+        a = [(ctypes.c_int * n)() for _ in range(4)]
+        k = lib().mhx_batch_shapes(self.h, ctypes.byref(form), *a, n)
+        if k < 0:
+            _check(k)
+        return form.value, [tuple(x[i] for x in a) for i in range(min(k, n))]
+
+    def status(self):
+        """Waits for the batch's executions so far; raises MhxError if a work
+        item of the persistent launch gave up waiting for its inputs."""
+        _check(lib().mhx_batch_status(self.h))
 
     def close(self):
         if getattr(self, "h", None):

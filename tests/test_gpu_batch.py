@@ -3,7 +3,11 @@ round trip synthesis -> grids -> analysis of its fields, which for
 band-limited inputs on the Gauss grid is the identity (scalar: s -> s;
 vector: W, dW, Z -> Q = l(l+1) W, S = dW, T = Z for l >= 1,
 src/sht_native.f90:114-123, :358-364).  Held to 1e-6 relative L2 per job
-and component; a replay of the batch's graph reproduces it bit for bit."""
+and component, in the persistent launch (default) and the per-job graph
+form; a replay of the batch's graph reproduces it bit for bit, and the two
+forms agree to the rounding of their kernel choices."""
+import os
+
 import numpy as np
 import pytest
 
@@ -61,8 +65,14 @@ def test_batch_round_trips_and_replays():
     tuning = b.tuning()
     assert sorted(t[0] for t in tuning) == [63, 127, 191, 255]
     for _, syn, ana, rows in tuning:
-        assert syn in (1, 2) and ana in (1, 2) and rows in (1, 2, 4, 8)
+        assert syn in (1, 2) and ana in (1, 2) and rows in (1, 2, 4, 8, 16)
+    form, shapes = b.shapes()
+    print("form %d, shapes %s" % (form, shapes), flush=True)
+    assert form == 1
+    for _, orders, groups, rows in shapes:
+        assert orders in (1, 2) and groups in (1, 2, 4) and rows in (4, 8, 16)
     b.execute()
+    b.status()
     first = [j[3].cpu().numpy() for j in jobs]
     for (L, kind, nf, nlev), x, got in zip(specs, host, first):
         want, mask = expected(kind, L, x)
@@ -77,6 +87,52 @@ def test_batch_round_trips_and_replays():
     st.wait_stream(torch.cuda.current_stream())
     b.execute(st)
     torch.cuda.synchronize()
+    b.status()
     for j, f in zip(jobs, first):
         assert np.array_equal(j[3].cpu().numpy(), f)
     b.close()
+
+
+def test_batch_persistent_launch_matches_graph_form():
+    """The same jobs through the one persistent launch and through the
+    per-job launches (MHX_BATCH_FORM=graph): within 2e-7 of each other per
+    job and component (the graph form may pick the chunked Legendre kernels
+    for a group; the persistent items run the whole-K and whole-theta ones),
+    and both within 1e-6 of the identity."""
+    import torch
+    rng = np.random.default_rng(11)
+    dev = torch.device("cuda")
+    specs = [(63, sht.JOB_VECTOR, 4, 17), (127, sht.JOB_SCALAR, 3, 9), (191, sht.JOB_VECTOR, 2, 5),
+             (255, sht.JOB_VECTOR, 1, 11), (255, sht.JOB_SCALAR, 2, 3)]
+    host = []
+    for L, kind, nf, nlev in specs:
+        nc = 3 if kind == sht.JOB_VECTOR else 1
+        host.append(job_input(rng, L, nc, nf, nlev))
+    outs = {}
+    for name in ("graph", "persistent"):
+        jobs = [(L, kind, torch.from_numpy(x).to(dev), torch.zeros(x.shape, dtype=torch.complex128, device=dev))
+                for (L, kind, nf, nlev), x in zip(specs, host)]
+        old = os.environ.pop("MHX_BATCH_FORM", None)
+        if name == "graph":
+            os.environ["MHX_BATCH_FORM"] = "graph"
+        try:
+            b = sht.Batch(jobs)
+        finally:
+            os.environ.pop("MHX_BATCH_FORM", None)
+            if old is not None:
+                os.environ["MHX_BATCH_FORM"] = old
+        assert b.shapes()[0] == (0 if name == "graph" else 1)
+        b.execute()
+        b.status()
+        outs[name] = [j[3].cpu().numpy() for j in jobs]
+        b.close()
+    for (L, kind, nf, nlev), x, g, p in zip(specs, host, outs["graph"], outs["persistent"]):
+        want, mask = expected(kind, L, x)
+        for c in range(x.shape[0]):
+            sel = mask if c > 0 else (mask if kind == sht.JOB_VECTOR else np.ones_like(mask))
+            d = rel_l2(p[c][..., sel], g[c][..., sel])
+            e = rel_l2(p[c][..., sel], want[c][..., sel])
+            print("l_max=%d kind=%d comp=%d: persistent vs graph %.2e, vs identity %.2e" % (L, kind, c, d, e),
+                  flush=True)
+            assert d < 2e-7, (L, kind, c, d)
+            assert e < SPLIT_TOL, (L, kind, c, e)

@@ -3,7 +3,8 @@
 * l_max=255, 384 x 768, 97 levels: the FP64 step against the oracle on two
   whole levels (the per-level oracle composes the reference's calls);
 * the uneven batch: all 64 round-trip jobs (l_max 63..255, 17..65 levels,
-  1..4 fields), every job back to its input within 1e-6, replay bit-exact;
+  1..4 fields) in the one persistent launch, every job back to its input
+  within 1e-6, replay bit-exact;
 * l_max=511, 768 x 1536, 161 levels, l^-5/3 with the l=1 poloidal field
   x100: the FP64 step against the oracle on one level, the split form
   against FP64 on all 161 levels (every field, and per 64-wide l band), the
@@ -121,7 +122,11 @@ def test_config3_full_uneven_batch():
         jobs.append((L, kind, torch.from_numpy(x).to(dev),
                      torch.zeros(x.shape, dtype=torch.complex128, device=dev)))
     b = sht.Batch(jobs)
+    form, shapes = b.shapes()
+    print("config 3: launch form %d, tile shapes per l_max %s" % (form, shapes), flush=True)
+    assert form == 1
     b.execute()
+    b.status()
     first = []
     worst = 0.0
     for (L, kind, nf, nlev), x, j in zip(specs, host, jobs):
@@ -144,6 +149,7 @@ def test_config3_full_uneven_batch():
     st.wait_stream(torch.cuda.current_stream())
     b.execute(st)
     torch.cuda.synchronize()
+    b.status()
     for j, f in zip(jobs, first):
         assert np.array_equal(j[3].cpu().numpy(), f)
     b.close()
