@@ -14,7 +14,9 @@
 // whole launch (autotuner, coordinate descent over the l_max groups):
 // orders per synthesis/analysis item (one m, or the pair m, m_max - m whose
 // l ranges add up to l_max + 1), level groups of 8 rows per item (1, 2, 4),
-// theta rows per row item (4, 8, 16 as the LDS allows).
+// and, where the row FFTs run on the generic LDS passes, theta rows per row
+// item (4, 8, 16 as the LDS allows; on wave slots, n_phi = 192, 384, 576,
+// 768, one theta row per slot: 32, 16, 8, 8).
 //
 // Graph form (MHX_BATCH_FORM=graph, or a plan the persistent kernel does not
 // admit): every job's four launches (Legendre synthesis, inverse row FFTs to
@@ -449,7 +451,8 @@ int autotune_p(mhx_batch *b)
     for (Group &g : b->groups) {
         g.fold = 1;
         g.lpi = 1;
-        g.tbr = tbr_fits(&g.plan->p, 8) ? 8 : 4;
+        const int ns = pbatch_slot_rows(g.plan->p.M);
+        g.tbr = ns ? ns : (tbr_fits(&g.plan->p, 8) ? 8 : 4);
     }
     int rc;
     float best;
@@ -472,6 +475,7 @@ int autotune_p(mhx_batch *b)
         }
         g.fold = bf;
         g.lpi = bl;
+        if (pbatch_slot_rows(g.plan->p.M)) continue;  // slot row items: theta rows fixed
         const int t0 = g.tbr;
         int bt = t0;
         for (int tbr : {4, 8, 16}) {

@@ -335,6 +335,95 @@ template <typename C> __device__ void load_twiddles(C *s_tw, C *s_twr, const FFT
     for (int k = threadIdx.x; k <= f.M; k += blockDim.x) s_twr[k] = twr[k];
 }
 
+// ---------------------------------------------------------------------------
+// Wave-slot FFTs: a slot of Q lanes of one wave transforms one row of M
+// points held in LDS, each lane holding P points.
+// slots of 12 points per lane pad their rows by one complex per 32 (index
+// i at i + i/32): the radix-12 pass writes 12 consecutive points per lane, a
+// 12-point lane stride that otherwise maps every 8th lane to the same LDS
+// banks (measured: 1.364 -> 1.327 ms at M = 384, 11.1 -> 10.5 ms at M = 768;
+// at 24 points per lane the padding cost 4 %, so M = 1536 is not padded)
+template <int P> __device__ __forceinline__ int pidx(int i) { return P == 12 ? i + (i >> 5) : i; }
+template <int M, int P> constexpr int padded_row() { return P == 12 ? M + (M >> 5) + 1 : M; }
+
+template <int R, int L, int M, int P, typename C>
+__device__ __forceinline__ void slot_pass(C *buf, int t, const C *tw, int sign)
+{
+    constexpr int Q = M / P, NB = P / R;  // lanes per slot, butterflies per lane
+    static_assert(P % R == 0 && Q * P == M, "pass does not tile the slot");
+    constexpr int T = M / R, tstride = M / (L * R);
+    // k = (t + Q b) mod L from t mod L (t < Q: no division when L >= Q)
+    const int tm = L >= Q ? t : t % L;
+    auto kof = [&](int b) {
+        const int k = tm + (Q * b) % L;
+        return k >= L ? k - L : k;
+    };
+    C x[NB][R];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = t + Q * b;
+        const int k = kof(b);
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            C v = buf[pidx<P>(j + s * T)];
+            if constexpr (L > 1) {
+                if (s > 0) {
+                    const int ti = s * k * tstride;
+                    C w = tw[ti + (ti >> 3)];  // table padded by one entry per 8 (banks)
+                    if (sign > 0) w = cconj(w);
+                    v = cmul(v, w);
+                }
+            }
+            x[b][s] = v;
+        }
+    }
+    wave_order();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = t + Q * b;
+        const int k = kof(b);
+        Dft<R, C>::run(x[b], sign);
+        const int o = (j - k) * R + k;
+#pragma unroll
+        for (int s = 0; s < R; ++s) buf[pidx<P>(o + s * L)] = x[b][s];
+    }
+    wave_order();
+}
+
+// M = 3 4^A 2^B = 12 4^(A-1) 2^B: a radix-12 pass (no twiddles), A - 1
+// radix-4 passes, B radix-2 passes; Q = min(M / 12, 64) lanes of P points
+constexpr int pow4(int a) { return a == 0 ? 1 : 4 * pow4(a - 1); }
+template <int A, int B> struct SlotFFT {
+    static constexpr int M = 3 * pow4(A) * (B ? 2 : 1);
+    static constexpr int Q = M / 12 <= 64 ? M / 12 : 64, P = M / Q;
+    // threads of the grid-stage block: six slots, at least one wave
+    static constexpr int NTHR = (6 * Q < 64 ? 64 : 6 * Q + 63) / 64 * 64;
+    static_assert(Q * P == M && P % 12 == 0 && 64 % Q == 0, "unsupported slot length");
+    template <int I, typename C>
+    __device__ __forceinline__ static void radix4(C *buf, int t, const C *tw, int sign)
+    {
+        if constexpr (I < A) {
+            slot_pass<4, 12 * pow4(I - 1), M, P>(buf, t, tw, sign);
+            radix4<I + 1>(buf, t, tw, sign);
+        }
+    }
+    template <typename C> __device__ __forceinline__ static void run(C *buf, int t, const C *tw, int sign)
+    {
+        slot_pass<12, 1, M, P>(buf, t, tw, sign);
+        radix4<1>(buf, t, tw, sign);
+        if constexpr (B) slot_pass<2, 12 * pow4(A - 1), M, P>(buf, t, tw, sign);
+    }
+};
+
+// the passes of a slot FFT of length M = 12 R1 R2 ... (Ri in 2, 3, 4): a
+// radix-12 pass, then the listed radices in order, 12 points per lane
+template <int M, int L, int R, int... Rest, typename C>
+__device__ __forceinline__ void slot_chain(C *buf, int t, const C *tw, int sign)
+{
+    slot_pass<R, L, M, 12>(buf, t, tw, sign);
+    if constexpr (sizeof...(Rest) > 0) slot_chain<M, L * R, Rest...>(buf, t, tw, sign);
+}
+
 }  // namespace
 
 }  // namespace mhx

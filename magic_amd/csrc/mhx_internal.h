@@ -421,6 +421,9 @@ struct PBQueues {
 constexpr int PB_DONE = PB_MAXQ, PB_ERR = PB_MAXQ + 1, PB_CTR0 = PB_MAXQ + 2;
 bool pbatch_supported(const Plan *p);
 size_t pbatch_row_lds(int M, int tbr);
+// theta rows per row item where the row FFTs run on wave slots (0: the
+// generic LDS FFT, tbr chosen by the autotuner)
+int pbatch_slot_rows(int M);
 size_t pbatch_lds();
 int fill_pbatch_job(const Plan *p, PBJob &j);
 // XCC_ID of each of n workgroups (device array ids), for the queue map

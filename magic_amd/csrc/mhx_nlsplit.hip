@@ -1677,6 +1677,123 @@ __device__ __forceinline__ void pb_row_item(const PBJob &J, int g, int tb, unsig
         }
 }
 
+// Row item on wave slots, for M = 12 R1 R2 ... (the batch's n_phi = 192,
+// 384, 576, 768): one slot of Q = M / 12 lanes (64 / Q slots per wave) per
+// theta row holds the row in LDS (padded by one point per 32) and runs its
+// inverse and its forward FFT in place with no block barrier between the
+// passes.  The block's NS = 4 (64 / Q) theta rows of one row and comp: the
+// Fourier tile (orders x theta rows) read in theta runs into LDS, the
+// pre-step per slot, both FFTs, the post-step back into the tile, the tile
+// written in theta runs of the theta-blocked F2.
+template <int M> struct PBSlot;
+template <> struct PBSlot<96> {
+    template <typename C> __device__ static void run(C *z, int t, const C *tw, int sign)
+    {
+        slot_chain<96, 1, 12, 4, 2>(z, t, tw, sign);
+    }
+};
+template <> struct PBSlot<192> {
+    template <typename C> __device__ static void run(C *z, int t, const C *tw, int sign)
+    {
+        slot_chain<192, 1, 12, 4, 4>(z, t, tw, sign);
+    }
+};
+template <> struct PBSlot<288> {
+    template <typename C> __device__ static void run(C *z, int t, const C *tw, int sign)
+    {
+        slot_chain<288, 1, 12, 4, 3, 2>(z, t, tw, sign);
+    }
+};
+template <> struct PBSlot<384> {
+    template <typename C> __device__ static void run(C *z, int t, const C *tw, int sign)
+    {
+        slot_chain<384, 1, 12, 4, 4, 2>(z, t, tw, sign);
+    }
+};
+
+// theta rows per slot row item (0: M has no slot form here)
+__host__ __device__ constexpr int pb_slot_rows(int M)
+{
+    return (M == 96 || M == 192 || M == 288 || M == 384) ? 4 * (64 / (M / 12)) : 0;
+}
+// LDS of a slot row item: padded twiddles, twr, NS padded rows, the tile
+__host__ __device__ constexpr size_t pb_slot_bytes(int M)
+{
+    return ((size_t)M + M / 8 + M + 2 + (size_t)pb_slot_rows(M) * (M + M / 32 + 1) +
+            (size_t)(2 * M / 3) * (pb_slot_rows(M) + 1)) * sizeof(float2);
+}
+static_assert(pb_slot_bytes(384) <= PB_LDS && pb_slot_bytes(288) <= PB_LDS && pb_slot_bytes(192) <= PB_LDS &&
+              pb_slot_bytes(96) <= PB_LDS, "slot row item exceeds the launch's LDS");
+
+template <int M>
+__device__ __forceinline__ void pb_row_slots(const PBJob &J, int g, int tb, unsigned char *lds, const int tid)
+{
+    typedef float2 C;
+    constexpr int P = 12, Q = M / P, SPW = 64 / Q, NS = 4 * SPW, MP = padded_row<M, P>();
+    constexpr int TS = NS + 1;  // tile stride per order (theta fastest)
+    static_assert(NS == pb_slot_rows(M), "slot rows");
+    C *s_tw = reinterpret_cast<C *>(lds);
+    C *s_twr = s_tw + M + M / 8;
+    C *rows = s_twr + M + 2;
+    C *tile = rows + NS * MP;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int sw = lane / Q, t = lane - sw * Q, slot = wave * SPW + sw;
+    const bool act = sw < SPW;
+    {
+        const C *tw = reinterpret_cast<const C *>(J.tw);
+        const C *twr = reinterpret_cast<const C *>(J.twr);
+        for (int e = tid; e < M; e += 256) s_tw[e + (e >> 3)] = tw[e];
+        for (int e = tid; e <= M; e += 256) s_twr[e] = twr[e];
+    }
+    const int n_m = J.n_m, nth = J.n_theta;
+    const int th0 = tb * NS, nrows = min(NS, nth - th0);
+    const int r0 = g * WLG, r1 = min(r0 + WLG, J.rows);
+    const float invN = (float)(1.0 / (double)J.N);
+    C *z = rows + (act ? slot : 0) * MP;
+    for (int r = r0; r < r1; ++r)
+        for (int c = 0; c < J.nc; ++c) {
+            __syncthreads();  // twiddles in; the previous tile written out
+            const C *Fp = reinterpret_cast<const C *>(J.F) + (size_t)r * J.f_ld + (size_t)c * n_m * nth + th0;
+            for (int idx = tid; idx < n_m * NS; idx += 256) {
+                const int m = idx / NS, tt = idx - m * NS;
+                tile[m * TS + tt] = tt < nrows ? Fp[(size_t)m * nth + tt] : make_float2(0.f, 0.f);
+            }
+            __syncthreads();
+            if (act) {
+                // Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/N} (X_k - conj X_{M-k})
+                for (int k = t; k <= M / 2; k += Q) {
+                    const int k2 = M - k;
+                    C a = k < n_m ? tile[k * TS + slot] : make_float2(0.f, 0.f);
+                    C b = k2 < n_m ? tile[k2 * TS + slot] : make_float2(0.f, 0.f);
+                    if (k == 0) {
+                        a.y = 0.f;
+                        b.y = 0.f;
+                    }
+                    z[pidx<P>(k)] = cadd(cadd(a, cconj(b)), cmuli(cmul(cconj(s_twr[k]), csub(a, cconj(b))), 1));
+                    if (k2 < M && k2 != k)
+                        z[pidx<P>(k2)] = cadd(cadd(b, cconj(a)), cmuli(cmul(cconj(s_twr[k2]), csub(b, cconj(a))), 1));
+                }
+                wave_order();
+                PBSlot<M>::run(z, t, s_tw, +1);
+                PBSlot<M>::run(z, t, s_tw, -1);
+            }
+            __syncthreads();  // every slot has read its tile column
+            if (act) {
+                float sc = invN;
+                if (((J.osin2_mask >> c) & 1u) && slot < nrows) sc *= (float)J.osin2[th0 + slot];
+                for (int k = t; k < n_m; k += Q) tile[k * TS + slot] = rfft_post_padded<P>(z, k, M, s_twr, sc);
+            }
+            __syncthreads();
+            C *Fo = reinterpret_cast<C *>(J.F2) + (size_t)r * J.f2_ld;
+            for (int idx = tid; idx < n_m * NS; idx += 256) {
+                const int k = idx / NS, tt = idx - k * NS;
+                if (tt >= nrows) continue;
+                const int th = th0 + tt;
+                Fo[(((size_t)c * J.nt16 + (th >> 4)) * n_m + k) * 16 + (th & 15)] = tile[k * TS + tt];
+            }
+        }
+}
+
 __device__ __forceinline__ void pb_syn_unit(const PBJob &J, int mc, int g, unsigned char *lds, int tid)
 {
     if (J.syn_kind == NS_S) {
@@ -1757,8 +1874,15 @@ __global__ __launch_bounds__(256, 2) void k_pbatch(const PBJob *__restrict__ job
         }
         if (w.kind == PB_ROW) {
             const PBJob *jp = &J;
-            asm volatile("" : "+s"(jp));
-            pb_row_item(*jp, w.g0, w.tb, lds);
+            int ltid = tid;
+            asm volatile("" : "+s"(jp), "+v"(ltid));
+            switch (jp->M) {
+            case 96: pb_row_slots<96>(*jp, w.g0, w.tb, lds, ltid); break;
+            case 192: pb_row_slots<192>(*jp, w.g0, w.tb, lds, ltid); break;
+            case 288: pb_row_slots<288>(*jp, w.g0, w.tb, lds, ltid); break;
+            case 384: pb_row_slots<384>(*jp, w.g0, w.tb, lds, ltid); break;
+            default: pb_row_item(*jp, w.g0, w.tb, lds); break;
+            }
         } else {
             for (int u = 0; u < 2; ++u) {
                 const int mc = u ? w.m1 : w.m0;
@@ -1964,6 +2088,7 @@ bool pbatch_supported(const Plan *p)
 }
 
 size_t pbatch_row_lds(int M, int tbr) { return pb_row_bytes(M, tbr); }
+int pbatch_slot_rows(int M) { return pb_slot_rows(M); }
 size_t pbatch_lds() { return PB_LDS; }
 
 // the plan's part of a job: tables and sizes of the whole-K synthesis and

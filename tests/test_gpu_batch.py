@@ -65,12 +65,12 @@ def test_batch_round_trips_and_replays():
     tuning = b.tuning()
     assert sorted(t[0] for t in tuning) == [63, 127, 191, 255]
     for _, syn, ana, rows in tuning:
-        assert syn in (1, 2) and ana in (1, 2) and rows in (1, 2, 4, 8, 16)
+        assert syn in (1, 2) and ana in (1, 2) and rows in (1, 2, 4, 8, 16, 32)
     form, shapes = b.shapes()
     print("form %d, shapes %s" % (form, shapes), flush=True)
     assert form == 1
     for _, orders, groups, rows in shapes:
-        assert orders in (1, 2) and groups in (1, 2, 4) and rows in (4, 8, 16)
+        assert orders in (1, 2) and groups in (1, 2, 4) and rows in (4, 8, 16, 32)
     b.execute()
     b.status()
     first = [j[3].cpu().numpy() for j in jobs]
@@ -95,10 +95,11 @@ def test_batch_round_trips_and_replays():
 
 def test_batch_persistent_launch_matches_graph_form():
     """The same jobs through the one persistent launch and through the
-    per-job launches (MHX_BATCH_FORM=graph): within 2e-7 of each other per
-    job and component (the graph form may pick the chunked Legendre kernels
-    for a group; the persistent items run the whole-K and whole-theta ones),
-    and both within 1e-6 of the identity."""
+    per-job launches (MHX_BATCH_FORM=graph): within 4e-7 of each other per
+    job and component, and both within 1e-6 of the identity.  The forms
+    round differently: wave-slot row FFTs (radix 12 first) against the
+    generic LDS passes, and the graph form may pick the chunked Legendre
+    kernels for a group; each form lies within about 2e-7 of the identity."""
     import torch
     rng = np.random.default_rng(11)
     dev = torch.device("cuda")
@@ -134,5 +135,5 @@ def test_batch_persistent_launch_matches_graph_form():
             e = rel_l2(p[c][..., sel], want[c][..., sel])
             print("l_max=%d kind=%d comp=%d: persistent vs graph %.2e, vs identity %.2e" % (L, kind, c, d, e),
                   flush=True)
-            assert d < 2e-7, (L, kind, c, d)
+            assert d < 4e-7, (L, kind, c, d)
             assert e < SPLIT_TOL, (L, kind, c, e)
