@@ -1745,20 +1745,36 @@ __device__ __forceinline__ void pb_row_slots(const PBJob &J, int g, int tb, unsi
         for (int e = tid; e < M; e += 256) s_tw[e + (e >> 3)] = tw[e];
         for (int e = tid; e <= M; e += 256) s_twr[e] = twr[e];
     }
-    const int n_m = J.n_m, nth = J.n_theta;
+    const int n_m = J.n_m, nth = J.n_theta, nc = J.nc;
     const int th0 = tb * NS, nrows = min(NS, nth - th0);
     const int r0 = g * WLG, r1 = min(r0 + WLG, J.rows);
     const float invN = (float)(1.0 / (double)J.N);
     C *z = rows + (act ? slot : 0) * MP;
+    // the tile of step (r, c) in registers: element idx = tid + 256 i is
+    // (order idx / NS, theta row idx % NS); the next step's tile is loaded
+    // while the slots work on the current one
+    constexpr int NV = ((2 * M / 3) * NS + 255) / 256;
+    C v[NV];
+    auto load = [&](int r, int c) {
+        const C *Fp = reinterpret_cast<const C *>(J.F) + (size_t)r * J.f_ld + (size_t)c * n_m * nth + th0;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int idx = tid + 256 * i, m = idx / NS, tt = idx - m * NS;
+            v[i] = (m < n_m && tt < nrows) ? Fp[(size_t)m * nth + tt] : make_float2(0.f, 0.f);
+        }
+    };
+    load(r0, 0);
     for (int r = r0; r < r1; ++r)
-        for (int c = 0; c < J.nc; ++c) {
+        for (int c = 0; c < nc; ++c) {
             __syncthreads();  // twiddles in; the previous tile written out
-            const C *Fp = reinterpret_cast<const C *>(J.F) + (size_t)r * J.f_ld + (size_t)c * n_m * nth + th0;
-            for (int idx = tid; idx < n_m * NS; idx += 256) {
-                const int m = idx / NS, tt = idx - m * NS;
-                tile[m * TS + tt] = tt < nrows ? Fp[(size_t)m * nth + tt] : make_float2(0.f, 0.f);
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const int idx = tid + 256 * i, m = idx / NS, tt = idx - m * NS;
+                if (m < n_m) tile[m * TS + tt] = v[i];
             }
             __syncthreads();
+            if (c + 1 < nc) load(r, c + 1);
+            else if (r + 1 < r1) load(r + 1, 0);
             if (act) {
                 // Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/N} (X_k - conj X_{M-k})
                 for (int k = t; k <= M / 2; k += Q) {
@@ -1811,15 +1827,17 @@ __device__ __forceinline__ void pb_ana_unit(const PBJob &J, int mc, int g, unsig
     else nana16p_item<NA_QST>(J.ana, mc, g, lds, tid);
 }
 
-// polls counter c until it reaches need (then an acquire at device scope) or
-// until the wait limit or another workgroup's failure: false
+// polls counter c until it reaches need, or until the wait limit or another
+// workgroup's failure: false.  No cache maintenance follows: the producers
+// ran on this XCD and completed their stores into its L2 before counting,
+// and no CU holds an earlier copy of the lines in its L1 (a level group's
+// Fourier rows are read only by its consumers, only after all its producers
+// have counted; the L1 starts each launch empty); the block barrier after
+// the wait orders the consumer's loads after it.
 __device__ bool pb_wait(int *c, int need, int *fail_flag)
 {
     for (int i = 0; i < PB_SPIN; ++i) {
-        if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            return true;
-        }
+        if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need) return true;
         if (__hip_atomic_load(fail_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
         __builtin_amdgcn_s_sleep(8);
     }
@@ -2084,7 +2102,8 @@ bool pbatch_supported(const Plan *p)
     const int ntlx = ((p->l_max + 1) / 2 + 1 + 15) / 16;
     const bool gfit = 3 * WLG * (2 * (p->l_max + 2) + 2) * 4 <= 6 * 16 * AROW;
     return p->l_max <= 255 && p->n_theta <= WNT && p->d_syn16x && p->d_ana16x && p->nh <= AKP &&
-           ntlx <= 4 * PTT && gfit && (int)p->fac.size() <= PB_MAXFAC && pb_row_bytes(p->M, 4) <= PB_LDS;
+           ntlx <= 4 * PTT && gfit && (int)p->fac.size() <= PB_MAXFAC && pb_row_bytes(p->M, 4) <= PB_LDS &&
+           (!pb_slot_rows(p->M) || 3 * p->n_m <= 2 * p->M);  // slot row items: tiles of 2 M / 3 orders
 }
 
 size_t pbatch_row_lds(int M, int tbr) { return pb_row_bytes(M, tbr); }
