@@ -138,6 +138,19 @@ int mhx_toraxi_to_spat(mhx_plan *p, const double *fl_ax, double *ft,
 int mhx_get_br_v_bcs(mhx_plan *p, const double *br, const double *vt, const double *vp,
                      double omega, double O_r_E_2, double O_rho, double *br_vt_lm,
                      double *br_vp_lm);
+/* src/nonlinear_bcs.f90:71-118 (get_b_nl_bcs, called from
+ * src/step_time.f90:619 and :631 on the products above): the nonlinear term
+ * of the magnetic boundary condition of a conducting mantle, bc =
+ * MHX_BC_CMB ('CMB'): b_nl_bc = -fac br_vt_lm, aj_nl_bc = -fac br_vp_lm with
+ * fac = conductance_ma prmag; or inner core, MHX_BC_ICB ('ICB'): aj_nl_bc =
+ * -fac br_vp_lm with fac = sigma_ratio prmag (b_nl_bc and br_vt_lm not
+ * used, may be null).  Entry lm = 1 (l = 0) is zero.  cond_or_sigma is
+ * conductance_ma or sigma_ratio.  Complex lm_max arrays, host or device;
+ * another bc is MHX_E_INVALID (the reference's abortRun). */
+#define MHX_BC_CMB 0
+#define MHX_BC_ICB 1
+int mhx_get_b_nl_bcs(mhx_plan *p, int bc, const double *br_vt_lm, const double *br_vp_lm,
+                     double cond_or_sigma, double prmag, double *b_nl_bc, double *aj_nl_bc);
 
 /* ---- batched over radial levels (device pointers, one launch per stage) ----
  * Level i of an input/output coefficient array starts at ptr + 2*i*ld_lm

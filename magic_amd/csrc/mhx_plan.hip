@@ -829,6 +829,42 @@ int mhx_get_br_v_bcs(mhx_plan *h, const double *br, const double *vt, const doub
     return analysis_call(P, ANA_SPHTOR, 2, gr, os, P->l_max, 2, out);
 }
 
+// nonlinear boundary terms: -fac x per complex entry, lm = 1 (l = 0) zero
+__global__ void k_b_nl_bcs(int n, const double *__restrict__ a, const double *__restrict__ b, double fac,
+                           double *__restrict__ bo, double *__restrict__ jo)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool l0 = i < 2;
+    if (bo) bo[i] = l0 ? 0.0 : -fac * a[i];
+    jo[i] = l0 ? 0.0 : -fac * b[i];
+}
+
+int mhx_get_b_nl_bcs(mhx_plan *h, int bc, const double *br_vt_lm, const double *br_vp_lm,
+                     double cond_or_sigma, double prmag, double *b_nl_bc, double *aj_nl_bc)
+{
+    PLAN_OR_FAIL(h);
+    if (bc != MHX_BC_CMB && bc != MHX_BC_ICB) return fail(MHX_E_INVALID, "Wrong input of bc into get_b_nl_bcs");
+    const bool cmb = bc == MHX_BC_CMB;
+    if (!br_vp_lm || !aj_nl_bc || (cmb && (!br_vt_lm || !b_nl_bc))) return fail(MHX_E_INVALID, "null array");
+    const size_t n = 2 * (size_t)P->lm_max;
+    const double *a = cmb ? stage_in(P, br_vt_lm, 0, n) : nullptr;
+    const double *b = stage_in(P, br_vp_lm, 1, n);
+    double *out[2] = {aj_nl_bc, b_nl_bc};
+    double *dst[2];
+    for (int k = 0; k < (cmb ? 2 : 1); ++k)
+        dst[k] = is_device_ptr(out[k]) ? out[k] : P->ws_C + (size_t)(4 + k) * n;
+    const double fac = cond_or_sigma * prmag;  // src/nonlinear_bcs.f90:90, :104
+    k_b_nl_bcs<<<(unsigned)((n + 255) / 256), 256, 0, P->stream>>>((int)n, a, b, fac, cmb ? dst[1] : nullptr,
+                                                                   dst[0]);
+    MHX_HIP_CHECK(hipGetLastError());
+    for (int k = 0; k < (cmb ? 2 : 1); ++k)
+        if (dst[k] != out[k])
+            MHX_HIP_CHECK(hipMemcpyAsync(out[k], dst[k], n * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    MHX_HIP_CHECK(hipStreamSynchronize(P->stream));
+    return MHX_OK;
+}
+
 int mhx_axi_to_spat(mhx_plan *h, const double *fl_ax, double *f)
 {
     PLAN_OR_FAIL(h);

@@ -47,6 +47,7 @@ _SIGS = {
     "mhx_spat_to_qst": "Pppppppi",
     "mhx_spat_to_sphertor": "Pppppi",
     "mhx_get_br_v_bcs": "Ppppdddpp",
+    "mhx_get_b_nl_bcs": "Pippddpp",
     "mhx_axi_to_spat": "Ppp",
     "mhx_toraxi_to_spat": "Ppppi",
     "mhx_torpol_to_spat_batch": "Pipppzpppzpp",
@@ -278,6 +279,18 @@ class Plan:
                                         _ptr(self._g(vp)), float(omega), float(O_r_E_2),
                                         float(O_rho), _ptr(a), _ptr(b)))
         return a, b
+
+    def get_b_nl_bcs(self, bc, br_vt_lm, br_vp_lm, cond_or_sigma, prmag):
+        """(b_nl_bc, aj_nl_bc): the nonlinear term of the magnetic boundary
+        condition at bc 'CMB' (cond_or_sigma = conductance_ma) or 'ICB'
+        (sigma_ratio; b_nl_bc None) (src/nonlinear_bcs.f90:71-118)."""
+        code = {"CMB": BC_CMB, "ICB": BC_ICB}[bc]
+        bo, jo = self.coef(), self.coef()
+        a = self._c(br_vt_lm, self.lm_max) if code == BC_CMB else None
+        _check(self._L.mhx_get_b_nl_bcs(self.h, code, _ptr(a) if a is not None else None,
+                                        _ptr(self._c(br_vp_lm, self.lm_max)), float(cond_or_sigma),
+                                        float(prmag), _ptr(bo) if code == BC_CMB else None, _ptr(jo)))
+        return (bo if code == BC_CMB else None), jo
 
     def axi_to_spat(self, fl_ax):
         f = np.zeros(self.n_theta)
@@ -559,6 +572,7 @@ def rloop(plan, phys, nlev, radial, fields, outputs, dtrkc, dthkc, ld_lm=None, s
 
 
 JOB_SCALAR, JOB_VECTOR = 0, 1
+BC_CMB, BC_ICB = 0, 1
 
 
 class _Job(ctypes.Structure):

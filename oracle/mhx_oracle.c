@@ -802,6 +802,25 @@ void orc_get_br_v_bcs(const orc_plan *p, const double *br, const double *vt, con
 }
 
 /* ------------------------------------------------------------------ */
+/* src/nonlinear_bcs.f90:71-118 (get_b_nl_bcs): the nonlinear term of the
+   magnetic boundary condition of a conducting mantle ('CMB', fac =
+   conductance_ma prmag, :88-98) or inner core ('ICB', fac = sigma_ratio
+   prmag, :100-110), lm = 1 (l = 0) zero. */
+int orc_get_b_nl_bcs(int lm_max, int bc, const double *br_vt_lm, const double *br_vp_lm,
+                     double cond_or_sigma, double prmag, double *b_nl_bc, double *aj_nl_bc)
+{
+    if (bc != 0 && bc != 1) return -1;
+    const double fac = cond_or_sigma * prmag;
+    for (int lm = 0; lm < lm_max; ++lm)
+        for (int ri = 0; ri < 2; ++ri) {
+            const size_t i = 2 * (size_t)lm + ri;
+            if (bc == 0) b_nl_bc[i] = lm == 0 ? 0.0 : -fac * br_vt_lm[i];
+            aj_nl_bc[i] = lm == 0 ? 0.0 : -fac * br_vp_lm[i];
+        }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* The radial loop of any physics switches, one interior level (nBc = 0).
    Grid components in[ORC_RI_*] and products out[ORC_RO_*] are n_phi x
    n_theta grids (theta fastest, scrambled rows); absent components are NULL.

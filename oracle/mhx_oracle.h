@@ -123,6 +123,12 @@ void orc_courant(const orc_plan *p, double or2, double or4, double orho1, double
 void orc_get_br_v_bcs(const orc_plan *p, const double *br, const double *vt, const double *vp,
                       double omega, double O_r_E_2, double O_rho, double *br_vt_lm,
                       double *br_vp_lm);
+/* src/nonlinear_bcs.f90:71-118 (get_b_nl_bcs): bc 0 'CMB': b_nl_bc =
+   -fac br_vt_lm, aj_nl_bc = -fac br_vp_lm; bc 1 'ICB': aj_nl_bc = -fac
+   br_vp_lm (b_nl_bc untouched); fac = cond_or_sigma * prmag; lm = 1 zero.
+   Complex arrays of lm_max; returns -1 for another bc. */
+int orc_get_b_nl_bcs(int lm_max, int bc, const double *br_vt_lm, const double *br_vp_lm,
+                     double cond_or_sigma, double prmag, double *b_nl_bc, double *aj_nl_bc);
 
 /* ---- the radial loop of any physics switches, one interior level ---- */
 /* grid components (the reference's grid_space_arrays_t fields) */

@@ -51,6 +51,7 @@ def lib():
         L.orc_get_td.argtypes = [P, I, D, D, D, D, ctypes.POINTER(dp), ctypes.POINTER(dp)]
         L.orc_courant.argtypes = [P, D, D, D, D, D, D, D, D, ctypes.POINTER(dp), dp, dp]
         L.orc_get_br_v_bcs.argtypes = [P, dp, dp, dp, D, D, D, dp, dp]
+        L.orc_get_b_nl_bcs.argtypes = [I, I, dp, dp, D, D, dp, dp]
         ip = ctypes.POINTER(ctypes.c_int)
         L.orc_rl_products.argtypes = [P, ip, dp, dp, ctypes.POINTER(dp), ctypes.POINTER(dp)]
         L.orc_rl_courant.argtypes = [P, I, dp, dp, ctypes.POINTER(dp), dp, dp]
@@ -289,6 +290,18 @@ class Plan:
                                  _ptr(self._g(vp)), float(omega), float(O_r_E_2), float(O_rho),
                                  _ptr(a), _ptr(b))
         return a, b
+
+    def get_b_nl_bcs(self, bc, br_vt_lm, br_vp_lm, cond_or_sigma, prmag):
+        """(b_nl_bc, aj_nl_bc) of src/nonlinear_bcs.f90 get_b_nl_bcs, bc
+        'CMB' or 'ICB' (b_nl_bc None at the ICB)."""
+        code = {"CMB": 0, "ICB": 1}[bc]
+        a = np.ascontiguousarray(br_vt_lm, dtype=np.complex128)
+        b = np.ascontiguousarray(br_vp_lm, dtype=np.complex128)
+        bo, jo = self.coef(), self.coef()
+        rc = self._L.orc_get_b_nl_bcs(int(self.lm_max), code, _ptr(a), _ptr(b), float(cond_or_sigma),
+                                      float(prmag), _ptr(bo), _ptr(jo))
+        assert rc == 0
+        return (bo if code == 0 else None), jo
 
 
     # ---- the radial loop of any physics switches, one interior level

@@ -11,7 +11,7 @@
 ! (src/get_td.f90, module nonlinear_lm_mod) in mode td and its Courant
 ! step limits (src/courant.f90) on the step's grids in mode courant, and
 ! the boundary products of a conducting mantle or inner core
-! (src/nonlinear_bcs.f90, get_br_v_bcs) in mode brv; mode rloop chains the
+! (src/nonlinear_bcs.f90, get_br_v_bcs and get_b_nl_bcs) in mode brv; mode rloop chains the
 ! three for the bulk levels of the radial loop (src/rIter.f90:188-470).
 !
 ! Usage:  ref_harness MODE l_max minc nalias infile outfile [nlev nrep]
@@ -32,10 +32,11 @@ program ref_harness
    use horizontal_data
    use radial_functions, only: or1, or2, or3, or4, orho1, orho2, beta, r, otemp1, &
        &                       visc, lambda, epscProf, l_R
-   use physical_parameters, only: LFfac, n_r_LCR, CorFac, epsc, opm
+   use physical_parameters, only: LFfac, n_r_LCR, CorFac, epsc, opm, conductance_ma, &
+       &                          prmag, sigma_ratio
    use num_param, only: delxr2, delxh2
    use courant_mod, only: courant
-   use nonlinear_bcs, only: get_br_v_bcs
+   use nonlinear_bcs, only: get_br_v_bcs, get_b_nl_bcs
    use fields, only: w_Rloc, dw_Rloc, ddw_Rloc, z_Rloc, dz_Rloc
    use nonlinear_lm_mod, only: nonlinear_lm_t
    use sht
@@ -465,26 +466,32 @@ contains
    end subroutine do_rloop
 
    subroutine do_brv()
-      !-- reads nlev levels; per level: omega, O_r_E_2, O_rho and the grids
-      !   br, vt, vp (n_theta, n_phi); writes br_vt_lm, br_vp_lm of
-      !   src/nonlinear_bcs.f90 get_br_v_bcs
+      !-- reads nlev levels; per level: omega, O_r_E_2, O_rho, conductance_ma,
+      !   prmag, sigma_ratio and the grids br, vt, vp (n_theta, n_phi);
+      !   writes br_vt_lm, br_vp_lm of src/nonlinear_bcs.f90 get_br_v_bcs,
+      !   then b_nl_bc, aj_nl_bc of get_b_nl_bcs('CMB') and aj_nl_bc of
+      !   get_b_nl_bcs('ICB') on those products
       real(cp), allocatable :: G(:,:,:)
-      complex(cp), allocatable :: a(:), b(:)
+      complex(cp), allocatable :: a(:), b(:), bc(:), jc(:), bi(:), ji(:)
       real(cp) :: om, o2, orh
       integer :: u, v, lev
 
       allocate( G(n_theta_max,n_phi_max,3), a(lm_max), b(lm_max) )
+      allocate( bc(lm_max), jc(lm_max), bi(lm_max), ji(lm_max) )
       open(newunit=u, file=trim(infile), access='stream', form='unformatted', &
       &    status='old')
       open(newunit=v, file=trim(outfile), access='stream', form='unformatted', &
       &    status='replace')
       call write_header(v)
       do lev=1,nlev
-         read(u) om, o2, orh
+         read(u) om, o2, orh, conductance_ma, prmag, sigma_ratio
          read(u) G(:,:,:)
          a(:)=(0.0_cp,0.0_cp); b(:)=(0.0_cp,0.0_cp)
          call get_br_v_bcs(G(:,:,1), G(:,:,2), G(:,:,3), om, o2, orh, a, b)
-         write(v) a, b
+         bc(:)=(0.0_cp,0.0_cp); jc(:)=(0.0_cp,0.0_cp); ji(:)=(0.0_cp,0.0_cp)
+         call get_b_nl_bcs('CMB', a, b, bc, jc)
+         call get_b_nl_bcs('ICB', a, b, bi, ji)
+         write(v) a, b, bc, jc, ji
       end do
       close(u)
       close(v)

@@ -236,12 +236,17 @@ def brv_case(tag, l_max, minc, config, write=True):
     """The reference's boundary products of a conducting mantle and inner
     core (src/nonlinear_bcs.f90 get_br_v_bcs, ref_harness brv): two levels,
     the outer boundary with omega_ma and the inner one with omega_ic, on
-    seeded grids uniform in [-1, 1]."""
+    seeded grids uniform in [-1, 1]; and on those products the nonlinear
+    boundary terms of get_b_nl_bcs for 'CMB' (b_nl_bc, aj_nl_bc) and 'ICB'
+    (aj_nl_bc) with conductance_ma, prmag, sigma_ratio per level."""
     n_theta, n_phi = workload.grid_size(l_max, 20, minc)
     nlev = 2
     omega = np.array([0.7, -1.3])
     or2 = np.array([1.0 / 1.538 ** 2, 1.0 / 0.538 ** 2])
     orho = np.array([1.0, 0.85])
+    cond = np.array([0.3, 1.7])
+    prmag = np.array([5.0, 0.45])
+    sigma = np.array([1.0, 0.61])
     grids = []
     for lev in range(nlev):
         u = workload.uniform01(workload.stream_seed(config, 77, lev), 3 * n_phi * n_theta)
@@ -251,16 +256,18 @@ def brv_case(tag, l_max, minc, config, write=True):
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         with open(fin, "wb") as f:
             for i in range(nlev):
-                np.array([omega[i], or2[i], orho[i]], dtype="<f8").tofile(f)
+                np.array([omega[i], or2[i], orho[i], cond[i], prmag[i], sigma[i]], dtype="<f8").tofile(f)
                 np.ascontiguousarray(grids[i], dtype="<f8").tofile(f)
         oracle.run_ref("brv", l_max, minc, 20, fin, fout, nlev=nlev, threads=1)
         with open(fout, "rb") as f:
             hdr = read_header(f)
             lm_max = hdr["lm_max"]
-            out = np.fromfile(f, "<c16", 2 * nlev * lm_max).reshape(nlev, 2, lm_max)
+            out = np.fromfile(f, "<c16", 5 * nlev * lm_max).reshape(nlev, 5, lm_max)
             assert f.read() == b""
     data = dict(l_max=l_max, minc=minc, n_theta=n_theta, n_phi=n_phi, omega=omega, or2=or2,
-                orho=orho, grids=grids, br_vt_lm=out[:, 0], br_vp_lm=out[:, 1])
+                orho=orho, grids=grids, br_vt_lm=out[:, 0], br_vp_lm=out[:, 1],
+                conductance_ma=cond, prmag=prmag, sigma_ratio=sigma, b_nl_cmb=out[:, 2],
+                aj_nl_cmb=out[:, 3], aj_nl_icb=out[:, 4])
     if write:
         path = os.path.join(OUT, "brv_%s.npz" % tag)
         np.savez_compressed(path, **data)

@@ -151,6 +151,26 @@ def test_oracle_get_br_v_bcs_matches_reference(fname):
         assert rel_l2(b, g["br_vp_lm"][lev]) < 1e-12, (fname, lev)
 
 
+@pytest.mark.parametrize("fname", ["brv_L16_m1.npz", "brv_L64_m4.npz"])
+def test_oracle_get_b_nl_bcs_matches_reference(fname):
+    """orc_get_b_nl_bcs against the reference's get_b_nl_bcs('CMB') and
+    ('ICB') on its own get_br_v_bcs products (ref_harness brv): bit for bit
+    (one product per entry, fac = conductance_ma prmag or sigma_ratio prmag);
+    another bc is refused like the reference's abortRun."""
+    g = load_golden(fname)
+    p = oracle.Plan(int(g["l_max"]), int(g["n_theta"]), int(g["n_phi"]), minc=int(g["minc"]))
+    for lev in range(g["br_vt_lm"].shape[0]):
+        a, b = g["br_vt_lm"][lev], g["br_vp_lm"][lev]
+        bc, jc = p.get_b_nl_bcs("CMB", a, b, g["conductance_ma"][lev], g["prmag"][lev])
+        _, ji = p.get_b_nl_bcs("ICB", a, b, g["sigma_ratio"][lev], g["prmag"][lev])
+        assert np.array_equal(bc, g["b_nl_cmb"][lev]) and np.array_equal(jc, g["aj_nl_cmb"][lev])
+        assert np.array_equal(ji, g["aj_nl_icb"][lev])
+        assert bc[0] == 0 and jc[0] == 0 and ji[0] == 0
+    z = np.zeros(p.lm_max, dtype=np.complex128)
+    assert p._L.orc_get_b_nl_bcs(int(p.lm_max), 2, oracle._ptr(z), oracle._ptr(z), 1.0, 1.0,
+                                 oracle._ptr(z), oracle._ptr(z)) == -1
+
+
 def test_br_v_bcs_solid_body_rotation():
     """A uniform br = 1 over a fluid at rest on a boundary rotating at
     omega: br_vp = -omega sin^2(theta) is solid-body rotation, so the
