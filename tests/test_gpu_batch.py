@@ -137,3 +137,52 @@ def test_batch_persistent_launch_matches_graph_form():
                   flush=True)
             assert d < 4e-7, (L, kind, c, d)
             assert e < SPLIT_TOL, (L, kind, c, e)
+
+
+def test_batch_small_and_ragged_jobs():
+    """Small truncations (l_max 7, 15, 23, 31: n_phi 24 .. 96, row FFTs on
+    the generic LDS passes rather than wave slots; n_theta = 3 (l_max + 1) / 2
+    must be a multiple of 4), single levels and fields, a level group cut
+    short (rows not a multiple of 8), a job of two whole level groups:
+    persistent launch against the identity and against the per-job graph
+    form."""
+    import torch
+    rng = np.random.default_rng(17)
+    dev = torch.device("cuda")
+    specs = [(7, sht.JOB_SCALAR, 1, 1), (15, sht.JOB_VECTOR, 1, 2), (23, sht.JOB_VECTOR, 2, 1),
+             (31, sht.JOB_SCALAR, 3, 3), (31, sht.JOB_VECTOR, 1, 16), (63, sht.JOB_SCALAR, 1, 1)]
+    host = []
+    for L, kind, nf, nlev in specs:
+        nc = 3 if kind == sht.JOB_VECTOR else 1
+        host.append(job_input(rng, L, nc, nf, nlev))
+    outs = {}
+    for name in ("graph", "persistent"):
+        jobs = [(L, kind, torch.from_numpy(x).to(dev), torch.zeros(x.shape, dtype=torch.complex128, device=dev))
+                for (L, kind, nf, nlev), x in zip(specs, host)]
+        old = os.environ.pop("MHX_BATCH_FORM", None)
+        if name == "graph":
+            os.environ["MHX_BATCH_FORM"] = "graph"
+        try:
+            b = sht.Batch(jobs)
+        finally:
+            os.environ.pop("MHX_BATCH_FORM", None)
+            if old is not None:
+                os.environ["MHX_BATCH_FORM"] = old
+        form, shapes = b.shapes()
+        print(name, "form", form, "shapes", shapes, flush=True)
+        assert form == (0 if name == "graph" else 1)
+        b.execute()
+        b.execute()
+        b.status()
+        outs[name] = [j[3].cpu().numpy() for j in jobs]
+        b.close()
+    for (L, kind, nf, nlev), x, g, p in zip(specs, host, outs["graph"], outs["persistent"]):
+        want, mask = expected(kind, L, x)
+        for c in range(x.shape[0]):
+            sel = mask if c > 0 else (mask if kind == sht.JOB_VECTOR else np.ones_like(mask))
+            d = rel_l2(p[c][..., sel], g[c][..., sel])
+            e = rel_l2(p[c][..., sel], want[c][..., sel])
+            print("l_max=%d kind=%d nf=%d nlev=%d comp=%d: persistent vs graph %.2e, vs identity %.2e"
+                  % (L, kind, nf, nlev, c, d, e), flush=True)
+            assert d < 4e-7, (L, kind, c, d)
+            assert e < SPLIT_TOL, (L, kind, c, e)
