@@ -424,7 +424,8 @@ def bench_batch(args, dev):
         "legendre_tflops_fp64_equiv": flops / dt / 1e12,
         "launch_form": "persistent" if one else "graph",
         "tile_shapes": [{"l_max": t[0], "orders_per_item": t[1], "level_groups_per_item": t[2],
-                         "theta_rows_per_row_item": t[3]} for t in shapes] if one else None,
+                         "level_groups_per_item_from_split": t[3], "split_order": t[4],
+                         "theta_rows_per_row_item": t[5]} for t in shapes] if one else None,
         "tuning": None if one else [{"l_max": t[0], "synthesis_form": t[1], "analysis_form": t[2],
                                      "fft_rows_per_block": t[3]} for t in tune],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -398,11 +398,12 @@ int mhx_batch_tuning(const mhx_batch *batch, int *l_max, int *syn_form, int *ana
 /* the launch form (*form: 1 one persistent launch, 0 per-job launches) and,
  * per l_max group (n entries at most), the tile shapes the autotuner chose
  * for the persistent launch: orders per synthesis/analysis item (1: one m;
- * 2: m and m_max - m), level groups of 8 rows (field x level) per item,
- * theta rows per row item (graph form: 1, 0, row-FFT rows per block);
+ * 2: m and m_max - m), level groups of 8 rows (field x level) per item for
+ * the orders below m_split (order index) and groups_high from it on, theta
+ * rows per row item (graph form: 1, 0, 0, 0, row-FFT rows per block);
  * returns the number of groups */
 int mhx_batch_shapes(const mhx_batch *batch, int *form, int *l_max, int *orders, int *groups,
-                     int *theta_rows, int n);
+                     int *groups_high, int *m_split, int *theta_rows, int n);
 /* waits until the batch's executions issued so far are done; MHX_E_HIP if
  * in one of them a work item of the persistent launch gave up waiting for
  * its inputs (its outputs are then not valid), MHX_OK otherwise */

@@ -51,7 +51,10 @@ struct Group {
     int l_max = 0;
     mhx_plan *plan = nullptr;
     int fft_rows = 4, syn_form = 1, ana_form = 1;  // graph form
-    int fold = 1, lpi = 1, tbr = 8;                // persistent form tile shapes
+    // persistent form tile shapes: orders per item (fold: m with m_max - m),
+    // level groups per item (lpi below order index msplit, lpi_hi from it
+    // on; lpi divides lpi_hi), theta rows per row item
+    int fold = 1, lpi = 1, lpi_hi = 1, msplit = 1 << 30, tbr = 8;
 };
 
 template <typename T> void dfree(T *&p)
@@ -283,12 +286,13 @@ void make_pjobs(const mhx_batch *b, std::vector<PBJob> &out)
 }
 
 // The work list with the groups' current tile shapes.  A chain is the
-// synthesis, row and analysis items of one job's run of `lpi` level groups;
-// chains are dealt to the XCD queues largest first, each to the least loaded
-// queue (estimated work: l range x rows x components for the Legendre items,
-// rows x components x n_phi for the row items); within a queue the
-// synthesis items come first, then the row items, then the analysis items,
-// each phase largest first.
+// synthesis, row and analysis items of one job's run of lpi_hi level groups
+// (its orders below msplit in items of lpi groups, the others in items of
+// the whole run); chains are dealt to the XCD queues largest first, each to
+// the least loaded queue (estimated work: l range x rows x components for
+// the Legendre items, rows x components x n_phi for the row items); within
+// a queue the synthesis items come first, then the row items, then the
+// analysis items, each phase largest first.
 void make_items(const mhx_batch *b, std::vector<PBItem> &items, PBQueues &qs)
 {
     struct W {
@@ -314,28 +318,37 @@ void make_items(const mhx_batch *b, std::vector<PBItem> &items, PBQueues &qs)
             for (int mc = 0; mc < n_m; ++mc) ords.push_back({mc, -1});
         }
         const int ntbr = (P->n_theta + g.tbr - 1) / g.tbr;
-        for (int g0 = 0; g0 < ng; g0 += g.lpi) {
-            Chain ch;
-            const int gn = std::min(g.lpi, ng - g0);
+        const int span = std::max(g.lpi, g.lpi_hi);
+        auto rows_in = [&](int g0, int gn) {
             int nr = 0;
             for (int gg = g0; gg < g0 + gn; ++gg) nr += std::min(PB_GROUP, j.rows - gg * PB_GROUP);
+            return nr;
+        };
+        for (int c0 = 0; c0 < ng; c0 += span) {
+            Chain ch;
+            const int cn = std::min(span, ng - c0);
             for (const auto &o : ords) {
-                PBItem it;
-                std::memset(&it, 0, sizeof(it));
-                it.kind = PB_SYN;
-                it.job = (short)ji;
-                it.m0 = (short)o.first;
-                it.m1 = (short)o.second;
-                it.g0 = (short)g0;
-                it.ng = (short)gn;
-                double kl = (double)(P->l_max + 2 - o.first * P->minc);
-                if (o.second >= 0) kl += (double)(P->l_max + 2 - o.second * P->minc);
-                const double c = kl * nr * wc * P->n_theta;
-                ch.syn.push_back({c, it});
-                it.kind = PB_ANA;
-                ch.ana.push_back({c, it});
-                ch.cost += 2 * c;
+                const int step = o.first >= g.msplit ? g.lpi_hi : g.lpi;
+                for (int g0 = c0; g0 < c0 + cn; g0 += step) {
+                    const int gn = std::min(step, c0 + cn - g0);
+                    PBItem it;
+                    std::memset(&it, 0, sizeof(it));
+                    it.kind = PB_SYN;
+                    it.job = (short)ji;
+                    it.m0 = (short)o.first;
+                    it.m1 = (short)o.second;
+                    it.g0 = (short)g0;
+                    it.ng = (short)gn;
+                    double kl = (double)(P->l_max + 2 - o.first * P->minc);
+                    if (o.second >= 0) kl += (double)(P->l_max + 2 - o.second * P->minc);
+                    const double c = kl * rows_in(g0, gn) * wc * P->n_theta;
+                    ch.syn.push_back({c, it});
+                    it.kind = PB_ANA;
+                    ch.ana.push_back({c, it});
+                    ch.cost += 2 * c;
+                }
             }
+            const int g0 = c0, gn = cn;
             for (int gg = g0; gg < g0 + gn; ++gg) {
                 const int nrg = std::min(PB_GROUP, j.rows - gg * PB_GROUP);
                 for (int tb = 0; tb < ntbr; ++tb) {
@@ -445,36 +458,44 @@ int time_p(mhx_batch *b, int reps, float &best)
 }
 
 // tile shapes per l_max group by coordinate descent on the whole launch:
-// (orders per item, level groups per item) then theta rows per row item
+// the Legendre items' shape (orders per item, level groups per item below
+// and from the split order), then theta rows per row item
 int autotune_p(mhx_batch *b)
 {
     for (Group &g : b->groups) {
         g.fold = 1;
-        g.lpi = 1;
+        g.lpi = g.lpi_hi = 1;
+        g.msplit = 1 << 30;
         const int ns = pbatch_slot_rows(g.plan->p.M);
         g.tbr = ns ? ns : (tbr_fits(&g.plan->p, 8) ? 8 : 4);
     }
     int rc;
     float best;
     if ((rc = stage(b)) || (rc = time_p(b, 2, best))) return rc;
-    const int shapes[4][2] = {{0, 1}, {1, 1}, {1, 2}, {1, 4}};
+    // {fold, lpi, lpi_hi, split at n_m / d (0: none)}: folded pairs of even
+    // cost, or single orders whose short high-m ones share items over more
+    // level groups
+    const int shapes[7][4] = {{1, 1, 1, 0}, {1, 2, 2, 0}, {1, 4, 4, 0}, {0, 1, 1, 0},
+                              {0, 1, 2, 2}, {0, 1, 4, 2}, {0, 1, 4, 4}};
     for (Group &g : b->groups) {
-        const int f0 = g.fold, l0 = g.lpi;
-        int bf = f0, bl = l0;
-        for (const auto &sh : shapes) {
-            if (sh[0] == f0 && sh[1] == l0) continue;
+        const int n_m = g.plan->p.n_m;
+        auto set = [&](const int *sh) {
             g.fold = sh[0];
             g.lpi = sh[1];
+            g.lpi_hi = sh[2];
+            g.msplit = sh[3] ? n_m / sh[3] : 1 << 30;
+        };
+        int bi = 0;  // the start shape is shapes[0]
+        for (int si = 1; si < 7; ++si) {
+            set(shapes[si]);
             float t;
             if ((rc = stage(b)) || (rc = time_p(b, 2, t))) return rc;
             if (t < best) {
                 best = t;
-                bf = sh[0];
-                bl = sh[1];
+                bi = si;
             }
         }
-        g.fold = bf;
-        g.lpi = bl;
+        set(shapes[bi]);
         if (pbatch_slot_rows(g.plan->p.M)) continue;  // slot row items: theta rows fixed
         const int t0 = g.tbr;
         int bt = t0;
@@ -666,8 +687,8 @@ int mhx_batch_tuning(const mhx_batch *b, int *l_max, int *syn_form, int *ana_for
     return (int)b->groups.size();
 }
 
-int mhx_batch_shapes(const mhx_batch *b, int *form, int *l_max, int *orders, int *groups, int *rows,
-                     int n)
+int mhx_batch_shapes(const mhx_batch *b, int *form, int *l_max, int *orders, int *groups,
+                     int *groups_high, int *m_split, int *rows, int n)
 {
     if (!b) return fail(MHX_E_INVALID, "null batch");
     if (form) *form = b->persistent ? 1 : 0;
@@ -677,6 +698,8 @@ int mhx_batch_shapes(const mhx_batch *b, int *form, int *l_max, int *orders, int
         if (l_max) l_max[i] = g.l_max;
         if (orders) orders[i] = b->persistent ? (g.fold ? 2 : 1) : 1;
         if (groups) groups[i] = b->persistent ? g.lpi : 0;
+        if (groups_high) groups_high[i] = b->persistent ? g.lpi_hi : 0;
+        if (m_split) m_split[i] = b->persistent ? std::min(g.msplit, g.plan->p.n_m) : 0;
         if (rows) rows[i] = b->persistent ? g.tbr : g.fft_rows;
     }
     return (int)b->groups.size();

@@ -67,7 +67,7 @@ _SIGS = {
     "mhx_batch_create": "piip",
     "mhx_batch_execute": "pp",
     "mhx_batch_tuning": "pppppi",
-    "mhx_batch_shapes": "ppppppi",
+    "mhx_batch_shapes": "ppppppppi",
     "mhx_batch_status": "p",
     "mhx_batch_destroy": "p",
 }
@@ -622,12 +622,13 @@ class Batch:
         return [tuple(x[i] for x in a) for i in range(min(k, n))]
 
     def shapes(self):
-        """(form, [(l_max, orders per item, level groups per item, theta rows
-        per row item)]): form 1 is the one persistent launch (the default),
-        0 the per-job launches (MHX_BATCH_FORM=graph)."""
+        """(form, [(l_max, orders per item, level groups per item below the
+        split order, level groups per item from it on, split order index,
+        theta rows per row item)]): form 1 is the one persistent launch (the
+        default), 0 the per-job launches (MHX_BATCH_FORM=graph)."""
         n = 16
         form = ctypes.c_int()
-        a = [(ctypes.c_int * n)() for _ in range(4)]
+        a = [(ctypes.c_int * n)() for _ in range(6)]
         k = lib().mhx_batch_shapes(self.h, ctypes.byref(form), *a, n)
         if k < 0:
             _check(k)

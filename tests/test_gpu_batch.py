@@ -69,8 +69,9 @@ def test_batch_round_trips_and_replays():
     form, shapes = b.shapes()
     print("form %d, shapes %s" % (form, shapes), flush=True)
     assert form == 1
-    for _, orders, groups, rows in shapes:
-        assert orders in (1, 2) and groups in (1, 2, 4) and rows in (4, 8, 16, 32)
+    for L, orders, groups, groups_hi, m_split, rows in shapes:
+        assert orders in (1, 2) and groups in (1, 2, 4) and groups_hi in (1, 2, 4) and rows in (4, 8, 16, 32)
+        assert groups_hi % groups == 0 and 0 < m_split <= L + 1
     b.execute()
     b.status()
     first = [j[3].cpu().numpy() for j in jobs]
