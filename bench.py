@@ -301,6 +301,11 @@ def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut, courant=False):
             "legendre_tflops": total_flop / (flop_ms * 1e-3) / 1e12, "outs": outs}
 
 
+# workloads of the bench and the seed stream each one's synthetic inputs are
+# drawn from (the stream number keeps the inputs identical across names)
+WORKLOADS = {"shell255": 2, "batch64": 3, "shell511": 4, "shell1023": 5}
+
+
 PROFILED = (255, 97)  # (l_max, n_r) of the committed PMC summary
 
 
@@ -452,18 +457,18 @@ def main():
                     help="time the step with the Courant maxima fused into its grid stage "
                          "(mhx_nl_step_courant; synchronous per step)")
     ap.add_argument("--tables", choices=["recurrence", "stored"], default="recurrence",
-                    help="--config 5: split-form Legendre tables generated in-kernel by the FP64 "
+                    help="shell1023: split-form Legendre tables generated in-kernel by the FP64 "
                          "recurrence (default) or read from stored FP16 tables")
     ap.add_argument("--spectrum", default=None,
                     help="amplitude spectrum of the synthesised fields: l^-5/3 (default), "
-                         "slopes-0-3 (default of --config 5: field f with (l+1)^-(f mod 4)), "
+                         "slopes-0-3 (default of shell1023: field f with (l+1)^-(f mod 4)), "
                          "white-to-l^-3 ((1+(l/64)^2)^-3/2), l+1^-<s>")
-    ap.add_argument("--config", type=int, choices=[2, 3, 4, 5], default=2,
-                    help="2: the nonlinear step at l_max=255 x 97 (headline); 3: the uneven "
-                         "batch of 64 transform jobs in one captured graph; 4: the nonlinear "
-                         "step at l_max=511 x 161 with the l=1 poloidal field x100; 5: the "
-                         "nonlinear step at l_max=1023 x 64, spectral slopes 0-3 per field, "
-                         "tables generated in-kernel")
+    ap.add_argument("--workload", choices=list(WORKLOADS), default="shell255",
+                    help="shell255: the nonlinear step at l_max=255 x 97 (headline); batch64: "
+                         "the uneven batch of 64 transform jobs in one launch; shell511: the "
+                         "nonlinear step at l_max=511 x 161 with the l=1 poloidal field x100; "
+                         "shell1023: the nonlinear step at l_max=1023 x 64, spectral slopes "
+                         "0-3 per field, tables generated in-kernel")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the bench, run one device-to-device copy of this many MiB "
                          "(a kernel of known traffic, for scaling hardware byte counters)")
@@ -483,19 +488,19 @@ def main():
         log("no GPU visible")
         sys.exit(1)
     dev = torch.device("cuda:0")
-    if args.config == 3:
+    if args.workload == "batch64":
         print(json.dumps(bench_batch(args, dev)))
         return
     L, nlev = args.l_max, args.nlev
     dipole, kind = 1.0, "l^-5/3"
-    if args.config == 4:
+    if args.workload == "shell511":
         L, nlev, dipole = 511, 161, 100.0
-    if args.config == 5:
+    if args.workload == "shell1023":
         L, nlev, kind = 1023, 64, "slopes-0-3"
     kind = args.spectrum or kind
     nt, nph = workload.grid_size(L)
     log("l_max=%d %dx%d nlev=%d spectrum %s" % (L, nt, nph, nlev, kind))
-    inputs = workload.nl_inputs(L, nlev, config=args.config, kind=kind, dipole=dipole)
+    inputs = workload.nl_inputs(L, nlev, stream=WORKLOADS[args.workload], kind=kind, dipole=dipole)
     r = np.linspace(0.55, 1.45, nlev)
     rad = (1.0 / r ** 2, 1.0 / r ** 4, 1.0 / (1.0 + 0.1 * r), 0.8)
     lcut = [L] * nlev
@@ -511,7 +516,7 @@ def main():
     if args.form in ("both", "split16"):
         # the l_max=1023 case generates its tables in-kernel (--tables stored
         # runs the stored-table kernels instead)
-        rec = args.config == 5 and args.tables == "recurrence"
+        rec = args.workload == "shell1023" and args.tables == "recurrence"
         forms["split16"] = run_form(sht.PREC_SPLIT16_REC if rec else sht.PREC_SPLIT16, L, nlev,
                                     args.steps, args.warmup, tin,
                                     rad, lcut, courant=args.courant)
@@ -569,7 +574,7 @@ def main():
                                "fused FFTs + grid products, 3 QST + scalar forward)",
                    "l_max": L, "n_theta": nt, "n_phi": nph, "n_r": nlev,
                    "dipole_l1_factor": dipole, "spectrum": kind, "precision": main_key,
-                   "tables": ("recurrence" if args.config == 5 and args.tables == "recurrence"
+                   "tables": ("recurrence" if args.workload == "shell1023" and args.tables == "recurrence"
                               else "stored"),
                    "courant_maxima": bool(args.courant)},
         "legendre_tflops_fp64_equiv": main["legendre_tflops"],

@@ -37,8 +37,8 @@ def uniform01(seed, n):
     return (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
 
 
-def stream_seed(config, field, level):
-    return (SEED_BASE + config) ^ ((field + 1) * 0x100000001B3) ^ ((level + 1) * 0xC2B2AE3D27D4EB4F)
+def stream_seed(stream, field, level):
+    return (SEED_BASE + stream) ^ ((field + 1) * 0x100000001B3) ^ ((level + 1) * 0xC2B2AE3D27D4EB4F)
 
 
 def lm_index(l_max, m_max=None, minc=1):
@@ -72,10 +72,10 @@ def spectrum(l, kind):
     raise ValueError(kind)
 
 
-def coefficients(l_max, config, field, level, kind="l+1^-2", potential=True, m_max=None,
+def coefficients(l_max, stream, field, level, kind="l+1^-2", potential=True, m_max=None,
                  minc=1, amp=1.0):
     ls, ms = lm_index(l_max, m_max, minc)
-    ph = 2.0 * np.pi * uniform01(stream_seed(config, field, level), ls.size)
+    ph = 2.0 * np.pi * uniform01(stream_seed(stream, field, level), ls.size)
     c = amp * spectrum(ls, kind) * np.exp(1j * ph)
     c[ms == 0] = c[ms == 0].real
     if potential:
@@ -89,7 +89,7 @@ def coefficients(l_max, config, field, level, kind="l+1^-2", potential=True, m_m
 NL_FIELD_ORDER = ("w", "dw", "ddw", "z", "dz", "s", "b", "db", "ddb", "aj", "dj")
 
 
-def nl_inputs(l_max, nlev, config=2, kind="l^-5/3", minc=1, dipole=1.0):
+def nl_inputs(l_max, nlev, stream=2, kind="l^-5/3", minc=1, dipole=1.0):
     """Coefficient inputs of the nonlinear step whose synthesised fields
     (velocity, vorticity, magnetic field, current density, temperature) have
     amplitude spectrum `kind` with random phases: each potential is drawn with
@@ -114,7 +114,7 @@ def nl_inputs(l_max, nlev, config=2, kind="l^-5/3", minc=1, dipole=1.0):
         arr = np.empty((nlev, ls.size), dtype=np.complex128)
 
         def fill(lev, f=f, kf=kf, fac=fac, name=name, arr=arr):
-            arr[lev] = coefficients(l_max, config, f, lev, kf, potential=(name != "s"),
+            arr[lev] = coefficients(l_max, stream, f, lev, kf, potential=(name != "s"),
                                     minc=minc) * fac
         # levels are independent streams: filled by a thread pool (numpy's
         # array operations release the interpreter lock)

@@ -24,28 +24,28 @@ from api_cases import read_header, write_api_input, read_api_output  # noqa: E40
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 
-def make_api_inputs(l_max, minc, n_theta, n_phi, lcuts, config, real_m0=True):
+def make_api_inputs(l_max, minc, n_theta, n_phi, lcuts, stream, real_m0=True):
     ls, ms = workload.lm_index(l_max, None, minc)
     levels = []
     for lev, lcut in enumerate(lcuts):
         coefs = []
         for fld in range(11):
-            c = workload.coefficients(l_max, config, fld, lev, "l+1^-2", potential=False,
+            c = workload.coefficients(l_max, stream, fld, lev, "l+1^-2", potential=False,
                                       minc=minc)
             if not real_m0:
-                u = workload.uniform01(workload.stream_seed(config, 50 + fld, lev), ls.size)
+                u = workload.uniform01(workload.stream_seed(stream, 50 + fld, lev), ls.size)
                 c = c + 1j * (ms == 0) * (u - 0.5) / (ls + 1.0) ** 2
             coefs.append(c)
-        u = workload.uniform01(workload.stream_seed(config, 99, lev), 3 * n_phi * n_theta)
+        u = workload.uniform01(workload.stream_seed(stream, 99, lev), 3 * n_phi * n_theta)
         grids = (2.0 * u - 1.0).reshape(3, n_phi, n_theta)
         levels.append(dict(lcut=lcut, r=0.3, r_icb=0.35, or2=1.0 / 0.81, coefs=coefs,
                            grids=grids))
     return levels
 
 
-def api_case(tag, l_max, minc, lcuts, config, real_m0=True):
+def api_case(tag, l_max, minc, lcuts, stream, real_m0=True):
     n_theta, n_phi = workload.grid_size(l_max, 20, minc)
-    levels = make_api_inputs(l_max, minc, n_theta, n_phi, lcuts, config, real_m0)
+    levels = make_api_inputs(l_max, minc, n_theta, n_phi, lcuts, stream, real_m0)
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         write_api_input(fin, levels)
@@ -151,10 +151,10 @@ TD_IN = ("AdvrLM", "AdvtLM", "AdvpLM", "VStLM", "VxBrLM", "VxBtLM", "VxBpLM", "w
 TD_OUT = ("dwdt", "dzdt", "dpdt", "dsdt", "dbdt", "djdt", "dVxBhLM")
 
 
-def td_inputs(l_max, nlev, config=21):
+def td_inputs(l_max, nlev, stream=21):
     """Seeded inputs of the lm-space assembly: the seven nonlinear outputs it
     reads and the fields w, dw/dr, z at each level (complex, (l+1)^-2)."""
-    return np.stack([np.stack([workload.coefficients(l_max, config, k, lev, "l+1^-2", potential=False)
+    return np.stack([np.stack([workload.coefficients(l_max, stream, k, lev, "l+1^-2", potential=False)
                                for lev in range(nlev)]) for k in range(len(TD_IN))])
 
 
@@ -232,7 +232,7 @@ def courant_case(tag, l_max, nlev, write=True):
     return data
 
 
-def brv_case(tag, l_max, minc, config, write=True):
+def brv_case(tag, l_max, minc, stream, write=True):
     """The reference's boundary products of a conducting mantle and inner
     core (src/nonlinear_bcs.f90 get_br_v_bcs, ref_harness brv): two levels,
     the outer boundary with omega_ma and the inner one with omega_ic, on
@@ -249,7 +249,7 @@ def brv_case(tag, l_max, minc, config, write=True):
     sigma = np.array([1.0, 0.61])
     grids = []
     for lev in range(nlev):
-        u = workload.uniform01(workload.stream_seed(config, 77, lev), 3 * n_phi * n_theta)
+        u = workload.uniform01(workload.stream_seed(stream, 77, lev), 3 * n_phi * n_theta)
         grids.append((2.0 * u - 1.0).reshape(3, n_phi, n_theta))
     grids = np.stack(grids)
     with tempfile.TemporaryDirectory() as td:
@@ -317,18 +317,18 @@ def main():
         sys.exit("oracle/_ref/ref_harness is not built (make -C oracle)")
     tables_case("L16_m1", 16, 1)
     tables_case("L64_m4", 64, 4)
-    api_case("L16_m1", 16, 1, [16, 11, 0], config=1, real_m0=True)
-    api_case("L16_m1_cplxm0", 16, 1, [16], config=11, real_m0=False)
-    api_case("L16_m4", 16, 4, [16, 9], config=12)
-    api_case("L64_m1", 64, 1, [64], config=1)
-    api_case("L64_m4", 64, 4, [64, 40], config=13)
+    api_case("L16_m1", 16, 1, [16, 11, 0], stream=1, real_m0=True)
+    api_case("L16_m1_cplxm0", 16, 1, [16], stream=11, real_m0=False)
+    api_case("L16_m4", 16, 4, [16, 9], stream=12)
+    api_case("L64_m1", 64, 1, [64], stream=1)
+    api_case("L64_m4", 64, 4, [64, 40], stream=13)
     nl_case("L16", 16)
     nlstep_case("L16", 16, 3)
     td_case("L16", 16, 3)
     td_case("L64", 64, 2)
     courant_case("L16", 16, 3)
-    brv_case("L16_m1", 16, 1, config=21)
-    brv_case("L64_m4", 64, 4, config=22)
+    brv_case("L16_m1", 16, 1, stream=21)
+    brv_case("L64_m4", 64, 4, stream=22)
     rloop_case("boussBenchSat")
 
 

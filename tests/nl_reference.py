@@ -8,11 +8,11 @@ from magic_amd import workload
 IN_SPECTRA = "l^-5/3"
 
 
-def make_inputs(l_max, nlev, config=2, kind=IN_SPECTRA, minc=1):
+def make_inputs(l_max, nlev, stream=2, kind=IN_SPECTRA, minc=1):
     """11 fields (w dw ddw z dz s b db ddb aj dj) x nlev, complex128, built
     by magic_amd.workload.nl_inputs (the synthesised fields have the given
     amplitude spectrum)."""
-    return workload.nl_inputs(l_max, nlev, config, kind, minc)
+    return workload.nl_inputs(l_max, nlev, stream, kind, minc)
 
 
 def radial(nlev):

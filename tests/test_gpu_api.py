@@ -51,7 +51,7 @@ def test_tables_match_oracle():
 
 @pytest.mark.parametrize("L", [64, 255])
 def test_api_matches_oracle_large(L):
-    """config-1 and config-2 shapes, every routine, one level."""
+    """l_max=64 and l_max=255 shapes, every routine, one level."""
     from oracle import oracle
     nt, nph = workload.grid_size(L)
     print("oracle compare", L, nt, nph, flush=True)

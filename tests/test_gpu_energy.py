@@ -41,7 +41,7 @@ def test_e_kin_matches_reference_output(sample):
 def test_e_kin_levels_large():
     L, nlev = 255, 97
     print("e_kin levels", L, nlev, flush=True)
-    w, dw, _, z = workload.nl_inputs(L, nlev, config=2, kind="l^-5/3")[:4]
+    w, dw, _, z = workload.nl_inputs(L, nlev, stream=2, kind="l^-5/3")[:4]
     r = np.linspace(0.54, 1.54, nlev)
     orho = 1.0 / (1.0 + 0.1 * r)
     p = sht.Plan(L)

@@ -73,7 +73,7 @@ def test_split_l511_dipole_vs_fp64():
     lcut); every output within 1e-6 of the FP64 form."""
     L, nlev = 511, 8
     nt, nph = workload.grid_size(L)
-    inputs = workload.nl_inputs(L, nlev, config=4, dipole=100.0)
+    inputs = workload.nl_inputs(L, nlev, stream=4, dipole=100.0)
     rad = radial(nlev)
     lcut = [L] * nlev
     lcut[2] = L - 40
