@@ -66,7 +66,8 @@ struct GtArgs {
 // inverse FFT of the components in tile rows r1, r2 (row pointers into the
 // tile, n_m coefficients each) into y[q] = x1[n] + i x2[n], n = j + 64 q
 __device__ __forceinline__ void gt_inverse(const C *__restrict__ r1, const C *__restrict__ r2, int n_m,
-                                           C *buf, const C *tw, int j, C (&y)[12])
+                                           C *buf, const C (&tw2)[3], const C (&tw3)[11], int j,
+                                           C (&y)[12])
 {
     // pass 1 (radix 16, L = 1, 48 butterflies): Y_k, k = j + 48 s
     if (j < 48) {
@@ -106,7 +107,7 @@ __device__ __forceinline__ void gt_inverse(const C *__restrict__ r1, const C *__
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 C v = ib[68 * i + 204 * s];
-                if (s > 0) v = cmul(v, tw[TW_I2 + (s - 1) * 16 + k]);
+                if (s > 0) v = cmul(v, tw2[s - 1]);
                 x[i][s] = v;
             }
         wave_order();
@@ -127,7 +128,7 @@ __device__ __forceinline__ void gt_inverse(const C *__restrict__ r1, const C *__
 #pragma unroll
         for (int s = 0; s < 12; ++s) {
             C v = ib[80 * s];
-            if (s > 0) v = cmul(v, tw[TW_I3 + (s - 1) * 64 + j]);
+            if (s > 0) v = cmul(v, tw3[s - 1]);
             y[s] = v;
         }
         Dft<12, C>::run(y, +1);
@@ -138,7 +139,7 @@ __device__ __forceinline__ void gt_inverse(const C *__restrict__ r1, const C *__
 // forward FFT of y[q] = p1[n] + i p2[n], n = j + 64 q, into tile rows o1, o2
 // (n_m coefficients each, scaled by 1/N)
 __device__ __forceinline__ void gt_forward(C (&y)[12], C *__restrict__ o1, C *__restrict__ o2, int n_m,
-                                           C *buf, const C *tw, int j)
+                                           C *buf, const C *tw, const C (&tw3)[11], int j)
 {
     // pass 1 (radix 12, L = 1): out 12 j + s, padded i + (i >> 2) = 15 j + s + (s >> 2)
     Dft<12, C>::run(y, -1);
@@ -176,19 +177,25 @@ __device__ __forceinline__ void gt_forward(C (&y)[12], C *__restrict__ o1, C *__
         }
     }
     wave_order();
-    // pass 3 (radix 16, L = 48, 48 butterflies): in j + 48 s, out j + 48 s
+    // pass 3 (radix 16, L = 48, 48 butterflies): in j + 48 s, out j + 48 s;
+    // twiddles exp(-2 pi i s j / N): for s <= 11 the conjugates of the
+    // inverse pass-3 twiddles the lane holds, above from the table
     if (j < 48) {
         C x[16];
 #pragma unroll
         for (int s = 0; s < 16; ++s) {
             C v = buf[j + 48 * s];
-            if (s > 0) v = cmul(v, tw[TW_F3 + (s - 1) * 48 + j]);
+            if (s > 0 && s <= 11) v = cmul(v, mk<C>(tw3[s - 1].x, -tw3[s - 1].y));
+            else if (s > 11) v = cmul(v, tw[TW_F3 + (s - 1) * 48 + j]);
             x[s] = v;
         }
         wave_order();
         Dft<16, C>::run(x, -1);
+        // outputs k = j + 48 s in [288, 480) (s = 6..9) are never read: the
+        // unpacking below reads k < n_m <= N / 3 and N - k > 2 N / 3
 #pragma unroll
-        for (int s = 0; s < 16; ++s) buf[j + 48 * s] = x[s];
+        for (int s = 0; s < 16; ++s)
+            if (s < 6 || s > 9) buf[j + 48 * s] = x[s];
     } else {
         wave_order();
     }
@@ -315,7 +322,7 @@ __global__ __launch_bounds__(64 * TR) void k_grid_tile(GtArgs a)
     };
 
     float u[3][12], acc[3][12], bb[3][12], pend[12];
-    C y[12];
+    C y[12], tw2[3], tw3[11];
     int slot_i = 0;
     auto slot = [&]() { return s_slot + (slot_i & 1) * SLOT; };
 
@@ -326,14 +333,14 @@ __global__ __launch_bounds__(64 * TR) void k_grid_tile(GtArgs a)
         store_pair(sl);
         __syncthreads();
         if (n0 >= 0) load_pair(n0, n1);
-        gt_inverse(row(sl, 0), row(sl, 1), n_m, buf, s_tw, j, y);
+        gt_inverse(row(sl, 0), row(sl, 1), n_m, buf, tw2, tw3, j, y);
         grids_out(y, c0, c1);
         ++slot_i;
     };
     // one forward pair: y already holds the products
     auto forward = [&](int c0, int c1) {
         C *sl = slot();
-        gt_forward(y, row(sl, 0), row(sl, 1), n_m, buf, s_tw, j);
+        gt_forward(y, row(sl, 0), row(sl, 1), n_m, buf, s_tw, tw3, j);
         __syncthreads();
         flush_pair(sl, c0, c1);
         ++slot_i;
@@ -341,6 +348,12 @@ __global__ __launch_bounds__(64 * TR) void k_grid_tile(GtArgs a)
 
     load_pair(0, 1);
     __syncthreads();  // twiddle tables
+    // the inverse passes' twiddles depend on the lane only: held in registers
+    // for the row's nine inverse FFTs
+#pragma unroll
+    for (int s = 0; s < 3; ++s) tw2[s] = s_tw[TW_I2 + s * 16 + (j & 15)];
+#pragma unroll
+    for (int s = 0; s < 11; ++s) tw3[s] = s_tw[TW_I3 + s * 64 + j];
 
     // u: (vr, vt), (vp, s); V s, forward (VSr, VSt)
     inverse(0, 1, 2, 15);
