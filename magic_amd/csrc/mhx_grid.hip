@@ -17,8 +17,9 @@
 // Y_k = X1_k + i X2_k with the Hermitian halves, and back X1_k = (Y_k + conj
 // Y_{N-k})/2, X2_k = (Y_k - conj Y_{N-k})/(2i)).  Each wave runs its FFTs
 // alone, Stockham passes in its own LDS buffer with no block barrier:
-//   inverse radices 16 (L = 1), 4 (L = 16), 12 (L = 64)
-//   forward radices 12 (L = 1), 4 (L = 12), 16 (L = 48)
+//   inverse radices 16 (L = 1), 4 (L = 16), 12 (L = 64), decimation in time
+//   forward radices 12, 4, 16, decimation in frequency (the same twiddles,
+//   conjugated, held in registers)
 // so that after the inverse (and before the forward) FFT lane j holds the
 // grid points n = j + 64 q, q < 12, of both components: the products of a
 // point are formed in the registers of the lane that owns it.  The buffer
@@ -37,10 +38,10 @@ constexpr int GT_BUF = 960;              // complex per wave FFT buffer (padded)
 constexpr int GT_MAXNM = 256;            // n_m <= N / 3
 constexpr int GT_ROWP = GT_MAXNM + 4;    // tile row pitch (complex)
 constexpr int GT_MT = GT_MAXNM / 64;     // tile loads per thread and comp
-// LDS twiddle tables (complex, lane-contiguous): inverse pass 2 [3][16],
-// inverse pass 3 [11][64], forward pass 2 [3][12], forward pass 3 [15][48]
-constexpr int TW_I2 = 0, TW_I3 = TW_I2 + 3 * 16, TW_F2 = TW_I3 + 11 * 64, TW_F3 = TW_F2 + 3 * 12;
-constexpr int TW_N = TW_F3 + 15 * 48;
+// LDS twiddle tables (complex, lane-contiguous), read once into registers:
+// inverse pass 2 [3][16], inverse pass 3 [11][64]
+constexpr int TW_I2 = 0, TW_I3 = TW_I2 + 3 * 16;
+constexpr int TW_N = TW_I3 + 11 * 64;
 // TR theta rows (waves) per workgroup: LDS of the twiddles, two tile slots
 // [2 comps][TR rows][ROWP] and TR FFT buffers
 template <int TR> constexpr size_t gt_lds()
@@ -137,58 +138,59 @@ __device__ __forceinline__ void gt_inverse(const C *__restrict__ r1, const C *__
 }
 
 // forward FFT of y[q] = p1[n] + i p2[n], n = j + 64 q, into tile rows o1, o2
-// (n_m coefficients each, scaled by 1/N)
+// (n_m coefficients each, scaled by 1/N).  Decimation in frequency, so that
+// every twiddle is the conjugate of an inverse twiddle the lane holds
+// (tw3[r - 1] = exp(2 pi i j r / N), tw2[s - 1] = exp(2 pi i (j mod 16) s / 64)):
+//   radix 12 over q, times exp(-2 pi i j r / N): z_r[j] at 64 r + j;
+//   radix 4 over j2 of j = j1 + 16 j2 (j1 = lane mod 16, r = 3 (lane / 16) + i),
+//     times exp(-2 pi i j1 k1 / 64): at 16 (r + 12 k1) + j1;
+//   radix 16 over j1 (lane b = r + 12 k1 < 48): X_{b + 48 k2} at b + 48 k2;
+// the first two buffers padded i + i / 16.
 __device__ __forceinline__ void gt_forward(C (&y)[12], C *__restrict__ o1, C *__restrict__ o2, int n_m,
-                                           C *buf, const C *tw, const C (&tw3)[11], int j)
+                                           C *buf, const C (&tw2)[3], const C (&tw3)[11], int j)
 {
-    // pass 1 (radix 12, L = 1): out 12 j + s, padded i + (i >> 2) = 15 j + s + (s >> 2)
+    // pass 1 (radix 12 over q): out padded 68 r + j + j / 16
     Dft<12, C>::run(y, -1);
     {
-        C *o = buf + 15 * j;
+        C *o = buf + (j + (j >> 4));
 #pragma unroll
-        for (int s = 0; s < 12; ++s) o[s + (s >> 2)] = y[s];
+        for (int r = 0; r < 12; ++r) {
+            C v = y[r];
+            if (r > 0) v = cmul(v, mk<C>(tw3[r - 1].x, -tw3[r - 1].y));
+            o[68 * r] = v;
+        }
     }
     wave_order();
-    // pass 2 (radix 4, L = 12): butterflies b = j + 64 i, k = b mod 12;
-    // in padded (j + (j >> 2)) + 80 i + 240 s; out 48 (b / 12) + k + 12 s
+    // pass 2 (radix 4 over j2, three r per lane): in padded 68 r + j1 + 17 j2,
+    // out padded 17 (r + 12 k1) + j1
     {
+        const int j1 = j & 15, rg = 3 * (j >> 4);
         C x[3][4];
-        int kk[3];
-        const C *ib = buf + (j + (j >> 2));
+        const C *ib = buf + (68 * rg + j1);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const int b = j + 64 * i;
-            kk[i] = b % 12;
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                C v = ib[80 * i + 240 * s];
-                if (s > 0) v = cmul(v, tw[TW_F2 + (s - 1) * 12 + kk[i]]);
-                x[i][s] = v;
-            }
-        }
+            for (int s = 0; s < 4; ++s) x[i][s] = ib[68 * i + 17 * s];
         wave_order();
+        C *ob = buf + (17 * rg + j1);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             Dft<4, C>::run(x[i], -1);
-            const int b = j + 64 * i;
-            C *ob = buf + (b - kk[i]) * 4 + kk[i];
 #pragma unroll
-            for (int s = 0; s < 4; ++s) ob[12 * s] = x[i][s];
+            for (int s = 0; s < 4; ++s) {
+                C v = x[i][s];
+                if (s > 0) v = cmul(v, mk<C>(tw2[s - 1].x, -tw2[s - 1].y));
+                ob[17 * i + 204 * s] = v;
+            }
         }
     }
     wave_order();
-    // pass 3 (radix 16, L = 48, 48 butterflies): in j + 48 s, out j + 48 s;
-    // twiddles exp(-2 pi i s j / N): for s <= 11 the conjugates of the
-    // inverse pass-3 twiddles the lane holds, above from the table
+    // pass 3 (radix 16 over j1, 48 butterflies): in padded 17 b + j1, out b + 48 s
     if (j < 48) {
         C x[16];
+        const C *ib = buf + 17 * j;
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            C v = buf[j + 48 * s];
-            if (s > 0 && s <= 11) v = cmul(v, mk<C>(tw3[s - 1].x, -tw3[s - 1].y));
-            else if (s > 11) v = cmul(v, tw[TW_F3 + (s - 1) * 48 + j]);
-            x[s] = v;
-        }
+        for (int s = 0; s < 16; ++s) x[s] = ib[s];
         wave_order();
         Dft<16, C>::run(x, -1);
         // outputs k = j + 48 s in [288, 480) (s = 6..9) are never read: the
@@ -242,22 +244,16 @@ __global__ __launch_bounds__(64 * TR) void k_grid_tile(GtArgs a)
     {
         const C *tn = reinterpret_cast<const C *>(a.twn);
         for (int e = tid; e < TW_N; e += NTHR) {
-            int idx, conj;
+            int idx;
             if (e < TW_I3) {
                 const int s = e / 16 + 1, k = e % 16;
-                idx = 12 * s * k, conj = 1;
-            } else if (e < TW_F2) {
-                const int s = (e - TW_I3) / 64 + 1, k = (e - TW_I3) % 64;
-                idx = s * k, conj = 1;
-            } else if (e < TW_F3) {
-                const int s = (e - TW_F2) / 12 + 1, k = (e - TW_F2) % 12;
-                idx = 16 * s * k, conj = 0;
+                idx = 12 * s * k;
             } else {
-                const int s = (e - TW_F3) / 48 + 1, k = (e - TW_F3) % 48;
-                idx = s * k, conj = 0;
+                const int s = (e - TW_I3) / 64 + 1, k = (e - TW_I3) % 64;
+                idx = s * k;
             }
             C v = tn[idx];
-            if (conj) v.y = -v.y;
+            v.y = -v.y;  // exp(+2 pi i idx / N)
             s_tw[e] = v;
         }
     }
@@ -340,7 +336,7 @@ __global__ __launch_bounds__(64 * TR) void k_grid_tile(GtArgs a)
     // one forward pair: y already holds the products
     auto forward = [&](int c0, int c1) {
         C *sl = slot();
-        gt_forward(y, row(sl, 0), row(sl, 1), n_m, buf, s_tw, tw3, j);
+        gt_forward(y, row(sl, 0), row(sl, 1), n_m, buf, tw2, tw3, j);
         __syncthreads();
         flush_pair(sl, c0, c1);
         ++slot_i;
