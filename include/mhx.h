@@ -375,6 +375,8 @@ int mhx_get_e_kin_noalloc(mhx_plan *p, int nlev, const double *w, const double *
  * environment variable MHX_BATCH_FORM=graph, or where a plan does not admit
  * that kernel, the jobs' per-job launches on parallel graph branches, their
  * kernel forms chosen per l_max by timing its largest job.
+ * The timing runs the jobs: during mhx_batch_create (which returns when they
+ * are done) the jobs' input arrays are read and their output arrays written.
  * mhx_batch_execute replays the graph (with a stream: returns at once,
  * ordered on the stream; without: returns when done).  The jobs' arrays
  * must stay allocated.  The reference runs these calls one level and field
