@@ -1879,7 +1879,11 @@ __global__ __launch_bounds__(256, 2) void k_pbatch(const PBJob *__restrict__ job
         if (w.kind != PB_SYN) {
             if (tid == 0) {
                 // a row item: every synthesis unit of its level group; an
-                // analysis item: every row item of each of its level groups
+                // analysis item: every row item of each of its level groups.
+                // No acquire fence: the arrays waited for are read only after
+                // their counts are complete (no item reads them earlier in
+                // the launch, and stores do not fill the CU's L1), so the
+                // CU's L1 holds no stale line of them
                 bool ok = true;
                 if (w.kind == PB_ROW) ok = pb_wait(&ctr[J.c_syn + w.g0], J.n_m, &ctr[PB_ERR]);
                 else
