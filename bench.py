@@ -176,6 +176,53 @@ def run_td(L, nlev, tout, rad, lcut, reps=20):
             "frac_of_hbm_peak": byts / (ms * 1e-3) / 8.0e12}
 
 
+# the Boussinesq MHD switch set with curl advection (the boussBenchSat
+# sample's: src/rIter.f90:483-707 with l_adv_curl, no double curl)
+RLOOP_BOUSS_MHD = {"l_conv": 1, "l_conv_nl": 1, "l_heat": 1, "l_heat_nl": 1, "l_mag": 1,
+                   "l_mag_LF": 1, "l_mag_nl": 1, "l_adv_curl": 1, "l_corr": 1,
+                   "l_cour_alf_damp": 1, "LFfac": 200.0, "CorFac": 1000.0, "opm": 0.2,
+                   "courfac": 0.8, "alffac": 0.35, "oek": 1000.0, "ra": 1.1e5, "opr": 1.0}
+
+
+def run_rloop(L, nlev, tin, reps=10):
+    """The reference-side radial loop (mhx_rloop, FP64, the body of the
+    rIter_batched_t plugin: syntheses, grid products, Courant steps,
+    analyses and the lm-space assembly of every interior level) on the same
+    fields, host-synchronous per call as the reference's loop."""
+    import numpy as np
+    import torch
+
+    from magic_amd import sht, workload
+    nt, nph = workload.grid_size(L)
+    p = sht.Plan(L, nt, nph, max_levels=nlev)
+    dev = tin[0].device
+    fields = dict(zip(sht.NL_INPUTS, tin))
+    names = ("dwdt", "dzdt", "dpdt", "dsdt", "dVSrLM", "dbdt", "djdt", "dVxBhLM")
+    outs = {n: torch.zeros((nlev, p.lm_max), dtype=torch.complex128, device=dev) for n in names}
+    r = np.linspace(0.55, 1.45, nlev)
+    one, zero = np.ones(nlev), np.zeros(nlev)
+    rad = {"r": r, "or1": 1 / r, "or2": 1 / r ** 2, "or4": 1 / r ** 4, "orho1": one, "beta": zero,
+           "otemp1": one, "visc": one, "lambda": one, "temp0": one, "epscProf": zero,
+           "delxr2": np.full(nlev, 1e-3), "delxh2": np.full(nlev, 2e-2),
+           "nR": np.arange(2, nlev + 2, dtype=np.int32), "l_R": np.full(nlev, L, dtype=np.int32)}
+    dtr = torch.zeros(nlev, dtype=torch.float64, device=dev)
+    dth = torch.zeros(nlev, dtype=torch.float64, device=dev)
+
+    def call():
+        sht.rloop(p, RLOOP_BOUSS_MHD, nlev, rad, fields, outs, dtr, dth)
+    for _ in range(3):
+        call()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        call()
+        ts.append(time.perf_counter() - t0)
+    p.close()
+    return {"entry": "mhx_rloop", "switches": "Boussinesq MHD, curl advection",
+            "levels": nlev, "median_ms_per_call": 1e3 * float(np.median(ts)),
+            "timing": "host clock, synchronous call"}
+
+
 def run_diag(L, nlev, tin, reps=20):
     """Two FP64 calls next to the step, host-synchronous as the reference
     makes them, timed per call with the host clock (launches, the copy of
@@ -295,8 +342,9 @@ def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut, courant=False):
     outs = [t.cpu().numpy() for t in tout]
     td = run_td(L, nlev, tout, rad, lcut) if precision == sht.PREC_FP64 else None
     diag = run_diag(L, nlev, tin) if precision == sht.PREC_FP64 else None
+    rl = run_rloop(L, nlev, tin) if precision == sht.PREC_FP64 else None
     p.close()
-    return {"td": td, "diag": diag, "pairs": nlev * steps / dt, "ms_per_step": 1e3 * dt / steps,
+    return {"td": td, "diag": diag, "rloop": rl, "pairs": nlev * steps / dt, "ms_per_step": 1e3 * dt / steps,
             "per_kernel": per_kernel, "kernels": kernels,
             "legendre_tflops": total_flop / (flop_ms * 1e-3) / 1e12, "outs": outs}
 
@@ -589,6 +637,10 @@ def main():
         res["td_assembly"] = forms["fp64"]["td"]
         if forms["fp64"].get("diag"):
             res["e_kin"], res["boundary_products"] = forms["fp64"]["diag"]
+        if forms["fp64"].get("rloop"):
+            # the radial loop the reference-side plugin runs (FP64, not in `value`)
+            res["radial_loop"] = forms["fp64"]["rloop"]
+            log("mhx_rloop: %.3f ms per call" % res["radial_loop"]["median_ms_per_call"])
         log("get_td: %.3f ms per call (%.0f GB/s)" % (res["td_assembly"]["avg_ms_per_call"],
                                                        res["td_assembly"]["GB/s"]))
     if "fp64" in forms and main_key != "fp64":
