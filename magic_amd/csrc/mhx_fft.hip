@@ -699,6 +699,70 @@ __device__ inline void rl_products(const RlPhys &ph, const double *v, double *o,
     }
 }
 
+// the get_nl products and Courant maxima of one (level, theta) block, point
+// by point in place over the N points of its component rows in LDS; R[idx(c,
+// j)] is point j of row c
+template <class Idx>
+__device__ __forceinline__ void rl_points(const RlGridArgs &a, double *R, Idx idx, int N, int lev, int th)
+{
+    const int n_theta = a.n_theta, nlev = a.nlev;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const double *rad = a.rad;
+    const double r = rad[RR_R * nlev + lev], or1 = rad[RR_OR1 * nlev + lev];
+    const double or2 = rad[RR_OR2 * nlev + lev], or4 = rad[RR_OR4 * nlev + lev];
+    const double orho1 = rad[RR_ORHO1 * nlev + lev], beta = rad[RR_BETA * nlev + lev];
+    const double otemp1 = rad[RR_OTEMP1 * nlev + lev], visc = rad[RR_VISC * nlev + lev];
+    const double lambda = rad[RR_LAMBDA * nlev + lev];
+    const bool lcr = rad[RR_LCR * nlev + lev] != 0.0;
+    const double o2 = a.th[RT_OSIN2 * n_theta + th], cosn2 = a.th[RT_COSN2 * n_theta + th];
+    const double sinT = a.th[RT_SIN * n_theta + th], cosT = a.th[RT_COS * n_theta + th];
+    const double osinT = a.th[RT_OSIN * n_theta + th];
+    double mr = 0.0, mh = 0.0;
+    const double vri2 = rad[RR_VALRI2 * nlev + lev], vhi2 = rad[RR_VALHI2 * nlev + lev];
+    const double orho2 = orho1 * orho1;
+    for (int j = tid; j < N; j += nthr) {
+        double v[RI_N], o[RO_N];
+#pragma unroll
+        for (int k = 0; k < RI_N; ++k) v[k] = a.in_of[k] >= 0 ? R[idx(a.in_of[k], j)] : 0.0;
+        rl_products(a.ph, v, o, o2, cosn2, sinT, cosT, osinT, j * a.dphi, r, or1, or2, or4, orho1,
+                    beta, otemp1, visc, lambda, lcr);
+        if (a.cmax) {
+            // src/courant.f90:209-275 (XSH_COURANT = 0)
+            const double vr = v[RI_VR], vt = v[RI_VT], vp = v[RI_VP];
+            const double vflr2 = orho2 * vr * vr;
+            const double vflh2 = (vt * vt + vp * vp) * o2 * orho2;
+            if (a.ph.cour_mag) {
+                const double br = v[RI_BR], bt = v[RI_BT], bp = v[RI_BP];
+                const double valr = br * br * a.ph.LFfac * orho1;
+                const double valr2 = valr * valr / (valr + vri2);
+                mr = fmax(mr, or4 * (a.ph.cf2 * vflr2 + a.ph.af2 * valr2));
+                const double valh2 = (bt * bt + bp * bp) * a.ph.LFfac * o2 * orho1;
+                const double valh2m = valh2 * valh2 / (valh2 + vhi2);
+                mh = fmax(mh, or2 * (a.ph.cf2 * vflh2 + a.ph.af2 * valh2m));
+            } else {
+                mr = fmax(mr, a.ph.cf2 * or4 * vflr2);
+                mh = fmax(mh, a.ph.cf2 * or2 * vflh2);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RO_N; ++k) {
+            const int row = a.out_of[k];
+            if (row >= 0) R[idx(row, j)] = ((a.out_osin2 >> row) & 1u) ? o[k] * o2 : o[k];
+        }
+    }
+    if (a.cmax) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            mr = fmax(mr, __shfl_xor(mr, off, 64));
+            mh = fmax(mh, __shfl_xor(mh, off, 64));
+        }
+        if ((tid & 63) == 0) {
+            atomicMax(a.cmax + 2 * lev, (unsigned long long)__double_as_longlong(mr));
+            atomicMax(a.cmax + 2 * lev + 1, (unsigned long long)__double_as_longlong(mh));
+        }
+    }
+}
+
 __global__ __launch_bounds__(512) void k_rl_grid(FFTArgs f, RlGridArgs a)
 {
     typedef double2 C;
@@ -738,61 +802,7 @@ __global__ __launch_bounds__(512) void k_rl_grid(FFTArgs f, RlGridArgs a)
     }
 
     // the products, point by point, in place
-    const double *rad = a.rad;
-    const double r = rad[RR_R * nlev + lev], or1 = rad[RR_OR1 * nlev + lev];
-    const double or2 = rad[RR_OR2 * nlev + lev], or4 = rad[RR_OR4 * nlev + lev];
-    const double orho1 = rad[RR_ORHO1 * nlev + lev], beta = rad[RR_BETA * nlev + lev];
-    const double otemp1 = rad[RR_OTEMP1 * nlev + lev], visc = rad[RR_VISC * nlev + lev];
-    const double lambda = rad[RR_LAMBDA * nlev + lev];
-    const bool lcr = rad[RR_LCR * nlev + lev] != 0.0;
-    const double o2 = a.th[RT_OSIN2 * n_theta + th], cosn2 = a.th[RT_COSN2 * n_theta + th];
-    const double sinT = a.th[RT_SIN * n_theta + th], cosT = a.th[RT_COS * n_theta + th];
-    const double osinT = a.th[RT_OSIN * n_theta + th];
-    double *R = reinterpret_cast<double *>(rows);  // row c: R[c * N + j]
-    double mr = 0.0, mh = 0.0;
-    const double vri2 = rad[RR_VALRI2 * nlev + lev], vhi2 = rad[RR_VALHI2 * nlev + lev];
-    const double orho2 = orho1 * orho1;
-    for (int j = tid; j < N; j += nthr) {
-        double v[RI_N], o[RO_N];
-#pragma unroll
-        for (int k = 0; k < RI_N; ++k) v[k] = a.in_of[k] >= 0 ? R[(size_t)a.in_of[k] * N + j] : 0.0;
-        rl_products(a.ph, v, o, o2, cosn2, sinT, cosT, osinT, j * a.dphi, r, or1, or2, or4, orho1,
-                    beta, otemp1, visc, lambda, lcr);
-        if (a.cmax) {
-            // src/courant.f90:209-275 (XSH_COURANT = 0)
-            const double vr = v[RI_VR], vt = v[RI_VT], vp = v[RI_VP];
-            const double vflr2 = orho2 * vr * vr;
-            const double vflh2 = (vt * vt + vp * vp) * o2 * orho2;
-            if (a.ph.cour_mag) {
-                const double br = v[RI_BR], bt = v[RI_BT], bp = v[RI_BP];
-                const double valr = br * br * a.ph.LFfac * orho1;
-                const double valr2 = valr * valr / (valr + vri2);
-                mr = fmax(mr, or4 * (a.ph.cf2 * vflr2 + a.ph.af2 * valr2));
-                const double valh2 = (bt * bt + bp * bp) * a.ph.LFfac * o2 * orho1;
-                const double valh2m = valh2 * valh2 / (valh2 + vhi2);
-                mh = fmax(mh, or2 * (a.ph.cf2 * vflh2 + a.ph.af2 * valh2m));
-            } else {
-                mr = fmax(mr, a.ph.cf2 * or4 * vflr2);
-                mh = fmax(mh, a.ph.cf2 * or2 * vflh2);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < RO_N; ++k) {
-            const int row = a.out_of[k];
-            if (row >= 0) R[(size_t)row * N + j] = ((a.out_osin2 >> row) & 1u) ? o[k] * o2 : o[k];
-        }
-    }
-    if (a.cmax) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            mr = fmax(mr, __shfl_xor(mr, off, 64));
-            mh = fmax(mh, __shfl_xor(mh, off, 64));
-        }
-        if ((tid & 63) == 0) {
-            atomicMax(a.cmax + 2 * lev, (unsigned long long)__double_as_longlong(mr));
-            atomicMax(a.cmax + 2 * lev + 1, (unsigned long long)__double_as_longlong(mh));
-        }
-    }
+    rl_points(a, reinterpret_cast<double *>(rows), [&](int c, int j) { return (size_t)c * N + j; }, N, lev, th);
     __syncthreads();
 
     // forward FFTs of rows[0, nout) into F2[lev][comp][theta][m]
@@ -806,6 +816,82 @@ __global__ __launch_bounds__(512) void k_rl_grid(FFTArgs f, RlGridArgs a)
             Fo[((size_t)(g0 + r2) * n_theta + th) * n_m + k] = rfft_post(res + (size_t)r2 * M, k, M, s_twr, invN);
         }
         __syncthreads();
+    }
+}
+
+// k_rl_grid with wave-slot FFTs (the row length M = 3 4^A 2^B): the rows
+// are padded (index i at pidx<P>(i)) and every row is transformed in place
+// by one slot of Q lanes of a wave, all rows at once; no scratch rows and no
+// block barrier inside the FFTs
+template <int A, int B>
+__global__ __launch_bounds__(512) void k_rl_grid_s(FFTArgs f, RlGridArgs a)
+{
+    typedef double2 C;
+    typedef SlotFFT<A, B> FFT;
+    constexpr int M = FFT::M, N = 2 * M, Q = FFT::Q, P = FFT::P;
+    constexpr int MP = padded_row<M, P>(), SPW = 64 / Q;
+    extern __shared__ char smem_raw[];
+    C *s_tw = reinterpret_cast<C *>(smem_raw);  // M + M / 8 (padded)
+    C *s_twr = s_tw + M + M / 8;                // M + 1
+    C *rows = s_twr + M + 1;                    // max(nin, nout) rows of MP
+    int lb;
+    {
+        const int nx = 8, nb = gridDim.x, b = blockIdx.x;
+        int per = nb / nx, rem = nb % nx, x = b % nx, i = b / nx;
+        lb = (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + i;
+    }
+    const int n_theta = a.n_theta, n_m = a.n_m;
+    const int th = lb % n_theta, lev = lb / n_theta;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int lane = tid & 63, sw = lane / Q, t = lane - sw * Q;
+    const int slot = (tid >> 6) * SPW + sw, nslot = (nthr >> 6) * SPW;
+    const bool act = sw < SPW;
+    {
+        const C *tw = reinterpret_cast<const C *>(f.tw);
+        const C *twr = reinterpret_cast<const C *>(f.twr);
+        for (int e = tid; e < M; e += nthr) s_tw[e + (e >> 3)] = tw[e];
+        for (int e = tid; e <= M; e += nthr) s_twr[e] = twr[e];
+    }
+    __syncthreads();
+
+    // inverse FFTs of rows [0, nin), one slot per row
+    const C *Fi = reinterpret_cast<const C *>(a.Fin) + (size_t)lev * a.nin * n_theta * n_m + th;
+    if (act)
+        for (int c = slot; c < a.nin; c += nslot) {
+            C *z = rows + (size_t)c * MP;
+            const C *X = Fi + (size_t)c * n_m * n_theta;
+            // Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/N} (X_k - conj X_{M-k})
+            for (int k = t; k <= M / 2; k += Q) {
+                const int k2 = M - k;
+                C xa = k < n_m ? X[(size_t)k * n_theta] : mk<C>(0.0, 0.0);
+                C xc = k2 < n_m ? X[(size_t)k2 * n_theta] : mk<C>(0.0, 0.0);
+                if (k == 0) {
+                    xa.y = 0.0;
+                    xc.y = 0.0;
+                }
+                z[pidx<P>(k)] = cadd(cadd(xa, cconj(xc)), cmuli(cmul(cconj(s_twr[k]), csub(xa, cconj(xc))), 1));
+                if (k2 < M && k2 != k)
+                    z[pidx<P>(k2)] = cadd(cadd(xc, cconj(xa)), cmuli(cmul(cconj(s_twr[k2]), csub(xc, cconj(xa))), 1));
+            }
+            wave_order();
+            FFT::run(z, t, s_tw, +1);
+        }
+    __syncthreads();
+
+    // the products: point j of row c is the (j mod 2) part of complex j / 2
+    rl_points(a, reinterpret_cast<double *>(rows),
+              [&](int c, int j) { return (size_t)c * 2 * MP + 2 * pidx<P>(j >> 1) + (j & 1); }, N, lev, th);
+    __syncthreads();
+
+    // forward FFTs of rows [0, nout) in place, then F2[lev][comp][theta][m]
+    if (act)
+        for (int c = slot; c < a.nout; c += nslot) FFT::run(rows + (size_t)c * MP, t, s_tw, -1);
+    __syncthreads();
+    C *Fo = reinterpret_cast<C *>(a.Fout) + (size_t)lev * a.nout * n_theta * n_m;
+    const double invN = 1.0 / (double)N;
+    for (int idx = tid; idx < a.nout * n_m; idx += nthr) {
+        const int r2 = idx / n_m, k = idx - r2 * n_m;
+        Fo[((size_t)r2 * n_theta + th) * n_m + k] = rfft_post_padded<P>(rows + (size_t)r2 * MP, k, M, s_twr, invN);
     }
 }
 
@@ -1020,6 +1106,21 @@ int launch_nl_fused(const Plan *p, int nlev, const void *Fin, void *Fout, const 
     return launch_fused_t<double>(p, nlev, Fin, Fout, rad, LFfac, dgrid, full_grids, s);
 }
 
+template <int A, int B>
+int launch_rl_slots(const Plan *p, const FFTArgs &f, const RlGridArgs &a, size_t shm, hipStream_t s)
+{
+    static int shm_set = 0;
+    if (shm > 64 * 1024 && (int)shm > shm_set) {
+        MHX_HIP_CHECK(hipFuncSetAttribute((const void *)k_rl_grid_s<A, B>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        shm_set = (int)shm;
+    }
+    dim3 grid((unsigned)(p->n_theta * a.nlev));
+    k_rl_grid_s<A, B><<<grid, 512, shm, s>>>(f, a);
+    MHX_HIP_CHECK(hipGetLastError());
+    return MHX_OK;
+}
+
 int launch_rl_grid(const Plan *p, const RlGridArgs &args, hipStream_t s)
 {
     if (args.nlev <= 0) return MHX_OK;
@@ -1029,15 +1130,40 @@ int launch_rl_grid(const Plan *p, const RlGridArgs &args, hipStream_t s)
     const int nrow = std::max(a.nin, a.nout);
     if (nrow <= 0 || nrow > 32) return fail(MHX_E_INVALID, "radial loop: bad component count");
     const size_t lds_max = 160 * 1024;
+    // rows of M = 3 4^A 2^B: wave-slot FFTs on padded rows, all rows at once,
+    // when they fit the LDS (measured at l_max = 255 x 97, Boussinesq MHD:
+    // 10.2 -> 8.7 ms per mhx_rloop call against the block-wide FFTs below)
+    {
+        int A, B;
+        if (slot_form(p, A, B)) {
+            const int M = p->M, P = M / 12 <= 64 ? 12 : M / 64;
+            const size_t MP = P == 12 ? (size_t)M + (M >> 5) + 1 : (size_t)M;
+            const size_t shm = ((size_t)2 * M + M / 8 + 1 + (size_t)nrow * MP) * cs;
+            if (shm <= lds_max) {
+                switch (2 * A + B) {
+                case 2: return launch_rl_slots<1, 0>(p, f, a, shm, s);
+                case 3: return launch_rl_slots<1, 1>(p, f, a, shm, s);
+                case 4: return launch_rl_slots<2, 0>(p, f, a, shm, s);
+                case 5: return launch_rl_slots<2, 1>(p, f, a, shm, s);
+                case 6: return launch_rl_slots<3, 0>(p, f, a, shm, s);
+                case 7: return launch_rl_slots<3, 1>(p, f, a, shm, s);
+                case 8: return launch_rl_slots<4, 0>(p, f, a, shm, s);
+                case 9: return launch_rl_slots<4, 1>(p, f, a, shm, s);
+                default: break;
+                }
+            }
+        }
+    }
     const size_t fixed = ((size_t)2 * p->M + 1 + (size_t)nrow * p->M) * cs;
     if (fixed + (size_t)p->M * cs > lds_max)
         return fail(MHX_E_UNSUPPORTED, "radial loop: the grid rows of this n_phi and these physics "
                                        "switches exceed the 160 KB of LDS of a workgroup");
-    // as many FFT rows per pass as the LDS holds, at most 8 (64 threads per
-    // row); 512 threads per block: the rows of one block use most of the LDS,
-    // so one block per CU, whose eight waves hide each other's latency
-    // (measured at l_max = 255 x 97: 256 threads 11.3 ms per mhx_rloop call,
-    // 512 threads 10.3 ms, 768 threads 10.4 ms)
+    // other lengths: block-wide Stockham passes over as many FFT rows per pass
+    // as the LDS holds, at most 8 (64 threads per row); 512 threads per block:
+    // the rows of one block use most of the LDS, so one block per CU, whose
+    // eight waves hide each other's latency (measured at l_max = 255 x 97:
+    // 256 threads 11.3 ms per mhx_rloop call, 512 threads 10.3 ms, 768
+    // threads 10.4 ms)
     int G = (int)std::min<size_t>(8, (lds_max - fixed) / ((size_t)p->M * cs));
     G = std::max(1, std::min(G, nrow));
     a.G = G;
