@@ -520,45 +520,67 @@ int mhx_rloop(mhx_plan *h, const mhx_rloop_phys *ph, int nlev, const mhx_rloop_r
         j.or2 = d_or2;
         return j;
     };
+    // independent launches (the syntheses, then the analyses) dealt over
+    // the plan's branch streams, forked from and joined back into s
+    int nl_launch = 0;
+    auto on = [&]() -> hipStream_t {
+        const int i = nl_launch++ % P->nbranch;
+        return i == 0 ? s : P->br[i - 1];
+    };
+    auto fork = [&]() -> int {
+        nl_launch = 0;
+        if (P->nbranch < 2) return MHX_OK;
+        MHX_HIP_CHECK(hipEventRecord(P->ev_fork, s));
+        for (int i = 0; i + 1 < P->nbranch; ++i) MHX_HIP_CHECK(hipStreamWaitEvent(P->br[i], P->ev_fork, 0));
+        return MHX_OK;
+    };
+    auto join = [&]() -> int {
+        for (int i = 0; i + 1 < P->nbranch; ++i) {
+            MHX_HIP_CHECK(hipEventRecord(P->ev_join[i], P->br[i]));
+            MHX_HIP_CHECK(hipStreamWaitEvent(s, P->ev_join[i], 0));
+        }
+        return MHX_OK;
+    };
+    if ((rc = fork())) return rc;
     if (conv) {
         // torpol_to_spat(w, dw, z): Q = l(l+1) w, S = dw, T = z
         SynJob j = syn(SYN_QST, {W, DW, Z}, in_of[RI_VR], in_of[RI_VT], in_of[RI_VP]);
         rl_qst_cols(j, rop(0, G_LL), rop(1), rop(2), true);
-        if ((rc = launch_syn(P, j, s))) return rc;
+        if ((rc = launch_syn(P, j, on()))) return rc;
         if (curl) {
             // torpol_to_curl_spat(or2, w, ddw, z, dz): Q = l(l+1) z, S = dz,
             // T = or2 l(l+1) w - ddw
             SynJob c = syn(SYN_QST, {W, DDW, Z, DZ}, in_of[RI_CVR], in_of[RI_CVT], in_of[RI_CVP]);
             rl_qst_cols(c, rop(2, G_LL), rop(3), rop(0, G_OR2LL, 0, 0, 1.0, 1, -1.0), true);
-            if ((rc = launch_syn(P, c, s))) return rc;
+            if ((rc = launch_syn(P, c, on()))) return rc;
         }
         if (need_grad) {
             if (!curl) {
                 // pol_to_curlr_spat(z): l(l+1) z
                 SynJob c = syn(SYN_SCAL, {Z}, in_of[RI_CVR], 0, 0);
                 c.col[0] = rop(0, G_LL);
-                if ((rc = launch_syn(P, c, s))) return rc;
+                if ((rc = launch_syn(P, c, on()))) return rc;
             }
             // torpol_to_spat(dw, ddw, dz)
             SynJob d = syn(SYN_QST, {DW, DDW, DZ}, in_of[RI_DVRDR], in_of[RI_DVTDR], in_of[RI_DVPDR]);
             rl_qst_cols(d, rop(0, G_LL), rop(1), rop(2), true);
-            if ((rc = launch_syn(P, d, s))) return rc;
+            if ((rc = launch_syn(P, d, on()))) return rc;
             // pol_to_grad_spat(w): gradient of l(l+1) w
             SynJob g = syn(SYN_GRAD, {W}, in_of[RI_DVRDT], in_of[RI_DVRDP], 0);
             g.col[0] = rop(0, G_LL);
             g.col[1] = rop(0, G_LL, 1, 1);
-            if ((rc = launch_syn(P, g, s))) return rc;
+            if ((rc = launch_syn(P, g, on()))) return rc;
             // torpol_to_dphspat(dw, z): S = i m dw, T = i m z, times 1/sin^2
             SynJob q = syn(SYN_SPHTOR, {DW, Z}, in_of[RI_DVTDP], in_of[RI_DVPDP], 0);
             rl_qst_cols(q, ColOp(), rop(0, G_ONE, 1, 1), rop(1, G_ONE, 1, 1), false);
             q.row_scale[0] = q.row_scale[1] = 1;
-            if ((rc = launch_syn(P, q, s))) return rc;
+            if ((rc = launch_syn(P, q, on()))) return rc;
         }
     }
     auto scalar = [&](int src, int comp) -> int {
         SynJob j = syn(SYN_SCAL, {src}, comp, 0, 0);
         j.col[0] = rop(0);
-        return launch_syn(P, j, s);
+        return launch_syn(P, j, on());
     };
     if (heat && (rc = scalar(S_, in_of[RI_S]))) return rc;
     if (chem && (rc = scalar(XI, in_of[RI_XI]))) return rc;
@@ -566,11 +588,13 @@ int mhx_rloop(mhx_plan *h, const mhx_rloop_phys *ph, int nlev, const mhx_rloop_r
     if (mag) {
         SynJob b = syn(SYN_QST, {BB, DB, AJ}, in_of[RI_BR], in_of[RI_BT], in_of[RI_BP]);
         rl_qst_cols(b, rop(0, G_LL), rop(1), rop(2), true);
-        if ((rc = launch_syn(P, b, s))) return rc;
+        if ((rc = launch_syn(P, b, on()))) return rc;
         SynJob c = syn(SYN_QST, {BB, DDB, AJ, DJ}, in_of[RI_CBR], in_of[RI_CBT], in_of[RI_CBP]);
         rl_qst_cols(c, rop(2, G_LL), rop(3), rop(0, G_OR2LL, 0, 0, 1.0, 1, -1.0), true);
-        if ((rc = launch_syn(P, c, s))) return rc;
+        if ((rc = launch_syn(P, c, on()))) return rc;
     }
+
+    if ((rc = join())) return rc;
 
     // ---- grid stage ----
     RlGridArgs g;
@@ -645,8 +669,9 @@ int mhx_rloop(mhx_plan *h, const mhx_rloop_phys *ph, int nlev, const mhx_rloop_r
         j.out[2] = o2;
         j.out_ld = ld;
         j.lcut = B->lcut;
-        return launch_ana(P, j, s);
+        return launch_ana(P, j, on());
     };
+    if ((rc = fork())) return rc;
     // the nonlinear_lm_t arrays live in nl[] with the caller's level stride
     // ld_lm, so that the outputs the reference writes straight from the
     // analysis (dVSrLM, dVXirLM, dphidt) share a launch with them
@@ -657,6 +682,7 @@ int mhx_rloop(mhx_plan *h, const mhx_rloop_phys *ph, int nlev, const mhx_rloop_r
         return rc;
     if (anel && (rc = ana(ANA_SH, out_of[RO_HEAT], nl[NL_HEAT], nullptr, nullptr, ld_lm))) return rc;
     if (phase && (rc = ana(ANA_SH, out_of[RO_PHI], dout[7], nullptr, nullptr, ld_lm))) return rc;
+    if ((rc = join())) return rc;
 
     // ---- lm-space assembly ----
     RlTdArgs t;
