@@ -184,17 +184,18 @@ RLOOP_BOUSS_MHD = {"l_conv": 1, "l_conv_nl": 1, "l_heat": 1, "l_heat_nl": 1, "l_
                    "courfac": 0.8, "alffac": 0.35, "oek": 1000.0, "ra": 1.1e5, "opr": 1.0}
 
 
-def run_rloop(L, nlev, tin, reps=10):
-    """The reference-side radial loop (mhx_rloop, FP64, the body of the
+def run_rloop(L, nlev, tin, precision, reps=10):
+    """The reference-side radial loop (mhx_rloop, the body of the
     rIter_batched_t plugin: syntheses, grid products, Courant steps,
     analyses and the lm-space assembly of every interior level) on the same
-    fields, host-synchronous per call as the reference's loop."""
+    fields in the given precision, host-synchronous per call as the
+    reference's loop."""
     import numpy as np
     import torch
 
     from magic_amd import sht, workload
     nt, nph = workload.grid_size(L)
-    p = sht.Plan(L, nt, nph, max_levels=nlev)
+    p = sht.Plan(L, nt, nph, precision=precision, max_levels=nlev)
     dev = tin[0].device
     fields = dict(zip(sht.NL_INPUTS, tin))
     names = ("dwdt", "dzdt", "dpdt", "dsdt", "dVSrLM", "dbdt", "djdt", "dVxBhLM")
@@ -342,7 +343,7 @@ def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut, courant=False):
     outs = [t.cpu().numpy() for t in tout]
     td = run_td(L, nlev, tout, rad, lcut) if precision == sht.PREC_FP64 else None
     diag = run_diag(L, nlev, tin) if precision == sht.PREC_FP64 else None
-    rl = run_rloop(L, nlev, tin) if precision == sht.PREC_FP64 else None
+    rl = run_rloop(L, nlev, tin, precision) if L <= 255 else None
     p.close()
     return {"td": td, "diag": diag, "rloop": rl, "pairs": nlev * steps / dt, "ms_per_step": 1e3 * dt / steps,
             "per_kernel": per_kernel, "kernels": kernels,
@@ -637,10 +638,13 @@ def main():
         res["td_assembly"] = forms["fp64"]["td"]
         if forms["fp64"].get("diag"):
             res["e_kin"], res["boundary_products"] = forms["fp64"]["diag"]
-        if forms["fp64"].get("rloop"):
-            # the radial loop the reference-side plugin runs (FP64, not in `value`)
-            res["radial_loop"] = forms["fp64"]["rloop"]
-            log("mhx_rloop: %.3f ms per call" % res["radial_loop"]["median_ms_per_call"])
+        # the radial loop the reference-side plugin runs, per plan precision
+        # (not in `value`)
+        rls = {k: f["rloop"] for k, f in forms.items() if f.get("rloop")}
+        if rls:
+            res["radial_loop"] = rls
+            for k, v in rls.items():
+                log("mhx_rloop %s: %.3f ms per call" % (k, v["median_ms_per_call"]))
         log("get_td: %.3f ms per call (%.0f GB/s)" % (res["td_assembly"]["avg_ms_per_call"],
                                                        res["td_assembly"]["GB/s"]))
     if "fp64" in forms and main_key != "fp64":

@@ -296,6 +296,66 @@ void rl_free(Plan *p)
     p->rl = nullptr;
 }
 
+// The switch sets the fused step covers (src/rIter.f90:483-707 with curl
+// advection, heat, the full MHD terms and no double curl, composition, phase
+// field, precession or centrifugal term): the fused step with its Courant
+// maxima (mhx_nl_step_courant), then its lm-space assembly (mhx_get_td), in
+// the plan's precision, as magic_amd/rloop.py composes them; returns when
+// done.  The Courant steps of src/courant.f90:271-272 (1e10 where a level has
+// no flow) from the maxima on the host.
+bool rl_fused_covers(const mhx_rloop_phys *ph)
+{
+    return (ph->l_conv || ph->l_mag_kin) && ph->l_conv_nl && ph->l_adv_curl && !ph->l_anel &&
+           !ph->l_anelastic_liquid && ph->l_heat && ph->l_heat_nl && ph->l_mag && ph->l_mag_LF &&
+           ph->l_mag_nl && !ph->l_mag_kin && !ph->l_chemical_conv && !ph->l_phase_field &&
+           !ph->l_precession && !ph->l_centrifuge && !ph->l_double_curl && !ph->l_single_matrix &&
+           ph->l_cour_alf_damp && ph->n_r_LCR == 0;
+}
+
+int rl_fused(mhx_plan *h, const mhx_rloop_phys *ph, int nlev, const mhx_rloop_radial *rad,
+             const mhx_rloop_fields *in, size_t ld_lm, const mhx_rloop_out *out, void *stream)
+{
+    Plan *P = &h->p;
+    if (!P->rl) P->rl = new RlBuffers();
+    RlBuffers *B = P->rl;
+    const size_t arr = (size_t)nlev * ld_lm * 2;
+    int rc;
+    if ((rc = grow(B->nl, B->nnl, 9 * arr))) return rc;
+    // the step's forward outputs: VSrLM is the caller's dVSrLM, the rest scratch
+    double *nl[10];
+    for (int k = 0, j = 0; k < 10; ++k) nl[k] = k == 3 ? out->dVSrLM : B->nl + (size_t)(j++) * arr;
+    const double *fin[11] = {in->w, in->dw, in->ddw, in->z, in->dz, in->s,
+                             in->b, in->db, in->ddb, in->aj, in->dj};
+    // valri2, valhi2 (src/courant.f90:177-183), epsc epscProf (src/get_td.f90:480)
+    const double damp = (0.5 * (1.0 + ph->opm)) * (0.5 * (1.0 + ph->opm));
+    std::vector<double> vri(nlev), vhi(nlev), eps(nlev), vr2(nlev), vh2(nlev), dt(2 * (size_t)nlev);
+    for (int i = 0; i < nlev; ++i) {
+        vri[i] = damp / rad->delxr2[i];
+        vhi[i] = damp / rad->delxh2[i];
+        eps[i] = ph->epsc * rad->epscProf[i];
+    }
+    if ((rc = mhx_nl_step_courant(h, nlev, fin, ld_lm, nl, rad->or2, rad->or4, rad->orho1, ph->LFfac,
+                                  rad->l_R, ph->courfac, ph->alffac, vri.data(), vhi.data(), vr2.data(),
+                                  vh2.data(), stream)))
+        return rc;
+    // AdvrLM AdvtLM AdvpLM VStLM VxBrLM VxBtLM VxBpLM, then w dw z
+    const double *tin[10] = {nl[0], nl[1], nl[2], nl[4], nl[6], nl[7], nl[8], in->w, in->dw, in->z};
+    double *tout[7] = {out->dwdt, out->dzdt, out->dpdt, out->dsdt, out->dbdt, out->djdt, out->dVxBhLM};
+    if ((rc = mhx_get_td(h, nlev, tin, ld_lm, tout, rad->or2, rad->or4, eps.data(),
+                         ph->l_corr ? ph->CorFac : 0.0, rad->l_R, stream)))
+        return rc;
+    for (int i = 0; i < nlev; ++i) {
+        dt[i] = vr2[i] != 0.0 ? std::min(1e10, std::sqrt(rad->delxr2[i] / vr2[i])) : 1e10;
+        dt[nlev + i] = vh2[i] != 0.0 ? std::min(1e10, std::sqrt(rad->delxh2[i] / vh2[i])) : 1e10;
+    }
+    MHX_HIP_CHECK(hipMemcpyAsync(out->dtrkc, dt.data(), (size_t)nlev * sizeof(double), hipMemcpyDefault,
+                                 P->stream));
+    MHX_HIP_CHECK(hipMemcpyAsync(out->dthkc, dt.data() + nlev, (size_t)nlev * sizeof(double),
+                                 hipMemcpyDefault, P->stream));
+    MHX_HIP_CHECK(hipStreamSynchronize(P->stream));  // dt leaves scope
+    return MHX_OK;
+}
+
 }  // namespace mhx
 
 using namespace mhx;
@@ -334,6 +394,7 @@ int mhx_rloop(mhx_plan *h, const mhx_rloop_phys *ph, int nlev, const mhx_rloop_r
     if (dcurl && !out->dVxVhLM) return fail(MHX_E_INVALID, "l_double_curl without dVxVhLM");
     for (int i = 0; i < nlev; ++i)
         if (rad->l_R[i] < 0 || rad->l_R[i] > P->l_max) return fail(MHX_E_INVALID, "l_R out of range");
+    if (rl_fused_covers(ph)) return rl_fused(h, ph, nlev, rad, in, ld_lm, out, stream);
 
     int rc;
     if (!P->rl) P->rl = new RlBuffers();

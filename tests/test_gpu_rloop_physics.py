@@ -20,11 +20,11 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def _run(d, device):
+def _run(d, device, precision=sht.PREC_FP64):
     import torch
     sel = rc.interior(d)
     nlev = sel.size
-    p = sht.Plan(**rc.plan_args(d), max_levels=nlev)
+    p = sht.Plan(**rc.plan_args(d), precision=precision, max_levels=nlev)
     fields = rc.fields_of(d, sel)
     names = rc.outputs_of(d)
     if device:
@@ -77,3 +77,24 @@ def test_rloop_refuses_missing_inputs():
         sht.rloop(p, rc.phys_of(d), sel.size, rc.radial_of(d, sel), fields, outs, np.zeros(sel.size),
                   np.zeros(sel.size))
     p.close()
+
+
+def test_rloop_split_plan_runs_the_fused_split_step():
+    """A split-precision plan with the Boussinesq MHD switches and curl
+    advection (the switch sets the fused step covers) runs the fused split
+    step inside mhx_rloop: its explicit terms within 2e-5 and its Courant
+    steps within 1e-5 of the reference's loop (the split form's bounds on
+    this state), and device arrays give the host arrays' numbers."""
+    d = rc.load("bouss_mhd")
+    print("rloop bouss_mhd, split plan", flush=True)
+    sel, outs, dtr, dth = _run(d, device=False, precision=sht.PREC_SPLIT16)
+    for n, got in outs.items():
+        e = rel_l2(got, d[n][sel])
+        print("  %-8s %.2e" % (n, e), flush=True)
+        assert e < 2e-5, (n, e)
+    np.testing.assert_allclose(dtr, d["dtrkc"][sel], rtol=1e-5, atol=0)
+    np.testing.assert_allclose(dth, d["dthkc"][sel], rtol=1e-5, atol=0)
+    _, dev, d1, d2 = _run(d, device=True, precision=sht.PREC_SPLIT16)
+    for n in outs:
+        assert np.array_equal(outs[n], dev[n]), n
+    assert np.array_equal(dtr, d1) and np.array_equal(dth, d2)
