@@ -184,7 +184,7 @@ RLOOP_BOUSS_MHD = {"l_conv": 1, "l_conv_nl": 1, "l_heat": 1, "l_heat_nl": 1, "l_
                    "courfac": 0.8, "alffac": 0.35, "oek": 1000.0, "ra": 1.1e5, "opr": 1.0}
 
 
-def run_rloop(L, nlev, tin, precision, reps=10):
+def run_rloop(L, nlev, tin, precision, switches=None, reps=10):
     """The reference-side radial loop (mhx_rloop, the body of the
     rIter_batched_t plugin: syntheses, grid products, Courant steps,
     analyses and the lm-space assembly of every interior level) on the same
@@ -209,8 +209,10 @@ def run_rloop(L, nlev, tin, precision, reps=10):
     dtr = torch.zeros(nlev, dtype=torch.float64, device=dev)
     dth = torch.zeros(nlev, dtype=torch.float64, device=dev)
 
+    sw = RLOOP_BOUSS_MHD if switches is None else switches
+
     def call():
-        sht.rloop(p, RLOOP_BOUSS_MHD, nlev, rad, fields, outs, dtr, dth)
+        sht.rloop(p, sw, nlev, rad, fields, outs, dtr, dth)
     for _ in range(3):
         call()
     ts = []
@@ -219,7 +221,8 @@ def run_rloop(L, nlev, tin, precision, reps=10):
         call()
         ts.append(time.perf_counter() - t0)
     p.close()
-    return {"entry": "mhx_rloop", "switches": "Boussinesq MHD, curl advection",
+    return {"entry": "mhx_rloop",
+            "switches": "Boussinesq MHD, " + ("curl" if sw["l_adv_curl"] else "u.grad u") + " advection",
             "levels": nlev, "median_ms_per_call": 1e3 * float(np.median(ts)),
             "timing": "host clock, synchronous call"}
 
@@ -343,7 +346,13 @@ def run_form(precision, L, nlev, steps, warmup, tin, rad, lcut, courant=False):
     outs = [t.cpu().numpy() for t in tout]
     td = run_td(L, nlev, tout, rad, lcut) if precision == sht.PREC_FP64 else None
     diag = run_diag(L, nlev, tin) if precision == sht.PREC_FP64 else None
-    rl = run_rloop(L, nlev, tin, precision) if L <= 255 else None
+    rl = None
+    if L <= 255:
+        rl = {"curl": run_rloop(L, nlev, tin, precision)}
+        if precision == sht.PREC_FP64:
+            # u.grad u advection: outside the fused step's switch sets, the
+            # loop's own kernels
+            rl["ugradu"] = run_rloop(L, nlev, tin, precision, dict(RLOOP_BOUSS_MHD, l_adv_curl=0))
     p.close()
     return {"td": td, "diag": diag, "rloop": rl, "pairs": nlev * steps / dt, "ms_per_step": 1e3 * dt / steps,
             "per_kernel": per_kernel, "kernels": kernels,
@@ -644,7 +653,8 @@ def main():
         if rls:
             res["radial_loop"] = rls
             for k, v in rls.items():
-                log("mhx_rloop %s: %.3f ms per call" % (k, v["median_ms_per_call"]))
+                for kk, vv in v.items():
+                    log("mhx_rloop %s %s: %.3f ms per call" % (k, kk, vv["median_ms_per_call"]))
         log("get_td: %.3f ms per call (%.0f GB/s)" % (res["td_assembly"]["avg_ms_per_call"],
                                                        res["td_assembly"]["GB/s"]))
     if "fp64" in forms and main_key != "fp64":

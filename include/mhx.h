@@ -256,7 +256,7 @@ int mhx_nl_step_td(mhx_plan *p, int nlev, const double *const *in, size_t ld_lm,
 /* ---- the radial loop for any physics switches, interior levels ----------
  * One call replaces the loop body of rIter_single_t%radialLoop
  * (src/rIter.f90:188-444) for nlev interior levels (nBc = 0) of a time step
- * stage, on the GPU in FP64 whatever the plan's precision: the inverse
+ * stage, on the GPU: the inverse
  * transforms of transform_to_grid_space (:466-622), the grid products of
  * get_nl (src/get_nl.f90:213-441) merged as transform_to_lm_space
  * (:644-680), the forward transforms (:682-707), the Courant steps
@@ -266,7 +266,12 @@ int mhx_nl_step_td(mhx_plan *p, int nlev, const double *const *in, size_t ld_lm,
  * caller runs the reference's own loop for such stages): boundary levels,
  * the diagnostics computed inside the loop (graph, movie, TO, RMS, power,
  * helicity, ...), l_full_sphere's centre, finite-difference l_parallel_solve
- * and the temperature-diffusion single-matrix form.
+ * and the temperature-diffusion single-matrix form.  The switch sets the
+ * fused step covers (curl advection, heat, the full MHD terms with
+ * l_cour_alf_damp; no anelastic, double-curl, composition, phase-field,
+ * precession or centrifugal terms, n_r_LCR = 0) run mhx_nl_step_courant and
+ * mhx_get_td in the plan's precision and return when done; the others run
+ * FP64 kernels whatever the plan's precision.
  *
  * The switches and constants the loop reads from the reference's logic,
  * physical_parameters and time-scheme data; booleans as 0/1. */
