@@ -137,7 +137,9 @@ contains
 
    subroutine initialize(this)
       class(rIter_batched_t) :: this
-      integer :: nR, n
+      integer :: nR, n, plen, pstat
+      integer(c_int) :: prec
+      character(len=16) :: pname
 
       call this%host%initialize()
       this%nR0 = max(nRstart, n_r_cmb+1)
@@ -148,8 +150,14 @@ contains
       do nR=this%nR0,this%nR1
          this%nR_arr(nR-this%nR0+1) = nR
       end do
+      ! the plan's precision: FP64 (MHX_PREC_FP64), or the split form
+      ! (MHX_PREC_SPLIT16) with the environment variable MHX_PRECISION=split;
+      ! the switch sets the fused step covers then run it in that precision
+      prec = 0_c_int
+      call get_environment_variable('MHX_PRECISION', pname, plen, pstat)
+      if ( pstat == 0 .and. trim(pname) == 'split' ) prec = 1_c_int
       call check(mhx_plan_create(int(l_max,c_int), int(m_max,c_int), int(minc,c_int),   &
-           &     int(n_theta_max,c_int), int(n_phi_max,c_int), 0_c_int, int(max(n,1),c_int), &
+           &     int(n_theta_max,c_int), int(n_phi_max,c_int), prec, int(max(n,1),c_int), &
            &     this%plan), 'plan')
    end subroutine initialize
 
