@@ -193,7 +193,7 @@ __global__ void k_nl_fused(FFTArgs f, const T *__restrict__ Fin, T *__restrict__
                         int comp = c0 + g0 + r2;
                         if (full_grids)
                             full_grids[(((size_t)lev * 18 + comp) * N + j) * n_theta + th] = v;
-                        if (comp >= 6 && comp < 9)
+                        if (comp >= 6 && comp < 9 && dgrid)
                             dgrid[(((size_t)lev * 3 + comp - 6) * n_theta + th) * N + j] = v;
                     }
                     val[g0 + r2][p] = v;
@@ -394,8 +394,10 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
 
     // inverse FFTs of six comps into val: round 0 = u, curl u (comps 0-5),
     // round 1 = B, curl B (9-14), round 2 = du/dr (6-8), s, grad s (15-17)
+    // without du/dr grids (dgrid null) round 2 transforms s and grad s only
+    // (the products read no du/dr)
     auto inverse6 = [&](int round) {
-        if (slot < 6) {
+        if (slot < 6 && (round != 2 || slot >= 3 || dgrid)) {
             C *z = rows + slot * MP;
             const int comp = round == 0 ? slot : (round == 1 ? 9 + slot : (slot < 3 ? 6 + slot : 12 + slot));
             const C *X = Fi + (size_t)comp * n_m * n_theta;
@@ -465,7 +467,7 @@ __global__ __launch_bounds__(384) void k_nl_slots(FFTArgs f, const T *__restrict
                         fg[0] = zz.x;
                         fg[n_theta] = zz.y;
                     }
-                    if (round == 2 && r < 3)
+                    if (round == 2 && r < 3 && dgrid)
                         *reinterpret_cast<C *>(dgrid + (((size_t)lev * 3 + r) * n_theta + th) * N + 2 * q) = zz;
                 }
                 val[r][2 * p2] = zz.x;

@@ -266,14 +266,19 @@ __global__ __launch_bounds__(64 * TR) void k_grid_tile(GtArgs a)
     const int tr = tid & (TR - 1), tmm = tid / TR;
     const C *Fi = reinterpret_cast<const C *>(a.Fin) + (size_t)lev * 18 * n_m * n_theta + th0 + tr;
     C pre[2][GT_MT];
+    // without du/dr grids (dgrid null) their components 6..8 are not read:
+    // zeros go into the FFT halves they share
+    const bool dudr = a.dgrid != nullptr;
     auto load_pair = [&](int c0, int c1) {
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            const C *src = Fi + (size_t)(c ? c1 : c0) * n_m * n_theta;
+            const int comp = c ? c1 : c0;
+            const bool skip = !dudr && comp >= 6 && comp <= 8;
+            const C *src = Fi + (size_t)comp * n_m * n_theta;
 #pragma unroll
             for (int t = 0; t < GT_MT; ++t) {
                 const int m = tmm + 64 * t;
-                pre[c][t] = m < n_m ? src[(size_t)m * n_theta] : mk<C>(0.f, 0.f);
+                pre[c][t] = (m < n_m && !skip) ? src[(size_t)m * n_theta] : mk<C>(0.f, 0.f);
             }
         }
     };
@@ -310,8 +315,9 @@ __global__ __launch_bounds__(64 * TR) void k_grid_tile(GtArgs a)
             fg[(((size_t)lev * 18 + c1) * GT_N + n) * n_theta + th] = y[q].y;
         }
     };
-    float *dg = a.dgrid + (size_t)lev * 3 * n_theta * GT_N;
+    float *dg = dudr ? a.dgrid + (size_t)lev * 3 * n_theta * GT_N : nullptr;
     auto dgrid_out = [&](int c, const C (&y)[12], bool imag) {
+        if (!dudr) return;
         float *d = dg + ((size_t)c * n_theta + th) * GT_N + j;
 #pragma unroll
         for (int q = 0; q < 12; ++q) d[64 * q] = imag ? y[q].y : y[q].x;

@@ -1103,6 +1103,8 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
     // a split-precision plan runs the Legendre stage in scaled FP16 x3 and
     // keeps Fourier data and grids in FP32; the per-call API stays FP64
     const bool split = P->precision == MHX_PREC_SPLIT16;
+    // du/dr grids: always when the caller asks for the grids
+    const bool dudr = P->dudr || grids;
     int nev = 0;
     auto mark = [&]() -> int {
         if (P->timing && nev < Plan::NEV) MHX_HIP_CHECK(hipEventRecord(P->ev[nev++], s));
@@ -1150,13 +1152,15 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
             const int qsrc[5][4] = {{W, DW, Z, -1}, {Z, DZ, W, DDW}, {DW, DDW, DZ, -1},
                                     {B, DB, AJ, -1}, {AJ, DJ, B, DDB}};
             for (int v = 0; v < 5; ++v) {
+                if (v == 2 && !dudr) continue;
                 for (int k = 0; k < 4; ++k) a.src[k] = qsrc[v][k] >= 0 ? din[qsrc[v][k]] : nullptr;
                 a.comp0 = 3 * v;
                 const int kind = qsrc[v][3] >= 0 ? NS_QSTC : NS_QST;
                 if ((rc2 = launch_nsyn16(P, kind, a, on(v))) || (rc2 = mark())) return rc2;
             }
             if ((rc2 = join())) return rc2;
-            if ((rc2 = launch_nl_fused(P, nlev, P->ws_F, P->ws_F2, P->d_rad, LFfac, P->ws_G, grids, s)) ||
+            if ((rc2 = launch_nl_fused(P, nlev, P->ws_F, P->ws_F2, P->d_rad, LFfac, dudr ? P->ws_G : nullptr,
+                                       grids, s)) ||
                 (rc2 = mark()))
                 return rc2;
             NAnaArgs b;
@@ -1200,6 +1204,7 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
                 {AJ, DJ, B, DDB, true, 12},   // curl B
             };
             for (int v = 0; v < 5; ++v) {
+                if (v == 2 && !dudr) continue;
                 const QSpec &q = qs[v];
                 SynJob j = syn_job(P, SYN_QST, nlev, ld_lm);
                 j.src[0] = din[q.q];
@@ -1213,7 +1218,7 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
             }
             if ((rc2 = join())) return rc2;
             // fused: inverse FFTs, grid products, forward FFTs; du/dr grids to ws_G
-            if (((rc2 = launch_nl_fused(P, nlev, P->ws_F, P->ws_F2, P->d_rad, LFfac, P->ws_G,
+            if (((rc2 = launch_nl_fused(P, nlev, P->ws_F, P->ws_F2, P->d_rad, LFfac, dudr ? P->ws_G : nullptr,
                                                  grids, s)) ||
                            (rc2 = mark())))
                 return rc2;
@@ -1269,6 +1274,7 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
             for (int i = 0; i < 11; ++i) key.push_back((unsigned long long)(uintptr_t)din[i]);
             for (int i = 0; i < 10; ++i) key.push_back((unsigned long long)(uintptr_t)dout[i]);
             key.push_back(ti ? 1ull : 0ull);
+            key.push_back(dudr ? 1ull : 0ull);
             if (ti) {
                 for (int i = 0; i < 7; ++i) key.push_back((unsigned long long)(uintptr_t)dtd[i]);
                 unsigned long long cf;

@@ -98,3 +98,67 @@ def test_rloop_split_plan_runs_the_fused_split_step():
     for n in outs:
         assert np.array_equal(outs[n], dev[n]), n
     assert np.array_equal(dtr, d1) and np.array_equal(dth, d2)
+
+
+@pytest.mark.parametrize("precision", [sht.PREC_FP64, sht.PREC_SPLIT16])
+def test_rloop_fused_route_skips_dudr_at_l255(precision):
+    """mhx_rloop's fused route runs the step without the du/dr grids (they
+    are not among the loop's outputs).  At l_max=255 (the theta-row tile grid
+    stage in the split form, the slot stage in FP64), against
+    magic_amd.rloop.RadialLoop, which runs the full step with them: in FP64
+    the explicit terms, dVSrLM and Courant steps are identical (the skipped
+    work feeds no output); in the split form each is as close to the FP64
+    full step as the split full step is, within a factor 2 (the tile stage
+    pairs components in one complex FFT, so the empty du/dr halves change
+    FP32 rounding, and a field's error follows its partner's size: VxBp
+    (dbdt) shares its forward FFT with Advr)."""
+    import torch
+    from magic_amd import rloop, workload
+    L, nlev = 255, 5
+    nt, nph = workload.grid_size(L)
+    print("rloop fused route l_max=255, precision", precision, flush=True)
+    ins = workload.nl_inputs(L, nlev, stream=7)
+    r = np.linspace(0.6, 1.4, nlev)
+    one, zero = np.ones(nlev), np.zeros(nlev)
+    dxr, dxh = np.full(nlev, 1e-3), np.full(nlev, 2e-2)
+    ph = {"l_conv": 1, "l_conv_nl": 1, "l_heat": 1, "l_heat_nl": 1, "l_mag": 1, "l_mag_LF": 1,
+          "l_mag_nl": 1, "l_adv_curl": 1, "l_corr": 1, "l_cour_alf_damp": 1, "LFfac": 200.0,
+          "CorFac": 1000.0, "opm": 0.2, "courfac": 0.8, "alffac": 0.35, "oek": 1000.0,
+          "epsc": 0.5}
+    rad = {"r": r, "or1": 1 / r, "or2": 1 / r ** 2, "or4": 1 / r ** 4, "orho1": one, "beta": zero,
+           "otemp1": one, "visc": one, "lambda": one, "temp0": one, "epscProf": one,
+           "delxr2": dxr, "delxh2": dxh, "nR": np.arange(2, nlev + 2, dtype=np.int32),
+           "l_R": np.full(nlev, L, dtype=np.int32)}
+    names = ("dwdt", "dzdt", "dpdt", "dsdt", "dbdt", "djdt", "dVxBhLM", "dVSrLM")
+
+    def full_step(prec):
+        p = sht.Plan(L, nt, nph, precision=prec, max_levels=nlev)
+        rl = rloop.RadialLoop(p, nlev)
+        td, nl, d1, d2 = rl.bulk(ins, rad["or2"], rad["or4"], rad["orho1"], ph["LFfac"], ph["CorFac"],
+                                 ph["epsc"] * rad["epscProf"], rad["l_R"], ph["courfac"], ph["alffac"],
+                                 ph["opm"], dxr, dxh)
+        torch.cuda.synchronize()
+        o = {n: td[k].cpu().numpy() for k, n in enumerate(sht.TD_OUTPUTS)}
+        o["dVSrLM"] = nl[3].cpu().numpy()
+        p.close()
+        return o, d1, d2
+
+    p = sht.Plan(L, nt, nph, precision=precision, max_levels=nlev)
+    fields = dict(zip(sht.NL_INPUTS, ins))
+    outs = {n: np.zeros((nlev, p.lm_max), dtype=np.complex128) for n in names}
+    dtr, dth = np.zeros(nlev), np.zeros(nlev)
+    sht.rloop(p, ph, nlev, rad, fields, outs, dtr, dth)
+    p.close()
+    want, dtr2, dth2 = full_step(precision)
+    if precision == sht.PREC_FP64:
+        for n in names:
+            assert np.array_equal(outs[n], want[n]), n
+        assert np.array_equal(dtr, dtr2) and np.array_equal(dth, dth2)
+        return
+    ref, dtr64, dth64 = full_step(sht.PREC_FP64)
+    for n in names:
+        e_route, e_full = rel_l2(outs[n], ref[n]), rel_l2(want[n], ref[n])
+        print("  %-8s vs FP64: route %.2e, full split step %.2e" % (n, e_route, e_full), flush=True)
+        assert e_route <= 2 * e_full + 1e-7, (n, e_route, e_full)
+    for a, b, c in ((dtr, dtr2, dtr64), (dth, dth2, dth64)):
+        assert np.max(np.abs(a / c - 1)) <= 2 * np.max(np.abs(b / c - 1)) + 1e-7
