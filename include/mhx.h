@@ -417,6 +417,18 @@ int mhx_batch_shapes(const mhx_batch *batch, int *form, int *l_max, int *orders,
 int mhx_batch_status(mhx_batch *batch);
 int mhx_batch_destroy(mhx_batch *batch);
 
+/* Whether the nonlinear step (mhx_nl_step, mhx_nl_step_td,
+ * mhx_nl_step_courant) computes the du/dr grids: the synthesis of dw, ddw,
+ * dz (torpol_to_spat into dvrdrc, dvtdrc, dvpdrc) and their transforms.  On
+ * by default; the products and forward outputs do not use them, and with
+ * curl advection the reference computes them only on steps that write
+ * outputs (src/rIter.f90:527-538).  Off, the step skips that synthesis and
+ * the grids' writes (FP64: identical outputs; split form: the tile grid
+ * stage transforms empty halves in their place, which changes FP32
+ * rounding only).  mhx_nl_step_grids, and a plan timing its launches, keep
+ * them.  No counterpart in the reference, whose loop decides per step. */
+int mhx_set_dudr(mhx_plan *p, int on);
+
 /* durations (ms) of the stages of the last mhx_nl_step on this plan,
  * measured with HIP events on the launch stream: [0] synthesis Legendre,
  * [1] fused grid stage (inverse FFTs, grid products, forward FFTs), [2] and

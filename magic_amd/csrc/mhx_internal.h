@@ -253,7 +253,7 @@ struct Plan {
     float *d_twn32 = nullptr;       // n_phi complex: exp(-2 pi i k/n_phi), FP32
     int grid_form = 0;              // grid stage of the split step: 0 tiles where supported, 1 slots
     // the nonlinear step's du/dr grids (synthesis of dw, ddw, dz and their
-    // transforms); off only inside mhx_rloop's fused route, for one call
+    // transforms): mhx_set_dudr; off inside mhx_rloop's fused route
     bool dudr = true;
 
     // split-precision tables (MHX_PREC_SPLIT16): FP16 hi/lo pairs in the

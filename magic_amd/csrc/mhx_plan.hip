@@ -1103,8 +1103,9 @@ static int nl_step_impl(mhx_plan *h, int nlev, const double *const *in, size_t l
     // a split-precision plan runs the Legendre stage in scaled FP16 x3 and
     // keeps Fourier data and grids in FP32; the per-call API stays FP64
     const bool split = P->precision == MHX_PREC_SPLIT16;
-    // du/dr grids: always when the caller asks for the grids
-    const bool dudr = P->dudr || grids;
+    // du/dr grids: always when the caller asks for the grids, or when the
+    // launches are timed one by one (their count assumes them)
+    const bool dudr = P->dudr || grids || P->timing;
     int nev = 0;
     auto mark = [&]() -> int {
         if (P->timing && nev < Plan::NEV) MHX_HIP_CHECK(hipEventRecord(P->ev[nev++], s));
@@ -1568,6 +1569,13 @@ int mhx_nl_step_grids(mhx_plan *h, int nlev, const double *const *in, size_t ld_
     int rc = nl_step_impl(h, nlev, in, ld_lm, out, or2, or4, orho1, LFfac, lcut, grids, stream);
     if (rc) return rc;
     return finish_step(h, stream);
+}
+
+int mhx_set_dudr(mhx_plan *h, int on)
+{
+    if (!h) return fail(MHX_E_INVALID, "null plan");
+    h->p.dudr = on != 0;
+    return MHX_OK;
 }
 
 int mhx_set_timing(mhx_plan *h, int on)

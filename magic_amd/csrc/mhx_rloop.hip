@@ -334,12 +334,12 @@ int rl_fused(mhx_plan *h, const mhx_rloop_phys *ph, int nlev, const mhx_rloop_ra
         vhi[i] = damp / rad->delxh2[i];
         eps[i] = ph->epsc * rad->epscProf[i];
     }
-    // the step's du/dr grids are not among the loop's outputs (kept when the
-    // plan times its launches one by one, whose count assumes them)
-    P->dudr = P->timing;
+    // the step's du/dr grids are not among the loop's outputs
+    const bool keep = P->dudr;
+    P->dudr = false;
     rc = mhx_nl_step_courant(h, nlev, fin, ld_lm, nl, rad->or2, rad->or4, rad->orho1, ph->LFfac, rad->l_R,
                              ph->courfac, ph->alffac, vri.data(), vhi.data(), vr2.data(), vh2.data(), stream);
-    P->dudr = true;
+    P->dudr = keep;
     if (rc) return rc;
     // AdvrLM AdvtLM AdvpLM VStLM VxBrLM VxBtLM VxBpLM, then w dw z
     const double *tin[10] = {nl[0], nl[1], nl[2], nl[4], nl[6], nl[7], nl[8], in->w, in->dw, in->z};

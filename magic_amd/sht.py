@@ -62,6 +62,7 @@ _SIGS = {
     "mhx_rloop": "Ppippzpp",
     "mhx_nl_step_courant": "Pipzppppdpddppppp",
     "mhx_set_timing": "Pi",
+    "mhx_set_dudr": "Pi",
     "mhx_stage_times": "Pp",
     "mhx_launch_times": "Ppi",
     "mhx_batch_create": "piip",
@@ -320,6 +321,10 @@ class Plan:
 
     def set_timing(self, on=True):
         _check(self._L.mhx_set_timing(self.h, 1 if on else 0))
+
+    def set_dudr(self, on=True):
+        """whether the nonlinear step computes the du/dr grids (mhx_set_dudr)"""
+        _check(self._L.mhx_set_dudr(self.h, 1 if on else 0))
 
     def stage_times(self):
         ms = (ctypes.c_double * 5)()

@@ -120,3 +120,40 @@ def test_nl_step_full_size_properties():
     q, _, _ = p.spat_to_qst(G[lev, 0], G[lev, 1], G[lev, 2], L)
     ll = workload.lm_index(L)[0]
     assert rel_l2(q, ll * (ll + 1) * inputs[0][lev]) < 1e-12
+
+
+def test_nl_step_without_dudr():
+    """mhx_set_dudr(p, 0): the step skips the du/dr grids (their synthesis and
+    grid writes), which none of its ten outputs uses.  At l_max=255 (the
+    theta-row tile grid stage in the split form, the slot stage in FP64),
+    two calls in a row (the second replays the step's captured graph): the
+    FP64 outputs are identical to the full step's, and the split outputs are
+    as close to the FP64 step as the full split step's, within a factor 2
+    (the tile stage transforms empty halves in place of the du/dr
+    components, which changes FP32 rounding only)."""
+    L, nlev = 255, 4
+    nt, nph = workload.grid_size(L)
+    print("nl_step without du/dr, l_max=255 x", nlev, flush=True)
+    inputs = workload.nl_inputs(L, nlev, stream=9)
+    rad = radial(nlev)
+    lcut = [L] * nlev
+
+    def run(precision, dudr):
+        p = sht.Plan(L, nt, nph, precision=precision, max_levels=nlev)
+        p.set_dudr(dudr)
+        _, first = _run(p, inputs, rad, lcut, with_grids=False)
+        _, second = _run(p, inputs, rad, lcut, with_grids=False)
+        p.close()
+        for a, b in zip(first, second):
+            assert np.array_equal(a, b)
+        return second
+
+    full64, skip64 = run(sht.PREC_FP64, True), run(sht.PREC_FP64, False)
+    for k in range(10):
+        assert np.array_equal(full64[k], skip64[k]), sht.NL_OUTPUTS[k]
+    full16, skip16 = run(sht.PREC_SPLIT16, True), run(sht.PREC_SPLIT16, False)
+    for k in range(10):
+        e_skip, e_full = rel_l2(skip16[k], full64[k]), rel_l2(full16[k], full64[k])
+        print("  %-7s vs FP64: without du/dr %.2e, full %.2e" % (sht.NL_OUTPUTS[k], e_skip, e_full),
+              flush=True)
+        assert e_skip <= 2 * e_full + 1e-7, (sht.NL_OUTPUTS[k], e_skip, e_full)
