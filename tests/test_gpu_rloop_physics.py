@@ -162,3 +162,41 @@ def test_rloop_fused_route_skips_dudr_at_l255(precision):
         assert e_route <= 2 * e_full + 1e-7, (n, e_route, e_full)
     for a, b, c in ((dtr, dtr2, dtr64), (dth, dth2, dth64)):
         assert np.max(np.abs(a / c - 1)) <= 2 * np.max(np.abs(b / c - 1)) + 1e-7
+
+
+def test_rloop_own_kernels_at_l255_match_oracle():
+    """The loop's own kernels at l_max=255 (n_phi = 768: the grid stage
+    k_rl_grid_s<3, 1>, rows of 384 points by wave slots), with the u.grad u
+    dump's switches (outside the fused step's sets), on two levels of
+    synthetic fields against the oracle's radial loop (FP64, 1e-11)."""
+    from oracle import oracle
+    from rloop_reference import oracle_rloop
+    from magic_amd import workload
+    L, nlev = 255, 2
+    nt, nph = workload.grid_size(L)
+    d = rc.load("bouss_nocurl")
+    ph = rc.phys_of(d)
+    assert not ph["l_adv_curl"]
+    print("rloop own kernels l_max=255 x", nlev, flush=True)
+    ins = workload.nl_inputs(L, nlev, stream=11)
+    fields = {n: None for n in sht.RLOOP_IN}
+    fields.update(dict(zip(sht.NL_INPUTS, ins)))
+    r = np.array([0.8, 1.2])
+    one, zero = np.ones(nlev), np.zeros(nlev)
+    rad = {"r": r, "or1": 1 / r, "or2": 1 / r ** 2, "or4": 1 / r ** 4, "orho1": one, "beta": zero,
+           "otemp1": one, "visc": one, "lambda": one, "temp0": one, "epscProf": one,
+           "delxr2": np.full(nlev, 1e-3), "delxh2": np.full(nlev, 2e-2),
+           "nR": np.array([5, 9], dtype=np.int32), "l_R": np.full(nlev, L, dtype=np.int32)}
+    names = rc.outputs_of(d)
+    p = sht.Plan(L, nt, nph, max_levels=nlev)
+    outs = {n: np.zeros((nlev, p.lm_max), dtype=np.complex128) for n in names}
+    dtr, dth = np.zeros(nlev), np.zeros(nlev)
+    sht.rloop(p, ph, nlev, rad, fields, outs, dtr, dth)
+    p.close()
+    o = oracle.Plan(L, nt, nph)
+    want, wtr, wth = oracle_rloop(o, ph, nlev, rad, fields, 1)
+    for n in names:
+        e = rel_l2(outs[n], want[n])
+        print("  %-8s %.2e" % (n, e), flush=True)
+        assert e < 1e-11, (n, e)
+    assert rel_l2(dtr, wtr) < 1e-11 and rel_l2(dth, wth) < 1e-11
